@@ -1,0 +1,214 @@
+/*
+ * ptgpu.h — C ABI of the MI355X (gfx950) per-grain tracking library (libptgpu.so).
+ *
+ * The library replaces the per-frame-pair hot path of p-tracker's particle_tracking()
+ * (src/p-tracker.cpp:460-581): precomputation, integer ZNCC search over shifts x rotation
+ * steps (main, rescue and super-rescue zones), bicubic sampling, the subpixel NCC
+ * objective and its optimizer, and a distortion-correction remap built on undistor().
+ *
+ * Conventions
+ *  - Every function returns int: PTG_OK (0) on success, a negative PTG_ERR_* code on
+ *    failure; ptg_last_error(ctx) then holds a one-line message. No exception crosses
+ *    this boundary.
+ *  - Frames are row-major uint16, element (x, y) at base[y * pitch + x]. This is the
+ *    reference's image[i][x][y] (src/p-tracker.hpp:80) relabelled, not transposed data.
+ *  - Grain records are caller-owned host arrays copied in and out; the context owns
+ *    all device memory except frames attached with ptg_set_image_device().
+ *  - One context per host thread. All work of a context runs in order on one HIP
+ *    stream; calls return before the GPU finishes, except those that hand results
+ *    back to host memory (ptg_get_grains, ptg_get_queue_counts, ptg_ncc_objective*,
+ *    ptg_interp_cubic, ptg_get_stage_times, ptg_sync).
+ *  - Reads outside a frame are undefined behaviour in the reference
+ *    (src/interpol.cpp:61, src/p-tracker.cpp:703). Here they are clamped to the frame
+ *    edge and the grain's `status` gets PTG_STATUS_CLAMPED, so no call can fault.
+ */
+#ifndef PTGPU_H
+#define PTGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTG_ABI_VERSION 1
+
+enum {
+  PTG_OK = 0,
+  PTG_ERR_ARG = -1,    /* bad argument (null pointer, size, index, zone) */
+  PTG_ERR_HIP = -2,    /* HIP runtime error */
+  PTG_ERR_STATE = -3,  /* call order: e.g. search before images/grains are set */
+  PTG_ERR_NOMEM = -4,  /* device allocation failed */
+  PTG_ERR_LIMIT = -5   /* a size above a documented kernel limit */
+};
+
+/* Per-grain status flags (ptg_grain.status). */
+enum { PTG_STATUS_CLAMPED = 1 };
+
+/* Search stages of one frame pair, src/p-tracker.cpp:472-537. */
+enum { PTG_STAGE_MAIN = 0, PTG_STAGE_RESCUE = 1, PTG_STAGE_SUPER = 2 };
+
+/* Integer-search arithmetic.
+ *  PLAIN: per candidate, the reference's two-pass FP64 sums in the reference's pixel
+ *         order, unfused; NCC is bit-identical to follow_pattern_pixel().
+ *  EXACT: per candidate, exact integer moments (sum I1, sum I1^2, sum I0*I1) from an
+ *         INT8 MFMA correlation, then FP64 normalisation. Picks equal PLAIN's except
+ *         where two different candidates' NCC differ by less than FP64 rounding. */
+enum { PTG_SEARCH_PLAIN = 0, PTG_SEARCH_EXACT = 1 };
+
+/* Subpixel optimizer: POWELL is derivative-free, as the reference
+ * (src/p-tracker.cpp:1114-1116); LM is Levenberg-Marquardt driven by the analytic
+ * gradient of the ZNCC residual through the bicubic sampler. */
+enum { PTG_SUBPIX_POWELL = 0, PTG_SUBPIX_LM = 1 };
+
+/* Same members, order and layout as search_zone_type (src/search_zone_type.hpp:20-27). */
+typedef struct ptg_search_zone {
+  int32_t left;    /* towards decreasing x */
+  int32_t right;   /* towards increasing x */
+  int32_t up;      /* towards decreasing y */
+  int32_t down;    /* towards increasing y */
+  int32_t num_rot; /* rotation steps each side */
+  double inc_rot;  /* rotation increment (rad) */
+} ptg_search_zone;
+
+/* The scalar members of grain_type_2D (src/grain_type_2D.hpp:23-56), in the order of
+ * the 13 dic_out columns (src/p-tracker.cpp:1331-1343) followed by the backups,
+ * the precomputed reference moments and the mask. The pattern offset list is passed
+ * separately (ptg_set_grains). */
+typedef struct ptg_grain {
+  int32_t refcoord_xpix; /* col 1 */
+  int32_t refcoord_ypix; /* col 2 */
+  double refrot;         /* col 3 */
+  double radius_pix;     /* col 4 */
+  double dx, dy, drot;   /* col 5-7: cumulative motion */
+  double upix, vpix, rot_inc;          /* col 8-10: increments of the last pair */
+  double NCC, NCC_rescue, NCC_subpix;  /* col 11-13 */
+  double dx_prev, dy_prev, drot_prev;  /* backup() */
+  double mean0, C0C0;                  /* do_precomputations() */
+  int32_t masked;
+  int32_t status; /* PTG_STATUS_* flags, written by the library */
+} ptg_grain;
+
+/* Subpixel stage parameters; defaults of src/p-tracker.hpp:122-127 in brackets. */
+typedef struct ptg_subpix_params {
+  double subpix_tol;              /* [1e-8] Powell relative tolerance on the objective */
+  double initial_direction[3];    /* [0.01, 0.01, 0.01] Powell initial step per axis */
+  double subpix_displacement_max; /* [3.0] */
+  double overflow_stiffness;      /* [0.0] */
+  int32_t method;                 /* PTG_SUBPIX_* */
+  int32_t max_iter;               /* Powell outer iterations / LM iterations cap, <=0: 200 */
+} ptg_subpix_params;
+
+/* One frame pair, src/p-tracker.cpp:472-581. */
+typedef struct ptg_track_params {
+  ptg_search_zone search_zone;
+  ptg_search_zone search_zone_rescue;
+  ptg_search_zone search_zone_super_rescue;
+  double NCC_min;       /* [0.7] */
+  double NCC_min_super; /* [0.5] */
+  int32_t rescue_level; /* [2] <0: no integer search; >=1 rescue; >=2 super-rescue */
+  int32_t subpixel;     /* [1] */
+  int32_t search_method;/* PTG_SEARCH_* */
+  int32_t reserved;
+  ptg_subpix_params subpix;
+} ptg_track_params;
+
+/* ---- context ------------------------------------------------------------------ */
+int ptg_create(void** ctx, int device);
+int ptg_destroy(void* ctx);
+const char* ptg_last_error(const void* ctx);
+int ptg_abi_version(void);
+
+/* ---- frames: image[index], index 0..3 (0/1 are the reference's image[0]/image[1]) -- */
+/* Replaces read_image() storage (src/p-tracker.cpp:75-201): copies a host frame. */
+int ptg_set_image(void* ctx, int index, const uint16_t* host, int32_t dimx, int32_t dimy,
+                  int64_t pitch, int32_t bits);
+/* Attaches a caller-owned device frame (no copy); it must outlive its use. */
+int ptg_set_image_device(void* ctx, int index, const uint16_t* dev, int32_t dimx, int32_t dimy,
+                         int64_t pitch, int32_t bits);
+/* im_index_ref / im_index_current, src/p-tracker.hpp:82-83. */
+int ptg_set_image_indices(void* ctx, int im_index_ref, int im_index_current);
+
+/* ---- grains ------------------------------------------------------------------- */
+/* grain[] and each grain's pattern (src/grain_type_2D.hpp:58): pattern_xy holds
+ * (dx, dy) int pairs, grain g owning pairs [pattern_begin[g], pattern_begin[g+1]). */
+int ptg_set_grains(void* ctx, int32_t num_grains, const ptg_grain* grains,
+                   const int32_t* pattern_xy, const int64_t* pattern_begin);
+/* Overwrites the scalar records only (patterns kept). */
+int ptg_put_grains(void* ctx, const ptg_grain* grains);
+/* Copies the records back (synchronises). */
+int ptg_get_grains(void* ctx, ptg_grain* out);
+
+/* ---- the per-pair stages -------------------------------------------------------- */
+/* grain.reset(); grain.backup() for every grain, src/p-tracker.cpp:478-479. */
+int ptg_begin_pair(void* ctx);
+/* do_precomputations(), src/p-tracker.cpp:624-656. */
+int ptg_precompute(void* ctx);
+/* follow_pattern_pixel / _rescue_pixel / _super_rescue_pixel over the stage's grain
+ * list, src/p-tracker.cpp:659-1016. MAIN runs every unmasked grain and queues those
+ * with best NCC < NCC_min; RESCUE runs that queue and queues those < NCC_min_super;
+ * SUPER runs the second queue. */
+int ptg_search(void* ctx, int stage, const ptg_search_zone* zone, double NCC_min,
+               double NCC_min_super, int method);
+/* follow_pattern_subpixel_xyR for every unmasked grain, src/p-tracker.cpp:1090-1128. */
+int ptg_subpixel(void* ctx, const ptg_subpix_params* params);
+/* upix = dx - dx_prev, ..., src/p-tracker.cpp:577-581. */
+int ptg_end_pair(void* ctx);
+/* The reference swap at the end of a pair, src/p-tracker.cpp:587-611. */
+int ptg_change_reference(void* ctx);
+/* One whole frame pair, src/p-tracker.cpp:460-581 (precomputation when needed, the
+ * stages gated by rescue_level and subpixel, then the increments). */
+int ptg_track_pair(void* ctx, const ptg_track_params* params);
+/* num_to_be_rescued / num_to_be_super_rescued of the last pair (synchronises). */
+int ptg_get_queue_counts(void* ctx, int32_t* num_to_be_rescued, int32_t* num_to_be_super_rescued);
+int ptg_sync(void* ctx);
+
+/* ---- the subpixel API points ----------------------------------------------------- */
+/* NCC_to_minimize_xyR(X) for grain igrain, src/p-tracker.cpp:1018-1088: the value the
+ * NCC_optimizer functor returns (src/NCC_optimizer_functor.cpp:20). Batched form:
+ * n evaluations, grain igrain[k] at X[3k..3k+2] -> f[k]. */
+int ptg_ncc_objective(void* ctx, int32_t igrain, const double* X, double subpix_displacement_max,
+                      double overflow_stiffness, double* f);
+int ptg_ncc_objective_batch(void* ctx, int32_t n, const int32_t* igrain, const double* X,
+                            double subpix_displacement_max, double overflow_stiffness, double* f);
+/* Gradient of the same objective with respect to X (analytic, through the sampler). */
+int ptg_ncc_objective_grad_batch(void* ctx, int32_t n, const int32_t* igrain, const double* X,
+                                 double* f, double* grad);
+/* image_interpCubic::getValue(image[index], x, y), src/interpol.cpp:55-86, for n points
+ * xy[2k], xy[2k+1] -> out[k]. */
+int ptg_interp_cubic(void* ctx, int index, int32_t n, const double* xy, double* out);
+
+/* ---- distortion ------------------------------------------------------------------ */
+/* undistor(X, xd, yd, xu, yu), src/distortion_correction.cpp:20-61, for n points. */
+int ptg_undistort_points(void* ctx, const double* params14, int32_t n, const double* xy_d,
+                         double* xy_u);
+/* Distortion-correction remap: image[dst_index](x, y) = bicubic sample of
+ * image[src_index] at the distorted point whose undistor() image is (x, y), rounded to
+ * the nearest integer and clamped to [0, 2^bits - 1]. The inverse is found by Newton
+ * iteration on undistor(). dst gets src's size and bits. */
+int ptg_remap(void* ctx, int src_index, int dst_index, const double* params14);
+
+/* ---- timing ---------------------------------------------------------------------- */
+/* Stage timers: HIP events around each stage on the context's stream. */
+enum {
+  PTG_T_PRECOMPUTE = 0,
+  PTG_T_SEARCH_MAIN = 1,
+  PTG_T_SEARCH_RESCUE = 2,
+  PTG_T_SEARCH_SUPER = 3,
+  PTG_T_SUBPIXEL = 4,
+  PTG_T_REMAP = 5,
+  PTG_T_SEARCH_KERNEL = 6, /* the candidate-scoring kernel alone, main stage */
+  PTG_T_COUNT = 7
+};
+int ptg_set_timing(void* ctx, int enable);
+/* Accumulated milliseconds and launch counts since the last reset (synchronises). */
+int ptg_get_stage_times(void* ctx, double* ms, int64_t* launches, int n);
+int ptg_reset_stage_times(void* ctx);
+/* Work counters of the last subpixel stage: total objective evaluations. */
+int ptg_get_subpix_evals(void* ctx, int64_t* evals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTGPU_H */

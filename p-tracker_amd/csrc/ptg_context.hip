@@ -1,0 +1,894 @@
+// ptg_context.hip — the C ABI of libptgpu: context, frames, grains, and the per-pair
+// stage sequence of p-tracker's particle_tracking() (src/p-tracker.cpp:460-581).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ptg_internal.hpp"
+
+using namespace ptg;
+
+namespace ptg {
+
+int Context::fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  err = buf;
+  return code;
+}
+
+int Context::check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PTG_OK;
+  return fail(PTG_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int Context::ensure(void** p, size_t* cap, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (*p && *cap >= bytes) return PTG_OK;
+  if (*p) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return check(e, "hipStreamSynchronize");
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(PTG_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  *cap = bytes;
+  return PTG_OK;
+}
+
+hipEvent_t Context::get_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void Context::mark_begin(int stage, hipEvent_t* ev) {
+  *ev = nullptr;
+  if (!timing) return;
+  *ev = get_event();
+  if (*ev) (void)hipEventRecord(*ev, stream);
+  (void)stage;
+}
+
+void Context::mark_end(int stage, hipEvent_t ev) {
+  if (!timing || !ev) return;
+  hipEvent_t e2 = get_event();
+  if (!e2) return;
+  (void)hipEventRecord(e2, stream);
+  pending.push_back({stage, ev, e2});
+}
+
+// Fills n int32 with a value (used instead of hipMemsetAsync so a captured stream
+// records a kernel node).
+__global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static hipError_t fill_i32(Context& c, int32_t* p, int n, int32_t v) {
+  k_fill_i32<<<(n + 255) / 256, 256, 0, c.stream>>>(p, n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_i64(long long* p, int n, long long v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace ptg
+
+namespace {
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(Context& c, size_t bytes) {
+    if (p && cap >= bytes) return PTG_OK;
+    if (p) {
+      (void)hipStreamSynchronize(c.stream);
+      (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    if (hipHostMalloc(&p, bytes < 64 ? 64 : bytes, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return c.fail(PTG_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+    }
+    cap = bytes;
+    return PTG_OK;
+  }
+};
+
+// The context as seen through the C ABI, with the host-side staging it owns.
+struct Ctx : Context {
+  PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list;
+  enum { SNAP_INVALID = 0, SNAP_VALID = 1, SNAP_QUEUES = 2 } snap = SNAP_INVALID;
+  std::vector<double> h_steps;
+  std::vector<int32_t> h_unmasked;
+  // recorded after the last upload from the pinned staging buffers, waited on before
+  // the host rewrites them
+  hipEvent_t staging_done = nullptr;
+  int wait_staging() {
+    if (staging_done) {
+      hipError_t e = hipEventSynchronize(staging_done);
+      if (e != hipSuccess) return check(e, "hipEventSynchronize");
+    }
+    return PTG_OK;
+  }
+  int mark_staging() {
+    if (!staging_done && hipEventCreateWithFlags(&staging_done, hipEventDisableTiming) != hipSuccess)
+      return fail(PTG_ERR_HIP, "hipEventCreate failed");
+    return check(hipEventRecord(staging_done, stream), "hipEventRecord");
+  }
+};
+
+Ctx* as_ctx(void* p) { return reinterpret_cast<Ctx*>(p); }
+
+#define CK(c, expr)                                                  \
+  do {                                                               \
+    hipError_t e_ = (expr);                                          \
+    if (e_ != hipSuccess) return (c)->check(e_, #expr);              \
+  } while (0)
+
+#define TRY(expr)             \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != PTG_OK) return r_; \
+  } while (0)
+
+int valid_index(Ctx* c, int index) {
+  if (index < 0 || index >= kMaxImages) return c->fail(PTG_ERR_ARG, "image index %d outside [0, %d)", index, kMaxImages);
+  return PTG_OK;
+}
+
+// Rotation steps of src/p-tracker.cpp:684-686 in double, as the reference.
+int make_steps(Ctx* c, const ptg_search_zone* z, std::vector<double>& out) {
+  out.clear();
+  if (!(z->inc_rot > 0.0))
+    return c->fail(PTG_ERR_ARG, "inc_rot must be > 0 (the reference loop does not end otherwise; "
+                                "`rotations 0` sets it to 1.0)");
+  if (z->num_rot < 0) return c->fail(PTG_ERR_ARG, "num_rot must be >= 0");
+  double rotmax = fabs(z->inc_rot * z->num_rot);
+  for (double drot = -rotmax; drot <= rotmax; drot += z->inc_rot) {
+    out.push_back(drot);
+    if (out.size() > 100000) return c->fail(PTG_ERR_LIMIT, "more than 100000 rotation steps");
+  }
+  return PTG_OK;
+}
+
+int sync_snapshot(Ctx* c) {
+  if (c->G == 0) return PTG_OK;
+  CK(c, hipMemcpyAsync(c->h_grains.data(), c->d_grains, sizeof(ptg_grain) * c->G, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  c->snap = Ctx::SNAP_VALID;
+  return PTG_OK;
+}
+
+int need_grains(Ctx* c) {
+  if (c->G <= 0) return c->fail(PTG_ERR_STATE, "no grains set (ptg_set_grains)");
+  return PTG_OK;
+}
+
+int need_images(Ctx* c, bool ref, bool cur) {
+  if (ref && !c->img[c->iref].d) return c->fail(PTG_ERR_STATE, "image[%d] (reference) not set", c->iref);
+  if (cur && !c->img[c->icur].d) return c->fail(PTG_ERR_STATE, "image[%d] (current) not set", c->icur);
+  if (ref && cur) {
+    const Image &a = c->img[c->iref], &b = c->img[c->icur];
+    if (a.dimx != b.dimx || a.dimy != b.dimy)
+      return c->fail(PTG_ERR_ARG, "image sizes differ: %dx%d vs %dx%d", a.dimx, a.dimy, b.dimx, b.dimy);
+  }
+  return PTG_OK;
+}
+
+int do_precompute(Ctx* c) {
+  TRY(need_grains(c));
+  TRY(need_images(c, true, false));
+  // cos/sin(refrot) per grain from the host's libm, as do_precomputations (:636-637)
+  TRY(c->wait_staging());
+  TRY(c->pin_cs0.ensure(*c, sizeof(double) * 2 * c->G));
+  double* cs0 = (double*)c->pin_cs0.p;
+  for (int g = 0; g < c->G; g++) {
+    cs0[2 * g] = cos(c->h_grains[g].refrot);
+    cs0[2 * g + 1] = sin(c->h_grains[g].refrot);
+  }
+  hipEvent_t ev;
+  c->mark_begin(PTG_T_PRECOMPUTE, &ev);
+  CK(c, hipMemcpyAsync(c->d_cs0, cs0, sizeof(double) * 2 * c->G, hipMemcpyHostToDevice, c->stream));
+  TRY(c->mark_staging());
+  CK(c, launch_precompute(*c));
+  c->mark_end(PTG_T_PRECOMPUTE, ev);
+  c->require_precomputations = false;
+  return PTG_OK;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, double NCC_min_super, int method) {
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (!z) return c->fail(PTG_ERR_ARG, "null search zone");
+  if (stage < PTG_STAGE_MAIN || stage > PTG_STAGE_SUPER) return c->fail(PTG_ERR_ARG, "bad stage %d", stage);
+  if (method != PTG_SEARCH_PLAIN && method != PTG_SEARCH_EXACT) return c->fail(PTG_ERR_ARG, "bad method %d", method);
+  if (z->left < 0 || z->right < 0 || z->up < 0 || z->down < 0)
+    return c->fail(PTG_ERR_ARG, "search zone extents must be >= 0");
+  if ((int64_t)(z->left + z->right + 1) * (z->up + z->down + 1) > (1 << 24))
+    return c->fail(PTG_ERR_LIMIT, "search zone too large");
+  if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(make_steps(c, z, c->h_steps));
+  const int nsteps = (int)c->h_steps.size();
+
+  // the stage's grain list on the host
+  std::vector<int32_t> list;
+  int32_t* d_list = c->d_list[stage];
+  int32_t* d_count = c->d_counts + stage;
+  if (stage == PTG_STAGE_MAIN) {
+    if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
+    list = c->h_unmasked;  // built at set/put time
+  } else {
+    // the queue was filled on the device by the previous stage
+    if (c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
+    int32_t n = 0;
+    CK(c, hipMemcpyAsync(&n, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+    list.resize(n);
+    if (n) CK(c, hipMemcpyAsync(list.data(), d_list, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+  }
+  const int nlist = (int)list.size();
+  // next queue counter starts at zero
+  if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
+  if (nlist == 0) return PTG_OK;
+
+  // rotation tables: cos/sin(refrot + drot + step) per (grain, step) from the host's
+  // libm, evaluated as the reference (:687-689)
+  TRY(c->wait_staging());
+  TRY(c->pin_cs.ensure(*c, sizeof(double) * 2 * (size_t)c->G * nsteps));
+  TRY(c->pin_steps.ensure(*c, sizeof(double) * nsteps));
+  double* cs = (double*)c->pin_cs.p;
+  memcpy(c->pin_steps.p, c->h_steps.data(), sizeof(double) * nsteps);
+  const double* steps = c->h_steps.data();
+#pragma omp parallel for schedule(static)
+  for (int li = 0; li < nlist; li++) {
+    const int g = list[li];
+    const ptg_grain& gr = c->h_grains[g];
+    double* row = cs + 2 * (size_t)g * nsteps;
+    for (int k = 0; k < nsteps; k++) {
+      double total_rot = gr.refrot + gr.drot + steps[k];
+      row[2 * k] = cos(total_rot);
+      row[2 * k + 1] = sin(total_rot);
+    }
+  }
+  TRY(c->ensure((void**)&c->d_cs, &c->cs_cap, sizeof(double) * 2 * (size_t)c->G * nsteps));
+  TRY(c->ensure((void**)&c->d_steps, &c->steps_cap, sizeof(double) * nsteps));
+  TRY(c->ensure((void**)&c->d_best, &c->best_cap, sizeof(StepBest) * (size_t)nlist * nsteps));
+  // upload only the rows in use: for the main stage that is nearly all of them
+  if (stage == PTG_STAGE_MAIN) {
+    CK(c, hipMemcpyAsync(c->d_cs, cs, sizeof(double) * 2 * (size_t)c->G * nsteps, hipMemcpyHostToDevice, c->stream));
+  } else {
+    for (int li = 0; li < nlist; li++) {
+      const size_t off = 2 * (size_t)list[li] * nsteps;
+      CK(c, hipMemcpyAsync(c->d_cs + off, cs + off, sizeof(double) * 2 * nsteps, hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
+  TRY(c->mark_staging());
+
+  // LDS: the largest window and pattern of the list
+  int64_t win_cap = 0, off_cap = 0;
+  const int U = z->left + z->right + 1, V = z->up + z->down + 1;
+  for (int li = 0; li < nlist; li++) {
+    const int g = list[li];
+    const int rho = c->h_prad[g];
+    win_cap = std::max<int64_t>(win_cap, (int64_t)(U + 2 * rho) * (V + 2 * rho));
+    off_cap = std::max<int64_t>(off_cap, c->h_pbegin[g + 1] - c->h_pbegin[g]);
+  }
+
+  SearchArgs a;
+  const Image &i0 = c->img[c->iref], &i1 = c->img[c->icur];
+  a.I0 = i0.d;
+  a.I1 = i1.d;
+  a.dimx = i1.dimx;
+  a.dimy = i1.dimy;
+  a.pitch0 = i0.pitch;
+  a.pitch1 = i1.pitch;
+  a.grains = c->d_grains;
+  a.pattern = c->d_pattern;
+  a.pbegin = c->d_pbegin;
+  a.p0 = c->d_p0;
+  a.d0 = c->d_d0;
+  a.prad = c->d_prad;
+  a.list = d_list;
+  a.count = d_count;
+  a.cs = c->d_cs;
+  a.steps = c->d_steps;
+  a.nsteps = nsteps;
+  a.left = z->left;
+  a.right = z->right;
+  a.up = z->up;
+  a.down = z->down;
+  a.win_cap = (int32_t)win_cap;
+  a.off_cap = (int32_t)off_cap;
+  a.best = c->d_best;
+
+  const int tstage = PTG_T_SEARCH_MAIN + stage;
+  hipEvent_t ev;
+  c->mark_begin(tstage, &ev);
+  if (method == PTG_SEARCH_PLAIN) {
+    const size_t lds = (size_t)(win_cap * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
+    if (lds > kMaxLds)
+      return c->fail(PTG_ERR_LIMIT, "search window + pattern need %zu B of LDS (max %zu)", lds, kMaxLds);
+    hipEvent_t ek;
+    if (stage == PTG_STAGE_MAIN) c->mark_begin(PTG_T_SEARCH_KERNEL, &ek);
+    CK(c, launch_search_plain(*c, a, nlist, lds));
+    if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
+  } else {
+    TRY(run_search_exact(*c, a, nlist));
+  }
+  int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
+  int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
+  CK(c, launch_search_commit(*c, a, nlist, stage, NCC_min, NCC_min_super, next_list, next_count));
+  c->mark_end(tstage, ev);
+  // successful grains changed their dx/drot; queued ones did not, so the snapshot
+  // still serves the following rescue stages
+  c->snap = Ctx::SNAP_QUEUES;
+  return PTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptg_abi_version(void) { return PTG_ABI_VERSION; }
+
+int ptg_create(void** out, int device) {
+  if (!out) return PTG_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PTG_ERR_HIP;
+  if (device < 0 || device >= n) return PTG_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return PTG_ERR_HIP;
+  Ctx* c = new Ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PTG_ERR_HIP;
+  }
+  if (hipMalloc((void**)&c->d_counts, sizeof(int32_t) * 4) != hipSuccess ||
+      hipMalloc((void**)&c->d_evals, sizeof(int64_t)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PTG_ERR_NOMEM;
+  }
+  k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_counts, 4, 0);
+  k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return PTG_ERR_HIP;
+  *out = c;
+  return PTG_OK;
+}
+
+static void free_grain_buffers(Ctx* c) {
+  void** bufs[] = {(void**)&c->d_grains, (void**)&c->d_pattern, (void**)&c->d_pbegin, (void**)&c->d_p0,
+                   (void**)&c->d_d0,     (void**)&c->d_prad,    (void**)&c->d_list[0], (void**)&c->d_list[1],
+                   (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0};
+  for (void** b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+}
+
+int ptg_destroy(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < kMaxImages; i++)
+    if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
+  free_grain_buffers(c);
+  void* more[] = {c->d_counts, c->d_cs, c->d_steps, c->d_best, c->d_evals, c->d_exact_scratch, c->d_tmp};
+  for (void* b : more)
+    if (b) (void)hipFree(b);
+  PinnedBuf* pins[] = {&c->pin_cs, &c->pin_steps, &c->pin_cs0, &c->pin_list};
+  for (PinnedBuf* b : pins)
+    if (b->p) (void)hipHostFree(b->p);
+  for (auto& pe : c->pending) {
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->staging_done) (void)hipEventDestroy(c->staging_done);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PTG_OK;
+}
+
+const char* ptg_last_error(const void* p) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(p);
+  if (!c) return "null context";
+  return c->err.c_str();
+}
+
+static int set_image_common(Ctx* c, int index, int32_t dimx, int32_t dimy, int64_t pitch, int32_t bits) {
+  TRY(valid_index(c, index));
+  if (dimx < 4 || dimy < 4) return c->fail(PTG_ERR_ARG, "image %dx%d too small", dimx, dimy);
+  if (pitch < dimx) return c->fail(PTG_ERR_ARG, "pitch %lld < dimx %d", (long long)pitch, dimx);
+  if (bits < 1 || bits > 16) return c->fail(PTG_ERR_ARG, "bits %d outside [1, 16]", bits);
+  if (index == c->iref) c->require_precomputations = true;
+  return PTG_OK;
+}
+
+int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_t dimy, int64_t pitch,
+                  int32_t bits) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  CK(c, hipSetDevice(c->device));
+  if (!host) return c->fail(PTG_ERR_ARG, "null host frame");
+  TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
+  Image& im = c->img[index];
+  if (!im.owned) {
+    im.d = nullptr;
+    im.cap = 0;
+  }
+  const size_t bytes = sizeof(uint16_t) * (size_t)dimx * dimy;
+  if (!im.owned || im.cap < bytes) {
+    if (im.owned && im.d) {
+      CK(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(im.d);
+    }
+    im.d = nullptr;
+    if (hipMalloc((void**)&im.d, bytes) != hipSuccess) {
+      im.owned = false;
+      return c->fail(PTG_ERR_NOMEM, "hipMalloc(%zu) for image[%d]", bytes, index);
+    }
+    im.cap = bytes;
+    im.owned = true;
+  }
+  im.dimx = dimx;
+  im.dimy = dimy;
+  im.pitch = dimx;
+  im.bits = bits;
+  CK(c, hipMemcpy2DAsync(im.d, sizeof(uint16_t) * dimx, host, sizeof(uint16_t) * pitch, sizeof(uint16_t) * dimx,
+                         dimy, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_set_image_device(void* p, int index, const uint16_t* dev, int32_t dimx, int32_t dimy, int64_t pitch,
+                         int32_t bits) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!dev) return c->fail(PTG_ERR_ARG, "null device frame");
+  TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
+  Image& im = c->img[index];
+  if (im.owned && im.d) {
+    CK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(im.d);
+  }
+  im.d = const_cast<uint16_t*>(dev);
+  im.owned = false;
+  im.cap = 0;
+  im.dimx = dimx;
+  im.dimy = dimy;
+  im.pitch = pitch;
+  im.bits = bits;
+  return PTG_OK;
+}
+
+int ptg_set_image_indices(void* p, int im_index_ref, int im_index_current) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(valid_index(c, im_index_ref));
+  TRY(valid_index(c, im_index_current));
+  if (im_index_ref != c->iref) c->require_precomputations = true;
+  c->iref = im_index_ref;
+  c->icur = im_index_current;
+  return PTG_OK;
+}
+
+static int upload_unmasked(Ctx* c) {
+  std::vector<int32_t>& um = c->h_unmasked;
+  um.clear();
+  for (int g = 0; g < c->G; g++)
+    if (!c->h_grains[g].masked) um.push_back(g);
+  c->n_unmasked = (int32_t)um.size();
+  if (!um.empty())
+    CK(c, hipMemcpyAsync(c->d_list[0], um.data(), sizeof(int32_t) * um.size(), hipMemcpyHostToDevice, c->stream));
+  CK(c, fill_i32(*c, c->d_counts, 1, c->n_unmasked));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* pattern_xy,
+                   const int64_t* pattern_begin) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  CK(c, hipSetDevice(c->device));
+  if (G < 0 || (G > 0 && (!grains || !pattern_begin))) return c->fail(PTG_ERR_ARG, "bad grain arguments");
+  if (G > 0 && pattern_begin[0] != 0) return c->fail(PTG_ERR_ARG, "pattern_begin[0] must be 0");
+  for (int g = 0; g < G; g++) {
+    const int64_t n = pattern_begin[g + 1] - pattern_begin[g];
+    if (n <= 0) return c->fail(PTG_ERR_ARG, "grain %d has an empty pattern", g);
+    if (n > (1 << 20)) return c->fail(PTG_ERR_LIMIT, "grain %d pattern has %lld pixels", g, (long long)n);
+  }
+  const int64_t total = G ? pattern_begin[G] : 0;
+  if (total > 0 && !pattern_xy) return c->fail(PTG_ERR_ARG, "null pattern_xy");
+  CK(c, hipStreamSynchronize(c->stream));
+  free_grain_buffers(c);
+  c->G = G;
+  c->total_px = total;
+  c->h_grains.assign(grains, grains + G);
+  c->h_pattern.assign(pattern_xy, pattern_xy + 2 * total);
+  c->h_pbegin.assign(pattern_begin, pattern_begin + G + 1);
+  c->h_prad.resize(G);
+  for (int g = 0; g < G; g++) {
+    int64_t m2 = 0;
+    for (int64_t i = pattern_begin[g]; i < pattern_begin[g + 1]; i++) {
+      const int64_t dx = pattern_xy[2 * i], dy = pattern_xy[2 * i + 1];
+      m2 = std::max(m2, dx * dx + dy * dy);
+    }
+    // every rotated offset round(c dx + s dy) lies within ceil(|(dx, dy)|) + 1
+    c->h_prad[g] = (int32_t)ceil(sqrt((double)m2)) + 1;
+  }
+  if (G == 0) return PTG_OK;
+  const size_t Gs = (size_t)G;
+  if (hipMalloc((void**)&c->d_grains, sizeof(ptg_grain) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_pattern, sizeof(int32_t) * 2 * total) != hipSuccess ||
+      hipMalloc((void**)&c->d_pbegin, sizeof(int64_t) * (Gs + 1)) != hipSuccess ||
+      hipMalloc((void**)&c->d_p0, sizeof(int32_t) * 2 * total) != hipSuccess ||
+      hipMalloc((void**)&c->d_d0, sizeof(double) * total) != hipSuccess ||
+      hipMalloc((void**)&c->d_prad, sizeof(int32_t) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_list[0], sizeof(int32_t) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_list[1], sizeof(int32_t) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_list[2], sizeof(int32_t) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_cs0, sizeof(double) * 2 * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_sd0, sizeof(double) * Gs) != hipSuccess) {
+    free_grain_buffers(c);
+    c->G = 0;
+    return c->fail(PTG_ERR_NOMEM, "grain buffers for %d grains / %lld pixels", G, (long long)total);
+  }
+  CK(c, hipMemcpyAsync(c->d_grains, grains, sizeof(ptg_grain) * Gs, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(c->d_pattern, pattern_xy, sizeof(int32_t) * 2 * total, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(c->d_pbegin, pattern_begin, sizeof(int64_t) * (Gs + 1), hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(c->d_prad, c->h_prad.data(), sizeof(int32_t) * Gs, hipMemcpyHostToDevice, c->stream));
+  CK(c, fill_i32(*c, c->d_counts, 4, 0));
+  TRY(upload_unmasked(c));
+  c->snap = Ctx::SNAP_VALID;
+  c->require_precomputations = true;
+  return PTG_OK;
+}
+
+int ptg_put_grains(void* p, const ptg_grain* grains) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  if (!grains) return c->fail(PTG_ERR_ARG, "null grains");
+  CK(c, hipStreamSynchronize(c->stream));
+  c->h_grains.assign(grains, grains + c->G);
+  CK(c, hipMemcpyAsync(c->d_grains, c->h_grains.data(), sizeof(ptg_grain) * c->G, hipMemcpyHostToDevice, c->stream));
+  TRY(upload_unmasked(c));
+  c->snap = Ctx::SNAP_VALID;
+  c->require_precomputations = true;
+  return PTG_OK;
+}
+
+int ptg_get_grains(void* p, ptg_grain* out) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!out) return c->fail(PTG_ERR_ARG, "null output");
+  if (c->G == 0) return PTG_OK;
+  TRY(sync_snapshot(c));
+  memcpy(out, c->h_grains.data(), sizeof(ptg_grain) * c->G);
+  return PTG_OK;
+}
+
+int ptg_begin_pair(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  CK(c, launch_begin_pair(*c));
+  CK(c, fill_i32(*c, c->d_counts + 1, 2, 0));
+  // NCC fields and backups changed; dx/drot did not
+  return PTG_OK;
+}
+
+int ptg_precompute(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  return do_precompute(c);
+}
+
+int ptg_search(void* p, int stage, const ptg_search_zone* zone, double NCC_min, double NCC_min_super, int method) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  return do_search(c, stage, zone, NCC_min, NCC_min_super, method);
+}
+
+int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!prm) return c->fail(PTG_ERR_ARG, "null subpixel params");
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (prm->method != PTG_SUBPIX_POWELL && prm->method != PTG_SUBPIX_LM)
+    return c->fail(PTG_ERR_ARG, "bad subpixel method %d", prm->method);
+  if (c->require_precomputations) TRY(do_precompute(c));
+  hipEvent_t ev;
+  c->mark_begin(PTG_T_SUBPIXEL, &ev);
+  k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
+  CK(c, hipGetLastError());
+  CK(c, launch_subpixel(*c, *prm));
+  c->mark_end(PTG_T_SUBPIXEL, ev);
+  c->snap = Ctx::SNAP_INVALID;
+  return PTG_OK;
+}
+
+int ptg_end_pair(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  CK(c, launch_end_pair(*c));
+  if (c->snap == Ctx::SNAP_VALID) {
+    for (auto& g : c->h_grains) {
+      g.upix = g.dx - g.dx_prev;
+      g.vpix = g.dy - g.dy_prev;
+      g.rot_inc = g.drot - g.drot_prev;
+    }
+  }
+  return PTG_OK;
+}
+
+int ptg_change_reference(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
+  CK(c, launch_change_reference(*c));
+  // the same update on the host snapshot keeps it valid (src/p-tracker.cpp:598-609)
+  for (auto& r : c->h_grains) {
+    double actual_pos = (double)(r.refcoord_xpix) + r.dx;
+    r.refcoord_xpix = (int32_t)round(actual_pos);
+    r.dx = actual_pos - r.refcoord_xpix;
+    actual_pos = (double)(r.refcoord_ypix) + r.dy;
+    r.refcoord_ypix = (int32_t)round(actual_pos);
+    r.dy = actual_pos - r.refcoord_ypix;
+    r.refrot += r.drot;
+    r.drot = 0.0;
+  }
+  std::swap(c->iref, c->icur);
+  c->require_precomputations = true;
+  return PTG_OK;
+}
+
+int ptg_track_pair(void* p, const ptg_track_params* prm) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!prm) return c->fail(PTG_ERR_ARG, "null track params");
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (c->require_precomputations) TRY(do_precompute(c));
+  if (prm->rescue_level >= 0) {
+    TRY(ptg_begin_pair(p));
+    TRY(do_search(c, PTG_STAGE_MAIN, &prm->search_zone, prm->NCC_min, prm->NCC_min_super, prm->search_method));
+  }
+  if (prm->rescue_level >= 1)
+    TRY(do_search(c, PTG_STAGE_RESCUE, &prm->search_zone_rescue, prm->NCC_min, prm->NCC_min_super,
+                  prm->search_method));
+  if (prm->rescue_level >= 2)
+    TRY(do_search(c, PTG_STAGE_SUPER, &prm->search_zone_super_rescue, prm->NCC_min, prm->NCC_min_super,
+                  prm->search_method));
+  if (prm->subpixel) TRY(ptg_subpixel(p, &prm->subpix));
+  TRY(ptg_end_pair(p));
+  return PTG_OK;
+}
+
+int ptg_get_queue_counts(void* p, int32_t* nres, int32_t* nsup) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  int32_t cnt[4] = {0, 0, 0, 0};
+  CK(c, hipMemcpyAsync(cnt, c->d_counts, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  if (nres) *nres = cnt[1];
+  if (nsup) *nsup = cnt[2];
+  return PTG_OK;
+}
+
+int ptg_sync(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+static int tmp_upload(Ctx* c, size_t bytes) { return c->ensure(&c->d_tmp, &c->tmp_cap, bytes); }
+
+static int objective_common(Ctx* c, int32_t n, const int32_t* igrain, const double* X, double dmax, double stiff,
+                            double* f, double* grad) {
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (n < 0 || (n > 0 && (!igrain || !X || !f))) return c->fail(PTG_ERR_ARG, "bad objective arguments");
+  for (int k = 0; k < n; k++)
+    if (igrain[k] < 0 || igrain[k] >= c->G) return c->fail(PTG_ERR_ARG, "grain index %d out of range", igrain[k]);
+  if (n == 0) return PTG_OK;
+  if (c->require_precomputations) TRY(do_precompute(c));
+  const size_t bi = ((size_t)n * 4 + 255) / 256 * 256, bx = (size_t)n * 24, bf = (size_t)n * 8, bg = (size_t)n * 24;
+  TRY(tmp_upload(c, bi + bx + bf + bg));
+  char* base = (char*)c->d_tmp;
+  int32_t* d_ig = (int32_t*)base;
+  double* d_X = (double*)(base + bi);
+  double* d_f = (double*)(base + bi + bx);
+  double* d_g = (double*)(base + bi + bx + bf);
+  CK(c, hipMemcpyAsync(d_ig, igrain, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(d_X, X, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, launch_objective_batch(*c, n, d_ig, d_X, dmax, stiff, d_f, grad ? d_g : nullptr));
+  CK(c, hipMemcpyAsync(f, d_f, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (grad) CK(c, hipMemcpyAsync(grad, d_g, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_ncc_objective(void* p, int32_t igrain, const double* X, double dmax, double stiff, double* f) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  return objective_common(c, 1, &igrain, X, dmax, stiff, f, nullptr);
+}
+
+int ptg_ncc_objective_batch(void* p, int32_t n, const int32_t* igrain, const double* X, double dmax, double stiff,
+                            double* f) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  return objective_common(c, n, igrain, X, dmax, stiff, f, nullptr);
+}
+
+int ptg_ncc_objective_grad_batch(void* p, int32_t n, const int32_t* igrain, const double* X, double* f,
+                                 double* grad) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!grad) return c->fail(PTG_ERR_ARG, "null gradient output");
+  return objective_common(c, n, igrain, X, 3.0, 0.0, f, grad);
+}
+
+int ptg_interp_cubic(void* p, int index, int32_t n, const double* xy, double* out) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(valid_index(c, index));
+  if (!c->img[index].d) return c->fail(PTG_ERR_STATE, "image[%d] not set", index);
+  if (n < 0 || (n > 0 && (!xy || !out))) return c->fail(PTG_ERR_ARG, "bad sampler arguments");
+  if (n == 0) return PTG_OK;
+  const size_t bx = ((size_t)n * 16 + 255) / 256 * 256;
+  TRY(tmp_upload(c, bx + (size_t)n * 8));
+  double* d_xy = (double*)c->d_tmp;
+  double* d_o = (double*)((char*)c->d_tmp + bx);
+  CK(c, hipMemcpyAsync(d_xy, xy, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, launch_interp_batch(*c, index, n, d_xy, d_o));
+  CK(c, hipMemcpyAsync(out, d_o, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_undistort_points(void* p, const double* params14, int32_t n, const double* xy_d, double* xy_u) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!params14 || n < 0 || (n > 0 && (!xy_d || !xy_u))) return c->fail(PTG_ERR_ARG, "bad undistort arguments");
+  if (n == 0) return PTG_OK;
+  const size_t bx = ((size_t)n * 16 + 255) / 256 * 256;
+  TRY(tmp_upload(c, 2 * bx));
+  double* d_in = (double*)c->d_tmp;
+  double* d_out = (double*)((char*)c->d_tmp + bx);
+  CK(c, hipMemcpyAsync(d_in, xy_d, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, launch_undistort_points(*c, params14, n, d_in, d_out));
+  CK(c, hipMemcpyAsync(xy_u, d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_remap(void* p, int src, int dst, const double* params14) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(valid_index(c, src));
+  TRY(valid_index(c, dst));
+  if (src == dst) return c->fail(PTG_ERR_ARG, "remap source and destination must differ");
+  if (!params14) return c->fail(PTG_ERR_ARG, "null distortion parameters");
+  const Image& s = c->img[src];
+  if (!s.d) return c->fail(PTG_ERR_STATE, "image[%d] not set", src);
+  Image& d = c->img[dst];
+  const size_t bytes = sizeof(uint16_t) * (size_t)s.dimx * s.dimy;
+  if (!d.owned || d.cap < bytes) {
+    CK(c, hipStreamSynchronize(c->stream));
+    if (d.owned && d.d) (void)hipFree(d.d);
+    d.d = nullptr;
+    if (hipMalloc((void**)&d.d, bytes) != hipSuccess) {
+      d.owned = false;
+      return c->fail(PTG_ERR_NOMEM, "hipMalloc(%zu) for image[%d]", bytes, dst);
+    }
+    d.owned = true;
+    d.cap = bytes;
+  }
+  d.dimx = s.dimx;
+  d.dimy = s.dimy;
+  d.pitch = s.dimx;
+  d.bits = s.bits;
+  if (dst == c->iref) c->require_precomputations = true;
+  hipEvent_t ev;
+  c->mark_begin(PTG_T_REMAP, &ev);
+  CK(c, launch_remap(*c, src, dst, params14));
+  c->mark_end(PTG_T_REMAP, ev);
+  return PTG_OK;
+}
+
+int ptg_set_timing(void* p, int enable) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  c->timing = enable != 0;
+  return PTG_OK;
+}
+
+static int drain_timing(Ctx* c) {
+  CK(c, hipStreamSynchronize(c->stream));
+  for (auto& pe : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+      c->stage_ms[pe.stage] += ms;
+      c->stage_launches[pe.stage] += 1;
+    }
+    c->event_pool.push_back(pe.a);
+    c->event_pool.push_back(pe.b);
+  }
+  c->pending.clear();
+  return PTG_OK;
+}
+
+int ptg_get_stage_times(void* p, double* ms, int64_t* launches, int n) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(drain_timing(c));
+  for (int i = 0; i < n && i < PTG_T_COUNT; i++) {
+    if (ms) ms[i] = c->stage_ms[i];
+    if (launches) launches[i] = c->stage_launches[i];
+  }
+  return PTG_OK;
+}
+
+int ptg_reset_stage_times(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(drain_timing(c));
+  for (int i = 0; i < PTG_T_COUNT; i++) {
+    c->stage_ms[i] = 0.0;
+    c->stage_launches[i] = 0;
+  }
+  return PTG_OK;
+}
+
+int ptg_get_subpix_evals(void* p, int64_t* evals) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!evals) return c->fail(PTG_ERR_ARG, "null output");
+  CK(c, hipMemcpyAsync(evals, c->d_evals, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+}  // extern "C"
+
+namespace ptg {
+// Placeholder until the MFMA form lands: reports that the exact form is unavailable.
+int run_search_exact(Context& c, const SearchArgs&, int) {
+  return c.fail(PTG_ERR_ARG, "PTG_SEARCH_EXACT is not available in this build");
+}
+}  // namespace ptg

@@ -1,0 +1,143 @@
+// ptg_internal.hpp — the context behind the C ABI of libptgpu and the launchers each
+// kernel translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ptgpu.h"
+
+namespace ptg {
+
+constexpr int kMaxImages = 4;
+
+struct Image {
+  uint16_t* d = nullptr;  // device frame, row-major
+  int32_t dimx = 0, dimy = 0;
+  int64_t pitch = 0;      // elements
+  int32_t bits = 16;
+  bool owned = false;
+  size_t cap = 0;         // bytes allocated when owned
+};
+
+// Per-candidate-step argmax of one (grain, rotation step) work item.
+struct StepBest {
+  double ncc;
+  int32_t idx;  // u-major candidate index within the step, INT32_MAX when none > 0
+  int32_t pad;
+};
+
+// Arguments of the integer-search kernels (plain and exact forms).
+struct SearchArgs {
+  const uint16_t* I0;
+  const uint16_t* I1;
+  int32_t dimx, dimy;
+  int64_t pitch0, pitch1;
+  ptg_grain* grains;
+  const int32_t* pattern;   // (dx, dy) pairs
+  const int64_t* pbegin;    // G + 1
+  const int32_t* p0;        // pattern0_rotated pairs, same indexing as pattern
+  const double* d0;         // I0 at p0 minus mean0, same indexing
+  const int32_t* prad;      // per grain: every rotated offset lies in [-prad, prad]^2
+  const int32_t* list;      // grains of this stage
+  const int32_t* count;     // device count of list entries
+  const double* cs;         // per list slot g: steps x (cos, sin) of refrot + drot + step
+  const double* steps;      // rotation-step values
+  int32_t nsteps;
+  int32_t left, right, up, down;
+  int32_t win_cap;          // LDS elements reserved for the window
+  int32_t off_cap;          // LDS elements reserved for rotated offsets
+  StepBest* best;           // list slot x nsteps
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  Image img[kMaxImages];
+  int iref = 0, icur = 1;
+
+  // grains
+  int32_t G = 0;
+  ptg_grain* d_grains = nullptr;
+  int32_t* d_pattern = nullptr;
+  int64_t* d_pbegin = nullptr;
+  int32_t* d_p0 = nullptr;
+  double* d_d0 = nullptr;
+  int32_t* d_prad = nullptr;
+  int64_t total_px = 0;
+  std::vector<int32_t> h_pattern;
+  std::vector<int64_t> h_pbegin;
+  std::vector<int32_t> h_prad;
+  std::vector<ptg_grain> h_grains;  // host snapshot, valid when h_valid
+  bool h_valid = false;
+  bool require_precomputations = true;
+
+  // stage lists: [0] unmasked grains, [1] rescue queue, [2] super-rescue queue
+  int32_t* d_list[3] = {nullptr, nullptr, nullptr};
+  int32_t* d_counts = nullptr;  // 3 counters
+  int32_t n_unmasked = 0;
+
+  // search scratch
+  double* d_cs = nullptr;
+  size_t cs_cap = 0;
+  double* d_steps = nullptr;
+  size_t steps_cap = 0;
+  StepBest* d_best = nullptr;
+  size_t best_cap = 0;
+  double* d_cs0 = nullptr;  // cos, sin of refrot per grain (precompute)
+
+  // subpixel
+  int64_t* d_evals = nullptr;
+  double* d_sd0 = nullptr;  // per grain: sum of d0 (used by the one-pass objective)
+
+  // exact-form scratch
+  void* d_exact_scratch = nullptr;
+  size_t exact_scratch_cap = 0;
+
+  // objective / sampler batches
+  void* d_tmp = nullptr;
+  size_t tmp_cap = 0;
+
+  // timing
+  bool timing = false;
+  struct Pending { int stage; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double stage_ms[PTG_T_COUNT] = {0};
+  int64_t stage_launches[PTG_T_COUNT] = {0};
+
+  int fail(int code, const char* fmt, ...);
+  int check(hipError_t e, const char* what);
+  int ensure(void** p, size_t* cap, size_t bytes);
+  hipEvent_t get_event();
+  void mark_begin(int stage, hipEvent_t* ev);
+  void mark_end(int stage, hipEvent_t ev);
+};
+
+// launchers (each returns hipError_t of the launch)
+hipError_t launch_begin_pair(Context& c);
+hipError_t launch_precompute(Context& c);
+hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
+hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
+                                double NCC_min_super, int32_t* next_list, int32_t* next_count);
+hipError_t launch_end_pair(Context& c);
+hipError_t launch_change_reference(Context& c);
+hipError_t launch_build_unmasked(Context& c);
+
+// exact (MFMA) form: returns an error code of the C ABI, writes message into c.err
+int run_search_exact(Context& c, const SearchArgs& a, int list_cap);
+
+// subpixel
+hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);
+hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, const double* d_X, double dmax,
+                                  double stiff, double* d_f, double* d_grad);
+hipError_t launch_interp_batch(Context& c, int index, int32_t n, const double* d_xy, double* d_out);
+hipError_t launch_undistort_points(Context& c, const double* params, int32_t n, const double* d_in, double* d_out);
+hipError_t launch_remap(Context& c, int src, int dst, const double* params);
+
+}  // namespace ptg

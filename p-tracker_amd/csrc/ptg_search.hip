@@ -1,0 +1,359 @@
+// ptg_search.hip — precomputation, the plain integer ZNCC search and the per-pair
+// bookkeeping kernels of libptgpu (gfx950).
+//
+// Plain search: one workgroup per (grain, rotation step). The workgroup stages the
+// current-frame search window and the step's rotated pattern offsets in LDS, then
+// each lane scores whole candidates with the reference's two-pass FP64 sums in the
+// reference's pixel order, unfused (src/p-tracker.cpp:698-716), so every candidate's
+// NCC is bit-identical to follow_pattern_pixel(). A commit kernel reduces the steps in
+// order and applies the reference's update rules.
+#include <limits.h>
+
+#include "ptg_device.hpp"
+#include "ptg_internal.hpp"
+
+namespace ptg {
+
+// grain.reset(); grain.backup(), src/grain_type_2D.cpp:51-57, as called at
+// src/p-tracker.cpp:478-479.
+__global__ void k_begin_pair(ptg_grain* __restrict__ g, int G) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  ptg_grain& r = g[i];
+  r.NCC = r.NCC_rescue = r.NCC_subpix = 0.0;
+  r.dx_prev = r.dx;
+  r.dy_prev = r.dy;
+  r.drot_prev = r.drot;
+  r.status = 0;
+}
+
+hipError_t launch_begin_pair(Context& c) {
+  if (c.G == 0) return hipSuccess;
+  k_begin_pair<<<(c.G + 255) / 256, 256, 0, c.stream>>>(c.d_grains, c.G);
+  return hipGetLastError();
+}
+
+// upix = dx - dx_prev, ..., src/p-tracker.cpp:577-581.
+__global__ void k_end_pair(ptg_grain* __restrict__ g, int G) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  ptg_grain& r = g[i];
+  r.upix = r.dx - r.dx_prev;
+  r.vpix = r.dy - r.dy_prev;
+  r.rot_inc = r.drot - r.drot_prev;
+}
+
+hipError_t launch_end_pair(Context& c) {
+  if (c.G == 0) return hipSuccess;
+  k_end_pair<<<(c.G + 255) / 256, 256, 0, c.stream>>>(c.d_grains, c.G);
+  return hipGetLastError();
+}
+
+// The reference swap, src/p-tracker.cpp:598-609.
+__global__ void k_change_reference(ptg_grain* __restrict__ g, int G) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  ptg_grain& r = g[i];
+  double actual_pos = (double)(r.refcoord_xpix) + r.dx;
+  r.refcoord_xpix = (int32_t)round(actual_pos);
+  r.dx = actual_pos - r.refcoord_xpix;
+  actual_pos = (double)(r.refcoord_ypix) + r.dy;
+  r.refcoord_ypix = (int32_t)round(actual_pos);
+  r.dy = actual_pos - r.refcoord_ypix;
+  r.refrot += r.drot;
+  r.drot = 0.0;
+}
+
+hipError_t launch_change_reference(Context& c) {
+  if (c.G == 0) return hipSuccess;
+  k_change_reference<<<(c.G + 255) / 256, 256, 0, c.stream>>>(c.d_grains, c.G);
+  return hipGetLastError();
+}
+
+// do_precomputations, src/p-tracker.cpp:624-656: one wave per grain. The sum of I0 is
+// a sum of integers, exact in any order, so mean0 matches the reference's bits; C0C0
+// is accumulated by one lane in the reference's pixel order from an LDS chunk that the
+// whole wave stages. Also stores d0[i] = I0(p0_i) - mean0 (the reference's
+// `(double)image[...] - mean0`, :708) and the per-grain sum of d0 for the one-pass
+// subpixel objective.
+constexpr int kPreChunk = 1024;
+
+__global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ I0, int dimx, int dimy,
+                                                   int64_t pitch, ptg_grain* __restrict__ grains,
+                                                   const int32_t* __restrict__ pattern,
+                                                   const int64_t* __restrict__ pbegin,
+                                                   const double* __restrict__ cs0, int32_t* __restrict__ p0,
+                                                   double* __restrict__ d0, double* __restrict__ sd0) {
+  __shared__ double chunk[kPreChunk];
+  const int g = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t b = pbegin[g];
+  const int n = (int)(pbegin[g + 1] - b);
+  ptg_grain& gr = grains[g];
+  const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
+  const double c0 = cs0[2 * g], s0 = cs0[2 * g + 1];
+  long long s = 0;
+  bool clamped = false;
+  for (int i = lane; i < n; i += 64) {
+    int x, y;
+    rotate_px(pattern[2 * (b + i)], pattern[2 * (b + i) + 1], c0, s0, x, y);
+    p0[2 * (b + i)] = x;
+    p0[2 * (b + i) + 1] = y;
+    int X = rx + x, Y = ry + y;
+    if (X < 0 || Y < 0 || X >= dimx || Y >= dimy) clamped = true;
+    X = clampi(X, 0, dimx - 1);
+    Y = clampi(Y, 0, dimy - 1);
+    s += I0[(int64_t)Y * pitch + X];
+  }
+  s = wave_sum_i64(s);
+  const double mean0 = (double)s / (double)n;
+  double C0C0 = 0.0, sd = 0.0;
+  for (int base = 0; base < n; base += kPreChunk) {
+    const int m = min(kPreChunk, n - base);
+    for (int t = lane; t < m; t += 64) {
+      const int i = base + t;
+      const int X = clampi(rx + p0[2 * (b + i)], 0, dimx - 1);
+      const int Y = clampi(ry + p0[2 * (b + i) + 1], 0, dimy - 1);
+      const double diff = (double)I0[(int64_t)Y * pitch + X] - mean0;
+      chunk[t] = diff;
+      d0[b + i] = diff;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      for (int t = 0; t < m; t++) {
+        const double diff = chunk[t];
+        C0C0 += diff * diff;
+        sd += diff;
+      }
+    }
+    __syncthreads();
+  }
+  if (clamped) atomicOr(&gr.status, (int)PTG_STATUS_CLAMPED);
+  if (lane == 0) {
+    gr.mean0 = mean0;
+    gr.C0C0 = C0C0;
+    sd0[g] = sd;
+  }
+}
+
+hipError_t launch_precompute(Context& c) {
+  if (c.G == 0) return hipSuccess;
+  const Image& im = c.img[c.iref];
+  k_precompute<<<c.G, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern, c.d_pbegin,
+                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0);
+  return hipGetLastError();
+}
+
+// Candidate order of the reference: rotation step, then u (outer), then v (inner),
+// src/p-tracker.cpp:686,695-696. Within a step, idx = (u + left) * V + (v + up).
+__device__ __forceinline__ bool better(double ncc, int idx, double best, int bidx) {
+  // strict improvement as `best_NCC < NCC_test` (:718); among equal values the lower
+  // index (earlier in the reference loop) wins; NaN and values <= 0 never win
+  return ncc > best || (ncc == best && ncc > 0.0 && idx < bidx);
+}
+
+constexpr int kPlainThreads = 256;
+constexpr int kPlainCPT = 4;  // candidates per lane per round
+
+__global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nsteps = a.nsteps;
+  const int li = blockIdx.x / nsteps;
+  const int k = blockIdx.x - li * nsteps;
+  if (li >= *a.count) return;
+  const int g = a.list[li];
+  const ptg_grain& gr = a.grains[g];
+  const int64_t b = a.pbegin[g];
+  const int n = (int)(a.pbegin[g + 1] - b);
+  const int rho = a.prad[g];
+  const int gdx = (int)round(gr.dx), gdy = (int)round(gr.dy);
+  const int U = a.left + a.right + 1, V = a.up + a.down + 1;
+  const int ww = U + 2 * rho, wh = V + 2 * rho;
+  const int X0 = gr.refcoord_xpix + gdx - a.left - rho;
+  const int Y0 = gr.refcoord_ypix + gdy - a.up - rho;
+  const double C0C0 = gr.C0C0;
+
+  uint16_t* win = reinterpret_cast<uint16_t*>(smem);
+  int32_t* off = reinterpret_cast<int32_t*>(smem + ((size_t)a.win_cap * 2 + 15) / 16 * 16);
+
+  bool clamped = false;
+  for (int t = threadIdx.x; t < ww * wh; t += kPlainThreads) {
+    const int wy = t / ww, wx = t - wy * ww;
+    int X = X0 + wx, Y = Y0 + wy;
+    if (X < 0 || Y < 0 || X >= a.dimx || Y >= a.dimy) clamped = true;
+    X = clampi(X, 0, a.dimx - 1);
+    Y = clampi(Y, 0, a.dimy - 1);
+    win[t] = a.I1[(int64_t)Y * a.pitch1 + X];
+  }
+  const double c = a.cs[2 * ((int64_t)g * nsteps + k)];
+  const double s = a.cs[2 * ((int64_t)g * nsteps + k) + 1];
+  for (int i = threadIdx.x; i < n; i += kPlainThreads) {
+    int x, y;
+    rotate_px(a.pattern[2 * (b + i)], a.pattern[2 * (b + i) + 1], c, s, x, y);
+    off[i] = (y + rho) * ww + (x + rho);
+  }
+  if (clamped) atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
+  __syncthreads();
+
+  const double* __restrict__ d0 = a.d0 + b;
+  const int S = U * V;
+  const double dn = (double)n;
+  double best = 0.0;
+  int bidx = INT_MAX;
+  for (int r0 = 0; r0 < S; r0 += kPlainThreads * kPlainCPT) {
+    int base[kPlainCPT], cidx[kPlainCPT];
+    bool valid[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      // work item w walks u fastest so neighbouring lanes read neighbouring LDS words
+      const int w = r0 + q * kPlainThreads + threadIdx.x;
+      valid[q] = w < S;
+      const int ww_ = valid[q] ? w : 0;
+      const int v = ww_ / U, u = ww_ - v * U;
+      base[q] = v * ww + u;
+      cidx[q] = u * V + v;
+    }
+    unsigned int isum[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) isum[q] = 0u;
+    for (int i = 0; i < n; i++) {
+      const int o = off[i];
+#pragma unroll
+      for (int q = 0; q < kPlainCPT; q++) isum[q] += win[o + base[q]];
+    }
+    // mean1 += (double)I1 over integers is exact in any order, then one division (:705)
+    double m1[kPlainCPT], c01[kPlainCPT], c11[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      m1[q] = (double)isum[q] / dn;
+      c01[q] = 0.0;
+      c11[q] = 0.0;
+    }
+    for (int i = 0; i < n; i++) {
+      const int o = off[i];
+      const double dd = d0[i];
+#pragma unroll
+      for (int q = 0; q < kPlainCPT; q++) {
+        const double diffC1 = (double)win[o + base[q]] - m1[q];
+        c01[q] = c01[q] + dd * diffC1;
+        c11[q] = c11[q] + diffC1 * diffC1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      if (!valid[q]) continue;
+      const double ncc = c01[q] / sqrt(C0C0 * c11[q]);
+      if (better(ncc, cidx[q], best, bidx)) {
+        best = ncc;
+        bidx = cidx[q];
+      }
+    }
+  }
+  // workgroup argmax with the same ordering
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double ob = __shfl_xor(best, m, 64);
+    const int oi = __shfl_xor(bidx, m, 64);
+    if (better(ob, oi, best, bidx)) {
+      best = ob;
+      bidx = oi;
+    }
+  }
+  __shared__ double wb[kPlainThreads / 64];
+  __shared__ int wi[kPlainThreads / 64];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wb[wave] = best;
+    wi[wave] = bidx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kPlainThreads / 64; w++)
+      if (better(wb[w], wi[w], best, bidx)) {
+        best = wb[w];
+        bidx = wi[w];
+      }
+    StepBest sb;
+    sb.ncc = best;
+    sb.idx = bidx;
+    sb.pad = 0;
+    a.best[(int64_t)li * nsteps + k] = sb;
+  }
+}
+
+hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes) {
+  if (list_cap == 0) return hipSuccess;
+  const int64_t blocks = (int64_t)list_cap * a.nsteps;
+  if (lds_bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_search_plain, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  k_search_plain<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a);
+  return hipGetLastError();
+}
+
+// Reduces the per-step bests in step order and applies the per-stage output rules:
+// main src/p-tracker.cpp:729-742, rescue :868-880, super-rescue :1005-1015.
+__global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double NCC_min_super,
+                                int32_t* __restrict__ next_list, int32_t* __restrict__ next_count) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= *a.count) return;
+  const int g = a.list[li];
+  ptg_grain& gr = a.grains[g];
+  double best = 0.0;
+  int bidx = INT_MAX, bk = -1;
+  for (int k = 0; k < a.nsteps; k++) {
+    const StepBest sb = a.best[(int64_t)li * a.nsteps + k];
+    if (sb.ncc > best) {
+      best = sb.ncc;
+      bidx = sb.idx;
+      bk = k;
+    }
+  }
+  const int V = a.up + a.down + 1;
+  int best_upix = 0, best_vpix = 0;
+  double best_drot = 0.0;
+  if (bk >= 0) {
+    best_upix = bidx / V - a.left;
+    best_vpix = bidx % V - a.up;
+    best_drot = a.steps[bk];
+  }
+  const int grain_dx = (int)round(gr.dx), grain_dy = (int)round(gr.dy);
+  const double rest_dx = gr.dx - grain_dx, rest_dy = gr.dy - grain_dy;
+  bool queued = false, update = false;
+  if (stage == PTG_STAGE_MAIN) {
+    gr.upix = gr.vpix = gr.rot_inc = 0.0;
+    gr.NCC = best;
+    gr.NCC_rescue = best;
+    if (best < NCC_min) queued = true; else update = true;
+  } else if (stage == PTG_STAGE_RESCUE) {
+    gr.NCC_rescue = best;
+    if (best < NCC_min_super) queued = true; else update = true;
+  } else {
+    gr.NCC_rescue = best;
+    if (best > NCC_min_super) update = true;
+  }
+  if (update) {
+    gr.upix = best_upix - rest_dx;
+    gr.vpix = best_vpix - rest_dy;
+    gr.rot_inc = best_drot;
+    gr.dx = grain_dx + best_upix;
+    gr.dy = grain_dy + best_vpix;
+    gr.drot += best_drot;
+  }
+  if (queued && next_list) {
+    const int pos = atomicAdd(next_count, 1);
+    next_list[pos] = g;
+  }
+}
+
+hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
+                                double NCC_min_super, int32_t* next_list, int32_t* next_count) {
+  if (list_cap == 0) return hipSuccess;
+  k_search_commit<<<(list_cap + 127) / 128, 128, 0, c.stream>>>(a, stage, NCC_min, NCC_min_super, next_list,
+                                                                 next_count);
+  return hipGetLastError();
+}
+
+}  // namespace ptg
