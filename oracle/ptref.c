@@ -388,6 +388,10 @@ double ptref_powell(ptref_fn3 f, void* user, double* p, const double* dx, double
   line_ctx L = {f, user, NULL, NULL, 0};
   L.nevals = 1;
   double fret = f(p, user);
+  if (fret != fret) { /* undefined ZNCC (flat pattern): nothing to minimise */
+    if (nevals) *nevals += L.nevals;
+    return fret;
+  }
   double pt[3] = {p[0], p[1], p[2]};
   for (int iter = 0; iter < max_iter; iter++) {
     double fp = fret, del = 0.0;

@@ -75,6 +75,11 @@ def _p(a):
     return a.ctypes.data_as(c_void_p)
 
 
+def _rec(grains, g):
+    """Pointer to record g of a grain array."""
+    return c_void_p(int(grains.ctypes.data) + int(g) * grains.itemsize)
+
+
 def rotation_steps(inc_rot, num_rot):
     n = lib().ptref_rotation_steps(inc_rot, num_rot, None, 0)
     if n < 0:
@@ -162,7 +167,7 @@ def precompute(w: Workload, grains=None):
     p0 = np.zeros_like(w.pattern_xy)
     for g in range(len(gr)):
         b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
-        lib().ptref_precompute(_p(w.I0), w.pitch, ctypes.c_void_p(gr.ctypes.data + g * gr.itemsize), int(e - b),
+        lib().ptref_precompute(_p(w.I0), w.pitch, _rec(gr, g), int(e - b),
                                _p(w.pattern_xy[b:e]), _p(p0[b:e]))
     return gr, p0
 
@@ -170,17 +175,17 @@ def precompute(w: Workload, grains=None):
 def follow_pattern(w: Workload, grains, p0, g, z, stage, NCC_min=0.7, NCC_min_super=0.5):
     b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
     scratch = np.zeros((e - b, 2), dtype=np.int32)
-    return lib().ptref_follow_pattern(_p(w.I0), _p(w.I1), w.pitch, ctypes.c_void_p(grains.ctypes.data + g * grains.itemsize),
+    return lib().ptref_follow_pattern(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                                       int(e - b), _p(w.pattern_xy[b:e]), _p(p0[b:e]), ctypes.byref(z), stage,
                                       NCC_min, NCC_min_super, _p(scratch))
 
 
 def candidate_ncc(w: Workload, grains, p0, g, z):
     b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
-    n = lib().ptref_candidate_ncc(_p(w.I0), _p(w.I1), w.pitch, ctypes.c_void_p(grains.ctypes.data + g * grains.itemsize),
+    n = lib().ptref_candidate_ncc(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                                   int(e - b), _p(w.pattern_xy[b:e]), _p(p0[b:e]), ctypes.byref(z), None, 0)
     out = np.zeros(n)
-    lib().ptref_candidate_ncc(_p(w.I0), _p(w.I1), w.pitch, ctypes.c_void_p(grains.ctypes.data + g * grains.itemsize),
+    lib().ptref_candidate_ncc(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                               int(e - b), _p(w.pattern_xy[b:e]), _p(p0[b:e]), ctypes.byref(z), _p(out), n)
     return out
 
@@ -188,14 +193,14 @@ def candidate_ncc(w: Workload, grains, p0, g, z):
 def ncc_objective(w: Workload, grains, p0, g, X, dmax=3.0, stiffness=0.0):
     b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
     X = np.ascontiguousarray(X, dtype=np.float64)
-    return lib().ptref_ncc_objective(_p(w.I0), _p(w.I1), w.pitch, ctypes.c_void_p(grains.ctypes.data + g * grains.itemsize),
+    return lib().ptref_ncc_objective(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                                      int(e - b), _p(p0[b:e]), _p(X), dmax, stiffness)
 
 
 def subpixel(w: Workload, grains, p0, g, prm):
     b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
     ne = c_int64(0)
-    lib().ptref_subpixel(_p(w.I0), _p(w.I1), w.pitch, ctypes.c_void_p(grains.ctypes.data + g * grains.itemsize),
+    lib().ptref_subpixel(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                          int(e - b), _p(p0[b:e]), ctypes.byref(prm), ctypes.byref(ne))
     return ne.value
 

@@ -44,7 +44,8 @@ __device__ __forceinline__ double interp_cubic(const uint16_t* __restrict__ im, 
   int iy = (int)floor(y);
   double y01 = y - (double)iy;
   double x01 = x - (double)ix;
-  bool inside = ix >= 1 && iy >= 1 && ix + 2 < dimx && iy + 2 < dimy;
+  // written without ix + 2 so that saturated conversions of huge coordinates cannot wrap
+  bool inside = ix >= 1 && iy >= 1 && ix < dimx - 2 && iy < dimy - 2;
   double arr[4];
   if (inside) {
     const uint16_t* r0 = im + (int64_t)(iy - 1) * pitch + (ix - 1);

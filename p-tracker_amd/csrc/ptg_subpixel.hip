@@ -107,7 +107,7 @@ __device__ __forceinline__ void interp_cubic_grad(const uint16_t* __restrict__ i
   const int ix = (int)floor(x), iy = (int)floor(y);
   const double y01 = y - (double)iy, x01 = x - (double)ix;
   double arr[4], darr[4];
-  const bool inside = ix >= 1 && iy >= 1 && ix + 2 < dimx && iy + 2 < dimy;
+  const bool inside = ix >= 1 && iy >= 1 && ix < dimx - 2 && iy < dimy - 2;
   if (!inside) clamped = true;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -297,6 +297,7 @@ __device__ double powell(Obj& o, double* p, const double* dx, double ftol, int m
   double u[3][3] = {{dx[0], 0.0, 0.0}, {0.0, dx[1], 0.0}, {0.0, 0.0, dx[2]}};
   if (max_iter <= 0) max_iter = 200;
   double fret = eval(o, p);
+  if (fret != fret) return fret;  // undefined ZNCC (flat pattern): nothing to minimise
   double pt[3] = {p[0], p[1], p[2]};
   for (int iter = 0; iter < max_iter; iter++) {
     const double fp = fret;
