@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--subpix", choices=["auto", "lm", "powell"], default="auto")
     ap.add_argument("--grains", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain search+Powell form (0: skip)")
-    ap.add_argument("--cpu-grains", type=int, default=24, help="grains in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-grains", type=int, default=480, help="grains in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:

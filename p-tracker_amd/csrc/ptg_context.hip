@@ -277,7 +277,8 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   }
   TRY(c->ensure((void**)&c->d_cs, &c->cs_cap, sizeof(double) * 2 * (size_t)c->G * nsteps));
   TRY(c->ensure((void**)&c->d_steps, &c->steps_cap, sizeof(double) * nsteps));
-  TRY(c->ensure((void**)&c->d_best, &c->best_cap, sizeof(StepBest) * (size_t)nlist * nsteps));
+  const int nparts_cap = std::max(nsteps, z->left + z->right + 1);
+  TRY(c->ensure((void**)&c->d_best, &c->best_cap, sizeof(StepBest) * (size_t)nlist * nparts_cap));
   // upload only the rows in use: for the main stage that is nearly all of them
   if (stage == PTG_STAGE_MAIN) {
     CK(c, hipMemcpyAsync(c->d_cs, cs, sizeof(double) * 2 * (size_t)c->G * nsteps, hipMemcpyHostToDevice, c->stream));
@@ -330,6 +331,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   const int tstage = PTG_T_SEARCH_MAIN + stage;
   hipEvent_t ev;
   c->mark_begin(tstage, &ev);
+  int nparts = nsteps;
   if (method == PTG_SEARCH_PLAIN) {
     const size_t lds = (size_t)(win_cap * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
     if (lds > kMaxLds)
@@ -339,11 +341,15 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     CK(c, launch_search_plain(*c, a, nlist, lds));
     if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
   } else {
-    TRY(run_search_exact(*c, a, nlist));
+    hipEvent_t ek;
+    if (stage == PTG_STAGE_MAIN) c->mark_begin(PTG_T_SEARCH_KERNEL, &ek);
+    TRY(run_search_exact(*c, a, nlist, list, c->d_best, &nparts));
+    if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
   int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
-  CK(c, launch_search_commit(*c, a, nlist, stage, NCC_min, NCC_min_super, next_list, next_count));
+  CK(c, launch_search_commit(*c, a, nlist, stage, NCC_min, NCC_min_super, next_list, next_count, c->d_best,
+                             nparts));
   c->mark_end(tstage, ev);
   // successful grains changed their dx/drot; queued ones did not, so the snapshot
   // still serves the following rescue stages
@@ -886,9 +892,3 @@ int ptg_get_subpix_evals(void* p, int64_t* evals) {
 
 }  // extern "C"
 
-namespace ptg {
-// Placeholder until the MFMA form lands: reports that the exact form is unavailable.
-int run_search_exact(Context& c, const SearchArgs&, int) {
-  return c.fail(PTG_ERR_ARG, "PTG_SEARCH_EXACT is not available in this build");
-}
-}  // namespace ptg

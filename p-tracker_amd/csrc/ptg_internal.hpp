@@ -123,14 +123,19 @@ struct Context {
 hipError_t launch_begin_pair(Context& c);
 hipError_t launch_precompute(Context& c);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
+// Reduces nparts partial argmaxes per list slot (global candidate index k * S + u * V + v,
+// first maximum wins) and applies the stage's update rules.
 hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
-                                double NCC_min_super, int32_t* next_list, int32_t* next_count);
+                                double NCC_min_super, int32_t* next_list, int32_t* next_count,
+                                const StepBest* parts, int nparts);
 hipError_t launch_end_pair(Context& c);
 hipError_t launch_change_reference(Context& c);
 hipError_t launch_build_unmasked(Context& c);
 
-// exact (MFMA) form: returns an error code of the C ABI, writes message into c.err
-int run_search_exact(Context& c, const SearchArgs& a, int list_cap);
+// exact (MFMA) form: returns an error code of the C ABI, writes message into c.err.
+// Writes nparts partial argmaxes per list slot into out_best.
+int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vector<int32_t>& list,
+                     StepBest* out_best, int* nparts_out);
 
 // subpixel
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);

@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
       }
     StepBest sb;
     sb.ncc = best;
-    sb.idx = bidx;
+    sb.idx = bidx == INT_MAX ? INT_MAX : k * S + bidx;  // global candidate index
     sb.pad = 0;
     a.best[(int64_t)li * nsteps + k] = sb;
   }
@@ -293,30 +293,32 @@ hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, si
   return hipGetLastError();
 }
 
-// Reduces the per-step bests in step order and applies the per-stage output rules:
+// Reduces the partial argmaxes of a slot (global candidate indices, the first maximum
+// in the reference's (step, u, v) order wins) and applies the per-stage output rules:
 // main src/p-tracker.cpp:729-742, rescue :868-880, super-rescue :1005-1015.
 __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double NCC_min_super,
-                                int32_t* __restrict__ next_list, int32_t* __restrict__ next_count) {
+                                int32_t* __restrict__ next_list, int32_t* __restrict__ next_count,
+                                const StepBest* __restrict__ parts, int nparts) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= *a.count) return;
   const int g = a.list[li];
   ptg_grain& gr = a.grains[g];
   double best = 0.0;
-  int bidx = INT_MAX, bk = -1;
-  for (int k = 0; k < a.nsteps; k++) {
-    const StepBest sb = a.best[(int64_t)li * a.nsteps + k];
-    if (sb.ncc > best) {
+  int bidx = INT_MAX;
+  for (int p = 0; p < nparts; p++) {
+    const StepBest sb = parts[(int64_t)li * nparts + p];
+    if (better(sb.ncc, sb.idx, best, bidx)) {
       best = sb.ncc;
       bidx = sb.idx;
-      bk = k;
     }
   }
-  const int V = a.up + a.down + 1;
+  const int U = a.left + a.right + 1, V = a.up + a.down + 1, S = U * V;
   int best_upix = 0, best_vpix = 0;
   double best_drot = 0.0;
-  if (bk >= 0) {
-    best_upix = bidx / V - a.left;
-    best_vpix = bidx % V - a.up;
+  if (bidx != INT_MAX) {
+    const int bk = bidx / S, rem = bidx - bk * S;
+    best_upix = rem / V - a.left;
+    best_vpix = rem % V - a.up;
     best_drot = a.steps[bk];
   }
   const int grain_dx = (int)round(gr.dx), grain_dy = (int)round(gr.dy);
@@ -349,10 +351,11 @@ __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double 
 }
 
 hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
-                                double NCC_min_super, int32_t* next_list, int32_t* next_count) {
+                                double NCC_min_super, int32_t* next_list, int32_t* next_count,
+                                const StepBest* parts, int nparts) {
   if (list_cap == 0) return hipSuccess;
   k_search_commit<<<(list_cap + 127) / 128, 128, 0, c.stream>>>(a, stage, NCC_min, NCC_min_super, next_list,
-                                                                 next_count);
+                                                                 next_count, parts, nparts);
   return hipGetLastError();
 }
 

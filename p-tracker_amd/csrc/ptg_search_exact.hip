@@ -1,0 +1,656 @@
+// ptg_search_exact.hip — the exact-integer integer-pixel search of libptgpu on the
+// gfx950 matrix cores.
+//
+// For a grain, rotation step k and shift s the reference's ZNCC needs three sums over
+// the rotated pattern (src/p-tracker.cpp:698-716):
+//   A = sum a_i I1(s + p_k,i),  B = sum I1(s + p_k,i),  C = sum I1(s + p_k,i)^2
+// with a_i = I0 at the reference pattern pixel. As exact integers,
+//   NCC = (N A - Sa B) / sqrt((N Saa - Sa^2) (N C - B^2)),
+// equal to the reference's two-pass FP64 value up to its rounding.
+//
+// A is a correlation of the rotated template T_k (pixel values scattered onto their
+// rounded offsets, duplicates summed) with the window, computed as a GEMM on
+// v_mfma_i32_16x16x64_i8: rows = (step, template digit), columns = vertical shifts of
+// one shift column, K = template cells. Values are split into 7-bit digits (T < 2^14,
+// I1 < 2^12), so every product is exact in the int32 accumulators.
+//  - Template fragments are built once per search in global memory in fragment order
+//    (one 1 KiB block per M-tile and K-chunk) and read coalesced; all shift columns of
+//    a grain run on one XCD so its L2 keeps them.
+//  - K-chunks are 4 segments of 16 template cells of one row, segment starts aligned to
+//    16. For each shift column the window digit planes are copied into LDS rows whose
+//    cell c is window column su + c, so every B-fragment is one aligned ds_read_b128.
+// B and C use the multiset mask M_mid of the middle rotation step: B_mid(s), C_mid(s)
+// come from row prefix sums, and each step adds the sparse difference M_k - M_mid
+// (a few tens of cells) on the vector units.
+#include <limits.h>
+
+#include "ptg_device.hpp"
+#include "ptg_internal.hpp"
+
+namespace ptg {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Per-slot geometry of the exact search (host computed).
+struct ExactSlot {
+  int32_t rho;       // every rotated offset lies in [-rho, rho]^2
+  int32_t cw;        // template row width in cells, 16 * ceil((2 rho + 1) / 16)
+  int32_t cs;        // LDS staging row stride (bytes), cw rounded to an odd multiple of 16
+  int32_t kc;        // K-chunks (4 segments each)
+  int64_t seg_off;   // first segment in the segment table
+  int64_t tmpl_off;  // first byte of this slot's template fragments
+  int64_t dl_off;    // first Delta entry (nsteps * dcap entries)
+  int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
+};
+
+struct ExactArgs {
+  SearchArgs s;
+  const ExactSlot* slots;      // per list slot
+  const int32_t* bucket;       // list slots of this launch
+  int32_t nbucket;
+  const int32_t* seg;          // segment table: (qy + rho) << 16 | c0
+  uint8_t* tmpl;               // template fragments
+  int32_t* dlist;              // Delta entries: offset << 3 | (delta + 4)
+  int32_t* dlen;               // per (slot, step)
+  int64_t* mid;                // B_mid, C_mid per (slot, shift)
+  int64_t* mom;                // per slot: N, Sa, Saa
+  int32_t* flags;              // [0]: template or Delta overflow
+  StepBest* out;               // per (slot, shift block), part_cap entries per slot
+  int32_t part_cap;
+  int32_t dcap;
+  int32_t nmt;                 // M-tiles (8 steps each)
+  int32_t nsub;                // shift blocks per grain
+  int32_t nsu;                 // shift columns per block (= waves)
+  int32_t vp;                  // vertical shifts padded to 16
+  int32_t ww, wh_pad;          // plain window width/height of the largest slot of the launch
+};
+
+__device__ __forceinline__ int win_x0(const ptg_grain& gr, int left, int rho) {
+  return gr.refcoord_xpix + (int)round(gr.dx) - left - rho;
+}
+__device__ __forceinline__ int win_y0(const ptg_grain& gr, int up, int rho) {
+  return gr.refcoord_ypix + (int)round(gr.dy) - up - rho;
+}
+
+// ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
+// LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2] =
+// count of the middle step. Rows beyond the last step are written as zeros.
+__global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int bi = blockIdx.x;
+  if (bi >= e.nbucket) return;
+  const int li = e.bucket[bi];
+  if (li >= *e.s.count) return;
+  const int k = blockIdx.y;  // 0 .. 8 * nmt - 1
+  const int g = e.s.list[li];
+  const ExactSlot sl = e.slots[li];
+  const int rho = sl.rho, side = 2 * rho + 1, ncell = side * side;
+  const int nsteps = e.s.nsteps;
+  int32_t* packed = reinterpret_cast<int32_t*>(smem);
+  int32_t* cmid = packed + ncell;
+  for (int t = threadIdx.x; t < 2 * ncell; t += blockDim.x) packed[t] = 0;
+  __syncthreads();
+  const bool real = k < nsteps;
+  const ptg_grain& gr = e.s.grains[g];
+  const int64_t b = e.s.pbegin[g];
+  const int n = (int)(e.s.pbegin[g + 1] - b);
+  const int kmid = nsteps / 2;
+  if (real) {
+    const double c = e.s.cs[2 * ((int64_t)g * nsteps + k)], s = e.s.cs[2 * ((int64_t)g * nsteps + k) + 1];
+    const double cm = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], sm = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
+    const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int px = e.s.pattern[2 * (b + i)], py = e.s.pattern[2 * (b + i) + 1];
+      const int X = clampi(rx + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
+      const int Y = clampi(ry + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
+      const int a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
+      int x, y;
+      rotate_px(px, py, c, s, x, y);
+      atomicAdd(&packed[(y + rho) * side + (x + rho)], a + (1 << 24));
+      rotate_px(px, py, cm, sm, x, y);
+      atomicAdd(&cmid[(y + rho) * side + (x + rho)], 1);
+    }
+  }
+  __syncthreads();
+  // fragments: rows 2k (digit 1) and 2k + 1 (digit 0) of M-tile k / 8
+  const int mt = k >> 3, m = (2 * k) & 15;
+  const int kc = sl.kc;
+  bool overflow = false;
+  for (int t = threadIdx.x; t < kc * 4 * 2; t += blockDim.x) {
+    const int d = t & 1, gg = (t >> 1) & 3, ch = t >> 3;
+    const int segv = e.seg[sl.seg_off + 4 * ch + gg];
+    const int qyr = segv >> 16, c0 = segv & 0x7fff;  // qy + rho, first cell column
+    const bool empty = (segv & 0x8000) != 0;          // padding segment: zero template bytes
+    unsigned int w[4] = {0u, 0u, 0u, 0u};
+    if (real && !empty) {
+      for (int j = 0; j < 16; j++) {
+        const int col = c0 + j;  // qx + rho
+        if (col < side) {
+          const int v = packed[qyr * side + col] & 0xffffff;
+          if (v >= (1 << 14)) overflow = true;
+          const unsigned int dig = d == 0 ? (unsigned)(v >> 7) & 127u : (unsigned)v & 127u;
+          w[j >> 2] |= dig << (8 * (j & 3));
+        }
+      }
+    }
+    const int lane = m + d + 16 * gg;
+    uint4 val = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(e.tmpl + sl.tmpl_off + (((int64_t)mt * kc + ch) * 64 + lane) * 16) = val;
+  }
+  if (real) {
+    // Delta list: cells whose multiplicity differs from the middle step's
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * e.dcap;
+    const int ww = e.s.left + e.s.right + sl.cw;  // plain window width of this slot
+    for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+      const int ck = (packed[t] >> 24) & 0xff, cmv = cmid[t];
+      if (ck != cmv) {
+        const int pos = atomicAdd(&cnt, 1);
+        if (pos < e.dcap) {
+          const int qy = t / side, qx = t - qy * side;  // both + rho
+          dl[pos] = ((qy * ww + qx) << 3) | (ck - cmv + 4);
+        }
+        if (ck - cmv > 3 || ck - cmv < -4) overflow = true;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (cnt > e.dcap) overflow = true;
+      e.dlen[(int64_t)li * nsteps + k] = min(cnt, e.dcap);
+    }
+  }
+  if (overflow) atomicOr(&e.flags[0], 1);
+}
+
+// ---- B_mid, C_mid for every shift and the reference moments: one workgroup per slot ----
+// LDS: prefix[(V + 2 rho) * (U + 2 rho + 1)] u32, the middle-step count grid, runs.
+constexpr int kMaxRuns = 1536;
+
+__global__ __launch_bounds__(256) void k_exact_mid(ExactArgs e) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int nruns;
+  __shared__ long long red[3][4];
+  const int bi = blockIdx.x;
+  if (bi >= e.nbucket) return;
+  const int li = e.bucket[bi];
+  if (li >= *e.s.count) return;
+  const int g = e.s.list[li];
+  const ExactSlot sl = e.slots[li];
+  const int rho = sl.rho, side = 2 * rho + 1;
+  const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1;
+  const int PW = U + 2 * rho + 1, PH = V + 2 * rho;
+  uint32_t* P = reinterpret_cast<uint32_t*>(smem);
+  int32_t* cm = reinterpret_cast<int32_t*>(P + (size_t)PW * PH);
+  int32_t* runs = cm + side * side;  // (row << 20 | lo << 10 | hi), weight in a second array
+  int32_t* runw = runs + kMaxRuns;
+  const ptg_grain& gr = e.s.grains[g];
+  const int64_t b = e.s.pbegin[g];
+  const int n = (int)(e.s.pbegin[g + 1] - b);
+  const int nsteps = e.s.nsteps, kmid = nsteps / 2;
+  for (int t = threadIdx.x; t < side * side; t += blockDim.x) cm[t] = 0;
+  if (threadIdx.x == 0) nruns = 0;
+  __syncthreads();
+  const double c = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], s = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
+  const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
+  long long Sa = 0, Saa = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int x, y;
+    rotate_px(e.s.pattern[2 * (b + i)], e.s.pattern[2 * (b + i) + 1], c, s, x, y);
+    atomicAdd(&cm[(y + rho) * side + (x + rho)], 1);
+    const int X = clampi(rx + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
+    const int Y = clampi(ry + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
+    const long long a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
+    Sa += a;
+    Saa += a * a;
+  }
+  Sa = wave_sum_i64(Sa);
+  Saa = wave_sum_i64(Saa);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = Sa;
+    red[1][threadIdx.x >> 6] = Saa;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long A = 0, AA = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      A += red[0][w];
+      AA += red[1][w];
+    }
+    e.mom[3 * li] = n;
+    e.mom[3 * li + 1] = A;
+    e.mom[3 * li + 2] = AA;
+  }
+  // runs of equal non-zero multiplicity along each template row
+  for (int r = threadIdx.x; r < side; r += blockDim.x) {
+    int x = 0;
+    while (x < side) {
+      const int w = cm[r * side + x];
+      if (w == 0) {
+        x++;
+        continue;
+      }
+      int h = x;
+      while (h + 1 < side && cm[r * side + h + 1] == w) h++;
+      const int pos = atomicAdd(&nruns, 1);
+      if (pos < kMaxRuns) {
+        runs[pos] = (r << 20) | (x << 10) | h;
+        runw[pos] = w;
+      }
+      x = h + 1;
+    }
+  }
+  __syncthreads();
+  const int nr = min(nruns, kMaxRuns);
+  if (threadIdx.x == 0 && nruns > kMaxRuns) atomicOr(&e.flags[0], 2);
+  const int X0 = gr.refcoord_xpix + (int)round(gr.dx) - e.s.left - rho;
+  const int Y0 = gr.refcoord_ypix + (int)round(gr.dy) - e.s.up - rho;
+  const int S = U * V;
+  for (int pass = 0; pass < 2; pass++) {
+    // row prefix sums of I1 (pass 0) or I1^2 (pass 1); u32 wraps, differences stay exact
+    for (int y = threadIdx.x; y < PH; y += blockDim.x) {
+      const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
+      uint32_t acc = 0u;
+      P[(size_t)y * PW] = 0u;
+      for (int x = 0; x + 1 < PW; x++) {
+        const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x, 0, e.s.dimx - 1)];
+        acc += pass == 0 ? v : v * v;
+        P[(size_t)y * PW + x + 1] = acc;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < S; t += blockDim.x) {
+      const int su = t / V, sv = t - su * V;
+      long long sum = 0;
+      for (int q = 0; q < nr; q++) {
+        const int rv = runs[q];
+        const int row = rv >> 20, lo = (rv >> 10) & 1023, hi = rv & 1023;
+        const uint32_t* pr = P + (size_t)(sv + row) * PW + su;
+        sum += (long long)runw[q] * (long long)(uint32_t)(pr[hi + 1] - pr[lo]);
+      }
+      e.mid[sl.mid_off + 2 * (int64_t)t + pass] = sum;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- the MFMA search ---------------------------------------------------------------
+__device__ __forceinline__ bool better_g(double ncc, int idx, double best, int bidx) {
+  return ncc > best || (ncc == best && ncc > 0.0 && idx < bidx);
+}
+
+template <int NMT, int NTP>
+__global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  // XCD-aware job mapping: all shift blocks of a grain share blockIdx % 8
+  const int bx = blockIdx.x;
+  const int xcd = bx & 7, t = bx >> 3;
+  const int grp = t / e.nsub, blk = t - grp * e.nsub;
+  const int bi = grp * 8 + xcd;
+  if (bi >= e.nbucket) return;
+  const int li = e.bucket[bi];
+  if (li >= *e.s.count) return;
+  const int g = e.s.list[li];
+  const ExactSlot sl = e.slots[li];
+  const int rho = sl.rho, cw = sl.cw, cs = sl.cs, kc = sl.kc;
+  const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1, S = U * V;
+  const int vp = e.vp;
+  const int WW = U - 1 + cw, WH = vp + 2 * rho;
+  const int nsteps = e.s.nsteps;
+  const ptg_grain& gr = e.s.grains[g];
+  const int X0 = win_x0(gr, e.s.left, rho), Y0 = win_y0(gr, e.s.up, rho);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nthreads = blockDim.x;
+
+  uint16_t* win = reinterpret_cast<uint16_t*>(smem);
+  const size_t win_bytes = ((size_t)e.ww * e.wh_pad * 2 + 15) / 16 * 16;
+  const size_t stage_bytes = (size_t)2 * e.wh_pad * cs;
+  uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * ((stage_bytes + 15) / 16 * 16);
+
+  bool clamped = false;
+  for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
+    const int y = q / WW, x = q - y * WW;
+    int X = X0 + x, Y = Y0 + y;
+    if (X < 0 || Y < 0 || X >= e.s.dimx || Y >= e.s.dimy) clamped = true;
+    X = clampi(X, 0, e.s.dimx - 1);
+    Y = clampi(Y, 0, e.s.dimy - 1);
+    win[q] = e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+  }
+  if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
+  __syncthreads();
+  const int su = blk * e.nsu + wave;
+  const bool active = su < U;
+  // stage this shift column: cell c of staged row y is window column su + c
+  if (active) {
+    const int nq = WH * (cw >> 2);
+    for (int q = lane; q < nq; q += 64) {
+      const int y = q / (cw >> 2), c4 = (q - y * (cw >> 2)) * 4;
+      unsigned int hi = 0u, lo = 0u;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const unsigned int v = win[y * WW + su + c4 + j];
+        hi |= ((v >> 7) & 127u) << (8 * j);
+        lo |= (v & 127u) << (8 * j);
+      }
+      *reinterpret_cast<unsigned int*>(stage + (size_t)y * cs + c4) = hi;
+      *reinterpret_cast<unsigned int*>(stage + (size_t)e.wh_pad * cs + (size_t)y * cs + c4) = lo;
+    }
+  }
+  __syncthreads();
+  const int gg = lane >> 4, col = lane & 15;
+  double best = 0.0;
+  int bidx = INT_MAX;
+  const int64_t* mom = e.mom + 3 * li;
+  const long long N = mom[0], Sa = mom[1], Saa = mom[2];
+  const double d0 = (double)(N * Saa - Sa * Sa);
+  const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
+  const int32_t* seg = e.seg + sl.seg_off;
+  if (active) {
+    const int ntile = vp >> 4;
+    for (int p0 = 0; p0 < ntile; p0 += NTP) {
+      v4i acc[NMT][NTP][2];
+#pragma unroll
+      for (int a = 0; a < NMT; a++)
+#pragma unroll
+        for (int j = 0; j < NTP; j++)
+#pragma unroll
+          for (int p = 0; p < 2; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
+      for (int ch = 0; ch < kc; ch++) {
+        v4i af[NMT];
+#pragma unroll
+        for (int a = 0; a < NMT; a++)
+          af[a] = *reinterpret_cast<const v4i*>(tmpl + (((int64_t)a * kc + ch) * 64 + lane) * 16);
+        const int segv = seg[4 * ch + gg];
+        const int rowoff = (segv >> 16) * cs + (segv & 0x7fff);  // padding segments read row 0
+#pragma unroll
+        for (int j = 0; j < NTP; j++) {
+          const int sv = (p0 + j) * 16 + col;
+          const uint8_t* bp = stage + (size_t)sv * cs + rowoff;
+          const v4i b1 = *reinterpret_cast<const v4i*>(bp);
+          const v4i b0 = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
+#pragma unroll
+          for (int a = 0; a < NMT; a++) {
+            acc[a][j][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], b1, acc[a][j][0], 0, 0, 0);
+            acc[a][j][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], b0, acc[a][j][1], 0, 0, 0);
+          }
+        }
+      }
+      // epilogue: this lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg, + 1 (digit 1, 0)
+#pragma unroll
+      for (int a = 0; a < NMT; a++)
+#pragma unroll
+        for (int j = 0; j < NTP; j++) {
+          const int sv = (p0 + j) * 16 + col;
+          if (sv >= V) continue;
+          const int sidx = su * V + sv;
+          const long long Bm = e.mid[sl.mid_off + 2 * (int64_t)sidx];
+          const long long Cm = e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
+          const int base = sv * WW + su;
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int k = 8 * a + 2 * gg + h;
+            if (k >= nsteps) continue;
+            const long long A = (long long)acc[a][j][0][2 * h] * 16384LL +
+                                ((long long)acc[a][j][1][2 * h] + (long long)acc[a][j][0][2 * h + 1]) * 128LL +
+                                (long long)acc[a][j][1][2 * h + 1];
+            long long B = Bm, C = Cm;
+            const int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * e.dcap;
+            const int nd = e.dlen[(int64_t)li * nsteps + k];
+            for (int q = 0; q < nd; q++) {
+              const int ent = dl[q];
+              const long long w = win[base + (ent >> 3)];
+              const int dlt = (ent & 7) - 4;
+              B += dlt * w;
+              C += dlt * w * w;
+            }
+            const long long num = N * A - Sa * B;
+            const long long d1 = N * C - B * B;
+            const double ncc = (double)num / sqrt(d0 * (double)d1);
+            const int gidx = k * S + sidx;
+            if (better_g(ncc, gidx, best, bidx)) {
+              best = ncc;
+              bidx = gidx;
+            }
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double ob = __shfl_xor(best, m, 64);
+    const int oi = __shfl_xor(bidx, m, 64);
+    if (better_g(ob, oi, best, bidx)) {
+      best = ob;
+      bidx = oi;
+    }
+  }
+  __shared__ double wb[8];
+  __shared__ int wi[8];
+  if (lane == 0) {
+    wb[wave] = best;
+    wi[wave] = bidx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (nthreads >> 6); w++)
+      if (better_g(wb[w], wi[w], best, bidx)) {
+        best = wb[w];
+        bidx = wi[w];
+      }
+    StepBest sb;
+    sb.ncc = best;
+    sb.idx = bidx;
+    sb.pad = 0;
+    e.out[(int64_t)li * e.part_cap + blk] = sb;
+  }
+}
+
+__global__ void k_fill_best(StepBest* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    StepBest sb;
+    sb.ncc = 0.0;
+    sb.idx = INT_MAX;
+    sb.pad = 0;
+    p[i] = sb;
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---- host side -----------------------------------------------------------------------
+template <int NMT, int NTP>
+static hipError_t launch_main(const ExactArgs& e, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  if (lds > 65536) {
+    hipError_t r = hipFuncSetAttribute((const void*)k_search_exact<NMT, NTP>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (r != hipSuccess) return r;
+  }
+  k_search_exact<NMT, NTP><<<grid, block, lds, st>>>(e);
+  return hipGetLastError();
+}
+
+static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
+                                hipStream_t st) {
+#define PTG_CASE(M, T) \
+  if (nmt == M && ntp == T) return launch_main<M, T>(e, grid, block, lds, st);
+  PTG_CASE(1, 3) PTG_CASE(2, 3) PTG_CASE(3, 3) PTG_CASE(4, 2) PTG_CASE(5, 2) PTG_CASE(6, 2)
+#undef PTG_CASE
+  return hipErrorInvalidValue;
+}
+
+namespace {
+struct Bucket {
+  int rho_max;
+  std::vector<int32_t> slots;
+};
+}  // namespace
+
+int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
+                     StepBest* out_best, int* nparts_out) {
+  const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
+  if (i0.bits > 12 || i1.bits > 12)
+    return c.fail(PTG_ERR_LIMIT, "the exact search takes frames of at most 12 bits (got %d/%d); use PTG_SEARCH_PLAIN",
+                  i0.bits, i1.bits);
+  const int nsteps = sa.nsteps;
+  const int nmt = (2 * nsteps + 15) / 16;
+  if (nmt > 6) return c.fail(PTG_ERR_LIMIT, "the exact search takes at most 48 rotation steps (got %d)", nsteps);
+  const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1, S = U * V;
+  if ((int64_t)nsteps * S >= INT_MAX / 2) return c.fail(PTG_ERR_LIMIT, "too many candidates");
+  const int vp = (V + 15) / 16 * 16;
+  const int ntile = vp / 16;
+  const int ntp = nmt <= 3 ? 3 : 2;
+  (void)ntile;
+  // per-slot geometry and segment tables
+  std::vector<ExactSlot> slots(nlist);
+  std::vector<int32_t> segs;
+  int64_t tmpl_total = 0, mid_total = 0;
+  int rho_max = 0;
+  for (int li = 0; li < nlist; li++) {
+    const int g = list[li];
+    const int rho = c.h_prad[g];
+    rho_max = std::max(rho_max, rho);
+    ExactSlot& s = slots[li];
+    s.rho = rho;
+    s.cw = (2 * rho + 1 + 15) / 16 * 16;
+    s.cs = ((s.cw / 16) % 2 == 1) ? s.cw : s.cw + 16;
+    s.seg_off = (int64_t)segs.size();
+    for (int qy = -rho; qy <= rho; qy++) {
+      const int xm = (int)floor(sqrt((double)(rho * rho - qy * qy)));
+      const int c_lo = (rho - xm) / 16 * 16, c_hi = rho + xm;
+      for (int c0 = c_lo; c0 <= c_hi; c0 += 16) segs.push_back(((qy + rho) << 16) | c0);
+    }
+    while ((segs.size() - s.seg_off) % 4) segs.push_back(0x8000);  // padding segment (zero template)
+    s.kc = (int)((segs.size() - s.seg_off) / 4);
+    s.tmpl_off = tmpl_total;
+    tmpl_total += (int64_t)nmt * s.kc * 1024;
+    s.mid_off = mid_total;
+    mid_total += 2 * (int64_t)S;
+  }
+  const int dcap = 8 * rho_max + 64;
+  for (int li = 0; li < nlist; li++) slots[li].dl_off = (int64_t)li * nsteps * dcap;
+  // buckets by rho so each launch reserves LDS for its own grains
+  const int bounds[] = {13, 21, 29, 37, 45, 1 << 20};
+  std::vector<Bucket> buckets(sizeof(bounds) / sizeof(bounds[0]), Bucket{0, {}});
+  for (int li = 0; li < nlist; li++) {
+    int bk = 0;
+    while (slots[li].rho > bounds[bk]) bk++;
+    buckets[bk].slots.push_back(li);
+    buckets[bk].rho_max = std::max(buckets[bk].rho_max, slots[li].rho);
+  }
+  // device buffers in one scratch allocation
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t b_slots = al(sizeof(ExactSlot) * nlist), b_seg = al(sizeof(int32_t) * segs.size()),
+               b_bucket = al(sizeof(int32_t) * nlist), b_tmpl = al((size_t)tmpl_total),
+               b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * dcap),
+               b_dlen = al(sizeof(int32_t) * (size_t)nlist * nsteps), b_mid = al(sizeof(int64_t) * (size_t)mid_total),
+               b_mom = al(sizeof(int64_t) * 3 * nlist), b_flags = al(sizeof(int32_t) * 4);
+  const size_t total = b_slots + b_seg + b_bucket + b_tmpl + b_dl + b_dlen + b_mid + b_mom + b_flags;
+  int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, total);
+  if (rc != PTG_OK) return rc;
+  char* base = (char*)c.d_exact_scratch;
+  ExactArgs e;
+  e.s = sa;
+  e.slots = (const ExactSlot*)base;
+  base += b_slots;
+  e.seg = (const int32_t*)base;
+  char* seg_dev = base;
+  base += b_seg;
+  int32_t* bucket_dev = (int32_t*)base;
+  base += b_bucket;
+  e.tmpl = (uint8_t*)base;
+  base += b_tmpl;
+  e.dlist = (int32_t*)base;
+  base += b_dl;
+  e.dlen = (int32_t*)base;
+  base += b_dlen;
+  e.mid = (int64_t*)base;
+  base += b_mid;
+  e.mom = (int64_t*)base;
+  base += b_mom;
+  e.flags = (int32_t*)base;
+  e.dcap = dcap;
+  e.nmt = nmt;
+  e.vp = vp;
+  // host staging for the small tables (synchronous copies: they are small)
+  std::vector<int32_t> bucket_cat;
+  for (auto& bk : buckets) bucket_cat.insert(bucket_cat.end(), bk.slots.begin(), bk.slots.end());
+  hipError_t he;
+  if ((he = hipMemcpyAsync((void*)e.slots, slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
+                           c.stream)) != hipSuccess ||
+      (he = hipMemcpyAsync(seg_dev, segs.data(), sizeof(int32_t) * segs.size(), hipMemcpyHostToDevice, c.stream)) !=
+          hipSuccess ||
+      (he = hipMemcpyAsync(bucket_dev, bucket_cat.data(), sizeof(int32_t) * bucket_cat.size(), hipMemcpyHostToDevice,
+                           c.stream)) != hipSuccess ||
+      (k_zero_i32<<<1, 64, 0, c.stream>>>(e.flags, 4), (he = hipGetLastError()) != hipSuccess))
+    return c.check(he, "exact search tables");
+  if ((he = hipStreamSynchronize(c.stream)) != hipSuccess) return c.check(he, "hipStreamSynchronize");
+  // shift blocks: up to 8 columns per block, balanced; results of block b of slot li go
+  // to out_best[li * U + b], the rest stay "none"
+  e.out = out_best;
+  e.part_cap = U;
+  *nparts_out = U;
+  k_fill_best<<<(nlist * U + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * U);
+  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
+  size_t off = 0;
+  for (auto& bk : buckets) {
+    const int nb = (int)bk.slots.size();
+    if (nb == 0) continue;
+    e.bucket = bucket_dev + off;
+    e.nbucket = nb;
+    off += nb;
+    const int rho = bk.rho_max, side = 2 * rho + 1;
+    const int cw = (side + 15) / 16 * 16, cs = ((cw / 16) % 2 == 1) ? cw : cw + 16;
+    // templates and Delta lists
+    const size_t lds_t = (size_t)2 * side * side * 4;
+    if (lds_t > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", rho);
+    if (lds_t > 65536 &&
+        (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_t)) != hipSuccess)
+      return c.check(he, "hipFuncSetAttribute");
+    k_exact_template<<<dim3(nb, 8 * nmt), 256, lds_t, c.stream>>>(e);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
+    // middle-step window moments
+    const size_t PW = U + 2 * rho + 1, PH = V + 2 * rho;
+    const size_t lds_m = PW * PH * 4 + (size_t)side * side * 4 + 2 * kMaxRuns * 4;
+    if (lds_m > 160 * 1024)
+      return c.fail(PTG_ERR_LIMIT, "search zone and pattern too large for the exact search (%zu B LDS)", lds_m);
+    if (lds_m > 65536 &&
+        (he = hipFuncSetAttribute((const void*)k_exact_mid, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_m)) != hipSuccess)
+      return c.check(he, "hipFuncSetAttribute");
+    k_exact_mid<<<nb, 256, lds_m, c.stream>>>(e);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
+    // the MFMA search
+    const int WW = U - 1 + cw, WH = vp + 2 * rho;
+    e.ww = WW;
+    e.wh_pad = WH;
+    const size_t win_bytes = ((size_t)WW * WH * 2 + 15) / 16 * 16;
+    const size_t stage_bytes = ((size_t)2 * WH * cs + 15) / 16 * 16;
+    int nsu_b = std::min(8, U);
+    while (nsu_b > 1 && win_bytes + nsu_b * stage_bytes > 150 * 1024) nsu_b--;
+    if (win_bytes + nsu_b * stage_bytes > 160 * 1024)
+      return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
+    ExactArgs eb = e;
+    eb.nsub = (U + nsu_b - 1) / nsu_b;
+    eb.nsu = (U + eb.nsub - 1) / eb.nsub;  // balance the columns over the blocks
+    const size_t lds = win_bytes + nsu_b * stage_bytes;
+    const int groups = (nb + 7) / 8;
+    dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
+    if ((he = dispatch_main(nmt, ntp, eb, grid, block, lds, c.stream)) != hipSuccess)
+      return c.check(he, "k_search_exact");
+  }
+  // overflow of a template digit or a Delta list: the exact form cannot serve this search
+  int32_t flags = 0;
+  if ((he = hipMemcpyAsync(&flags, e.flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
+      (he = hipStreamSynchronize(c.stream)) != hipSuccess)
+    return c.check(he, "exact search flags");
+  if (flags) return c.fail(PTG_ERR_LIMIT, "exact search capacity exceeded (flags %d): use PTG_SEARCH_PLAIN", flags);
+  return PTG_OK;
+}
+
+}  // namespace ptg

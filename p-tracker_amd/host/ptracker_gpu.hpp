@@ -1,0 +1,219 @@
+// ptracker_gpu.hpp — C++ host side over libptgpu with the reference's names.
+//
+// The state and member functions a caller of p-tracker's per-grain path uses
+// (src/p-tracker.hpp:66-237): grain[], the three search zones, the thresholds, the
+// subpixel settings, do_precomputations(), the per-pair body of particle_tracking()
+// (track_pair), NCC_to_minimize_xyR() and the reference swap. The GPU work goes through
+// the C ABI of include/ptgpu.h; errors raise std::runtime_error with the library's
+// message.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ptgpu.h"
+
+namespace ptgpu {
+
+// src/relative_coord_type.hpp:19-25
+struct relative_coord_type {
+  int dx{0};
+  int dy{0};
+};
+
+// src/search_zone_type.hpp:20-27; the same layout as ptg_search_zone
+struct search_zone_type {
+  int left{0}, right{0}, up{0}, down{0}, num_rot{0};
+  double inc_rot{1.0};
+};
+static_assert(sizeof(search_zone_type) == sizeof(ptg_search_zone), "search zone layouts differ");
+
+// src/grain_type_2D.hpp:23-71 (the neighbour list, dead in the reference, is left out)
+struct grain_type_2D {
+  int refcoord_xpix{0}, refcoord_ypix{0};
+  double refrot{0.0}, radius_pix{0.0};
+  double dx{0.0}, dy{0.0}, drot{0.0};
+  double dx_prev{0.0}, dy_prev{0.0}, drot_prev{0.0};
+  double upix{0.0}, vpix{0.0}, rot_inc{0.0};
+  double NCC{0.0}, NCC_rescue{0.0}, NCC_subpix{0.0};
+  double mean0{0.0}, C0C0{0.0};
+  bool masked{false};
+  int status{0};
+  std::vector<relative_coord_type> pattern;
+  void backup() { dx_prev = dx; dy_prev = dy; drot_prev = drot; }
+  void reset() { NCC = NCC_rescue = NCC_subpix = 0.0; }
+};
+
+class ptracker_gpu {
+ public:
+  std::vector<grain_type_2D> grain;
+  search_zone_type search_zone, search_zone_rescue, search_zone_super_rescue;
+  int rescue_level{2};  // src/p-tracker.hpp:109-127 defaults
+  double NCC_min{0.7};
+  double NCC_min_super{0.5};
+  int subpixel{1};
+  double subpix_tol{1e-8};
+  double initial_direction_dx{0.01}, initial_direction_dy{0.01}, initial_direction_dr{0.01};
+  double subpix_displacement_max{3.0};
+  double overflow_stiffness{0.0};
+  int im_index_ref{0}, im_index_current{1};
+  int search_method{PTG_SEARCH_EXACT};
+  int subpix_method{PTG_SUBPIX_POWELL};
+
+  explicit ptracker_gpu(int device = 0) {
+    if (ptg_create(&ctx_, device) != PTG_OK) throw std::runtime_error("ptg_create failed");
+  }
+  ~ptracker_gpu() { ptg_destroy(ctx_); }
+  ptracker_gpu(const ptracker_gpu&) = delete;
+  ptracker_gpu& operator=(const ptracker_gpu&) = delete;
+
+  // image[index] from a row-major frame (element (x, y) at frame[y * dimx + x])
+  void set_image(int index, const uint16_t* frame, int dimx, int dimy, int bits) {
+    check(ptg_set_image(ctx_, index, frame, dimx, dimy, dimx, bits), "ptg_set_image");
+  }
+
+  // grain[] and patterns to the device; call after changing patterns or the grain count
+  void upload_grains() {
+    std::vector<int32_t> pxy;
+    std::vector<int64_t> pbeg{0};
+    for (auto& g : grain) {
+      for (auto& p : g.pattern) {
+        pxy.push_back(p.dx);
+        pxy.push_back(p.dy);
+      }
+      pbeg.push_back((int64_t)pxy.size() / 2);
+    }
+    std::vector<ptg_grain> rec = records();
+    check(ptg_set_grains(ctx_, (int32_t)grain.size(), rec.data(), pxy.data(), pbeg.data()), "ptg_set_grains");
+  }
+
+  void download_grains() {
+    std::vector<ptg_grain> rec(grain.size());
+    check(ptg_get_grains(ctx_, rec.data()), "ptg_get_grains");
+    for (size_t g = 0; g < grain.size(); g++) from_record(rec[g], grain[g]);
+  }
+
+  // do_precomputations, src/p-tracker.cpp:624-656
+  void do_precomputations() { check(ptg_precompute(ctx_), "ptg_precompute"); }
+
+  // the per-pair body of particle_tracking, src/p-tracker.cpp:460-581; grain[] holds the
+  // 13 columns afterwards
+  void track_pair() {
+    check(ptg_set_image_indices(ctx_, im_index_ref, im_index_current), "ptg_set_image_indices");
+    ptg_track_params tp{};
+    std::memcpy(&tp.search_zone, &search_zone, sizeof(search_zone));
+    std::memcpy(&tp.search_zone_rescue, &search_zone_rescue, sizeof(search_zone));
+    std::memcpy(&tp.search_zone_super_rescue, &search_zone_super_rescue, sizeof(search_zone));
+    tp.NCC_min = NCC_min;
+    tp.NCC_min_super = NCC_min_super;
+    tp.rescue_level = rescue_level;
+    tp.subpixel = subpixel;
+    tp.search_method = search_method;
+    tp.subpix = subpix_params();
+    check(ptg_track_pair(ctx_, &tp), "ptg_track_pair");
+    download_grains();
+  }
+
+  int num_to_be_rescued() {
+    int32_t a = 0, b = 0;
+    check(ptg_get_queue_counts(ctx_, &a, &b), "ptg_get_queue_counts");
+    return a;
+  }
+
+  // NCC_to_minimize_xyR, src/p-tracker.cpp:1018-1088, for an explicit grain
+  double NCC_to_minimize_xyR(int igrain, std::vector<double>& X) {
+    double f = 0.0;
+    check(ptg_ncc_objective(ctx_, igrain, X.data(), subpix_displacement_max, overflow_stiffness, &f),
+          "ptg_ncc_objective");
+    return f;
+  }
+
+  // the reference swap, src/p-tracker.cpp:587-611 (swaps the image indices too)
+  void change_reference() {
+    check(ptg_change_reference(ctx_), "ptg_change_reference");
+    std::swap(im_index_ref, im_index_current);
+    download_grains();
+  }
+
+  void* context() { return ctx_; }
+
+ private:
+  void* ctx_ = nullptr;
+
+  void check(int rc, const char* what) {
+    if (rc != PTG_OK) throw std::runtime_error(std::string(what) + ": " + ptg_last_error(ctx_));
+  }
+
+  ptg_subpix_params subpix_params() const {
+    ptg_subpix_params p{};
+    p.subpix_tol = subpix_tol;
+    p.initial_direction[0] = initial_direction_dx;
+    p.initial_direction[1] = initial_direction_dy;
+    p.initial_direction[2] = initial_direction_dr;
+    p.subpix_displacement_max = subpix_displacement_max;
+    p.overflow_stiffness = overflow_stiffness;
+    p.method = subpix_method;
+    p.max_iter = 0;
+    return p;
+  }
+
+  std::vector<ptg_grain> records() const {
+    std::vector<ptg_grain> rec(grain.size());
+    for (size_t i = 0; i < grain.size(); i++) {
+      const grain_type_2D& g = grain[i];
+      ptg_grain& r = rec[i];
+      r = ptg_grain{};
+      r.refcoord_xpix = g.refcoord_xpix;
+      r.refcoord_ypix = g.refcoord_ypix;
+      r.refrot = g.refrot;
+      r.radius_pix = g.radius_pix;
+      r.dx = g.dx;
+      r.dy = g.dy;
+      r.drot = g.drot;
+      r.dx_prev = g.dx_prev;
+      r.dy_prev = g.dy_prev;
+      r.drot_prev = g.drot_prev;
+      r.upix = g.upix;
+      r.vpix = g.vpix;
+      r.rot_inc = g.rot_inc;
+      r.NCC = g.NCC;
+      r.NCC_rescue = g.NCC_rescue;
+      r.NCC_subpix = g.NCC_subpix;
+      r.masked = g.masked ? 1 : 0;
+    }
+    return rec;
+  }
+
+  static void from_record(const ptg_grain& r, grain_type_2D& g) {
+    g.refcoord_xpix = r.refcoord_xpix;
+    g.refcoord_ypix = r.refcoord_ypix;
+    g.refrot = r.refrot;
+    g.dx = r.dx;
+    g.dy = r.dy;
+    g.drot = r.drot;
+    g.dx_prev = r.dx_prev;
+    g.dy_prev = r.dy_prev;
+    g.drot_prev = r.drot_prev;
+    g.upix = r.upix;
+    g.vpix = r.vpix;
+    g.rot_inc = r.rot_inc;
+    g.NCC = r.NCC;
+    g.NCC_rescue = r.NCC_rescue;
+    g.NCC_subpix = r.NCC_subpix;
+    g.mean0 = r.mean0;
+    g.C0C0 = r.C0C0;
+    g.status = r.status;
+  }
+};
+
+// The NCC_optimizer functor (src/NCC_optimizer_functor.hpp:22-25) with an explicit grain.
+struct NCC_optimizer {
+  ptracker_gpu* ptrackerObject;
+  int igrain;
+  double operator()(std::vector<double>& X) { return ptrackerObject->NCC_to_minimize_xyR(igrain, X); }
+};
+
+}  // namespace ptgpu

@@ -244,3 +244,46 @@ def test_change_reference_sequence(tracker):
     for col in INT_COLS:
         assert bits_equal(g2[col], ref2[col]), col
     assert np.array_equal(g2["refcoord_xpix"], ref2["refcoord_xpix"])
+
+
+# ---- exact-integer MFMA search -------------------------------------------------------------
+@pytest.mark.parametrize("which", ["cfg1", "cfg2s"])
+def test_main_search_exact_picks_equal_oracle(tracker, which, request):
+    w = request.getfixturevalue(which)
+    z = w.params.search_zone
+    say(f"main search exact {which}: G={w.G}")
+    load(tracker, w)
+    tracker.begin_pair()
+    tracker.search(pk.STAGE_MAIN, z, 0.7, 0.5, pk.SEARCH_EXACT)
+    got = tracker.get_grains()
+    ref, queued = _oracle_main(w, z)
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+    assert tracker.queue_counts()[0] == len(queued)
+
+
+def test_rescue_exact_picks_equal_oracle(tracker):
+    w = synth.workload(2, n_grains=24, size=(800, 700), max_disp=6.0, seed=77)
+    z, zr, zs = pk.zone(2, num_rot=2, inc_rot=0.02), pk.zone(4, num_rot=2, inc_rot=0.02), pk.zone(8, num_rot=1, inc_rot=0.05)
+    prm = pk.track_params(z, zr, zs, NCC_min=0.7, NCC_min_super=0.6, rescue_level=2, subpixel=0,
+                          search_method=pk.SEARCH_EXACT)
+    say("rescue + super-rescue exact")
+    load(tracker, w)
+    tracker.track_pair(prm)
+    got = tracker.get_grains()
+    ref, counts, _ = ptref.track_pair(oracle_workload(w), prm, nthreads=4)
+    assert tracker.queue_counts() == counts
+    for col in ("dx", "dy", "drot"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC_rescue"] - ref["NCC_rescue"]).max() < 1e-12
+
+
+def test_exact_rejects_16bit_frames(tracker, cfg1):
+    say("exact search refuses 16-bit frames")
+    tracker.set_image(0, cfg1.I0, 16)
+    tracker.set_image(1, cfg1.I1, 16)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(cfg1.grains, cfg1.pattern_xy, cfg1.pattern_begin)
+    with pytest.raises(pk.PtgError):
+        tracker.search(pk.STAGE_MAIN, cfg1.params.search_zone, 0.7, 0.5, pk.SEARCH_EXACT)
