@@ -110,7 +110,10 @@ def main():
     R = len(ptref.rotation_steps(z.inc_rot, z.num_rot))
     S = (z.left + z.right + 1) * (z.up + z.down + 1)
     flops = 2.0 * R * S * P.sum()
-    k_ms = ms[pk.T_SEARCH_KERNEL] / max(1, launches[pk.T_SEARCH_KERNEL])
+    # the scoring kernel's GPU time per pair (one launch for the plain form, one per
+    # pattern-radius bucket for the exact form), HIP events on the library stream
+    k_ms = ms[pk.T_SEARCH_KERNEL] / args.steps
+    k_launches = launches[pk.T_SEARCH_KERNEL] / args.steps
     achieved = flops / (k_ms * 1e-3) / 1e12
     if method == "exact":
         roof = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
@@ -118,9 +121,10 @@ def main():
     else:
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
-    roof["kernel"] = "search candidate scoring, main stage"
-    roof["algorithmic_flops_per_launch"] = flops
-    roof["avg_launch_ms"] = k_ms
+    roof["kernel"] = "k_search_exact (MFMA)" if method == "exact" else "k_search_plain (FP64)"
+    roof["algorithmic_flops_per_pair"] = flops
+    roof["kernel_ms_per_pair"] = k_ms
+    roof["launches_per_pair"] = k_launches
 
     result = {
         "metric": METRIC, "value": value, "unit": "grains/s", "n_gpus": 1, "steps": args.steps,
@@ -173,6 +177,22 @@ def main():
                     break
         except OSError:
             pass
+        # the fast form's error against the FP64 oracle (the oracle's Powell run to a tight
+        # tolerance) on the same sample, untimed
+        tp = pk.track_params(w.params.search_zone, w.params.search_zone_rescue, w.params.search_zone_super_rescue,
+                             NCC_min=0.7, rescue_level=1, subpixel=1, search_method=pk.SEARCH_PLAIN,
+                             subpix=pk.default_subpix(pk.SUBPIX_POWELL, subpix_tol=1e-13))
+        ot, _, _ = ptref.track_pair(W, tp, nthreads=threads, subset=sample)
+        fs = out[sample]
+        epx = np.hypot(fs["dx"] - ot["dx"], fs["dy"] - ot["dy"])
+        erad = np.abs(fs["drot"] - ot["drot"])
+        encc = np.abs(fs["NCC_subpix"] - ot["NCC_subpix"])
+        result["fp64_oracle_error"] = {
+            "grains": int(len(sample)), "max_px": float(epx.max()), "max_rad": float(erad.max()),
+            "max_ncc_subpix": float(encc.max()), "over_1e-3_px": int((epx > 1e-3).sum()),
+            "over_1e-5_rad": int((erad > 1e-5).sum())}
+        log(f"fast form vs FP64 oracle on {len(sample)} grains: max {epx.max():.2e} px, {erad.max():.2e} rad, "
+            f"{(epx > 1e-3).sum()} grains over 1e-3 px, {(erad > 1e-5).sum()} over 1e-5 rad")
         result["cpu_baseline"] = {"value": len(sample) / ce, "unit": "grains/s", "cores": threads, "kind": "port",
                                   "sample": f"{len(sample)} grains drawn from the same pair (seed 0), whole "
                                             f"per-pair path (precompute, search, rescue, Powell) through the "

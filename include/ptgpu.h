@@ -197,15 +197,19 @@ enum {
   PTG_T_SEARCH_SUPER = 3,
   PTG_T_SUBPIXEL = 4,
   PTG_T_REMAP = 5,
-  PTG_T_SEARCH_KERNEL = 6, /* the candidate-scoring kernel alone, main stage */
+  PTG_T_SEARCH_KERNEL = 6, /* the candidate-scoring kernel(s) alone, main stage: the FP64
+                              kernel (plain) or the MFMA kernel, one launch per radius
+                              bucket (exact) */
   PTG_T_COUNT = 7
 };
 int ptg_set_timing(void* ctx, int enable);
 /* Accumulated milliseconds and launch counts since the last reset (synchronises). */
 int ptg_get_stage_times(void* ctx, double* ms, int64_t* launches, int n);
 int ptg_reset_stage_times(void* ctx);
-/* Work counters of the last subpixel stage: total objective evaluations. */
+/* Work counters of the last subpixel stage: total objective evaluations, and per grain
+ * (per_grain may be NULL; it receives G counts, 0 for masked grains). */
 int ptg_get_subpix_evals(void* ctx, int64_t* evals);
+int ptg_get_subpix_evals_per_grain(void* ctx, int32_t* per_grain);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_get_stage_times": ([vp, vp, vp, c_int], c_int),
         "ptg_reset_stage_times": ([vp], c_int),
         "ptg_get_subpix_evals": ([vp, POINTER(c_int64)], c_int),
+        "ptg_get_subpix_evals_per_grain": ([vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -163,7 +164,8 @@ def exported_symbols():
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
-            "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals"]
+            "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
+            "ptg_get_subpix_evals_per_grain"]
 
 
 class PtgError(RuntimeError):
@@ -312,6 +314,11 @@ class Tracker:
 
     def reset_stage_times(self):
         self._ck(self.lib.ptg_reset_stage_times(self.h), "ptg_reset_stage_times")
+
+    def subpix_evals_per_grain(self) -> np.ndarray:
+        out = np.zeros(self.G, dtype=np.int32)
+        self._ck(self.lib.ptg_get_subpix_evals_per_grain(self.h, _ptr(out)), "ptg_get_subpix_evals_per_grain")
+        return out
 
     def subpix_evals(self) -> int:
         e = c_int64()

@@ -121,7 +121,6 @@ struct Ctx : Context {
   PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list;
   enum { SNAP_INVALID = 0, SNAP_VALID = 1, SNAP_QUEUES = 2 } snap = SNAP_INVALID;
   std::vector<double> h_steps;
-  std::vector<int32_t> h_unmasked;
   // recorded after the last upload from the pinned staging buffers, waited on before
   // the host rewrites them
   hipEvent_t staging_done = nullptr;
@@ -341,10 +340,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     CK(c, launch_search_plain(*c, a, nlist, lds));
     if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
   } else {
-    hipEvent_t ek;
-    if (stage == PTG_STAGE_MAIN) c->mark_begin(PTG_T_SEARCH_KERNEL, &ek);
-    TRY(run_search_exact(*c, a, nlist, list, c->d_best, &nparts));
-    if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
+    TRY(run_search_exact(*c, a, nlist, list, c->d_best, &nparts, stage == PTG_STAGE_MAIN));
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
   int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
@@ -407,7 +403,7 @@ int ptg_destroy(void* p) {
   for (int i = 0; i < kMaxImages; i++)
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
-  void* more[] = {c->d_counts, c->d_cs, c->d_steps, c->d_best, c->d_evals, c->d_exact_scratch, c->d_tmp};
+  void* more[] = {c->d_counts, c->d_cs, c->d_steps, c->d_best, c->d_evals, c->d_evals_g, c->d_exact_scratch, c->d_tmp};
   for (void* b : more)
     if (b) (void)hipFree(b);
   PinnedBuf* pins[] = {&c->pin_cs, &c->pin_steps, &c->pin_cs0, &c->pin_list};
@@ -638,6 +634,8 @@ int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
   c->mark_begin(PTG_T_SUBPIXEL, &ev);
   k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
   CK(c, hipGetLastError());
+  TRY(c->ensure((void**)&c->d_evals_g, &c->evals_g_cap, sizeof(int32_t) * c->G));
+  CK(c, fill_i32(*c, c->d_evals_g, c->G, 0));
   CK(c, launch_subpixel(*c, *prm));
   c->mark_end(PTG_T_SUBPIXEL, ev);
   c->snap = Ctx::SNAP_INVALID;
@@ -878,6 +876,19 @@ int ptg_reset_stage_times(void* p) {
     c->stage_ms[i] = 0.0;
     c->stage_launches[i] = 0;
   }
+  return PTG_OK;
+}
+
+int ptg_get_subpix_evals_per_grain(void* p, int32_t* per_grain) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!per_grain) return c->fail(PTG_ERR_ARG, "null output");
+  if (!c->d_evals_g || c->G == 0) {
+    for (int g = 0; g < c->G; g++) per_grain[g] = 0;
+    return PTG_OK;
+  }
+  CK(c, hipMemcpyAsync(per_grain, c->d_evals_g, sizeof(int32_t) * c->G, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
   return PTG_OK;
 }
 

@@ -81,6 +81,8 @@ struct Context {
   int32_t* d_list[3] = {nullptr, nullptr, nullptr};
   int32_t* d_counts = nullptr;  // 3 counters
   int32_t n_unmasked = 0;
+  std::vector<int32_t> h_unmasked;  // host copy of list [0]
+  const std::vector<int32_t>& h_unmasked_ref() const { return h_unmasked; }
 
   // search scratch
   double* d_cs = nullptr;
@@ -93,6 +95,8 @@ struct Context {
 
   // subpixel
   int64_t* d_evals = nullptr;
+  int32_t* d_evals_g = nullptr;  // per grain, last subpixel stage
+  size_t evals_g_cap = 0;
   double* d_sd0 = nullptr;  // per grain: sum of d0 (used by the one-pass objective)
 
   // exact-form scratch
@@ -134,8 +138,9 @@ hipError_t launch_build_unmasked(Context& c);
 
 // exact (MFMA) form: returns an error code of the C ABI, writes message into c.err.
 // Writes nparts partial argmaxes per list slot into out_best.
+// time_kernel: bracket each MFMA kernel launch with the PTG_T_SEARCH_KERNEL timer.
 int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vector<int32_t>& list,
-                     StepBest* out_best, int* nparts_out);
+                     StepBest* out_best, int* nparts_out, bool time_kernel);
 
 // subpixel
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);

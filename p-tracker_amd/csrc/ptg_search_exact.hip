@@ -39,7 +39,7 @@ struct ExactSlot {
   int32_t kc;        // K-chunks (4 segments each)
   int64_t seg_off;   // first segment in the segment table
   int64_t tmpl_off;  // first byte of this slot's template fragments
-  int64_t dl_off;    // first Delta entry (nsteps * dcap entries)
+  int64_t dl_off;    // first Delta entry (nsteps * 2 * dcap entries)
   int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
 };
 
@@ -50,20 +50,30 @@ struct ExactArgs {
   int32_t nbucket;
   const int32_t* seg;          // segment table: (qy + rho) << 16 | c0
   uint8_t* tmpl;               // template fragments
-  int32_t* dlist;              // Delta entries: offset << 3 | (delta + 4)
-  int32_t* dlen;               // per (slot, step)
+  int32_t* dlist;              // Delta cell offsets: per step dcap cells gained, then dcap cells lost
+                               // (a cell whose multiplicity changes by 2 appears twice)
+  int32_t* dlen;               // per (slot, step): cells gained, cells lost
   int64_t* mid;                // B_mid, C_mid per (slot, shift)
   int64_t* mom;                // per slot: N, Sa, Saa
   int32_t* flags;              // [0]: template or Delta overflow
   StepBest* out;               // per (slot, shift block), part_cap entries per slot
   int32_t part_cap;
-  int32_t dcap;
+  int32_t dcap;                // per sign; at most 255 so 32-bit sums of I1^2 stay exact
   int32_t nmt;                 // M-tiles (8 steps each)
   int32_t nsub;                // shift blocks per grain
   int32_t nsu;                 // shift columns per block (= waves)
   int32_t vp;                  // vertical shifts padded to 16
   int32_t ww, wh_pad;          // plain window width/height of the largest slot of the launch
+  int32_t cs_max;              // staging row stride of the largest slot of the launch
 };
+
+// Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
+// that a row is an odd number of dwords and lanes reading consecutive rows hit
+// distinct LDS banks.
+__host__ __device__ __forceinline__ int win_width(int U, int cw) {
+  const int w = U - 1 + cw;
+  return w + ((2 - (w & 3)) & 3);
+}
 
 __device__ __forceinline__ int win_x0(const ptg_grain& gr, int left, int rho) {
   return gr.refcoord_xpix + (int)round(gr.dx) - left - rho;
@@ -138,27 +148,27 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
     *reinterpret_cast<uint4*>(e.tmpl + sl.tmpl_off + (((int64_t)mt * kc + ch) * 64 + lane) * 16) = val;
   }
   if (real) {
-    // Delta list: cells whose multiplicity differs from the middle step's
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
+    // Delta lists: cells whose multiplicity differs from the middle step's, split by sign
+    __shared__ int cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
     __syncthreads();
-    int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * e.dcap;
-    const int ww = e.s.left + e.s.right + sl.cw;  // plain window width of this slot
+    int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * e.dcap;
+    const int ww = win_width(e.s.left + e.s.right + 1, sl.cw);  // plain window width of this slot
     for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
-      const int ck = (packed[t] >> 24) & 0xff, cmv = cmid[t];
-      if (ck != cmv) {
-        const int pos = atomicAdd(&cnt, 1);
-        if (pos < e.dcap) {
-          const int qy = t / side, qx = t - qy * side;  // both + rho
-          dl[pos] = ((qy * ww + qx) << 3) | (ck - cmv + 4);
-        }
-        if (ck - cmv > 3 || ck - cmv < -4) overflow = true;
+      const int d = ((packed[t] >> 24) & 0xff) - cmid[t];
+      if (d != 0) {
+        const int sg = d > 0 ? 0 : 1, m = d > 0 ? d : -d;
+        const int pos = atomicAdd(&cnt[sg], m);
+        const int qy = t / side, qx = t - qy * side;  // both + rho
+        for (int r = 0; r < m; r++)
+          if (pos + r < e.dcap) dl[sg * e.dcap + pos + r] = qy * ww + qx;
       }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (cnt > e.dcap) overflow = true;
-      e.dlen[(int64_t)li * nsteps + k] = min(cnt, e.dcap);
+      if (cnt[0] > e.dcap || cnt[1] > e.dcap) overflow = true;
+      e.dlen[2 * ((int64_t)li * nsteps + k)] = min(cnt[0], e.dcap);
+      e.dlen[2 * ((int64_t)li * nsteps + k) + 1] = min(cnt[1], e.dcap);
     }
   }
   if (overflow) atomicOr(&e.flags[0], 1);
@@ -296,7 +306,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const int rho = sl.rho, cw = sl.cw, cs = sl.cs, kc = sl.kc;
   const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1, S = U * V;
   const int vp = e.vp;
-  const int WW = U - 1 + cw, WH = vp + 2 * rho;
+  const int WW = win_width(U, cw), WH = vp + 2 * rho;
   const int nsteps = e.s.nsteps;
   const ptg_grain& gr = e.s.grains[g];
   const int X0 = win_x0(gr, e.s.left, rho), Y0 = win_y0(gr, e.s.up, rho);
@@ -305,8 +315,12 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
 
   uint16_t* win = reinterpret_cast<uint16_t*>(smem);
   const size_t win_bytes = ((size_t)e.ww * e.wh_pad * 2 + 15) / 16 * 16;
-  const size_t stage_bytes = (size_t)2 * e.wh_pad * cs;
-  uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * ((stage_bytes + 15) / 16 * 16);
+  const size_t stage_bytes = ((size_t)2 * e.wh_pad * e.cs_max + 15) / 16 * 16;
+  uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * stage_bytes;
+  unsigned char* tail = smem + win_bytes + (size_t)e.nsu * stage_bytes;
+  double* abuf = reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
+  int32_t* segl = reinterpret_cast<int32_t*>(tail + (size_t)e.nsu * 8 * (NTP * 16) * 8);
+  for (int q = threadIdx.x; q < 4 * kc; q += nthreads) segl[q] = e.seg[sl.seg_off + q];
 
   bool clamped = false;
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
@@ -344,8 +358,10 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const int64_t* mom = e.mom + 3 * li;
   const long long N = mom[0], Sa = mom[1], Saa = mom[2];
   const double d0 = (double)(N * Saa - Sa * Sa);
+  const double Nd = (double)N, Sad = (double)Sa;
+  double bnum = 0.0, bd1 = 1.0;  // this lane's best candidate: numerator and N C - B^2
   const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
-  const int32_t* seg = e.seg + sl.seg_off;
+  const int32_t* segs = segl;
   if (active) {
     const int ntile = vp >> 4;
     for (int p0 = 0; p0 < ntile; p0 += NTP) {
@@ -356,66 +372,132 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         for (int j = 0; j < NTP; j++)
 #pragma unroll
           for (int p = 0; p < 2; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
-      for (int ch = 0; ch < kc; ch++) {
-        v4i af[NMT];
+      // Software pipeline: the template fragments (global) and window fragments (LDS)
+      // of chunk ch + 1 are requested before chunk ch multiplies; two named buffer sets
+      // and scheduling barriers keep the loads ahead of their use.
+      v4i fa0[NMT], fb0[NTP][2], fa1[NMT], fb1[NTP][2];
+      auto fetch = [&](int chq, v4i* fa, v4i (*fb)[2]) {
+        const int chc = chq < kc ? chq : kc - 1;
 #pragma unroll
         for (int a = 0; a < NMT; a++)
-          af[a] = *reinterpret_cast<const v4i*>(tmpl + (((int64_t)a * kc + ch) * 64 + lane) * 16);
-        const int segv = seg[4 * ch + gg];
+          fa[a] = *reinterpret_cast<const v4i*>(tmpl + (((int64_t)a * kc + chc) * 64 + lane) * 16);
+        const int segv = segs[4 * chc + gg];
         const int rowoff = (segv >> 16) * cs + (segv & 0x7fff);  // padding segments read row 0
 #pragma unroll
         for (int j = 0; j < NTP; j++) {
-          const int sv = (p0 + j) * 16 + col;
-          const uint8_t* bp = stage + (size_t)sv * cs + rowoff;
-          const v4i b1 = *reinterpret_cast<const v4i*>(bp);
-          const v4i b0 = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
+          const uint8_t* bp = stage + (size_t)((p0 + j) * 16 + col) * cs + rowoff;
+          fb[j][0] = *reinterpret_cast<const v4i*>(bp);
+          fb[j][1] = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
+        }
+      };
+      auto mult = [&](const v4i* fa, v4i (*fb)[2]) {
+#pragma unroll
+        for (int j = 0; j < NTP; j++)
 #pragma unroll
           for (int a = 0; a < NMT; a++) {
-            acc[a][j][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], b1, acc[a][j][0], 0, 0, 0);
-            acc[a][j][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], b0, acc[a][j][1], 0, 0, 0);
+            acc[a][j][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][0], acc[a][j][0], 0, 0, 0);
+            acc[a][j][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][1], acc[a][j][1], 0, 0, 0);
           }
-        }
+      };
+      fetch(0, fa0, fb0);
+      for (int ch = 0; ch < kc; ch += 2) {  // kc is even (host pads with empty chunks)
+        fetch(ch + 1, fa1, fb1);
+        __builtin_amdgcn_sched_barrier(0);
+        mult(fa0, fb0);
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(ch + 2, fa0, fb0);
+        __builtin_amdgcn_sched_barrier(0);
+        mult(fa1, fb1);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      // epilogue: this lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg, + 1 (digit 1, 0)
+      // epilogue, one M-tile (8 steps) at a time: the cross terms go through LDS so that
+      // lanes become shifts and the step (and its correction list) is wave-uniform
+      const int svl = p0 * 16 + lane;  // this lane's shift in the epilogue
+      const bool lane_ok = lane < NTP * 16 && svl < V;
+      double Bm = 0.0, Cm = 0.0;
+      int base = 0, sidx = 0;
+      if (lane_ok) {
+        sidx = su * V + svl;
+        Bm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx];
+        Cm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
+        base = svl * WW + su;
+      }
 #pragma unroll
-      for (int a = 0; a < NMT; a++)
+      for (int a = 0; a < NMT; a++) {
+        // lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg + h, digits (1, 0) per step
 #pragma unroll
-        for (int j = 0; j < NTP; j++) {
-          const int sv = (p0 + j) * 16 + col;
-          if (sv >= V) continue;
-          const int sidx = su * V + sv;
-          const long long Bm = e.mid[sl.mid_off + 2 * (int64_t)sidx];
-          const long long Cm = e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
-          const int base = sv * WW + su;
+        for (int j = 0; j < NTP; j++)
 #pragma unroll
           for (int h = 0; h < 2; h++) {
-            const int k = 8 * a + 2 * gg + h;
-            if (k >= nsteps) continue;
-            const long long A = (long long)acc[a][j][0][2 * h] * 16384LL +
-                                ((long long)acc[a][j][1][2 * h] + (long long)acc[a][j][0][2 * h + 1]) * 128LL +
-                                (long long)acc[a][j][1][2 * h + 1];
-            long long B = Bm, C = Cm;
-            const int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * e.dcap;
-            const int nd = e.dlen[(int64_t)li * nsteps + k];
-            for (int q = 0; q < nd; q++) {
-              const int ent = dl[q];
-              const long long w = win[base + (ent >> 3)];
-              const int dlt = (ent & 7) - 4;
-              B += dlt * w;
-              C += dlt * w * w;
-            }
-            const long long num = N * A - Sa * B;
-            const long long d1 = N * C - B * B;
-            const double ncc = (double)num / sqrt(d0 * (double)d1);
+            // T I1 from its four digit products, exact in double (< 2^53)
+            const double A = (double)acc[a][j][0][2 * h] * 16384.0 +
+                             ((double)acc[a][j][1][2 * h] + (double)acc[a][j][0][2 * h + 1]) * 128.0 +
+                             (double)acc[a][j][1][2 * h + 1];
+            abuf[(2 * gg + h) * (NTP * 16) + j * 16 + col] = A;
+          }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int kl = 0; kl < 8; kl++) {
+          const int k = 8 * a + kl;
+          if (k >= nsteps) break;
+          const int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * e.dcap;
+          const int npos = e.dlen[2 * ((int64_t)li * nsteps + k)];
+          const int nneg = e.dlen[2 * ((int64_t)li * nsteps + k) + 1];
+          unsigned int bp = 0u, bn = 0u, cp = 0u, cn = 0u;
+          int q = 0;
+          for (; q + 4 <= npos; q += 4) {
+            const unsigned int w0 = win[base + dl[q]], w1 = win[base + dl[q + 1]];
+            const unsigned int w2 = win[base + dl[q + 2]], w3 = win[base + dl[q + 3]];
+            bp += (w0 + w1) + (w2 + w3);
+            cp += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
+          }
+          for (; q < npos; q++) {
+            const unsigned int w = win[base + dl[q]];
+            bp += w;
+            cp += w * w;
+          }
+          const int32_t* dn = dl + e.dcap;
+          q = 0;
+          for (; q + 4 <= nneg; q += 4) {
+            const unsigned int w0 = win[base + dn[q]], w1 = win[base + dn[q + 1]];
+            const unsigned int w2 = win[base + dn[q + 2]], w3 = win[base + dn[q + 3]];
+            bn += (w0 + w1) + (w2 + w3);
+            cn += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
+          }
+          for (; q < nneg; q++) {
+            const unsigned int w = win[base + dn[q]];
+            bn += w;
+            cn += w * w;
+          }
+          if (lane_ok) {
+            // every quantity is an integer below 2^53, so these double operations are exact
+            const double A = abuf[kl * (NTP * 16) + lane];
+            const double B = Bm + ((double)bp - (double)bn);
+            const double C = Cm + ((double)cp - (double)cn);
+            const double num = Nd * A - Sad * B;
+            const double d1 = Nd * C - B * B;
             const int gidx = k * S + sidx;
-            if (better_g(ncc, gidx, best, bidx)) {
-              best = ncc;
-              bidx = gidx;
+            // rank by NCC |NCC| * d0 = num |num| / d1 by cross-multiplication (no divide);
+            // only num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680)
+            if (num > 0.0 && d1 > 0.0) {
+              const double lhs = num * num * bd1, rhs = bnum * bnum * d1;
+              if (bidx == INT_MAX || lhs > rhs || (lhs == rhs && gidx < bidx)) {
+                bnum = num;
+                bd1 = d1;
+                bidx = gidx;
+              }
             }
           }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
   }
+  // the lane's best as the reference's quantity, NCC = num / sqrt(d0 (N C - B^2))
+  if (bidx != INT_MAX) best = bnum / sqrt(d0 * bd1);
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     const double ob = __shfl_xor(best, m, 64);
@@ -491,7 +573,7 @@ struct Bucket {
 }  // namespace
 
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
-                     StepBest* out_best, int* nparts_out) {
+                     StepBest* out_best, int* nparts_out, bool time_kernel) {
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
   if (i0.bits > 12 || i1.bits > 12)
     return c.fail(PTG_ERR_LIMIT, "the exact search takes frames of at most 12 bits (got %d/%d); use PTG_SEARCH_PLAIN",
@@ -524,15 +606,16 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       const int c_lo = (rho - xm) / 16 * 16, c_hi = rho + xm;
       for (int c0 = c_lo; c0 <= c_hi; c0 += 16) segs.push_back(((qy + rho) << 16) | c0);
     }
-    while ((segs.size() - s.seg_off) % 4) segs.push_back(0x8000);  // padding segment (zero template)
+    // padding segments (zero template): a whole number of chunk pairs
+    while ((segs.size() - s.seg_off) % 8) segs.push_back(0x8000);
     s.kc = (int)((segs.size() - s.seg_off) / 4);
     s.tmpl_off = tmpl_total;
     tmpl_total += (int64_t)nmt * s.kc * 1024;
     s.mid_off = mid_total;
     mid_total += 2 * (int64_t)S;
   }
-  const int dcap = 8 * rho_max + 64;
-  for (int li = 0; li < nlist; li++) slots[li].dl_off = (int64_t)li * nsteps * dcap;
+  const int dcap = std::min(255, 4 * rho_max + 64);
+  for (int li = 0; li < nlist; li++) slots[li].dl_off = (int64_t)li * nsteps * 2 * dcap;
   // buckets by rho so each launch reserves LDS for its own grains
   const int bounds[] = {13, 21, 29, 37, 45, 1 << 20};
   std::vector<Bucket> buckets(sizeof(bounds) / sizeof(bounds[0]), Bucket{0, {}});
@@ -546,8 +629,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   const size_t b_slots = al(sizeof(ExactSlot) * nlist), b_seg = al(sizeof(int32_t) * segs.size()),
                b_bucket = al(sizeof(int32_t) * nlist), b_tmpl = al((size_t)tmpl_total),
-               b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * dcap),
-               b_dlen = al(sizeof(int32_t) * (size_t)nlist * nsteps), b_mid = al(sizeof(int64_t) * (size_t)mid_total),
+               b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * 2 * dcap),
+               b_dlen = al(sizeof(int32_t) * 2 * (size_t)nlist * nsteps), b_mid = al(sizeof(int64_t) * (size_t)mid_total),
                b_mom = al(sizeof(int64_t) * 3 * nlist), b_flags = al(sizeof(int32_t) * 4);
   const size_t total = b_slots + b_seg + b_bucket + b_tmpl + b_dl + b_dlen + b_mid + b_mom + b_flags;
   int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, total);
@@ -626,23 +709,31 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     k_exact_mid<<<nb, 256, lds_m, c.stream>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     // the MFMA search
-    const int WW = U - 1 + cw, WH = vp + 2 * rho;
+    const int WW = win_width(U, cw), WH = vp + 2 * rho;
     e.ww = WW;
     e.wh_pad = WH;
     const size_t win_bytes = ((size_t)WW * WH * 2 + 15) / 16 * 16;
     const size_t stage_bytes = ((size_t)2 * WH * cs + 15) / 16 * 16;
+    const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
+    int max_kc = 0;
+    for (int li : bk.slots) max_kc = std::max(max_kc, slots[li].kc);
+    const size_t seg_bytes = (size_t)4 * max_kc * 4;
+    auto need = [&](int w) { return win_bytes + w * (stage_bytes + abuf_bytes) + seg_bytes; };
     int nsu_b = std::min(8, U);
-    while (nsu_b > 1 && win_bytes + nsu_b * stage_bytes > 150 * 1024) nsu_b--;
-    if (win_bytes + nsu_b * stage_bytes > 160 * 1024)
-      return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
+    while (nsu_b > 1 && need(nsu_b) > 150 * 1024) nsu_b--;
+    if (need(nsu_b) > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
     ExactArgs eb = e;
     eb.nsub = (U + nsu_b - 1) / nsu_b;
     eb.nsu = (U + eb.nsub - 1) / eb.nsub;  // balance the columns over the blocks
-    const size_t lds = win_bytes + nsu_b * stage_bytes;
+    eb.cs_max = cs;
+    const size_t lds = need(eb.nsu);
     const int groups = (nb + 7) / 8;
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
+    hipEvent_t ev = nullptr;
+    if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev);
     if ((he = dispatch_main(nmt, ntp, eb, grid, block, lds, c.stream)) != hipSuccess)
       return c.check(he, "k_search_exact");
+    if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev);
   }
   // overflow of a template digit or a Delta list: the exact form cannot serve this search
   int32_t flags = 0;

@@ -8,6 +8,8 @@
 // identical values; the objective's pixel sums are split across the 64 lanes and
 // combined with butterfly reductions whose result is bit-identical in every lane, so
 // all branches stay wave-uniform.
+#include <algorithm>
+
 #include "ptg_device.hpp"
 #include "ptg_internal.hpp"
 
@@ -17,6 +19,9 @@ struct Obj {
   const uint16_t* __restrict__ I1;
   int dimx, dimy;
   int64_t pitch;
+  // the grain's neighbourhood of I1 copied into LDS (taps outside it read global memory)
+  const uint32_t* tile;  // one 32-bit word per pixel: aligned LDS reads
+  int tx0, ty0, tw, th;
   const int32_t* __restrict__ p0;  // pattern0_rotated pairs of this grain
   const double* __restrict__ d0;   // I0(p0) - mean0 of this grain
   int n;
@@ -35,6 +40,9 @@ __device__ __forceinline__ void load_obj(Obj& o, const ptg_grain& g, const uint1
   o.dimx = dimx;
   o.dimy = dimy;
   o.pitch = pitch;
+  o.tile = nullptr;
+  o.tx0 = o.ty0 = 0;
+  o.tw = o.th = 0;
   o.p0 = p0;
   o.d0 = d0;
   o.n = n;
@@ -50,6 +58,25 @@ __device__ __forceinline__ void load_obj(Obj& o, const ptg_grain& g, const uint1
   o.evals = 0;
 }
 
+// image_interpCubic::getValue (src/interpol.cpp:55-86) reading the 4x4 stencil from the
+// LDS tile when it lies inside it (the tile holds the frame's values, edge-clamped as the
+// global path clamps), else from global memory.
+__device__ __forceinline__ double sample(const Obj& o, double x, double y, bool& clamped) {
+  const int ix = (int)floor(x), iy = (int)floor(y);
+  const int lx = ix - 1 - o.tx0, ly = iy - 1 - o.ty0;
+  if (o.tile && lx >= 0 && ly >= 0 && lx < o.tw - 3 && ly < o.th - 3) {
+    if (!(ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2)) clamped = true;
+    const double y01 = y - (double)iy, x01 = x - (double)ix;
+    const uint32_t* r0 = o.tile + ly * o.tw + lx;
+    double arr[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      arr[i] = cubic1((double)r0[i], (double)r0[o.tw + i], (double)r0[2 * o.tw + i], (double)r0[3 * o.tw + i], y01);
+    return cubic1(arr[0], arr[1], arr[2], arr[3], x01);
+  }
+  return interp_cubic(o.I1, o.dimx, o.dimy, o.pitch, x, y, clamped);
+}
+
 __device__ __forceinline__ double weight(const Obj& o, const double* X) {
   double W = 1.0;
   const double disp = sqrt(X[0] * X[0] + X[1] * X[1]);
@@ -59,7 +86,7 @@ __device__ __forceinline__ double weight(const Obj& o, const double* X) {
 
 // (1 - NCC) * W at X. The reference's two passes (mean, then centred products) are
 // replaced by one pass over shifted sums: C1C1 = Sgg - Sg^2/N, C0C1 = Sdg - Sg*Sd0/N.
-__device__ double eval(Obj& o, const double* X) {
+__device__ __noinline__ double eval(Obj& o, const double* X) {
   o.evals++;
   const int lane = threadIdx.x & 63;
   const double xpix1 = o.xr + X[0], ypix1 = o.yr + X[1];
@@ -71,7 +98,7 @@ __device__ double eval(Obj& o, const double* X) {
     const double x = (double)o.p0[2 * i], y = (double)o.p0[2 * i + 1];
     const double xs = xpix1 + c1 * x + s1 * y;
     const double ys = ypix1 - s1 * x + c1 * y;
-    const double v = interp_cubic(o.I1, o.dimx, o.dimy, o.pitch, xs, ys, cl) - o.shift;
+    const double v = sample(o, xs, ys, cl) - o.shift;
     Sg += v;
     Sgg = fma(v, v, Sgg);
     Sdg = fma(o.d0[i], v, Sdg);
@@ -101,22 +128,30 @@ __device__ __forceinline__ double cubic1_d(double p0, double p1, double p2, doub
   return (3.0 * a * x + 2.0 * b) * x + fp0;
 }
 
-__device__ __forceinline__ void interp_cubic_grad(const uint16_t* __restrict__ im, int dimx, int dimy, int64_t pitch,
-                                                  double x, double y, double& v, double& gx, double& gy,
-                                                  bool& clamped) {
+__device__ __forceinline__ void interp_cubic_grad(const Obj& o, double x, double y, double& v, double& gx,
+                                                  double& gy, bool& clamped) {
+  const uint16_t* __restrict__ im = o.I1;
+  const int dimx = o.dimx, dimy = o.dimy;
+  const int64_t pitch = o.pitch;
   const int ix = (int)floor(x), iy = (int)floor(y);
   const double y01 = y - (double)iy, x01 = x - (double)ix;
   double arr[4], darr[4];
   const bool inside = ix >= 1 && iy >= 1 && ix < dimx - 2 && iy < dimy - 2;
   if (!inside) clamped = true;
+  const int lx = ix - 1 - o.tx0, ly = iy - 1 - o.ty0;
+  const bool in_tile = o.tile && lx >= 0 && ly >= 0 && lx < o.tw - 3 && ly < o.th - 3;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     double p[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int X = inside ? ix - 1 + i : clampi(ix - 1 + i, 0, dimx - 1);
-      const int Y = inside ? iy - 1 + j : clampi(iy - 1 + j, 0, dimy - 1);
-      p[j] = (double)im[(int64_t)Y * pitch + X];
+      if (in_tile) {
+        p[j] = (double)o.tile[(ly + j) * o.tw + lx + i];
+      } else {
+        const int X = inside ? ix - 1 + i : clampi(ix - 1 + i, 0, dimx - 1);
+        const int Y = inside ? iy - 1 + j : clampi(iy - 1 + j, 0, dimy - 1);
+        p[j] = (double)im[(int64_t)Y * pitch + X];
+      }
     }
     arr[i] = cubic1(p[0], p[1], p[2], p[3], y01);
     darr[i] = cubic1_d(p[0], p[1], p[2], p[3], y01);
@@ -133,7 +168,7 @@ struct Derivs {
   double JTr[3];
 };
 
-__device__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
+__device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
   o.evals++;
   const int lane = threadIdx.x & 63;
   const double xpix1 = o.xr + X[0], ypix1 = o.yr + X[1];
@@ -149,7 +184,7 @@ __device__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
     const double xs = xpix1 + c1 * x + s1 * y;
     const double ys = ypix1 - s1 * x + c1 * y;
     double v, gx, gy;
-    interp_cubic_grad(o.I1, o.dimx, o.dimy, o.pitch, xs, ys, v, gx, gy, cl);
+    interp_cubic_grad(o, xs, ys, v, gx, gy, cl);
     v -= o.shift;
     const double dd = o.d0[i];
     const double gt = gx * (-s1 * x + c1 * y) + gy * (-c1 * x - s1 * y);
@@ -231,7 +266,8 @@ __device__ double line_eval(Obj& o, const double* p, const double* xi, double a)
   return eval(o, x);
 }
 
-__device__ double line_minimize(Obj& o, const double* p, const double* xi, double f0, double* fmin) {
+__device__ __noinline__ double line_minimize(Obj& o, const double* p, const double* xi, double f0, double* fmin,
+                                double line_tol = PW_LINE_TOL) {
   double a = 0.0, fa = f0, b = 1.0, fb = line_eval(o, p, xi, b);
   if (fb > fa) {
     double t = a; a = b; b = t;
@@ -249,7 +285,7 @@ __device__ double line_minimize(Obj& o, const double* p, const double* xi, doubl
   double d = 0.0, e = 0.0;
   for (int it = 0; it < PW_BRENT_MAX; it++) {
     const double m = 0.5 * (lo + hi);
-    const double tol1 = PW_LINE_TOL * fabs(x) + PW_ZEPS;
+    const double tol1 = line_tol * fabs(x) + PW_ZEPS;
     const double t2 = 2.0 * tol1;
     if (fabs(x - m) <= t2 - 0.5 * (hi - lo)) break;
     bool golden = true;
@@ -341,7 +377,9 @@ __device__ double powell(Obj& o, double* p, const double* dx, double ftol, int m
         fret = fmin;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          u[ibig][j] = u[2][j];
+          // u[ibig] = u[2] with static indices (no scratch for a dynamically indexed array)
+          if (ibig == 0) u[0][j] = u[2][j];
+          if (ibig == 1) u[1][j] = u[2][j];
           u[2][j] = xit[j];
         }
       }
@@ -350,9 +388,10 @@ __device__ double powell(Obj& o, double* p, const double* dx, double ftol, int m
   return fret;
 }
 
-// Levenberg-Marquardt on (u, v, theta) with the Gauss-Newton matrix of the ZNCC
-// residual. Stops when the step is below 1e-7 px (and 1e-9 rad), when the relative
-// decrease of f falls under ftol, or after max_iter accepted/rejected trials.
+// The LM form: the first sweep of Powell's method (line minimisations along the three
+// initial directions, which is where Powell's run picks its basin when the objective has
+// several minima in theta), then Levenberg-Marquardt on (u, v, theta) with the
+// Gauss-Newton matrix of the ZNCC residual until the step is below 1e-8 px and 1e-10 rad.
 __device__ __forceinline__ bool solve3(const double* H, const double* g, double lambda, double* dx) {
   // (H + lambda diag(H)) dx = -g, symmetric 3x3 by Cramer's rule
   double a = H[0] * (1.0 + lambda), b = H[1], c = H[2];
@@ -367,33 +406,42 @@ __device__ __forceinline__ bool solve3(const double* H, const double* g, double 
   return isfinite(dx[0]) && isfinite(dx[1]) && isfinite(dx[2]);
 }
 
-__device__ double lm(Obj& o, double* X, double ftol, int max_iter) {
-  if (max_iter <= 0) max_iter = 50;
+#define LM_SWEEP_TOL 1.0e-4
+#define LM_STEP_PX 1.0e-8
+#define LM_STEP_RAD 1.0e-10
+
+__device__ double lm(Obj& o, double* X, const double* dx, int max_iter) {
+  if (max_iter <= 0) max_iter = 40;
+  double fret = eval(o, X);
+  if (fret != fret) return fret;
+  for (int i = 0; i < 3; i++) {
+    double u[3] = {0.0, 0.0, 0.0};
+    u[i] = dx[i];
+    double fmin;
+    const double a = line_minimize(o, X, u, fret, &fmin, LM_SWEEP_TOL);
+    X[i] += a * u[i];
+    fret = fmin;
+  }
   Derivs d;
   eval_derivs(o, X, d);
   double lambda = 1e-3;
   for (int it = 0; it < max_iter; it++) {
-    // gradient of (1 - NCC) is -dNCC = JTr; the overflow weight, when active, uses the
-    // full gradient
-    const double* g = d.grad;
     double step[3];
-    if (!solve3(d.H, g, lambda, step)) break;
+    if (!solve3(d.H, d.grad, lambda, step)) break;
+    const bool small = fabs(step[0]) < LM_STEP_PX && fabs(step[1]) < LM_STEP_PX && fabs(step[2]) < LM_STEP_RAD;
     double Xn[3] = {X[0] + step[0], X[1] + step[1], X[2] + step[2]};
     Derivs dn;
     eval_derivs(o, Xn, dn);
-    if (dn.f < d.f) {
-      const double df = d.f - dn.f;
+    if (dn.f <= d.f) {
       X[0] = Xn[0];
       X[1] = Xn[1];
       X[2] = Xn[2];
       d = dn;
       lambda = fmax(lambda * 0.1, 1e-12);
-      const bool small = fabs(step[0]) < 1e-7 && fabs(step[1]) < 1e-7 && fabs(step[2]) < 1e-9;
-      if (small || 2.0 * df <= ftol * (fabs(d.f) + fabs(d.f + df)) + PW_TINY) break;
+      if (small) break;
     } else {
       lambda *= 10.0;
-      const bool small = fabs(step[0]) < 1e-7 && fabs(step[1]) < 1e-7 && fabs(step[2]) < 1e-9;
-      if (small || lambda > 1e12) break;
+      if (small || lambda > 1e10) break;
     }
   }
   return d.f;
@@ -407,7 +455,9 @@ __global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1
                                                  const int32_t* __restrict__ p0, const double* __restrict__ d0,
                                                  const double* __restrict__ sd0, const int32_t* __restrict__ list,
                                                  int nlist, ptg_subpix_params prm,
-                                                 unsigned long long* __restrict__ evals) {
+                                                 unsigned long long* __restrict__ evals,
+                                                 const int32_t* __restrict__ prad, int tile_margin,
+                                                 int32_t* __restrict__ evals_g) {
   const int li = blockIdx.x;
   if (li >= nlist) return;
   const int g = list[li];
@@ -417,10 +467,25 @@ __global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1
   Obj o;
   load_obj(o, gr, I1, dimx, dimy, pitch, p0 + 2 * b, d0 + b, n, sd0[g], prm.subpix_displacement_max,
            prm.overflow_stiffness);
+  // the tile: the grain's pattern extent plus tile_margin pixels, clamped to the frame edge
+  extern __shared__ uint32_t tile[];
+  {
+    const int h = prad[g] + tile_margin;
+    o.tx0 = (int)floor(o.xr) - h;
+    o.ty0 = (int)floor(o.yr) - h;
+    o.tw = o.th = 2 * h + 2;
+    for (int t = threadIdx.x; t < o.tw * o.th; t += 64) {
+      const int yy = t / o.tw, xx = t - yy * o.tw;
+      const int X = clampi(o.tx0 + xx, 0, dimx - 1), Y = clampi(o.ty0 + yy, 0, dimy - 1);
+      tile[t] = I1[(int64_t)Y * pitch + X];
+    }
+    o.tile = tile;
+    __syncthreads();
+  }
   double X[3] = {0.0, 0.0, 0.0};
   double fret;
   if (prm.method == PTG_SUBPIX_LM)
-    fret = lm(o, X, prm.subpix_tol, prm.max_iter);
+    fret = lm(o, X, prm.initial_direction, prm.max_iter);
   else
     fret = powell(o, X, prm.initial_direction, prm.subpix_tol, prm.max_iter);
   if ((threadIdx.x & 63) == 0) {
@@ -433,15 +498,27 @@ __global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1
     gr.drot += X[2];
     if (o.clamped) gr.status |= PTG_STATUS_CLAMPED;
     atomicAdd(evals, (unsigned long long)o.evals);
+    evals_g[g] = (int32_t)o.evals;
   }
 }
 
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
   if (c.n_unmasked == 0) return hipSuccess;
   const Image& im = c.img[c.icur];
-  k_subpixel<<<c.n_unmasked, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0,
-                                                 c.d_d0, c.d_sd0, c.d_list[0], c.n_unmasked, p,
-                                                 reinterpret_cast<unsigned long long*>(c.d_evals));
+  // LDS tile: pattern radius bound + margin for the subpixel excursion and the 4x4 stencil
+  const int margin = 12;
+  int rmax = 0;
+  for (int g : c.h_unmasked_ref()) rmax = std::max(rmax, c.h_prad[g]);
+  const int tw = 2 * (rmax + margin) + 2;
+  const size_t lds = (size_t)tw * tw * sizeof(uint32_t);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_subpixel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  k_subpixel<<<c.n_unmasked, 64, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0,
+                                                   c.d_d0, c.d_sd0, c.d_list[0], c.n_unmasked, p,
+                                                   reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, margin,
+                                                   c.d_evals_g);
   return hipGetLastError();
 }
 

@@ -178,7 +178,11 @@ def test_track_pair_cfg1_against_oracle(tracker, cfg1, method):
     load(tracker, cfg1)
     tracker.track_pair(prm)
     got = tracker.get_grains()
-    ref, _, _ = ptref.track_pair(oracle_workload(cfg1), cfg1.params, nthreads=4)
+    # Powell parity: the oracle's Powell at the reference tolerance; LM: the FP64 oracle,
+    # the oracle's Powell continued to a tight tolerance (same basin, converged minimum)
+    oprm = cfg1.params if method == "powell" else pk.track_params(
+        cfg1.params.search_zone, rescue_level=0, subpixel=1, subpix=pk.default_subpix(subpix_tol=1e-13))
+    ref, _, _ = ptref.track_pair(oracle_workload(cfg1), oprm, nthreads=4)
     assert bits_equal(got["NCC"], ref["NCC"]) and bits_equal(got["NCC_rescue"], ref["NCC_rescue"])
     assert np.abs(got["dx"] - ref["dx"]).max() < TOL_PX
     assert np.abs(got["dy"] - ref["dy"]).max() < TOL_PX
@@ -186,6 +190,23 @@ def test_track_pair_cfg1_against_oracle(tracker, cfg1, method):
     assert np.abs(got["NCC_subpix"] - ref["NCC_subpix"]).max() < TOL_NCC
     for col in ("upix", "vpix", "rot_inc"):
         assert np.abs(got[col] - ref[col]).max() < (TOL_RAD if col == "rot_inc" else TOL_PX), col
+
+
+def test_track_pair_cfg2_small_lm_against_fp64_oracle(tracker, cfg2s):
+    say("track_pair cfg2 small, exact search + LM")
+    p = cfg2s.params
+    prm = pk.track_params(p.search_zone, p.search_zone_rescue, p.search_zone_super_rescue, rescue_level=1,
+                          search_method=pk.SEARCH_EXACT, subpix=pk.default_subpix(pk.SUBPIX_LM))
+    load(tracker, cfg2s)
+    tracker.track_pair(prm)
+    got = tracker.get_grains()
+    oprm = pk.track_params(p.search_zone, p.search_zone_rescue, p.search_zone_super_rescue, rescue_level=1,
+                           subpix=pk.default_subpix(subpix_tol=1e-13))
+    ref, _, _ = ptref.track_pair(oracle_workload(cfg2s), oprm, nthreads=8)
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+    assert np.abs(got["dx"] - ref["dx"]).max() < TOL_PX and np.abs(got["dy"] - ref["dy"]).max() < TOL_PX
+    assert np.abs(got["drot"] - ref["drot"]).max() < TOL_RAD
+    assert np.abs(got["NCC_subpix"] - ref["NCC_subpix"]).max() < TOL_NCC
 
 
 def test_track_pair_cfg2_small_against_oracle(tracker, cfg2s):
