@@ -388,7 +388,7 @@ int ptg_create(void** out, int device) {
 static void free_grain_buffers(Ctx* c) {
   void** bufs[] = {(void**)&c->d_grains, (void**)&c->d_pattern, (void**)&c->d_pbegin, (void**)&c->d_p0,
                    (void**)&c->d_d0,     (void**)&c->d_prad,    (void**)&c->d_list[0], (void**)&c->d_list[1],
-                   (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0};
+                   (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0,  (void**)&c->d_list_lpt};
   for (void** b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -509,8 +509,18 @@ static int upload_unmasked(Ctx* c) {
   for (int g = 0; g < c->G; g++)
     if (!c->h_grains[g].masked) um.push_back(g);
   c->n_unmasked = (int32_t)um.size();
-  if (!um.empty())
+  std::vector<int32_t>& lpt = c->h_list_lpt;
+  lpt = um;
+  auto size_of = [c](int32_t g) { return c->h_pbegin[g + 1] - c->h_pbegin[g]; };
+  std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
+  c->max_pattern_unmasked = lpt.empty() ? 0 : (int32_t)size_of(lpt[0]);
+  c->max_prad_unmasked = 0;
+  for (int32_t g : um) c->max_prad_unmasked = std::max(c->max_prad_unmasked, c->h_prad[g]);
+  if (!um.empty()) {
     CK(c, hipMemcpyAsync(c->d_list[0], um.data(), sizeof(int32_t) * um.size(), hipMemcpyHostToDevice, c->stream));
+    CK(c, hipMemcpyAsync(c->d_list_lpt, lpt.data(), sizeof(int32_t) * lpt.size(), hipMemcpyHostToDevice,
+                         c->stream));
+  }
   CK(c, fill_i32(*c, c->d_counts, 1, c->n_unmasked));
   CK(c, hipStreamSynchronize(c->stream));
   return PTG_OK;
@@ -558,6 +568,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
       hipMalloc((void**)&c->d_list[0], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[1], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[2], sizeof(int32_t) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_list_lpt, sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_cs0, sizeof(double) * 2 * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_sd0, sizeof(double) * Gs) != hipSuccess) {
     free_grain_buffers(c);

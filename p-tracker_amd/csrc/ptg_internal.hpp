@@ -82,6 +82,12 @@ struct Context {
   int32_t* d_counts = nullptr;  // 3 counters
   int32_t n_unmasked = 0;
   std::vector<int32_t> h_unmasked;  // host copy of list [0]
+  // list [0] ordered by decreasing pattern size (ties by index): the subpixel stage
+  // schedules its longest grains first
+  int32_t* d_list_lpt = nullptr;
+  std::vector<int32_t> h_list_lpt;
+  int32_t max_pattern_unmasked = 0;
+  int32_t max_prad_unmasked = 0;
   const std::vector<int32_t>& h_unmasked_ref() const { return h_unmasked; }
 
   // search scratch

@@ -1,13 +1,18 @@
 // ptg_subpixel.hip — the subpixel stage of libptgpu (gfx950): the ZNCC objective
-// NCC_to_minimize_xyR (src/p-tracker.cpp:1018-1088) evaluated by one wave per grain,
-// Powell's method with Brent line searches (the derivative-free optimizer of
-// follow_pattern_subpixel_xyR, :1090-1128), and a Levenberg-Marquardt form driven by
-// the analytic derivative of the ZNCC residual through the bicubic sampler.
+// NCC_to_minimize_xyR (src/p-tracker.cpp:1018-1088), Powell's method with Brent line
+// searches (the derivative-free optimizer of follow_pattern_subpixel_xyR, :1090-1128),
+// and a Levenberg-Marquardt form driven by the analytic derivative of the ZNCC residual
+// through the bicubic sampler.
 //
-// One wave owns one grain. Every lane runs the optimizer's scalar control flow on
-// identical values; the objective's pixel sums are split across the 64 lanes and
-// combined with butterfly reductions whose result is bit-identical in every lane, so
-// all branches stay wave-uniform.
+// One workgroup of kSpWaves waves owns one grain. At its start the workgroup stages the
+// grain in LDS: a tile of the current frame around the grain (16-bit pixels) and the
+// pattern as one 32-bit word per pixel (x, y offsets as int8 and the reference image
+// value I0, from which d0 = I0 - mean0 is recomputed bit-exactly). The optimizers are
+// written in reverse-communication form: wave 0 holds their state in LDS, consumes the
+// objective at the last requested point and publishes the next point (or "done"); every
+// wave then sums its share of the pattern pixels for that point. So the objective is
+// evaluated at one place in the code, the workgroup's registers hold only the pixel loop,
+// and the optimizer's scalar control flow runs once per workgroup.
 #include <algorithm>
 
 #include "ptg_device.hpp"
@@ -15,111 +20,76 @@
 
 namespace ptg {
 
+constexpr int kSpWaves = 4;
+constexpr int kSpThreads = 64 * kSpWaves;
+constexpr int kSpSums = 18;             // per-pixel sums of the value-and-derivatives pass
+constexpr int kSpWordsCap = 8192;       // pattern pixels staged in LDS per grain
+constexpr int kSpTileBytesCap = 65536;  // the frame tile is skipped above this size
+
+enum { RQ_VALUE = 0, RQ_DERIVS = 1, RQ_DONE = 2 };
+
+// The workgroup's dynamic LDS. Tile and pattern words are addressed as offsets into it
+// (not through generic pointers) so that their loads are LDS loads.
+extern __shared__ __align__(16) unsigned char sp_lds[];
+
+// The point an evaluation is requested at, as the pixel loop needs it.
+struct Request {
+  double xpix1, ypix1;  // refcoord + dx + X[0], refcoord + dy + X[1]
+  double c1, s1;        // cos, sin of drot + X[2]
+  int32_t kind;
+  int32_t pad;
+};
+
+// Per-grain inputs of the objective, uniform across the workgroup.
 struct Obj {
   const uint16_t* __restrict__ I1;
   int dimx, dimy;
   int64_t pitch;
-  // the grain's neighbourhood of I1 copied into LDS (taps outside it read global memory)
-  const uint32_t* tile;  // one 32-bit word per pixel: aligned LDS reads
+  bool tile;           // LDS tile of I1 present: 16-bit pixels, row stride tw, at kSpOffTile
   int tx0, ty0, tw, th;
-  const int32_t* __restrict__ p0;  // pattern0_rotated pairs of this grain
+  int words_off;        // byte offset of the packed pattern words [0, nw) in LDS
+  int nw;
+  const int32_t* __restrict__ p0;  // pattern0_rotated pairs of this grain (pixels [nw, n))
   const double* __restrict__ d0;   // I0(p0) - mean0 of this grain
   int n;
-  double xr, yr;   // refcoord + dx, refcoord + dy (:1031-1032 before adding X)
+  double xr, yr;  // refcoord + dx, refcoord + dy (:1031-1032 before adding X)
   double drot;
   double C0C0, sd0, shift;
   double dmax, stiff;
-  bool clamped;
-  long long evals;
 };
 
-__device__ __forceinline__ void load_obj(Obj& o, const ptg_grain& g, const uint16_t* I1, int dimx, int dimy,
-                                         int64_t pitch, const int32_t* p0, const double* d0, int n, double sd0,
-                                         double dmax, double stiff) {
-  o.I1 = I1;
-  o.dimx = dimx;
-  o.dimy = dimy;
-  o.pitch = pitch;
-  o.tile = nullptr;
-  o.tx0 = o.ty0 = 0;
-  o.tw = o.th = 0;
-  o.p0 = p0;
-  o.d0 = d0;
-  o.n = n;
-  o.xr = (double)g.refcoord_xpix + (double)g.dx;
-  o.yr = (double)g.refcoord_ypix + (double)g.dy;
-  o.drot = g.drot;
-  o.C0C0 = g.C0C0;
-  o.sd0 = sd0;
-  o.shift = g.mean0;  // sums run on I1 - mean0 to keep the one-pass variance well conditioned
-  o.dmax = dmax;
-  o.stiff = stiff;
-  o.clamped = false;
-  o.evals = 0;
+
+// One LDS pixel: a plain 16-bit load (kept from being merged into wider, unaligned
+// loads of neighbouring taps).
+__device__ __forceinline__ double tile_px(int idx);
+
+// Tile coordinates of the 4x4 stencil at (ix, iy) when it lies inside the tile.
+__device__ __forceinline__ bool in_tile(const Obj& o, int ix, int iy, int& off) {
+  const unsigned lx = (unsigned)ix - 1u - (unsigned)o.tx0, ly = (unsigned)iy - 1u - (unsigned)o.ty0;
+  off = (int)(ly * (unsigned)o.tw + lx);
+  return o.tile && lx < (unsigned)(o.tw - 3) && ly < (unsigned)(o.th - 3);
 }
 
-// image_interpCubic::getValue (src/interpol.cpp:55-86) reading the 4x4 stencil from the
-// LDS tile when it lies inside it (the tile holds the frame's values, edge-clamped as the
+// image_interpCubic::getValue (src/interpol.cpp:55-86) reading the stencil from the LDS
+// tile when it lies inside it (the tile holds the frame's values, edge-clamped as the
 // global path clamps), else from global memory.
 __device__ __forceinline__ double sample(const Obj& o, double x, double y, bool& clamped) {
   const int ix = (int)floor(x), iy = (int)floor(y);
-  const int lx = ix - 1 - o.tx0, ly = iy - 1 - o.ty0;
-  if (o.tile && lx >= 0 && ly >= 0 && lx < o.tw - 3 && ly < o.th - 3) {
+  int off;
+  if (in_tile(o, ix, iy, off)) {
     if (!(ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2)) clamped = true;
     const double y01 = y - (double)iy, x01 = x - (double)ix;
-    const uint32_t* r0 = o.tile + ly * o.tw + lx;
     double arr[4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      arr[i] = cubic1((double)r0[i], (double)r0[o.tw + i], (double)r0[2 * o.tw + i], (double)r0[3 * o.tw + i], y01);
+      arr[i] = cubic1(tile_px(off + i), tile_px(off + o.tw + i), tile_px(off + 2 * o.tw + i),
+                      tile_px(off + 3 * o.tw + i), y01);
     return cubic1(arr[0], arr[1], arr[2], arr[3], x01);
   }
   return interp_cubic(o.I1, o.dimx, o.dimy, o.pitch, x, y, clamped);
 }
 
-__device__ __forceinline__ double weight(const Obj& o, const double* X) {
-  double W = 1.0;
-  const double disp = sqrt(X[0] * X[0] + X[1] * X[1]);
-  if (disp > o.dmax) W = o.stiff * (disp - o.dmax) + 1.0;
-  return W;
-}
-
-// (1 - NCC) * W at X. The reference's two passes (mean, then centred products) are
-// replaced by one pass over shifted sums: C1C1 = Sgg - Sg^2/N, C0C1 = Sdg - Sg*Sd0/N.
-__device__ __noinline__ double eval(Obj& o, const double* X) {
-  o.evals++;
-  const int lane = threadIdx.x & 63;
-  const double xpix1 = o.xr + X[0], ypix1 = o.yr + X[1];
-  const double rot = o.drot + X[2];
-  const double c1 = cos(rot), s1 = sin(rot);
-  double Sg = 0.0, Sgg = 0.0, Sdg = 0.0;
-  bool cl = false;
-  for (int i = lane; i < o.n; i += 64) {
-    const double x = (double)o.p0[2 * i], y = (double)o.p0[2 * i + 1];
-    const double xs = xpix1 + c1 * x + s1 * y;
-    const double ys = ypix1 - s1 * x + c1 * y;
-    const double v = sample(o, xs, ys, cl) - o.shift;
-    Sg += v;
-    Sgg = fma(v, v, Sgg);
-    Sdg = fma(o.d0[i], v, Sdg);
-  }
-  if (cl) o.clamped = true;
-  Sg = wave_sum(Sg);
-  Sgg = wave_sum(Sgg);
-  Sdg = wave_sum(Sdg);
-  const double N = (double)o.n;
-  const double C1C1 = Sgg - Sg * Sg / N;
-  const double C0C1 = Sdg - Sg * o.sd0 / N;
-  const double NCC = C0C1 / sqrt(o.C0C0 * C1C1);
-  return (1.0 - NCC) * weight(o, X);
-}
-
-// Value and gradient at X in one pass. g_i = I1(p_i(X)); its derivatives come from
-// differentiating the Catmull-Rom sampler: d/dx of cubic1(arr, x01) and cubic1 along y
-// of the y-derivatives. Returns f; writes grad[3]; optionally the Gauss-Newton matrix
-// of the ZNCC residual r_i = (g_i - m1)/sqrt(C1C1) - d0_i/sqrt(C0C0), whose squared
-// norm is 2 (1 - NCC): H = (sum h h^T - beta beta^T) / C1C1 with h_i = dg_i - mean(dg)
-// and beta = sum ghat_i h_i.
+// d/dx of cubic1 (the Catmull-Rom segment of src/interpol.cpp:42-53).
 __device__ __forceinline__ double cubic1_d(double p0, double p1, double p2, double p3, double x) {
   const double fp0 = 0.5 * (p2 - p0);
   const double fp1 = 0.5 * (p3 - p1);
@@ -128,29 +98,27 @@ __device__ __forceinline__ double cubic1_d(double p0, double p1, double p2, doub
   return (3.0 * a * x + 2.0 * b) * x + fp0;
 }
 
-__device__ __forceinline__ void interp_cubic_grad(const Obj& o, double x, double y, double& v, double& gx,
-                                                  double& gy, bool& clamped) {
-  const uint16_t* __restrict__ im = o.I1;
-  const int dimx = o.dimx, dimy = o.dimy;
-  const int64_t pitch = o.pitch;
+// The sampler's value and its x and y derivatives.
+__device__ __forceinline__ void sample_grad(const Obj& o, double x, double y, double& v, double& gx, double& gy,
+                                            bool& clamped) {
   const int ix = (int)floor(x), iy = (int)floor(y);
   const double y01 = y - (double)iy, x01 = x - (double)ix;
-  double arr[4], darr[4];
-  const bool inside = ix >= 1 && iy >= 1 && ix < dimx - 2 && iy < dimy - 2;
+  const bool inside = ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2;
   if (!inside) clamped = true;
-  const int lx = ix - 1 - o.tx0, ly = iy - 1 - o.ty0;
-  const bool in_tile = o.tile && lx >= 0 && ly >= 0 && lx < o.tw - 3 && ly < o.th - 3;
+  int off;
+  const bool t = in_tile(o, ix, iy, off);
+  double arr[4], darr[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     double p[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (in_tile) {
-        p[j] = (double)o.tile[(ly + j) * o.tw + lx + i];
+      if (t) {
+        p[j] = tile_px(off + j * o.tw + i);
       } else {
-        const int X = inside ? ix - 1 + i : clampi(ix - 1 + i, 0, dimx - 1);
-        const int Y = inside ? iy - 1 + j : clampi(iy - 1 + j, 0, dimy - 1);
-        p[j] = (double)im[(int64_t)Y * pitch + X];
+        const int X = inside ? ix - 1 + i : clampi(ix - 1 + i, 0, o.dimx - 1);
+        const int Y = inside ? iy - 1 + j : clampi(iy - 1 + j, 0, o.dimy - 1);
+        p[j] = (double)o.I1[(int64_t)Y * o.pitch + X];
       }
     }
     arr[i] = cubic1(p[0], p[1], p[2], p[3], y01);
@@ -161,32 +129,45 @@ __device__ __forceinline__ void interp_cubic_grad(const Obj& o, double x, double
   gy = cubic1(darr[0], darr[1], darr[2], darr[3], x01);
 }
 
-struct Derivs {
-  double f, NCC;
-  double grad[3];
-  double H[6];  // xx, xy, xt, yy, yt, tt of the Gauss-Newton matrix (W = 1)
-  double JTr[3];
-};
+// Calls f(x, y, d0) for this thread's share of the grain's pattern pixels.
+template <class F>
+__device__ __forceinline__ void for_pixels(const Obj& o, F& f) {
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(sp_lds + o.words_off);
+  for (int i = threadIdx.x; i < o.nw; i += kSpThreads) {
+    const uint32_t w = words[i];
+    f((int)(int8_t)(w & 0xffu), (int)(int8_t)((w >> 8) & 0xffu), (double)(w >> 16) - o.shift);
+  }
+  for (int i = o.nw + (int)threadIdx.x; i < o.n; i += kSpThreads) f(o.p0[2 * i], o.p0[2 * i + 1], o.d0[i]);
+}
 
-__device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
-  o.evals++;
-  const int lane = threadIdx.x & 63;
-  const double xpix1 = o.xr + X[0], ypix1 = o.yr + X[1];
-  const double rot = o.drot + X[2];
-  const double c1 = cos(rot), s1 = sin(rot);
-  // sums: g, gg, dg, dG[3], gdG[3], d0 dG[3], dG dG^T[6]
-  double S[18];
+// Per-pixel sums at a request. Value pass: sum g, g^2, d0 g with g = I1(p(X)) - mean0.
+// Derivative pass adds sum dG, g dG, d0 dG and dG dG^T for dG = dg/d(u, v, theta).
+__device__ __forceinline__ void pixel_sums_value(const Obj& o, const Request& rq, double* S, bool& cl) {
+  S[0] = S[1] = S[2] = 0.0;
+  const double xpix1 = rq.xpix1, ypix1 = rq.ypix1, c1 = rq.c1, s1 = rq.s1;
+  auto px = [&](int xi, int yi, double dd) {
+    const double x = (double)xi, y = (double)yi;
+    const double xs = xpix1 + c1 * x + s1 * y;
+    const double ys = ypix1 - s1 * x + c1 * y;
+    const double v = sample(o, xs, ys, cl) - o.shift;
+    S[0] += v;
+    S[1] = fma(v, v, S[1]);
+    S[2] = fma(dd, v, S[2]);
+  };
+  for_pixels(o, px);
+}
+
+__device__ __forceinline__ void pixel_sums_derivs(const Obj& o, const Request& rq, double* S, bool& cl) {
 #pragma unroll
-  for (int k = 0; k < 18; k++) S[k] = 0.0;
-  bool cl = false;
-  for (int i = lane; i < o.n; i += 64) {
-    const double x = (double)o.p0[2 * i], y = (double)o.p0[2 * i + 1];
+  for (int k = 0; k < kSpSums; k++) S[k] = 0.0;
+  const double xpix1 = rq.xpix1, ypix1 = rq.ypix1, c1 = rq.c1, s1 = rq.s1;
+  auto px = [&](int xi, int yi, double dd) {
+    const double x = (double)xi, y = (double)yi;
     const double xs = xpix1 + c1 * x + s1 * y;
     const double ys = ypix1 - s1 * x + c1 * y;
     double v, gx, gy;
-    interp_cubic_grad(o, xs, ys, v, gx, gy, cl);
+    sample_grad(o, xs, ys, v, gx, gy, cl);
     v -= o.shift;
-    const double dd = o.d0[i];
     const double gt = gx * (-s1 * x + c1 * y) + gy * (-c1 * x - s1 * y);
     const double G[3] = {gx, gy, gt};
     S[0] += v;
@@ -204,11 +185,91 @@ __device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
     S[15] = fma(G[1], G[1], S[15]);
     S[16] = fma(G[1], G[2], S[16]);
     S[17] = fma(G[2], G[2], S[17]);
-  }
-  if (cl) o.clamped = true;
+  };
+  for_pixels(o, px);
+}
+
+// Workgroup sums in two halves around a barrier: each wave's butterfly sum goes to
+// red[wave * NS + k]; after the barrier one thread adds the waves' partials in wave order.
+template <int NS>
+__device__ __forceinline__ void wave_partials(double* S, double* red) {
 #pragma unroll
-  for (int k = 0; k < 18; k++) S[k] = wave_sum(S[k]);
-  const double N = (double)o.n;
+  for (int k = 0; k < NS; k++) S[k] = wave_sum(S[k]);
+  if ((threadIdx.x & 63) == 0) {
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NS; k++) red[wv * NS + k] = S[k];
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void reduce_partials(const double* red, double* S) {
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    double t = red[k];
+#pragma unroll
+    for (int w = 1; w < kSpWaves; w++) t += red[w * NS + k];
+    S[k] = t;
+  }
+}
+
+// The grain's constants of the objective.
+struct ObjConst {
+  double N, sd0, C0C0, dmax, stiff;
+  double xr, yr, drot;
+};
+
+__device__ __forceinline__ ObjConst obj_const(const Obj& o) {
+  ObjConst k;
+  k.N = (double)o.n;
+  k.sd0 = o.sd0;
+  k.C0C0 = o.C0C0;
+  k.dmax = o.dmax;
+  k.stiff = o.stiff;
+  k.xr = o.xr;
+  k.yr = o.yr;
+  k.drot = o.drot;
+  return k;
+}
+
+__device__ __forceinline__ double weight(const ObjConst& o, const double* X) {
+  double W = 1.0;
+  const double disp = sqrt(X[0] * X[0] + X[1] * X[1]);
+  if (disp > o.dmax) W = o.stiff * (disp - o.dmax) + 1.0;
+  return W;
+}
+
+__device__ __forceinline__ void make_request(const ObjConst& o, const double* X, int kind, Request* rq) {
+  const double rot = o.drot + X[2];
+  rq->xpix1 = o.xr + X[0];
+  rq->ypix1 = o.yr + X[1];
+  rq->c1 = cos(rot);
+  rq->s1 = sin(rot);
+  rq->kind = kind;
+}
+
+// (1 - NCC) * W at X from the value sums. The reference's two passes (mean, then
+// centred products) are replaced by one pass over shifted sums:
+// C1C1 = Sgg - Sg^2/N, C0C1 = Sdg - Sg*Sd0/N.
+__device__ __forceinline__ double finish_value(const ObjConst& o, const double* S, const double* X) {
+  const double N = o.N;
+  const double C1C1 = S[1] - S[0] * S[0] / N;
+  const double C0C1 = S[2] - S[0] * o.sd0 / N;
+  const double NCC = C0C1 / sqrt(o.C0C0 * C1C1);
+  return (1.0 - NCC) * weight(o, X);
+}
+
+// Value and gradient at X; also the Gauss-Newton matrix of the ZNCC residual
+// r_i = (g_i - m1)/sqrt(C1C1) - d0_i/sqrt(C0C0), whose squared norm is 2 (1 - NCC):
+// H = (sum h h^T - beta beta^T) / C1C1 with h_i = dg_i - mean(dg), beta = sum ghat_i h_i.
+struct Derivs {
+  double f, NCC;
+  double grad[3];
+  double H[6];  // xx, xy, xt, yy, yt, tt of the Gauss-Newton matrix (W = 1)
+};
+
+__device__ __forceinline__ void finish_derivs(const ObjConst& o, const double* S, const double* X, Derivs& d) {
+  const double N = o.N;
   const double m1 = S[0] / N;
   const double C1C1 = S[1] - S[0] * m1;
   const double C0C1 = S[2] - m1 * o.sd0;
@@ -218,8 +279,8 @@ __device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     mG[k] = S[3 + k] / N;
-    beta[k] = (S[6 + k] - m1 * S[3 + k]) / sig1;   // sum ghat_i h_i
-    hf[k] = (S[9 + k] - mG[k] * o.sd0) / sig0;     // sum h_i fhat_i
+    beta[k] = (S[6 + k] - m1 * S[3 + k]) / sig1;  // sum ghat_i h_i
+    hf[k] = (S[9 + k] - mG[k] * o.sd0) / sig0;    // sum h_i fhat_i
   }
   // dNCC/dX = (sum h fhat - beta NCC) / sig1
   double dN[3];
@@ -235,10 +296,7 @@ __device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
   d.NCC = NCC;
   d.f = (1.0 - NCC) * W;
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    d.grad[k] = -dN[k] * W + (1.0 - NCC) * dW[k];
-    d.JTr[k] = -dN[k];
-  }
+  for (int k = 0; k < 3; k++) d.grad[k] = -dN[k] * W + (1.0 - NCC) * dW[k];
   const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
 #pragma unroll
   for (int q = 0; q < 6; q++) {
@@ -249,8 +307,16 @@ __device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
 }
 
 // ---------------------------------------------------------------------------------
-// Powell's method with Brent line searches; the same control flow and constants as
-// the oracle's restatement, with eval() as the objective.
+// The optimizers, in reverse communication. Their state lives in the workgroup's LDS
+// (SpState) and only thread 0 runs them: each step consumes the objective at the point
+// last requested (st.Xq) and writes the next point into st.Xq, or finishes. The steps
+// are out-of-line functions without pointer arguments, so that their code and registers
+// stay apart from the pixel loop's.
+//
+// Brent line minimisation with golden-section bracketing (Brent 1973): line_start
+// requests f(p + xi); line_feed consumes the value at the last requested point and either
+// requests the next one or finishes with (x, fx). The sequence of points is that of a
+// plain loop with the same constants.
 #define PW_GOLD 1.618033988749895
 #define PW_CGOLD 0.3819660112501051
 #define PW_LINE_TOL 3.0e-8
@@ -259,139 +325,259 @@ __device__ __noinline__ void eval_derivs(Obj& o, const double* X, Derivs& d) {
 #define PW_BRACKET_MAX 60
 #define PW_BRENT_MAX 100
 
-__device__ double line_eval(Obj& o, const double* p, const double* xi, double a) {
-  double x[3];
-#pragma unroll
-  for (int j = 0; j < 3; j++) x[j] = p[j] + a * xi[j];
-  return eval(o, x);
-}
+struct Line {
+  double p[3], xi[3];
+  double a, b, c, fa, fb, fc;
+  double lo, hi, x, w, v, fx, fw, fv, d, e, u;
+  double tol;
+  int32_t phase, it;
+};
 
-__device__ __noinline__ double line_minimize(Obj& o, const double* p, const double* xi, double f0, double* fmin,
-                                double line_tol = PW_LINE_TOL) {
-  double a = 0.0, fa = f0, b = 1.0, fb = line_eval(o, p, xi, b);
-  if (fb > fa) {
-    double t = a; a = b; b = t;
-    t = fa; fa = fb; fb = t;
-  }
-  double c = b + PW_GOLD * (b - a), fc = line_eval(o, p, xi, c);
-  for (int it = 0; it < PW_BRACKET_MAX && fc < fb; it++) {
-    a = b; fa = fb;
-    b = c; fb = fc;
-    c = b + PW_GOLD * (b - a);
-    fc = line_eval(o, p, xi, c);
-  }
-  double lo = a < c ? a : c, hi = a < c ? c : a;
-  double x = b, w = b, v = b, fx = fb, fw = fb, fv = fb;
-  double d = 0.0, e = 0.0;
-  for (int it = 0; it < PW_BRENT_MAX; it++) {
-    const double m = 0.5 * (lo + hi);
-    const double tol1 = line_tol * fabs(x) + PW_ZEPS;
-    const double t2 = 2.0 * tol1;
-    if (fabs(x - m) <= t2 - 0.5 * (hi - lo)) break;
-    bool golden = true;
-    if (fabs(e) > tol1) {
-      double r = (x - w) * (fx - fv);
-      double q = (x - v) * (fx - fw);
-      double pp = (x - v) * q - (x - w) * r;
-      q = 2.0 * (q - r);
-      if (q > 0.0) pp = -pp; else q = -q;
-      const double etemp = e;
-      e = d;
-      if (fabs(pp) < fabs(0.5 * q * etemp) && pp > q * (lo - x) && pp < q * (hi - x)) {
-        d = pp / q;
-        const double u = x + d;
-        if (u - lo < t2 || hi - u < t2) d = (x < m) ? tol1 : -tol1;
-        golden = false;
-      }
-    }
-    if (golden) {
-      e = (x < m) ? hi - x : lo - x;
-      d = PW_CGOLD * e;
-    }
-    const double u = x + (fabs(d) >= tol1 ? d : (d > 0.0 ? tol1 : -tol1));
-    const double fu = line_eval(o, p, xi, u);
-    if (fu <= fx) {
-      if (u < x) hi = x; else lo = x;
-      v = w; fv = fw;
-      w = x; fw = fx;
-      x = u; fx = fu;
-    } else {
-      if (u < x) lo = u; else hi = u;
-      if (fu <= fw || w == x) {
-        v = w; fv = fw;
-        w = u; fw = fu;
-      } else if (fu <= fv || v == x || v == w) {
-        v = u; fv = fu;
-      }
-    }
-  }
-  *fmin = fx;
-  return x;
-}
+// Powell's method (Powell 1964) over (u, v, theta): line minimisations along a set of
+// directions, replacing the direction of largest decrease by the average direction when
+// the extrapolation test allows it. The same control flow and constants as the oracle's
+// restatement.
+enum { PS_INIT = 0, PS_LINE, PS_PTT, PS_XLINE };
 
-__device__ double powell(Obj& o, double* p, const double* dx, double ftol, int max_iter) {
-  double u[3][3] = {{dx[0], 0.0, 0.0}, {0.0, dx[1], 0.0}, {0.0, 0.0, dx[2]}};
-  if (max_iter <= 0) max_iter = 200;
-  double fret = eval(o, p);
-  if (fret != fret) return fret;  // undefined ZNCC (flat pattern): nothing to minimise
-  double pt[3] = {p[0], p[1], p[2]};
-  for (int iter = 0; iter < max_iter; iter++) {
-    const double fp = fret;
-    double del = 0.0;
-    int ibig = 0;
-    for (int i = 0; i < 3; i++) {
-      const double fstart = fret;
-      double fmin;
-      const double a = line_minimize(o, p, u[i], fret, &fmin);
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        u[i][j] *= a;
-        p[j] += u[i][j];
-      }
-      fret = fmin;
-      if (fstart - fret > del) {
-        del = fstart - fret;
-        ibig = i;
-      }
-    }
-    if (2.0 * (fp - fret) <= ftol * (fabs(fp) + fabs(fret)) + PW_TINY) break;
-    double ptt[3], xit[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      ptt[j] = 2.0 * p[j] - pt[j];
-      xit[j] = p[j] - pt[j];
-      pt[j] = p[j];
-    }
-    const double fptt = eval(o, ptt);
-    if (fptt < fp) {
-      const double t1 = fp - fret - del, t3 = fp - fptt;
-      const double t = 2.0 * (fp - 2.0 * fret + fptt) * t1 * t1 - del * t3 * t3;
-      if (t < 0.0) {
-        double fmin;
-        const double a = line_minimize(o, p, xit, fret, &fmin);
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          xit[j] *= a;
-          p[j] += xit[j];
-        }
-        fret = fmin;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          // u[ibig] = u[2] with static indices (no scratch for a dynamically indexed array)
-          if (ibig == 0) u[0][j] = u[2][j];
-          if (ibig == 1) u[1][j] = u[2][j];
-          u[2][j] = xit[j];
-        }
-      }
-    }
-  }
-  return fret;
-}
+struct Powell {
+  double u[3][3];
+  double p[3], pt[3], xit[3];
+  double fret, fp, del, fstart, ftol;
+  int32_t ibig, i, iter, max_iter, state, pad;
+};
 
 // The LM form: the first sweep of Powell's method (line minimisations along the three
 // initial directions, which is where Powell's run picks its basin when the objective has
 // several minima in theta), then Levenberg-Marquardt on (u, v, theta) with the
 // Gauss-Newton matrix of the ZNCC residual until the step is below 1e-8 px and 1e-10 rad.
+#define LM_SWEEP_TOL 1.0e-4
+#define LM_STEP_PX 1.0e-8
+#define LM_STEP_RAD 1.0e-10
+
+enum { LS_INIT = 0, LS_SWEEP, LS_D0, LS_DN };
+
+struct LevMar {
+  double X[3], Xn[3], dx[3];
+  double fret, lambda;
+  int32_t i, it, max_iter, state, small, pad;
+  Derivs d;
+};
+
+// The workgroup's shared state, at the start of its LDS.
+struct SpState {
+  Request rq;     // the published request
+  ObjConst k;     // the grain's constants
+  double Xq[3];   // the point of the request
+  int32_t lm, nevals;
+  Line L;         // the line search in progress
+  Derivs dq;      // derivatives at Xq, for derivative requests
+  double sums[kSpSums];  // the workgroup's pixel sums at Xq
+  union {
+    Powell P;
+    LevMar M;
+  } opt;
+};
+
+__device__ __forceinline__ SpState& sp_state() { return *reinterpret_cast<SpState*>(sp_lds); }
+
+__device__ __forceinline__ void line_point(SpState& st, double t) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) st.Xq[j] = st.L.p[j] + t * st.L.xi[j];
+}
+
+__device__ __forceinline__ void line_start(SpState& st, double p0, double p1, double p2, double x0, double x1,
+                                           double x2, double f0, double tol) {
+  Line& L = st.L;
+  L.p[0] = p0; L.p[1] = p1; L.p[2] = p2;
+  L.xi[0] = x0; L.xi[1] = x1; L.xi[2] = x2;
+  L.a = 0.0;
+  L.fa = f0;
+  L.b = 1.0;
+  L.tol = tol;
+  L.phase = 0;
+  line_point(st, L.b);
+}
+
+// Consumes f at the last requested point; returns true when the line search has ended
+// (st.L.x, st.L.fx), else requests the next point.
+__device__ __noinline__ bool line_feed(double f) {
+  SpState& st = sp_state();
+  Line& L = st.L;
+  if (L.phase == 0) {  // f(b = 1)
+    L.fb = f;
+    if (L.fb > L.fa) {
+      double t = L.a; L.a = L.b; L.b = t;
+      t = L.fa; L.fa = L.fb; L.fb = t;
+    }
+    L.c = L.b + PW_GOLD * (L.b - L.a);
+    L.phase = 1;
+    L.it = 0;
+    line_point(st, L.c);
+    return false;
+  }
+  if (L.phase == 1) {  // f(c) of the bracketing walk
+    L.fc = f;
+    if (L.it < PW_BRACKET_MAX && L.fc < L.fb) {
+      L.a = L.b; L.fa = L.fb;
+      L.b = L.c; L.fb = L.fc;
+      L.c = L.b + PW_GOLD * (L.b - L.a);
+      L.it++;
+      line_point(st, L.c);
+      return false;
+    }
+    L.lo = L.a < L.c ? L.a : L.c;
+    L.hi = L.a < L.c ? L.c : L.a;
+    L.x = L.w = L.v = L.b;
+    L.fx = L.fw = L.fv = L.fb;
+    L.d = L.e = 0.0;
+    L.it = 0;
+    L.phase = 2;
+  } else {  // f(u) of a Brent step
+    const double u = L.u, fu = f;
+    if (fu <= L.fx) {
+      if (u < L.x) L.hi = L.x; else L.lo = L.x;
+      L.v = L.w; L.fv = L.fw;
+      L.w = L.x; L.fw = L.fx;
+      L.x = u; L.fx = fu;
+    } else {
+      if (u < L.x) L.lo = u; else L.hi = u;
+      if (fu <= L.fw || L.w == L.x) {
+        L.v = L.w; L.fv = L.fw;
+        L.w = u; L.fw = fu;
+      } else if (fu <= L.fv || L.v == L.x || L.v == L.w) {
+        L.v = u; L.fv = fu;
+      }
+    }
+    L.it++;
+  }
+  // the head of Brent's loop: convergence test, then a parabolic or golden step
+  if (L.it >= PW_BRENT_MAX) return true;
+  const double x = L.x, lo = L.lo, hi = L.hi;
+  const double m = 0.5 * (lo + hi);
+  const double tol1 = L.tol * fabs(x) + PW_ZEPS;
+  const double t2 = 2.0 * tol1;
+  if (fabs(x - m) <= t2 - 0.5 * (hi - lo)) return true;
+  bool golden = true;
+  double d = L.d, e = L.e;
+  if (fabs(e) > tol1) {
+    double r = (x - L.w) * (L.fx - L.fv);
+    double q = (x - L.v) * (L.fx - L.fw);
+    double pp = (x - L.v) * q - (x - L.w) * r;
+    q = 2.0 * (q - r);
+    if (q > 0.0) pp = -pp; else q = -q;
+    const double etemp = e;
+    e = d;
+    if (fabs(pp) < fabs(0.5 * q * etemp) && pp > q * (lo - x) && pp < q * (hi - x)) {
+      d = pp / q;
+      const double u = x + d;
+      if (u - lo < t2 || hi - u < t2) d = (x < m) ? tol1 : -tol1;
+      golden = false;
+    }
+  }
+  if (golden) {
+    e = (x < m) ? hi - x : lo - x;
+    d = PW_CGOLD * e;
+  }
+  L.d = d;
+  L.e = e;
+  L.u = x + (fabs(d) >= tol1 ? d : (d > 0.0 ? tol1 : -tol1));
+  line_point(st, L.u);
+  return false;
+}
+
+__device__ __forceinline__ void pw_begin_line(SpState& st) {
+  Powell& P = st.opt.P;
+  P.fstart = P.fret;
+  line_start(st, P.p[0], P.p[1], P.p[2], P.u[P.i][0], P.u[P.i][1], P.u[P.i][2], P.fret, PW_LINE_TOL);
+  P.state = PS_LINE;
+}
+
+__device__ __forceinline__ void pw_begin_iteration(SpState& st) {
+  Powell& P = st.opt.P;
+  P.fp = P.fret;
+  P.del = 0.0;
+  P.ibig = 0;
+  P.i = 0;
+  pw_begin_line(st);
+}
+
+// Returns false when finished (P.p, P.fret hold the result), else a value request is
+// pending at st.Xq.
+__device__ __noinline__ bool pw_feed(double f) {
+  SpState& st = sp_state();
+  Powell& P = st.opt.P;
+  switch (P.state) {
+    case PS_INIT:
+      P.fret = f;
+      if (f != f) return false;  // undefined ZNCC (flat pattern): nothing to minimise
+#pragma unroll
+      for (int j = 0; j < 3; j++) P.pt[j] = P.p[j];
+      P.iter = 0;
+      pw_begin_iteration(st);
+      return true;
+    case PS_LINE: {
+      if (!line_feed(f)) return true;
+      const double a = st.L.x;
+      const int i = P.i;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        P.u[i][j] *= a;
+        P.p[j] += P.u[i][j];
+      }
+      P.fret = st.L.fx;
+      if (P.fstart - P.fret > P.del) {
+        P.del = P.fstart - P.fret;
+        P.ibig = i;
+      }
+      P.i = i + 1;
+      if (P.i < 3) {
+        pw_begin_line(st);
+        return true;
+      }
+      if (2.0 * (P.fp - P.fret) <= P.ftol * (fabs(P.fp) + fabs(P.fret)) + PW_TINY) return false;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        st.Xq[j] = 2.0 * P.p[j] - P.pt[j];
+        P.xit[j] = P.p[j] - P.pt[j];
+        P.pt[j] = P.p[j];
+      }
+      P.state = PS_PTT;
+      return true;
+    }
+    case PS_PTT: {
+      const double fptt = f;
+      if (fptt < P.fp) {
+        const double t1 = P.fp - P.fret - P.del, t3 = P.fp - fptt;
+        const double t = 2.0 * (P.fp - 2.0 * P.fret + fptt) * t1 * t1 - P.del * t3 * t3;
+        if (t < 0.0) {
+          line_start(st, P.p[0], P.p[1], P.p[2], P.xit[0], P.xit[1], P.xit[2], P.fret, PW_LINE_TOL);
+          P.state = PS_XLINE;
+          return true;
+        }
+      }
+      break;
+    }
+    default: {  // PS_XLINE
+      if (!line_feed(f)) return true;
+      const double a = st.L.x;
+      const int ib = P.ibig;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        P.xit[j] *= a;
+        P.p[j] += P.xit[j];
+        P.u[ib][j] = P.u[2][j];
+        P.u[2][j] = P.xit[j];
+      }
+      P.fret = st.L.fx;
+      break;
+    }
+  }
+  P.iter++;
+  if (P.iter >= P.max_iter) return false;
+  pw_begin_iteration(st);
+  return true;
+}
+
 __device__ __forceinline__ bool solve3(const double* H, const double* g, double lambda, double* dx) {
   // (H + lambda diag(H)) dx = -g, symmetric 3x3 by Cramer's rule
   double a = H[0] * (1.0 + lambda), b = H[1], c = H[2];
@@ -406,60 +592,188 @@ __device__ __forceinline__ bool solve3(const double* H, const double* g, double 
   return isfinite(dx[0]) && isfinite(dx[1]) && isfinite(dx[2]);
 }
 
-#define LM_SWEEP_TOL 1.0e-4
-#define LM_STEP_PX 1.0e-8
-#define LM_STEP_RAD 1.0e-10
-
-__device__ double lm(Obj& o, double* X, const double* dx, int max_iter) {
-  if (max_iter <= 0) max_iter = 40;
-  double fret = eval(o, X);
-  if (fret != fret) return fret;
-  for (int i = 0; i < 3; i++) {
-    double u[3] = {0.0, 0.0, 0.0};
-    u[i] = dx[i];
-    double fmin;
-    const double a = line_minimize(o, X, u, fret, &fmin, LM_SWEEP_TOL);
-    X[i] += a * u[i];
-    fret = fmin;
-  }
-  Derivs d;
-  eval_derivs(o, X, d);
-  double lambda = 1e-3;
-  for (int it = 0; it < max_iter; it++) {
-    double step[3];
-    if (!solve3(d.H, d.grad, lambda, step)) break;
-    const bool small = fabs(step[0]) < LM_STEP_PX && fabs(step[1]) < LM_STEP_PX && fabs(step[2]) < LM_STEP_RAD;
-    double Xn[3] = {X[0] + step[0], X[1] + step[1], X[2] + step[2]};
-    Derivs dn;
-    eval_derivs(o, Xn, dn);
-    if (dn.f <= d.f) {
-      X[0] = Xn[0];
-      X[1] = Xn[1];
-      X[2] = Xn[2];
-      d = dn;
-      lambda = fmax(lambda * 0.1, 1e-12);
-      if (small) break;
-    } else {
-      lambda *= 10.0;
-      if (small || lambda > 1e10) break;
-    }
-  }
-  return d.f;
+// The next Levenberg-Marquardt trial point (a derivative request), or RQ_DONE.
+__device__ __forceinline__ int lm_step(SpState& st) {
+  LevMar& M = st.opt.M;
+  if (M.it >= M.max_iter) return RQ_DONE;
+  double H[6], g[3], step[3];
+#pragma unroll
+  for (int q = 0; q < 6; q++) H[q] = M.d.H[q];
+#pragma unroll
+  for (int j = 0; j < 3; j++) g[j] = M.d.grad[j];
+  if (!solve3(H, g, M.lambda, step)) return RQ_DONE;
+  M.small = fabs(step[0]) < LM_STEP_PX && fabs(step[1]) < LM_STEP_PX && fabs(step[2]) < LM_STEP_RAD;
+#pragma unroll
+  for (int j = 0; j < 3; j++) st.Xq[j] = M.Xn[j] = M.X[j] + step[j];
+  M.state = LS_DN;
+  return RQ_DERIVS;
 }
 
-// follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128, one wave per grain of the
-// unmasked list.
-__global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy, int64_t pitch,
-                                                 ptg_grain* __restrict__ grains,
-                                                 const int64_t* __restrict__ pbegin,
-                                                 const int32_t* __restrict__ p0, const double* __restrict__ d0,
-                                                 const double* __restrict__ sd0, const int32_t* __restrict__ list,
-                                                 int nlist, ptg_subpix_params prm,
-                                                 unsigned long long* __restrict__ evals,
-                                                 const int32_t* __restrict__ prad, int tile_margin,
-                                                 int32_t* __restrict__ evals_g) {
+__device__ __forceinline__ int lm_begin_sweep_line(SpState& st) {
+  LevMar& M = st.opt.M;
+  const int i = M.i;
+  line_start(st, M.X[0], M.X[1], M.X[2], i == 0 ? M.dx[0] : 0.0, i == 1 ? M.dx[1] : 0.0, i == 2 ? M.dx[2] : 0.0,
+             M.fret, LM_SWEEP_TOL);
+  M.state = LS_SWEEP;
+  return RQ_VALUE;
+}
+
+// Consumes the value f (value requests) or st.dq (derivative requests) at the last
+// request; returns the kind of the next request, or RQ_DONE (M.X holds the result, its
+// objective in M.d.f).
+__device__ __noinline__ int lm_feed(double f) {
+  SpState& st = sp_state();
+  LevMar& M = st.opt.M;
+  switch (M.state) {
+    case LS_INIT:
+      M.fret = f;
+      M.d.f = f;
+      if (f != f) return RQ_DONE;
+      M.i = 0;
+      return lm_begin_sweep_line(st);
+    case LS_SWEEP: {
+      if (!line_feed(f)) return RQ_VALUE;
+      const int i = M.i;
+      M.X[i] += st.L.x * M.dx[i];
+      M.fret = st.L.fx;
+      M.i = i + 1;
+      if (M.i < 3) return lm_begin_sweep_line(st);
+#pragma unroll
+      for (int j = 0; j < 3; j++) st.Xq[j] = M.X[j];
+      M.state = LS_D0;
+      return RQ_DERIVS;
+    }
+    case LS_D0:
+      M.d = st.dq;
+      M.lambda = 1e-3;
+      M.it = 0;
+      return lm_step(st);
+    default:  // LS_DN
+      if (st.dq.f <= M.d.f) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) M.X[j] = M.Xn[j];
+        M.d = st.dq;
+        M.lambda = fmax(M.lambda * 0.1, 1e-12);
+        if (M.small) return RQ_DONE;
+      } else {
+        M.lambda *= 10.0;
+        if (M.small || M.lambda > 1e10) return RQ_DONE;
+      }
+      M.it++;
+      return lm_step(st);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Workgroup staging of one grain: the frame tile and the packed pattern words in LDS.
+struct SpLayout {
+  int32_t tile_elems;  // 16-bit elements reserved for the tile (0: no tile)
+  int32_t words_cap;   // 32-bit pattern words reserved
+  int32_t margin;      // tile pixels beyond the pattern's extent
+};
+
+// LDS: [SpState][kSpWaves x kSpSums doubles][tile][words]
+constexpr size_t sp_align(size_t b) { return (b + 15) & ~(size_t)15; }
+constexpr size_t kSpOffRed = sp_align(sizeof(SpState));
+constexpr size_t kSpOffTile = kSpOffRed + sp_align(sizeof(double) * kSpWaves * kSpSums);
+
+// LDS-qualified: a volatile access through a generic pointer would become a flat load.
+typedef __attribute__((address_space(3))) const volatile uint16_t lds_u16;
+
+__device__ __forceinline__ double tile_px(int idx) {
+  return (double)*(lds_u16*)(sp_lds + kSpOffTile + 2 * idx);
+}
+
+__device__ __forceinline__ void stage_grain(Obj& o, const ptg_grain& gr, int prad, const SpLayout& lay) {
+  uint16_t* tile = reinterpret_cast<uint16_t*>(sp_lds + kSpOffTile);
+  const int words_off = (int)(kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems));
+  uint32_t* words = reinterpret_cast<uint32_t*>(sp_lds + words_off);
+  // the tile: the grain's pattern extent plus margin pixels, clamped to the frame edge
+  const int h = prad + lay.margin;
+  const int tw = 2 * h + 2;
+  o.tile = false;
+  o.tx0 = (int)floor(o.xr) - h;
+  o.ty0 = (int)floor(o.yr) - h;
+  o.tw = o.th = tw;
+  if (lay.tile_elems > 0 && tw * tw <= lay.tile_elems && o.xr == o.xr && o.yr == o.yr) {
+    for (int t = threadIdx.x; t < tw * tw; t += kSpThreads) {
+      const int yy = t / tw, xx = t - yy * tw;
+      const int X = clampi(o.tx0 + xx, 0, o.dimx - 1), Y = clampi(o.ty0 + yy, 0, o.dimy - 1);
+      tile[t] = o.I1[(int64_t)Y * o.pitch + X];
+    }
+    o.tile = true;
+  }
+  // the pattern words: x, y as int8 and I0 = d0 + mean0 (recovered exactly: the pixel's
+  // word is used only when (double)I0 - mean0 reproduces its d0 bit for bit)
+  int nw = prad <= 127 ? min(o.n, lay.words_cap) : 0;
+  int bad = 0;
+  for (int i = threadIdx.x; i < nw; i += kSpThreads) {
+    const double dd = o.d0[i];
+    const double iv = rint(dd + o.shift);
+    const uint32_t I0 = (iv >= 0.0 && iv <= 65535.0) ? (uint32_t)iv : 0u;
+    if (!((double)I0 - o.shift == dd)) bad = 1;
+    words[i] = ((uint32_t)o.p0[2 * i] & 0xffu) | (((uint32_t)o.p0[2 * i + 1] & 0xffu) << 8) | (I0 << 16);
+  }
+  if (__syncthreads_or(bad)) nw = 0;
+  o.words_off = words_off;
+  o.nw = nw;
+}
+
+__device__ __forceinline__ void load_obj(Obj& o, const ptg_grain& g, const uint16_t* I1, int dimx, int dimy,
+                                         int64_t pitch, const int32_t* p0, const double* d0, int n, double sd0,
+                                         double dmax, double stiff) {
+  o.I1 = I1;
+  o.dimx = dimx;
+  o.dimy = dimy;
+  o.pitch = pitch;
+  o.p0 = p0;
+  o.d0 = d0;
+  o.n = n;
+  o.xr = (double)g.refcoord_xpix + (double)g.dx;
+  o.yr = (double)g.refcoord_ypix + (double)g.dy;
+  o.drot = g.drot;
+  o.C0C0 = g.C0C0;
+  o.sd0 = sd0;
+  o.shift = g.mean0;  // sums run on I1 - mean0 to keep the one-pass variance well conditioned
+  o.dmax = dmax;
+  o.stiff = stiff;
+}
+
+// One optimizer step, run by thread 0 once the workgroup's sums for the request in
+// st.rq are in st.sums: finishes the objective (or its derivatives) at st.Xq, feeds
+// it to the optimizer and publishes the next request or RQ_DONE.
+__device__ __noinline__ void sp_step() {
+  SpState& st = sp_state();
+  const double X[3] = {st.Xq[0], st.Xq[1], st.Xq[2]};
+  st.nevals++;
+  int next;
+  if (st.rq.kind == RQ_DERIVS) {
+    finish_derivs(st.k, st.sums, X, st.dq);
+    next = lm_feed(st.dq.f);
+  } else {
+    const double f = finish_value(st.k, st.sums, X);
+    next = st.lm ? lm_feed(f) : (pw_feed(f) ? RQ_VALUE : RQ_DONE);
+  }
+  if (next == RQ_DONE)
+    st.rq.kind = RQ_DONE;
+  else
+    make_request(st.k, st.Xq, next, &st.rq);
+}
+
+// follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128: one workgroup per grain of the
+// unmasked list, taken in the order of `list` (largest patterns first).
+__global__ __launch_bounds__(kSpThreads) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy,
+                                                         int64_t pitch, ptg_grain* __restrict__ grains,
+                                                         const int64_t* __restrict__ pbegin,
+                                                         const int32_t* __restrict__ p0, const double* __restrict__ d0,
+                                                         const double* __restrict__ sd0,
+                                                         const int32_t* __restrict__ list, int nlist,
+                                                         ptg_subpix_params prm, unsigned long long* __restrict__ evals,
+                                                         const int32_t* __restrict__ prad, SpLayout lay,
+                                                         int32_t* __restrict__ evals_g) {
   const int li = blockIdx.x;
   if (li >= nlist) return;
+  double* red = reinterpret_cast<double*>(sp_lds + kSpOffRed);
   const int g = list[li];
   ptg_grain& gr = grains[g];
   const int64_t b = pbegin[g];
@@ -467,28 +781,71 @@ __global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1
   Obj o;
   load_obj(o, gr, I1, dimx, dimy, pitch, p0 + 2 * b, d0 + b, n, sd0[g], prm.subpix_displacement_max,
            prm.overflow_stiffness);
-  // the tile: the grain's pattern extent plus tile_margin pixels, clamped to the frame edge
-  extern __shared__ uint32_t tile[];
-  {
-    const int h = prad[g] + tile_margin;
-    o.tx0 = (int)floor(o.xr) - h;
-    o.ty0 = (int)floor(o.yr) - h;
-    o.tw = o.th = 2 * h + 2;
-    for (int t = threadIdx.x; t < o.tw * o.th; t += 64) {
-      const int yy = t / o.tw, xx = t - yy * o.tw;
-      const int X = clampi(o.tx0 + xx, 0, dimx - 1), Y = clampi(o.ty0 + yy, 0, dimy - 1);
-      tile[t] = I1[(int64_t)Y * pitch + X];
+  stage_grain(o, gr, prad[g], lay);
+  SpState& st = sp_state();
+  if (threadIdx.x == 0) {
+    st.k = obj_const(o);
+    st.lm = prm.method == PTG_SUBPIX_LM;
+    st.nevals = 0;
+    st.Xq[0] = st.Xq[1] = st.Xq[2] = 0.0;
+    if (st.lm) {
+      LevMar& M = st.opt.M;
+      M.X[0] = M.X[1] = M.X[2] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) M.dx[j] = prm.initial_direction[j];
+      M.max_iter = prm.max_iter > 0 ? prm.max_iter : 40;
+      M.state = LS_INIT;
+    } else {
+      Powell& P = st.opt.P;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) P.u[i][j] = i == j ? prm.initial_direction[j] : 0.0;
+      P.p[0] = P.p[1] = P.p[2] = 0.0;
+      P.ftol = prm.subpix_tol;
+      P.max_iter = prm.max_iter > 0 ? prm.max_iter : 200;
+      P.state = PS_INIT;
     }
-    o.tile = tile;
-    __syncthreads();
+    make_request(st.k, st.Xq, RQ_VALUE, &st.rq);
   }
-  double X[3] = {0.0, 0.0, 0.0};
-  double fret;
-  if (prm.method == PTG_SUBPIX_LM)
-    fret = lm(o, X, prm.initial_direction, prm.max_iter);
-  else
-    fret = powell(o, X, prm.initial_direction, prm.subpix_tol, prm.max_iter);
-  if ((threadIdx.x & 63) == 0) {
+  bool cl = false;
+  for (;;) {
+    __syncthreads();  // the request is published
+    const Request q = st.rq;
+    if (q.kind == RQ_DONE) break;
+    if (q.kind == RQ_DERIVS) {
+      double S[kSpSums];
+      pixel_sums_derivs(o, q, S, cl);
+      wave_partials<kSpSums>(S, red);
+    } else {
+      double S[3];
+      pixel_sums_value(o, q, S, cl);
+      wave_partials<3>(S, red);
+    }
+    __syncthreads();  // the partials are written
+    if (threadIdx.x < 64) {
+      // wave 0: lane k adds the waves' partials of sum k in wave order, then thread 0
+      // steps the optimizer (LDS operations of one wave complete in order)
+      const int ns = q.kind == RQ_DERIVS ? kSpSums : 3;
+      if ((int)threadIdx.x < ns) {
+        double t = red[threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kSpWaves; w++) t += red[w * ns + threadIdx.x];
+        st.sums[threadIdx.x] = t;
+      }
+      if (threadIdx.x == 0) sp_step();
+    }
+  }
+  const bool any_clamped = __syncthreads_or(cl ? 1 : 0);
+  if (threadIdx.x == 0) {
+    double X[3], fret;
+    if (st.lm) {
+      X[0] = st.opt.M.X[0]; X[1] = st.opt.M.X[1]; X[2] = st.opt.M.X[2];
+      fret = st.opt.M.d.f;
+    } else {
+      X[0] = st.opt.P.p[0]; X[1] = st.opt.P.p[1]; X[2] = st.opt.P.p[2];
+      fret = st.opt.P.fret;
+    }
     gr.NCC_subpix = 1.0 - fret;
     gr.upix += X[0];
     gr.vpix += X[1];
@@ -496,63 +853,98 @@ __global__ __launch_bounds__(64) void k_subpixel(const uint16_t* __restrict__ I1
     gr.dx += X[0];
     gr.dy += X[1];
     gr.drot += X[2];
-    if (o.clamped) gr.status |= PTG_STATUS_CLAMPED;
-    atomicAdd(evals, (unsigned long long)o.evals);
-    evals_g[g] = (int32_t)o.evals;
+    if (any_clamped) gr.status |= PTG_STATUS_CLAMPED;
+    atomicAdd(evals, (unsigned long long)st.nevals);
+    evals_g[g] = st.nevals;
   }
+}
+
+static SpLayout sp_layout(const Context& c, int max_prad, int max_n, double dmax) {
+  SpLayout lay;
+  // the tile covers the pattern's extent, the optimizer's excursion up to the displacement
+  // bound, and the 4x4 stencil; samples outside it read global memory
+  const double dm = dmax == dmax ? std::min(std::max(dmax, 0.0), 16.0) : 16.0;
+  lay.margin = 3 + (int)ceil(dm);
+  const int tw = 2 * (max_prad + lay.margin) + 2;
+  const size_t tile_bytes = (size_t)tw * tw * sizeof(uint16_t);
+  lay.tile_elems = tile_bytes <= (size_t)kSpTileBytesCap ? tw * tw : 0;
+  lay.words_cap = std::min(max_n, kSpWordsCap);
+  (void)c;
+  return lay;
+}
+
+static size_t sp_lds_bytes(const SpLayout& lay) {
+  return kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems) + sizeof(uint32_t) * lay.words_cap;
+}
+
+static hipError_t sp_set_lds(const void* fn, size_t lds) {
+  if (lds > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipSuccess;
 }
 
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
   if (c.n_unmasked == 0) return hipSuccess;
   const Image& im = c.img[c.icur];
-  // LDS tile: pattern radius bound + margin for the subpixel excursion and the 4x4 stencil
-  const int margin = 12;
-  int rmax = 0;
-  for (int g : c.h_unmasked_ref()) rmax = std::max(rmax, c.h_prad[g]);
-  const int tw = 2 * (rmax + margin) + 2;
-  const size_t lds = (size_t)tw * tw * sizeof(uint32_t);
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_subpixel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  k_subpixel<<<c.n_unmasked, 64, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0,
-                                                   c.d_d0, c.d_sd0, c.d_list[0], c.n_unmasked, p,
-                                                   reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, margin,
-                                                   c.d_evals_g);
+  const SpLayout lay = sp_layout(c, c.max_prad_unmasked, c.max_pattern_unmasked, p.subpix_displacement_max);
+  const size_t lds = sp_lds_bytes(lay);
+  hipError_t e = sp_set_lds((const void*)k_subpixel, lds);
+  if (e != hipSuccess) return e;
+  k_subpixel<<<c.n_unmasked, kSpThreads, lds, c.stream>>>(
+      im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0, c.d_d0, c.d_sd0, c.d_list_lpt, c.n_unmasked,
+      p, reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, lay, c.d_evals_g);
   return hipGetLastError();
 }
 
 // NCC_to_minimize_xyR for a batch of (grain, X); with grad != nullptr also the
-// analytic gradient. One wave per evaluation.
-__global__ __launch_bounds__(64) void k_objective(const uint16_t* __restrict__ I1, int dimx, int dimy, int64_t pitch,
-                                                  const ptg_grain* __restrict__ grains,
-                                                  const int64_t* __restrict__ pbegin,
-                                                  const int32_t* __restrict__ p0, const double* __restrict__ d0,
-                                                  const double* __restrict__ sd0, int n_eval,
-                                                  const int32_t* __restrict__ ig, const double* __restrict__ Xs,
-                                                  double dmax, double stiff, double* __restrict__ f,
-                                                  double* __restrict__ grad) {
+// analytic gradient. One workgroup per evaluation, staged as the subpixel stage stages
+// its grains, so both run the same objective code.
+__global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __restrict__ I1, int dimx, int dimy,
+                                                          int64_t pitch, const ptg_grain* __restrict__ grains,
+                                                          const int64_t* __restrict__ pbegin,
+                                                          const int32_t* __restrict__ p0,
+                                                          const double* __restrict__ d0,
+                                                          const double* __restrict__ sd0, int n_eval,
+                                                          const int32_t* __restrict__ ig,
+                                                          const double* __restrict__ Xs, double dmax, double stiff,
+                                                          const int32_t* __restrict__ prad, SpLayout lay,
+                                                          double* __restrict__ f, double* __restrict__ grad) {
   const int e = blockIdx.x;
   if (e >= n_eval) return;
+  double* red = reinterpret_cast<double*>(sp_lds + kSpOffRed);
   const int g = ig[e];
   const ptg_grain& gr = grains[g];
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   Obj o;
   load_obj(o, gr, I1, dimx, dimy, pitch, p0 + 2 * b, d0 + b, n, sd0[g], dmax, stiff);
+  stage_grain(o, gr, prad[g], lay);
   const double X[3] = {Xs[3 * e], Xs[3 * e + 1], Xs[3 * e + 2]};
+  const ObjConst k = obj_const(o);
+  Request q;
+  make_request(k, X, grad ? RQ_DERIVS : RQ_VALUE, &q);
+  bool cl = false;
+  double S[kSpSums];
   if (grad) {
-    Derivs d;
-    eval_derivs(o, X, d);
-    if ((threadIdx.x & 63) == 0) {
+    pixel_sums_derivs(o, q, S, cl);
+    wave_partials<kSpSums>(S, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      reduce_partials<kSpSums>(red, S);
+      Derivs d;
+      finish_derivs(k, S, X, d);
       f[e] = d.f;
       grad[3 * e] = d.grad[0];
       grad[3 * e + 1] = d.grad[1];
       grad[3 * e + 2] = d.grad[2];
     }
   } else {
-    const double v = eval(o, X);
-    if ((threadIdx.x & 63) == 0) f[e] = v;
+    pixel_sums_value(o, q, S, cl);
+    wave_partials<3>(S, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      reduce_partials<3>(red, S);
+      f[e] = finish_value(k, S, X);
+    }
   }
 }
 
@@ -560,8 +952,17 @@ hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, co
                                   double stiff, double* d_f, double* d_grad) {
   if (n == 0) return hipSuccess;
   const Image& im = c.img[c.icur];
-  k_objective<<<n, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0, c.d_d0,
-                                      c.d_sd0, n, d_ig, d_X, dmax, stiff, d_f, d_grad);
+  int max_prad = 0;
+  for (int32_t r : c.h_prad) max_prad = std::max(max_prad, r);
+  int64_t max_n = 0;
+  for (int g = 0; g < c.G; g++) max_n = std::max(max_n, c.h_pbegin[g + 1] - c.h_pbegin[g]);
+  const SpLayout lay = sp_layout(c, max_prad, (int)std::min<int64_t>(max_n, kSpWordsCap), dmax);
+  const size_t lds = sp_lds_bytes(lay);
+  hipError_t e = sp_set_lds((const void*)k_objective, lds);
+  if (e != hipSuccess) return e;
+  k_objective<<<n, kSpThreads, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0,
+                                                c.d_d0, c.d_sd0, n, d_ig, d_X, dmax, stiff, c.d_prad, lay, d_f,
+                                                d_grad);
   return hipGetLastError();
 }
 
