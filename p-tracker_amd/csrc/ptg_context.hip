@@ -118,7 +118,29 @@ struct PinnedBuf {
 
 // The context as seen through the C ABI, with the host-side staging it owns.
 struct Ctx : Context {
-  PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list;
+  PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list, pin_flags;
+  // capacity flags of exact searches copied to pin_flags[stage], read at the next
+  // synchronisation of the stream
+  bool flags_pending[3] = {false, false, false};
+  int check_flags() {
+    for (int st = 0; st < 3; st++) {
+      if (!flags_pending[st]) continue;
+      flags_pending[st] = false;
+      const int32_t f = static_cast<const int32_t*>(pin_flags.p)[st];
+      if (f)
+        return fail(PTG_ERR_LIMIT,
+                    "exact search capacity exceeded in stage %d (flags %d): the stage changed nothing; use "
+                    "PTG_SEARCH_PLAIN",
+                    st, f);
+    }
+    return PTG_OK;
+  }
+  int sync_and_check() {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return check(e, "hipStreamSynchronize");
+    return check_flags();
+  }
+  bool any_flags_pending() const { return flags_pending[0] || flags_pending[1] || flags_pending[2]; }
   enum { SNAP_INVALID = 0, SNAP_VALID = 1, SNAP_QUEUES = 2 } snap = SNAP_INVALID;
   std::vector<double> h_steps;
   // recorded after the last upload from the pinned staging buffers, waited on before
@@ -177,7 +199,7 @@ int sync_snapshot(Ctx* c) {
   CK(c, hipMemcpyAsync(c->h_grains.data(), c->d_grains, sizeof(ptg_grain) * c->G, hipMemcpyDeviceToHost, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   c->snap = Ctx::SNAP_VALID;
-  return PTG_OK;
+  return c->check_flags();
 }
 
 int need_grains(Ctx* c) {
@@ -245,7 +267,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     if (c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
     int32_t n = 0;
     CK(c, hipMemcpyAsync(&n, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    CK(c, hipStreamSynchronize(c->stream));
+    TRY(c->sync_and_check());
     list.resize(n);
     if (n) CK(c, hipMemcpyAsync(list.data(), d_list, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
     CK(c, hipStreamSynchronize(c->stream));
@@ -331,6 +353,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   hipEvent_t ev;
   c->mark_begin(tstage, &ev);
   int nparts = nsteps;
+  const int32_t* flag_dev = nullptr;
   if (method == PTG_SEARCH_PLAIN) {
     const size_t lds = (size_t)(win_cap * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
     if (lds > kMaxLds)
@@ -340,12 +363,18 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     CK(c, launch_search_plain(*c, a, nlist, lds));
     if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
   } else {
-    TRY(run_search_exact(*c, a, nlist, list, c->d_best, &nparts, stage == PTG_STAGE_MAIN));
+    // the unmasked list is the same on every main-stage call until it is rebuilt
+    const uint64_t key = stage == PTG_STAGE_MAIN ? c->unmasked_version : 0;
+    TRY(run_search_exact(*c, a, nlist, list, key, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN));
+    TRY(c->pin_flags.ensure(*c, 4 * sizeof(int32_t)));
+    CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + stage, flag_dev, sizeof(int32_t),
+                         hipMemcpyDeviceToHost, c->stream));
+    c->flags_pending[stage] = true;
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
   int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
   CK(c, launch_search_commit(*c, a, nlist, stage, NCC_min, NCC_min_super, next_list, next_count, c->d_best,
-                             nparts));
+                             nparts, flag_dev));
   c->mark_end(tstage, ev);
   // successful grains changed their dx/drot; queued ones did not, so the snapshot
   // still serves the following rescue stages
@@ -406,7 +435,8 @@ int ptg_destroy(void* p) {
   void* more[] = {c->d_counts, c->d_cs, c->d_steps, c->d_best, c->d_evals, c->d_evals_g, c->d_exact_scratch, c->d_tmp};
   for (void* b : more)
     if (b) (void)hipFree(b);
-  PinnedBuf* pins[] = {&c->pin_cs, &c->pin_steps, &c->pin_cs0, &c->pin_list};
+  free_exact_plans(*c);
+  PinnedBuf* pins[] = {&c->pin_cs, &c->pin_steps, &c->pin_cs0, &c->pin_list, &c->pin_flags};
   for (PinnedBuf* b : pins)
     if (b->p) (void)hipHostFree(b->p);
   for (auto& pe : c->pending) {
@@ -503,26 +533,34 @@ int ptg_set_image_indices(void* p, int im_index_ref, int im_index_current) {
   return PTG_OK;
 }
 
-static int upload_unmasked(Ctx* c) {
-  std::vector<int32_t>& um = c->h_unmasked;
-  um.clear();
+// force: the device lists were reallocated or the patterns changed (ptg_set_grains)
+static int upload_unmasked(Ctx* c, bool force) {
+  std::vector<int32_t> um;
+  um.reserve(c->G);
   for (int g = 0; g < c->G; g++)
     if (!c->h_grains[g].masked) um.push_back(g);
-  c->n_unmasked = (int32_t)um.size();
-  std::vector<int32_t>& lpt = c->h_list_lpt;
-  lpt = um;
-  auto size_of = [c](int32_t g) { return c->h_pbegin[g + 1] - c->h_pbegin[g]; };
-  std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
-  c->max_pattern_unmasked = lpt.empty() ? 0 : (int32_t)size_of(lpt[0]);
-  c->max_prad_unmasked = 0;
-  for (int32_t g : um) c->max_prad_unmasked = std::max(c->max_prad_unmasked, c->h_prad[g]);
-  if (!um.empty()) {
-    CK(c, hipMemcpyAsync(c->d_list[0], um.data(), sizeof(int32_t) * um.size(), hipMemcpyHostToDevice, c->stream));
-    CK(c, hipMemcpyAsync(c->d_list_lpt, lpt.data(), sizeof(int32_t) * lpt.size(), hipMemcpyHostToDevice,
-                         c->stream));
+  // a new list (not only new records) invalidates the plans keyed on it
+  const bool changed = force || um != c->h_unmasked || c->unmasked_version == 0;
+  c->h_unmasked.swap(um);
+  const std::vector<int32_t>& ul = c->h_unmasked;
+  c->n_unmasked = (int32_t)ul.size();
+  if (changed) {
+    c->unmasked_version++;
+    std::vector<int32_t>& lpt = c->h_list_lpt;
+    lpt = ul;
+    auto size_of = [c](int32_t g) { return c->h_pbegin[g + 1] - c->h_pbegin[g]; };
+    std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
+    c->max_pattern_unmasked = lpt.empty() ? 0 : (int32_t)size_of(lpt[0]);
+    c->max_prad_unmasked = 0;
+    for (int32_t g : ul) c->max_prad_unmasked = std::max(c->max_prad_unmasked, c->h_prad[g]);
+    if (!ul.empty()) {
+      CK(c, hipMemcpyAsync(c->d_list[0], ul.data(), sizeof(int32_t) * ul.size(), hipMemcpyHostToDevice, c->stream));
+      CK(c, hipMemcpyAsync(c->d_list_lpt, lpt.data(), sizeof(int32_t) * lpt.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    }
   }
   CK(c, fill_i32(*c, c->d_counts, 1, c->n_unmasked));
-  CK(c, hipStreamSynchronize(c->stream));
+  if (changed) CK(c, hipStreamSynchronize(c->stream));
   return PTG_OK;
 }
 
@@ -580,7 +618,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
   CK(c, hipMemcpyAsync(c->d_pbegin, pattern_begin, sizeof(int64_t) * (Gs + 1), hipMemcpyHostToDevice, c->stream));
   CK(c, hipMemcpyAsync(c->d_prad, c->h_prad.data(), sizeof(int32_t) * Gs, hipMemcpyHostToDevice, c->stream));
   CK(c, fill_i32(*c, c->d_counts, 4, 0));
-  TRY(upload_unmasked(c));
+  TRY(upload_unmasked(c, true));
   c->snap = Ctx::SNAP_VALID;
   c->require_precomputations = true;
   return PTG_OK;
@@ -594,7 +632,7 @@ int ptg_put_grains(void* p, const ptg_grain* grains) {
   CK(c, hipStreamSynchronize(c->stream));
   c->h_grains.assign(grains, grains + c->G);
   CK(c, hipMemcpyAsync(c->d_grains, c->h_grains.data(), sizeof(ptg_grain) * c->G, hipMemcpyHostToDevice, c->stream));
-  TRY(upload_unmasked(c));
+  TRY(upload_unmasked(c, false));
   c->snap = Ctx::SNAP_VALID;
   c->require_precomputations = true;
   return PTG_OK;
@@ -629,7 +667,10 @@ int ptg_precompute(void* p) {
 int ptg_search(void* p, int stage, const ptg_search_zone* zone, double NCC_min, double NCC_min_super, int method) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
-  return do_search(c, stage, zone, NCC_min, NCC_min_super, method);
+  TRY(do_search(c, stage, zone, NCC_min, NCC_min_super, method));
+  // an exact search reports its capacity flag before returning
+  if (c->any_flags_pending()) TRY(c->sync_and_check());
+  return PTG_OK;
 }
 
 int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
@@ -709,6 +750,8 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
                   prm->search_method));
   if (prm->subpixel) TRY(ptg_subpixel(p, &prm->subpix));
   TRY(ptg_end_pair(p));
+  // a stage that could not be served exactly is reported here, after the pair's work
+  if (c->any_flags_pending()) TRY(c->sync_and_check());
   return PTG_OK;
 }
 
@@ -717,7 +760,7 @@ int ptg_get_queue_counts(void* p, int32_t* nres, int32_t* nsup) {
   if (!c) return PTG_ERR_ARG;
   int32_t cnt[4] = {0, 0, 0, 0};
   CK(c, hipMemcpyAsync(cnt, c->d_counts, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-  CK(c, hipStreamSynchronize(c->stream));
+  TRY(c->sync_and_check());
   if (nres) *nres = cnt[1];
   if (nsup) *nsup = cnt[2];
   return PTG_OK;
@@ -726,8 +769,7 @@ int ptg_get_queue_counts(void* p, int32_t* nres, int32_t* nsup) {
 int ptg_sync(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
-  CK(c, hipStreamSynchronize(c->stream));
-  return PTG_OK;
+  return c->sync_and_check();
 }
 
 static int tmp_upload(Ctx* c, size_t bytes) { return c->ensure(&c->d_tmp, &c->tmp_cap, bytes); }

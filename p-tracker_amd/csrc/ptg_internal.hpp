@@ -105,9 +105,12 @@ struct Context {
   size_t evals_g_cap = 0;
   double* d_sd0 = nullptr;  // per grain: sum of d0 (used by the one-pass objective)
 
-  // exact-form scratch
+  // exact-form scratch and the cached host plan of the last keyed list
   void* d_exact_scratch = nullptr;
   size_t exact_scratch_cap = 0;
+  void* exact_plan = nullptr;
+  uint64_t exact_dev_key = 0;       // plan whose tables the scratch holds (0: none)
+  uint64_t unmasked_version = 0;    // bumped whenever the unmasked list is rebuilt
 
   // objective / sampler batches
   void* d_tmp = nullptr;
@@ -135,18 +138,25 @@ hipError_t launch_precompute(Context& c);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
 // Reduces nparts partial argmaxes per list slot (global candidate index k * S + u * V + v,
 // first maximum wins) and applies the stage's update rules.
+// abort_flag (device, may be null): when non-zero the commit changes nothing.
 hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
                                 double NCC_min_super, int32_t* next_list, int32_t* next_count,
-                                const StepBest* parts, int nparts);
+                                const StepBest* parts, int nparts, const int32_t* abort_flag);
 hipError_t launch_end_pair(Context& c);
 hipError_t launch_change_reference(Context& c);
 hipError_t launch_build_unmasked(Context& c);
 
 // exact (MFMA) form: returns an error code of the C ABI, writes message into c.err.
-// Writes nparts partial argmaxes per list slot into out_best.
+// Writes nparts partial argmaxes per list slot into out_best. The capacity flag (template
+// digit or Delta list overflow) is left in *flag_dev on the device; the caller passes it
+// to the commit and checks it at its next synchronisation.
+// plan_key != 0 names a list that is the same on every call with that key (the unmasked
+// list of one grain set): its host tables are then built once and reused.
 // time_kernel: bracket each MFMA kernel launch with the PTG_T_SEARCH_KERNEL timer.
 int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vector<int32_t>& list,
-                     StepBest* out_best, int* nparts_out, bool time_kernel);
+                     uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
+                     bool time_kernel);
+void free_exact_plans(Context& c);
 
 // subpixel
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);

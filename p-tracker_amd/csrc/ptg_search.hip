@@ -298,9 +298,12 @@ hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, si
 // main src/p-tracker.cpp:729-742, rescue :868-880, super-rescue :1005-1015.
 __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double NCC_min_super,
                                 int32_t* __restrict__ next_list, int32_t* __restrict__ next_count,
-                                const StepBest* __restrict__ parts, int nparts) {
+                                const StepBest* __restrict__ parts, int nparts,
+                                const int32_t* __restrict__ abort_flag) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= *a.count) return;
+  // a search that could not be served exactly (capacity flag set) changes nothing
+  if (abort_flag && *abort_flag) return;
   const int g = a.list[li];
   ptg_grain& gr = a.grains[g];
   double best = 0.0;
@@ -352,10 +355,10 @@ __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double 
 
 hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, int stage, double NCC_min,
                                 double NCC_min_super, int32_t* next_list, int32_t* next_count,
-                                const StepBest* parts, int nparts) {
+                                const StepBest* parts, int nparts, const int32_t* abort_flag) {
   if (list_cap == 0) return hipSuccess;
   k_search_commit<<<(list_cap + 127) / 128, 128, 0, c.stream>>>(a, stage, NCC_min, NCC_min_super, next_list,
-                                                                 next_count, parts, nparts);
+                                                                 next_count, parts, nparts, abort_flag);
   return hipGetLastError();
 }
 

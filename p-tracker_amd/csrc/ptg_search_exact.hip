@@ -23,6 +23,10 @@
 // come from row prefix sums, and each step adds the sparse difference M_k - M_mid
 // (a few tens of cells) on the vector units.
 #include <limits.h>
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "ptg_device.hpp"
 #include "ptg_internal.hpp"
@@ -65,6 +69,8 @@ struct ExactArgs {
   int32_t vp;                  // vertical shifts padded to 16
   int32_t ww, wh_pad;          // plain window width/height of the largest slot of the launch
   int32_t cs_max;              // staging row stride of the largest slot of the launch
+  int32_t dl_lds_cap;          // Delta entries staged in LDS per slot (the rest is read from global)
+  int32_t seg_cap;             // K-chunks of the largest slot of the launch (segment table in LDS)
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -286,6 +292,42 @@ __global__ __launch_bounds__(256) void k_exact_mid(ExactArgs e) {
 }
 
 // ---- the MFMA search ---------------------------------------------------------------
+// Sums of I1 and I1^2 over one Delta list (ks = 2 step + sign) at this lane's shift: the
+// entries held in LDS (4 per 16-byte read, 8 in flight), then any beyond the LDS
+// capacity from global memory. 32-bit sums stay exact: a list has at most 255 cells.
+__device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
+                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int ks,
+                                           uint32_t wbase, int base, unsigned int& bsum, unsigned int& csum) {
+  const int n = dcnt[ks], o = dofs[ks];
+  const int nl = max(0, min(n, e.dl_lds_cap - o));
+  const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
+  auto px = [&](uint32_t off) { return (unsigned int)*reinterpret_cast<const uint16_t*>(wb + off); };
+  int q = 0;
+  for (; q + 8 <= nl; q += 8) {
+    const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
+    const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
+    const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
+    const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
+    bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+    csum += ((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7));
+  }
+  for (; q < nl; q += 4) {
+    const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
+    const unsigned int w0 = px(e0.x);
+    const unsigned int w1 = q + 1 < nl ? px(e0.y) : 0u;
+    const unsigned int w2 = q + 2 < nl ? px(e0.z) : 0u;
+    const unsigned int w3 = q + 3 < nl ? px(e0.w) : 0u;
+    bsum += (w0 + w1) + (w2 + w3);
+    csum += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
+  }
+  const int32_t* dg = e.dlist + dl_off + (int64_t)ks * e.dcap;
+  for (int r = nl; r < n; r++) {
+    const unsigned int w = win[base + dg[r]];
+    bsum += w;
+    csum += w * w;
+  }
+}
+
 __device__ __forceinline__ bool better_g(double ncc, int idx, double best, int bidx) {
   return ncc > best || (ncc == best && ncc > 0.0 && idx < bidx);
 }
@@ -321,6 +363,14 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   double* abuf = reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
   int32_t* segl = reinterpret_cast<int32_t*>(tail + (size_t)e.nsu * 8 * (NTP * 16) * 8);
   for (int q = threadIdx.x; q < 4 * kc; q += nthreads) segl[q] = e.seg[sl.seg_off + q];
+  // Delta lists of this slot in LDS: counts, offsets of the lists in a compact table
+  // (each list starting on a 16-byte boundary) and the cells as byte offsets into the
+  // window; entries beyond dl_lds_cap stay in global memory
+  const int nls = 2 * nsteps;
+  int32_t* dcnt = segl + ((4 * e.seg_cap + 3) & ~3);
+  int32_t* dofs = dcnt + ((nls + 3) & ~3);
+  uint32_t* dent = reinterpret_cast<uint32_t*>(dofs + ((nls + 4) & ~3));
+  for (int q = threadIdx.x; q < nls; q += nthreads) dcnt[q] = e.dlen[(int64_t)li * nls + q];
 
   bool clamped = false;
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
@@ -333,6 +383,29 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   }
   if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
   __syncthreads();
+  if (wave == 0) {
+    // exclusive prefix of the list lengths rounded up to 4 entries
+    int run = 0;
+    for (int q0 = 0; q0 < nls; q0 += 64) {
+      const int q = q0 + lane;
+      const int len = q < nls ? ((dcnt[q] + 3) & ~3) : 0;
+      int inc = len;
+#pragma unroll
+      for (int m = 1; m < 64; m <<= 1) {
+        const int o = __shfl_up(inc, m, 64);
+        if (lane >= m) inc += o;
+      }
+      if (q < nls) dofs[q] = run + inc - len;
+      run += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) dofs[nls] = run;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nls * e.dcap; t += nthreads) {
+    const int ks = t / e.dcap, q = t - ks * e.dcap;
+    const int pos = dofs[ks] + q;
+    if (q < dcnt[ks] && pos < e.dl_lds_cap) dent[pos] = 2u * (uint32_t)e.dlist[sl.dl_off + t];
+  }
   const int su = blk * e.nsu + wave;
   const bool active = su < U;
   // stage this shift column: cell c of staged row y is window column su + c
@@ -422,6 +495,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         Cm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
         base = svl * WW + su;
       }
+      const uint32_t wbase = 2u * (uint32_t)base;  // byte offset of this lane's window origin
 #pragma unroll
       for (int a = 0; a < NMT; a++) {
         // lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg + h, digits (1, 0) per step
@@ -441,35 +515,9 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         for (int kl = 0; kl < 8; kl++) {
           const int k = 8 * a + kl;
           if (k >= nsteps) break;
-          const int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * e.dcap;
-          const int npos = e.dlen[2 * ((int64_t)li * nsteps + k)];
-          const int nneg = e.dlen[2 * ((int64_t)li * nsteps + k) + 1];
           unsigned int bp = 0u, bn = 0u, cp = 0u, cn = 0u;
-          int q = 0;
-          for (; q + 4 <= npos; q += 4) {
-            const unsigned int w0 = win[base + dl[q]], w1 = win[base + dl[q + 1]];
-            const unsigned int w2 = win[base + dl[q + 2]], w3 = win[base + dl[q + 3]];
-            bp += (w0 + w1) + (w2 + w3);
-            cp += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
-          }
-          for (; q < npos; q++) {
-            const unsigned int w = win[base + dl[q]];
-            bp += w;
-            cp += w * w;
-          }
-          const int32_t* dn = dl + e.dcap;
-          q = 0;
-          for (; q + 4 <= nneg; q += 4) {
-            const unsigned int w0 = win[base + dn[q]], w1 = win[base + dn[q + 1]];
-            const unsigned int w2 = win[base + dn[q + 2]], w3 = win[base + dn[q + 3]];
-            bn += (w0 + w1) + (w2 + w3);
-            cn += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
-          }
-          for (; q < nneg; q++) {
-            const unsigned int w = win[base + dn[q]];
-            bn += w;
-            cn += w * w;
-          }
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, 2 * k, wbase, base, bp, cp);
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, 2 * k + 1, wbase, base, bn, cn);
           if (lane_ok) {
             // every quantity is an integer below 2^53, so these double operations are exact
             const double A = abuf[kl * (NTP * 16) + lane];
@@ -567,13 +615,101 @@ static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid,
 
 namespace {
 struct Bucket {
-  int rho_max;
+  int rho_max = 0;
+  int max_kc = 0;
   std::vector<int32_t> slots;
+};
+
+// Host tables of one search list: per-slot geometry, segment table, radius buckets and
+// the layout of the device scratch. They depend on the list, the patterns' extents and
+// the search zone's shape only, so a keyed list reuses them across calls.
+struct ExactPlan {
+  uint64_t key = 0;
+  int nlist = 0, U = 0, V = 0, nsteps = 0;
+  int nmt = 0, vp = 0, dcap = 0;
+  std::vector<ExactSlot> slots;
+  std::vector<int32_t> segs;
+  std::vector<Bucket> buckets;
+  std::vector<int32_t> bucket_cat;
+  size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
+  size_t total = 0;
+  void* dev_base = nullptr;  // scratch the tables were last uploaded to
 };
 }  // namespace
 
+void free_exact_plans(Context& c) {
+  delete static_cast<ExactPlan*>(c.exact_plan);
+  c.exact_plan = nullptr;
+  c.exact_dev_key = 0;
+}
+
+static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t>& list, int nlist, int U, int V,
+                       int nsteps) {
+  P.nlist = nlist;
+  P.U = U;
+  P.V = V;
+  P.nsteps = nsteps;
+  P.nmt = (2 * nsteps + 15) / 16;
+  P.vp = (V + 15) / 16 * 16;
+  const int S = U * V;
+  P.slots.assign(nlist, ExactSlot{});
+  P.segs.clear();
+  int64_t tmpl_total = 0, mid_total = 0;
+  int rho_max = 0;
+  for (int li = 0; li < nlist; li++) {
+    const int g = list[li];
+    const int rho = c.h_prad[g];
+    rho_max = std::max(rho_max, rho);
+    ExactSlot& s = P.slots[li];
+    s.rho = rho;
+    s.cw = (2 * rho + 1 + 15) / 16 * 16;
+    s.cs = ((s.cw / 16) % 2 == 1) ? s.cw : s.cw + 16;
+    s.seg_off = (int64_t)P.segs.size();
+    for (int qy = -rho; qy <= rho; qy++) {
+      const int xm = (int)floor(sqrt((double)(rho * rho - qy * qy)));
+      const int c_lo = (rho - xm) / 16 * 16, c_hi = rho + xm;
+      for (int c0 = c_lo; c0 <= c_hi; c0 += 16) P.segs.push_back(((qy + rho) << 16) | c0);
+    }
+    // padding segments (zero template): a whole number of chunk pairs
+    while ((P.segs.size() - s.seg_off) % 8) P.segs.push_back(0x8000);
+    s.kc = (int)((P.segs.size() - s.seg_off) / 4);
+    s.tmpl_off = tmpl_total;
+    tmpl_total += (int64_t)P.nmt * s.kc * 1024;
+    s.mid_off = mid_total;
+    mid_total += 2 * (int64_t)S;
+  }
+  P.dcap = std::min(255, 4 * rho_max + 64);
+  for (int li = 0; li < nlist; li++) P.slots[li].dl_off = (int64_t)li * nsteps * 2 * P.dcap;
+  // buckets by rho so each launch reserves LDS for its own grains
+  const int bounds[] = {13, 21, 29, 37, 45, 1 << 20};
+  P.buckets.assign(sizeof(bounds) / sizeof(bounds[0]), Bucket{});
+  for (int li = 0; li < nlist; li++) {
+    int bk = 0;
+    while (P.slots[li].rho > bounds[bk]) bk++;
+    Bucket& b = P.buckets[bk];
+    b.slots.push_back(li);
+    b.rho_max = std::max(b.rho_max, P.slots[li].rho);
+    b.max_kc = std::max(b.max_kc, P.slots[li].kc);
+  }
+  P.bucket_cat.clear();
+  for (auto& bk : P.buckets) P.bucket_cat.insert(P.bucket_cat.end(), bk.slots.begin(), bk.slots.end());
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  P.b_slots = al(sizeof(ExactSlot) * nlist);
+  P.b_seg = al(sizeof(int32_t) * P.segs.size());
+  P.b_bucket = al(sizeof(int32_t) * nlist);
+  P.b_tmpl = al((size_t)tmpl_total);
+  P.b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * 2 * P.dcap);
+  P.b_dlen = al(sizeof(int32_t) * 2 * (size_t)nlist * nsteps);
+  P.b_mid = al(sizeof(int64_t) * (size_t)mid_total);
+  P.b_mom = al(sizeof(int64_t) * 3 * nlist);
+  P.b_flags = al(sizeof(int32_t) * 4);
+  P.total = P.b_slots + P.b_seg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
+  P.dev_base = nullptr;
+}
+
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
-                     StepBest* out_best, int* nparts_out, bool time_kernel) {
+                     uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
+                     bool time_kernel) {
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
   if (i0.bits > 12 || i1.bits > 12)
     return c.fail(PTG_ERR_LIMIT, "the exact search takes frames of at most 12 bits (got %d/%d); use PTG_SEARCH_PLAIN",
@@ -581,106 +717,86 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   const int nsteps = sa.nsteps;
   const int nmt = (2 * nsteps + 15) / 16;
   if (nmt > 6) return c.fail(PTG_ERR_LIMIT, "the exact search takes at most 48 rotation steps (got %d)", nsteps);
-  const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1, S = U * V;
-  if ((int64_t)nsteps * S >= INT_MAX / 2) return c.fail(PTG_ERR_LIMIT, "too many candidates");
-  const int vp = (V + 15) / 16 * 16;
-  const int ntile = vp / 16;
+  const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1;
+  if ((int64_t)nsteps * U * V >= INT_MAX / 2) return c.fail(PTG_ERR_LIMIT, "too many candidates");
   const int ntp = nmt <= 3 ? 3 : 2;
-  (void)ntile;
-  // per-slot geometry and segment tables
-  std::vector<ExactSlot> slots(nlist);
-  std::vector<int32_t> segs;
-  int64_t tmpl_total = 0, mid_total = 0;
-  int rho_max = 0;
-  for (int li = 0; li < nlist; li++) {
-    const int g = list[li];
-    const int rho = c.h_prad[g];
-    rho_max = std::max(rho_max, rho);
-    ExactSlot& s = slots[li];
-    s.rho = rho;
-    s.cw = (2 * rho + 1 + 15) / 16 * 16;
-    s.cs = ((s.cw / 16) % 2 == 1) ? s.cw : s.cw + 16;
-    s.seg_off = (int64_t)segs.size();
-    for (int qy = -rho; qy <= rho; qy++) {
-      const int xm = (int)floor(sqrt((double)(rho * rho - qy * qy)));
-      const int c_lo = (rho - xm) / 16 * 16, c_hi = rho + xm;
-      for (int c0 = c_lo; c0 <= c_hi; c0 += 16) segs.push_back(((qy + rho) << 16) | c0);
+  // the host plan: cached for a keyed list, else built for this call
+  ExactPlan local;
+  ExactPlan* P = &local;
+  if (plan_key != 0) {
+    ExactPlan* cached = static_cast<ExactPlan*>(c.exact_plan);
+    if (!cached || cached->key != plan_key || cached->nlist != nlist || cached->U != U || cached->V != V ||
+        cached->nsteps != nsteps) {
+      if (!cached) {
+        cached = new ExactPlan();
+        c.exact_plan = cached;
+      }
+      build_plan(c, *cached, list, nlist, U, V, nsteps);
+      cached->key = plan_key;
+      c.exact_dev_key = 0;
     }
-    // padding segments (zero template): a whole number of chunk pairs
-    while ((segs.size() - s.seg_off) % 8) segs.push_back(0x8000);
-    s.kc = (int)((segs.size() - s.seg_off) / 4);
-    s.tmpl_off = tmpl_total;
-    tmpl_total += (int64_t)nmt * s.kc * 1024;
-    s.mid_off = mid_total;
-    mid_total += 2 * (int64_t)S;
+    P = cached;
+  } else {
+    build_plan(c, local, list, nlist, U, V, nsteps);
   }
-  const int dcap = std::min(255, 4 * rho_max + 64);
-  for (int li = 0; li < nlist; li++) slots[li].dl_off = (int64_t)li * nsteps * 2 * dcap;
-  // buckets by rho so each launch reserves LDS for its own grains
-  const int bounds[] = {13, 21, 29, 37, 45, 1 << 20};
-  std::vector<Bucket> buckets(sizeof(bounds) / sizeof(bounds[0]), Bucket{0, {}});
-  for (int li = 0; li < nlist; li++) {
-    int bk = 0;
-    while (slots[li].rho > bounds[bk]) bk++;
-    buckets[bk].slots.push_back(li);
-    buckets[bk].rho_max = std::max(buckets[bk].rho_max, slots[li].rho);
-  }
-  // device buffers in one scratch allocation
-  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
-  const size_t b_slots = al(sizeof(ExactSlot) * nlist), b_seg = al(sizeof(int32_t) * segs.size()),
-               b_bucket = al(sizeof(int32_t) * nlist), b_tmpl = al((size_t)tmpl_total),
-               b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * 2 * dcap),
-               b_dlen = al(sizeof(int32_t) * 2 * (size_t)nlist * nsteps), b_mid = al(sizeof(int64_t) * (size_t)mid_total),
-               b_mom = al(sizeof(int64_t) * 3 * nlist), b_flags = al(sizeof(int32_t) * 4);
-  const size_t total = b_slots + b_seg + b_bucket + b_tmpl + b_dl + b_dlen + b_mid + b_mom + b_flags;
-  int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, total);
+  int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
   if (rc != PTG_OK) return rc;
   char* base = (char*)c.d_exact_scratch;
   ExactArgs e;
   e.s = sa;
   e.slots = (const ExactSlot*)base;
-  base += b_slots;
+  base += P->b_slots;
   e.seg = (const int32_t*)base;
   char* seg_dev = base;
-  base += b_seg;
+  base += P->b_seg;
   int32_t* bucket_dev = (int32_t*)base;
-  base += b_bucket;
+  base += P->b_bucket;
   e.tmpl = (uint8_t*)base;
-  base += b_tmpl;
+  base += P->b_tmpl;
   e.dlist = (int32_t*)base;
-  base += b_dl;
+  base += P->b_dl;
   e.dlen = (int32_t*)base;
-  base += b_dlen;
+  base += P->b_dlen;
   e.mid = (int64_t*)base;
-  base += b_mid;
+  base += P->b_mid;
   e.mom = (int64_t*)base;
-  base += b_mom;
+  base += P->b_mom;
   e.flags = (int32_t*)base;
-  e.dcap = dcap;
+  e.dcap = P->dcap;
   e.nmt = nmt;
-  e.vp = vp;
-  // host staging for the small tables (synchronous copies: they are small)
-  std::vector<int32_t> bucket_cat;
-  for (auto& bk : buckets) bucket_cat.insert(bucket_cat.end(), bk.slots.begin(), bk.slots.end());
+  e.vp = P->vp;
   hipError_t he;
-  if ((he = hipMemcpyAsync((void*)e.slots, slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
-                           c.stream)) != hipSuccess ||
-      (he = hipMemcpyAsync(seg_dev, segs.data(), sizeof(int32_t) * segs.size(), hipMemcpyHostToDevice, c.stream)) !=
-          hipSuccess ||
-      (he = hipMemcpyAsync(bucket_dev, bucket_cat.data(), sizeof(int32_t) * bucket_cat.size(), hipMemcpyHostToDevice,
-                           c.stream)) != hipSuccess ||
-      (k_zero_i32<<<1, 64, 0, c.stream>>>(e.flags, 4), (he = hipGetLastError()) != hipSuccess))
-    return c.check(he, "exact search tables");
-  if ((he = hipStreamSynchronize(c.stream)) != hipSuccess) return c.check(he, "hipStreamSynchronize");
-  // shift blocks: up to 8 columns per block, balanced; results of block b of slot li go
-  // to out_best[li * U + b], the rest stay "none"
+  // the tables: uploaded unless the scratch already holds this keyed plan's
+  const bool fresh = plan_key != 0 && c.exact_dev_key == plan_key && P->dev_base == c.d_exact_scratch;
+  if (!fresh) {
+    if ((he = hipMemcpyAsync((void*)e.slots, P->slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
+                             c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(seg_dev, P->segs.data(), sizeof(int32_t) * P->segs.size(), hipMemcpyHostToDevice,
+                             c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(bucket_dev, P->bucket_cat.data(), sizeof(int32_t) * P->bucket_cat.size(),
+                             hipMemcpyHostToDevice, c.stream)) != hipSuccess)
+      return c.check(he, "exact search tables");
+    if (plan_key != 0) {
+      // the cached host tables outlive the copies
+      P->dev_base = c.d_exact_scratch;
+      c.exact_dev_key = plan_key;
+    } else {
+      // this call's tables are freed on return: let the copies finish first
+      if ((he = hipStreamSynchronize(c.stream)) != hipSuccess) return c.check(he, "hipStreamSynchronize");
+      c.exact_dev_key = 0;
+    }
+  }
+  k_zero_i32<<<1, 64, 0, c.stream>>>(e.flags, 4);
+  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_zero_i32");
+  // shift blocks: results of block b of slot li go to out_best[li * U + b], the rest stay
+  // "none"
   e.out = out_best;
   e.part_cap = U;
   *nparts_out = U;
   k_fill_best<<<(nlist * U + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * U);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
   size_t off = 0;
-  for (auto& bk : buckets) {
+  for (auto& bk : P->buckets) {
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
     e.bucket = bucket_dev + off;
@@ -709,16 +825,19 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     k_exact_mid<<<nb, 256, lds_m, c.stream>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     // the MFMA search
-    const int WW = win_width(U, cw), WH = vp + 2 * rho;
+    const int WW = win_width(U, cw), WH = P->vp + 2 * rho;
     e.ww = WW;
     e.wh_pad = WH;
     const size_t win_bytes = ((size_t)WW * WH * 2 + 15) / 16 * 16;
     const size_t stage_bytes = ((size_t)2 * WH * cs + 15) / 16 * 16;
     const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
-    int max_kc = 0;
-    for (int li : bk.slots) max_kc = std::max(max_kc, slots[li].kc);
-    const size_t seg_bytes = (size_t)4 * max_kc * 4;
-    auto need = [&](int w) { return win_bytes + w * (stage_bytes + abuf_bytes) + seg_bytes; };
+    const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
+    // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
+    // 16-byte read); typical lists total about 1.5 rho entries per rotation step
+    const int nls = 2 * nsteps;
+    const int dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
+    const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + dl_cap + 4) * 4;
+    auto need = [&](int w) { return win_bytes + w * (stage_bytes + abuf_bytes) + seg_bytes + dl_bytes; };
     int nsu_b = std::min(8, U);
     while (nsu_b > 1 && need(nsu_b) > 150 * 1024) nsu_b--;
     if (need(nsu_b) > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
@@ -726,6 +845,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     eb.nsub = (U + nsu_b - 1) / nsu_b;
     eb.nsu = (U + eb.nsub - 1) / eb.nsub;  // balance the columns over the blocks
     eb.cs_max = cs;
+    eb.dl_lds_cap = dl_cap;
+    eb.seg_cap = bk.max_kc;
     const size_t lds = need(eb.nsu);
     const int groups = (nb + 7) / 8;
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
@@ -735,12 +856,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev);
   }
-  // overflow of a template digit or a Delta list: the exact form cannot serve this search
-  int32_t flags = 0;
-  if ((he = hipMemcpyAsync(&flags, e.flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
-      (he = hipStreamSynchronize(c.stream)) != hipSuccess)
-    return c.check(he, "exact search flags");
-  if (flags) return c.fail(PTG_ERR_LIMIT, "exact search capacity exceeded (flags %d): use PTG_SEARCH_PLAIN", flags);
+  // a template digit or Delta list overflow leaves a flag: the commit then changes nothing
+  // and the caller reports PTG_ERR_LIMIT at its next synchronisation
+  *flag_dev = e.flags;
   return PTG_OK;
 }
 
