@@ -45,6 +45,9 @@ struct ExactSlot {
   int64_t tmpl_off;  // first byte of this slot's template fragments
   int64_t dl_off;    // first Delta entry (nsteps * 2 * dcap entries)
   int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
+  int64_t mseg_off;  // first segment of the middle-step GEMM's segment table
+  int32_t mkc;       // its K-chunks (4 segments each)
+  int32_t pad;
 };
 
 struct ExactArgs {
@@ -71,6 +74,10 @@ struct ExactArgs {
   int32_t cs_max;              // staging row stride of the largest slot of the launch
   int32_t dl_lds_cap;          // Delta entries staged in LDS per slot (the rest is read from global)
   int32_t seg_cap;             // K-chunks of the largest slot of the launch (segment table in LDS)
+  const int32_t* mseg;         // middle-step GEMM segments: (qy << 16) | x0, 0x8000 = empty
+  int32_t mid_cbw;             // 16-column blocks per middle-step workgroup
+  int32_t mid_stride;          // digit-plane row stride (bytes) of the launch
+  int32_t mid_mkc_cap;         // largest mkc of the launch
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -180,14 +187,35 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   if (overflow) atomicOr(&e.flags[0], 1);
 }
 
-// ---- B_mid, C_mid for every shift and the reference moments: one workgroup per slot ----
-// LDS: prefix[(V + 2 rho) * (U + 2 rho + 1)] u32, the middle-step count grid, runs.
-constexpr int kMaxRuns = 1536;
+// ---- B_mid, C_mid for every shift and the reference moments -------------------------
+// For a block of 16 shift columns su0 + m (m < 16), B_mid and C_mid of every vertical shift
+// come from one GEMM on v_mfma_i32_16x16x64_i8: row m of A is the middle step's multiset
+// mask moved right by m cells (A_m[qy][x] = M_mid(x - m, qy)), columns are vertical shifts,
+// K runs over (qy, x) in 16-cell segments, and the B operands are the window's digit planes
+// starting at column su0: I1 = 128 h + l and I1^2 = sum_d 128^d q_d (7-bit digits), so
+//   B_mid = 128 acc_h + acc_l,  C_mid = sum_d 128^d acc_qd,
+// every product at most 4 * 127 and every sum exact in int32.
+// One workgroup per (slot, column block); wave w takes N-tiles w, w + 4, ...
+// LDS: 6 digit planes [WHm rows x stride] | mask rows [side x MLR] bytes (16 zero bytes on
+// the left) | segment table.
+constexpr int kMidPlanes = 6;
 
-__global__ __launch_bounds__(256) void k_exact_mid(ExactArgs e) {
+// Row stride of the digit planes for cbw column blocks: an odd multiple of 16 bytes that
+// holds columns su0 .. su0 + 16 cbw + 2 rho + 15.
+__host__ __device__ __forceinline__ int mid_stride(int rho, int cbw) {
+  const int w = (16 * cbw + 2 * rho + 16 + 15) / 16 * 16;
+  return ((w / 16) % 2 == 1) ? w : w + 16;
+}
+__host__ __device__ __forceinline__ int mid_mask_row(int rho) { return (2 * rho + 1 + 50 + 15) / 16 * 16; }
+__host__ __device__ __forceinline__ size_t mid_lds_bytes(int rho, int vp, int cbw, int mkc) {
+  const size_t whm = vp + 2 * rho;
+  return (size_t)kMidPlanes * whm * mid_stride(rho, cbw) + (size_t)(2 * rho + 1) * mid_mask_row(rho) +
+         (size_t)16 * mkc + 64;
+}
+
+__global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int nruns;
-  __shared__ long long red[3][4];
+  __shared__ long long red[2][16];
   const int bi = blockIdx.x;
   if (bi >= e.nbucket) return;
   const int li = e.bucket[bi];
@@ -196,41 +224,74 @@ __global__ __launch_bounds__(256) void k_exact_mid(ExactArgs e) {
   const ExactSlot sl = e.slots[li];
   const int rho = sl.rho, side = 2 * rho + 1;
   const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1;
-  const int PW = U + 2 * rho + 1, PH = V + 2 * rho;
-  uint32_t* P = reinterpret_cast<uint32_t*>(smem);
-  int32_t* cm = reinterpret_cast<int32_t*>(P + (size_t)PW * PH);
-  int32_t* runs = cm + side * side;  // (row << 20 | lo << 10 | hi), weight in a second array
-  int32_t* runw = runs + kMaxRuns;
+  const int vp = e.vp, WHm = vp + 2 * rho;
+  const int stride = e.mid_stride, MLR = mid_mask_row(rho);
+  const int cbw = e.mid_cbw;
+  const int su0 = 16 * cbw * blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  uint8_t* planes = smem;
+  uint8_t* mrow = smem + (size_t)kMidPlanes * WHm * stride;
+  int32_t* segt = reinterpret_cast<int32_t*>(mrow + (size_t)side * MLR);
   const ptg_grain& gr = e.s.grains[g];
   const int64_t b = e.s.pbegin[g];
   const int n = (int)(e.s.pbegin[g + 1] - b);
   const int nsteps = e.s.nsteps, kmid = nsteps / 2;
-  for (int t = threadIdx.x; t < side * side; t += blockDim.x) cm[t] = 0;
-  if (threadIdx.x == 0) nruns = 0;
+  // the middle step's mask (bytes) and, in the first column block, the reference moments
+  for (int t = threadIdx.x; t < side * MLR / 4; t += blockDim.x) reinterpret_cast<uint32_t*>(mrow)[t] = 0u;
   __syncthreads();
   const double c = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], s = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
-  const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
   long long Sa = 0, Saa = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     int x, y;
     rotate_px(e.s.pattern[2 * (b + i)], e.s.pattern[2 * (b + i) + 1], c, s, x, y);
-    atomicAdd(&cm[(y + rho) * side + (x + rho)], 1);
-    const int X = clampi(rx + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
-    const int Y = clampi(ry + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
-    const long long a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
-    Sa += a;
-    Saa += a * a;
+    // byte counters in 32-bit words; a count above 127 (not an int8 operand) raises the
+    // capacity flag, which voids the search before a byte could carry into its neighbour
+    const int off = (y + rho) * MLR + 16 + (x + rho);
+    const unsigned int old = atomicAdd(reinterpret_cast<unsigned int*>(mrow + (off & ~3)), 1u << (8 * (off & 3)));
+    if (((old >> (8 * (off & 3))) & 0xffu) >= 127u) atomicOr(&e.flags[0], 8);
+    if (blockIdx.y == 0) {
+      const int X = clampi(gr.refcoord_xpix + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
+      const int Y = clampi(gr.refcoord_ypix + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
+      const long long a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
+      Sa += a;
+      Saa += a * a;
+    }
   }
-  Sa = wave_sum_i64(Sa);
-  Saa = wave_sum_i64(Saa);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = Sa;
-    red[1][threadIdx.x >> 6] = Saa;
+  if (blockIdx.y == 0) {
+    Sa = wave_sum_i64(Sa);
+    Saa = wave_sum_i64(Saa);
+    if (lane == 0) {
+      red[0][wave] = Sa;
+      red[1][wave] = Saa;
+    }
+  }
+  // this slot's segment table (host built)
+  const int nch = sl.mkc;
+  for (int q = threadIdx.x; q < 4 * nch; q += blockDim.x) segt[q] = e.mseg[sl.mseg_off + q];
+  // the window's digit planes for columns su0 .. su0 + stride - 1
+  const int X0 = gr.refcoord_xpix + (int)round(gr.dx) - e.s.left - rho + su0;
+  const int Y0 = gr.refcoord_ypix + (int)round(gr.dy) - e.s.up - rho;
+  const size_t psz = (size_t)WHm * stride;
+  for (int t = threadIdx.x; t < WHm * (stride / 4); t += blockDim.x) {
+    const int y = t / (stride / 4), x4 = (t - y * (stride / 4)) * 4;
+    const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
+    uint32_t w[kMidPlanes] = {0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
+      const uint32_t q = v * v;
+      const uint32_t dig[kMidPlanes] = {(v >> 7) & 127u, v & 127u, q & 127u, (q >> 7) & 127u, (q >> 14) & 127u,
+                                        (q >> 21) & 127u};
+#pragma unroll
+      for (int p = 0; p < kMidPlanes; p++) w[p] |= dig[p] << (8 * j);
+    }
+#pragma unroll
+    for (int p = 0; p < kMidPlanes; p++) *reinterpret_cast<uint32_t*>(planes + p * psz + (size_t)y * stride + x4) = w[p];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
     long long A = 0, AA = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    for (int w = 0; w < nwave; w++) {
       A += red[0][w];
       AA += red[1][w];
     }
@@ -238,56 +299,54 @@ __global__ __launch_bounds__(256) void k_exact_mid(ExactArgs e) {
     e.mom[3 * li + 1] = A;
     e.mom[3 * li + 2] = AA;
   }
-  // runs of equal non-zero multiplicity along each template row
-  for (int r = threadIdx.x; r < side; r += blockDim.x) {
-    int x = 0;
-    while (x < side) {
-      const int w = cm[r * side + x];
-      if (w == 0) {
-        x++;
-        continue;
+  const int m = lane & 15, gq = lane >> 4;
+  const int ntile = vp >> 4;
+  for (int job = wave; job < ntile * cbw; job += nwave) {
+    const int nt = job % ntile, cbl = job / ntile;  // N-tile, column block within the workgroup
+    if (su0 + 16 * cbl >= U) continue;
+    v4i acc[kMidPlanes];
+#pragma unroll
+    for (int p = 0; p < kMidPlanes; p++) acc[p] = v4i{0, 0, 0, 0};
+    for (int ch = 0; ch < nch; ch++) {
+      const int sv = segt[4 * ch + gq];
+      v4i af = v4i{0, 0, 0, 0};
+      int rowoff = 0;
+      if (!(sv & 0x8000)) {
+        const int qy = sv >> 16, c0 = sv & 0x7fff;
+        // A: bytes M(c0 + j - m, qy) = mask row byte 16 + c0 - m + j: five aligned words,
+        // realigned per lane
+        const int bs = 16 + c0 - m;
+        const uint32_t* wp = reinterpret_cast<const uint32_t*>(mrow + (size_t)qy * MLR + (bs & ~3));
+        const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3], w4 = wp[4];
+        const uint32_t sh = (uint32_t)(bs & 3);
+        af[0] = (int)__builtin_amdgcn_alignbyte(w1, w0, sh);
+        af[1] = (int)__builtin_amdgcn_alignbyte(w2, w1, sh);
+        af[2] = (int)__builtin_amdgcn_alignbyte(w3, w2, sh);
+        af[3] = (int)__builtin_amdgcn_alignbyte(w4, w3, sh);
+        rowoff = qy * stride + 16 * cbl + c0;
       }
-      int h = x;
-      while (h + 1 < side && cm[r * side + h + 1] == w) h++;
-      const int pos = atomicAdd(&nruns, 1);
-      if (pos < kMaxRuns) {
-        runs[pos] = (r << 20) | (x << 10) | h;
-        runw[pos] = w;
-      }
-      x = h + 1;
-    }
-  }
-  __syncthreads();
-  const int nr = min(nruns, kMaxRuns);
-  if (threadIdx.x == 0 && nruns > kMaxRuns) atomicOr(&e.flags[0], 2);
-  const int X0 = gr.refcoord_xpix + (int)round(gr.dx) - e.s.left - rho;
-  const int Y0 = gr.refcoord_ypix + (int)round(gr.dy) - e.s.up - rho;
-  const int S = U * V;
-  for (int pass = 0; pass < 2; pass++) {
-    // row prefix sums of I1 (pass 0) or I1^2 (pass 1); u32 wraps, differences stay exact
-    for (int y = threadIdx.x; y < PH; y += blockDim.x) {
-      const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
-      uint32_t acc = 0u;
-      P[(size_t)y * PW] = 0u;
-      for (int x = 0; x + 1 < PW; x++) {
-        const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x, 0, e.s.dimx - 1)];
-        acc += pass == 0 ? v : v * v;
-        P[(size_t)y * PW + x + 1] = acc;
+      // B: plane p, row (16 nt + col) + qy, bytes c0 .. c0 + 15 (empty segments read row 0)
+      const size_t bo = (size_t)(16 * nt + m) * stride + rowoff;
+#pragma unroll
+      for (int p = 0; p < kMidPlanes; p++) {
+        const v4i bf = *reinterpret_cast<const v4i*>(planes + p * psz + bo);
+        acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[p], 0, 0, 0);
       }
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < S; t += blockDim.x) {
-      const int su = t / V, sv = t - su * V;
-      long long sum = 0;
-      for (int q = 0; q < nr; q++) {
-        const int rv = runs[q];
-        const int row = rv >> 20, lo = (rv >> 10) & 1023, hi = rv & 1023;
-        const uint32_t* pr = P + (size_t)(sv + row) * PW + su;
-        sum += (long long)runw[q] * (long long)(uint32_t)(pr[hi + 1] - pr[lo]);
+    // lane holds rows 4 gq + i (columns su0 + 16 cbl + 4 gq + i) at vertical shift 16 nt + m
+    const int svv = 16 * nt + m;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int su = su0 + 16 * cbl + 4 * gq + i;
+      if (su < U && svv < V) {
+        const long long Bv = (long long)acc[0][i] * 128 + acc[1][i];
+        const long long Cv = (long long)acc[2][i] + ((long long)acc[3][i] << 7) + ((long long)acc[4][i] << 14) +
+                             ((long long)acc[5][i] << 21);
+        const int64_t t = (int64_t)su * V + svv;
+        e.mid[sl.mid_off + 2 * t] = Bv;
+        e.mid[sl.mid_off + 2 * t + 1] = Cv;
       }
-      e.mid[sl.mid_off + 2 * (int64_t)t + pass] = sum;
     }
-    __syncthreads();
   }
 }
 
@@ -617,6 +676,7 @@ namespace {
 struct Bucket {
   int rho_max = 0;
   int max_kc = 0;
+  int max_mkc = 0;
   std::vector<int32_t> slots;
 };
 
@@ -629,8 +689,10 @@ struct ExactPlan {
   int nmt = 0, vp = 0, dcap = 0;
   std::vector<ExactSlot> slots;
   std::vector<int32_t> segs;
+  std::vector<int32_t> msegs;
   std::vector<Bucket> buckets;
   std::vector<int32_t> bucket_cat;
+  size_t b_mseg = 0;
   size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
   size_t total = 0;
   void* dev_base = nullptr;  // scratch the tables were last uploaded to
@@ -677,6 +739,16 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     tmpl_total += (int64_t)P.nmt * s.kc * 1024;
     s.mid_off = mid_total;
     mid_total += 2 * (int64_t)S;
+    // middle-step GEMM: per mask row, 16-aligned segments covering x in
+    // [rho - xm, rho + xm + 15] (the row's cells moved right by up to 15)
+    s.mseg_off = (int64_t)P.msegs.size();
+    for (int qy = -rho; qy <= rho; qy++) {
+      const int xm = (int)floor(sqrt((double)(rho * rho - qy * qy)));
+      const int c_lo = (rho - xm) & ~15, c_hi = rho + xm + 15;
+      for (int c0 = c_lo; c0 <= c_hi; c0 += 16) P.msegs.push_back(((qy + rho) << 16) | c0);
+    }
+    while ((P.msegs.size() - s.mseg_off) % 4) P.msegs.push_back(0x8000);
+    s.mkc = (int)((P.msegs.size() - s.mseg_off) / 4);
   }
   P.dcap = std::min(255, 4 * rho_max + 64);
   for (int li = 0; li < nlist; li++) P.slots[li].dl_off = (int64_t)li * nsteps * 2 * P.dcap;
@@ -690,12 +762,14 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     b.slots.push_back(li);
     b.rho_max = std::max(b.rho_max, P.slots[li].rho);
     b.max_kc = std::max(b.max_kc, P.slots[li].kc);
+    b.max_mkc = std::max(b.max_mkc, P.slots[li].mkc);
   }
   P.bucket_cat.clear();
   for (auto& bk : P.buckets) P.bucket_cat.insert(P.bucket_cat.end(), bk.slots.begin(), bk.slots.end());
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   P.b_slots = al(sizeof(ExactSlot) * nlist);
   P.b_seg = al(sizeof(int32_t) * P.segs.size());
+  P.b_mseg = al(sizeof(int32_t) * P.msegs.size());
   P.b_bucket = al(sizeof(int32_t) * nlist);
   P.b_tmpl = al((size_t)tmpl_total);
   P.b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * 2 * P.dcap);
@@ -703,7 +777,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   P.b_mid = al(sizeof(int64_t) * (size_t)mid_total);
   P.b_mom = al(sizeof(int64_t) * 3 * nlist);
   P.b_flags = al(sizeof(int32_t) * 4);
-  P.total = P.b_slots + P.b_seg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
+  P.total = P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
   P.dev_base = nullptr;
 }
 
@@ -749,6 +823,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   e.seg = (const int32_t*)base;
   char* seg_dev = base;
   base += P->b_seg;
+  e.mseg = (const int32_t*)base;
+  char* mseg_dev = base;
+  base += P->b_mseg;
   int32_t* bucket_dev = (int32_t*)base;
   base += P->b_bucket;
   e.tmpl = (uint8_t*)base;
@@ -772,6 +849,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     if ((he = hipMemcpyAsync((void*)e.slots, P->slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
                              c.stream)) != hipSuccess ||
         (he = hipMemcpyAsync(seg_dev, P->segs.data(), sizeof(int32_t) * P->segs.size(), hipMemcpyHostToDevice,
+                             c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(mseg_dev, P->msegs.data(), sizeof(int32_t) * P->msegs.size(), hipMemcpyHostToDevice,
                              c.stream)) != hipSuccess ||
         (he = hipMemcpyAsync(bucket_dev, P->bucket_cat.data(), sizeof(int32_t) * P->bucket_cat.size(),
                              hipMemcpyHostToDevice, c.stream)) != hipSuccess)
@@ -813,17 +892,27 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return c.check(he, "hipFuncSetAttribute");
     k_exact_template<<<dim3(nb, 8 * nmt), 256, lds_t, c.stream>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
-    // middle-step window moments
-    const size_t PW = U + 2 * rho + 1, PH = V + 2 * rho;
-    const size_t lds_m = PW * PH * 4 + (size_t)side * side * 4 + 2 * kMaxRuns * 4;
+    // middle-step window sums B_mid, C_mid and the reference moments: as many 16-column
+    // blocks per workgroup as keep two workgroups per CU (at least one block)
+    const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
+    int cbw = ncb;
+    while (cbw > 1 && (mid_lds_bytes(rho, P->vp, cbw, bk.max_mkc) > 80 * 1024 || ntile_m * cbw > 16)) cbw--;
+    const size_t lds_m = mid_lds_bytes(rho, P->vp, cbw, bk.max_mkc);
     if (lds_m > 160 * 1024)
       return c.fail(PTG_ERR_LIMIT, "search zone and pattern too large for the exact search (%zu B LDS)", lds_m);
     if (lds_m > 65536 &&
         (he = hipFuncSetAttribute((const void*)k_exact_mid, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_m)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
-    k_exact_mid<<<nb, 256, lds_m, c.stream>>>(e);
-    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
+    {
+      ExactArgs em = e;
+      em.mid_cbw = cbw;
+      em.mid_stride = mid_stride(rho, cbw);
+      em.mid_mkc_cap = bk.max_mkc;
+      const int nwave_m = std::min(16, ntile_m * cbw);
+      k_exact_mid<<<dim3(nb, (ncb + cbw - 1) / cbw), 64 * nwave_m, lds_m, c.stream>>>(em);
+      if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
+    }
     // the MFMA search
     const int WW = win_width(U, cw), WH = P->vp + 2 * rho;
     e.ww = WW;
