@@ -78,6 +78,7 @@ struct ExactArgs {
   int32_t mid_cbw;             // 16-column blocks per middle-step workgroup
   int32_t mid_stride;          // digit-plane row stride (bytes) of the launch
   int32_t mid_mkc_cap;         // largest mkc of the launch
+  int32_t abuf_alias;          // epilogue buffer in the wave's staging area (one vertical pass)
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -419,8 +420,12 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const size_t stage_bytes = ((size_t)2 * e.wh_pad * e.cs_max + 15) / 16 * 16;
   uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * stage_bytes;
   unsigned char* tail = smem + win_bytes + (size_t)e.nsu * stage_bytes;
-  double* abuf = reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
-  int32_t* segl = reinterpret_cast<int32_t*>(tail + (size_t)e.nsu * 8 * (NTP * 16) * 8);
+  // the epilogue's exchange buffer: the wave's own staging area when the column takes one
+  // vertical pass (the staged planes are dead once its MFMAs are issued), else its own area
+  double* abuf = e.abuf_alias ? reinterpret_cast<double*>(stage)
+                              : reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
+  if (!e.abuf_alias) tail += (size_t)e.nsu * 8 * (NTP * 16) * 8;
+  int32_t* segl = reinterpret_cast<int32_t*>(tail);
   for (int q = threadIdx.x; q < 4 * kc; q += nthreads) segl[q] = e.seg[sl.seg_off + q];
   // Delta lists of this slot in LDS: counts, offsets of the lists in a compact table
   // (each list starting on a 16-byte boundary) and the cells as byte offsets into the
@@ -465,10 +470,22 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     const int pos = dofs[ks] + q;
     if (q < dcnt[ks] && pos < e.dl_lds_cap) dent[pos] = 2u * (uint32_t)e.dlist[sl.dl_off + t];
   }
-  const int su = blk * e.nsu + wave;
-  const bool active = su < U;
-  // stage this shift column: cell c of staged row y is window column su + c
-  if (active) {
+  __syncthreads();
+  const int gg = lane >> 4, col = lane & 15;
+  double best = 0.0;
+  int bidx = INT_MAX;
+  const int64_t* mom = e.mom + 3 * li;
+  const long long N = mom[0], Sa = mom[1], Saa = mom[2];
+  const double d0 = (double)(N * Saa - Sa * Sa);
+  const double Nd = (double)N, Sad = (double)Sa;
+  double bnum = 0.0, bd1 = 1.0;  // this lane's best candidate: numerator and N C - B^2
+  const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
+  const int32_t* segs = segl;
+  // this workgroup's shift columns [c_lo, c_hi); wave w takes c_lo + w, c_lo + w + nsu, ...
+  const int cpb = (U + e.nsub - 1) / e.nsub;
+  const int c_lo = blk * cpb, c_hi = min(U, c_lo + cpb);
+  for (int su = c_lo + wave; su < c_hi; su += e.nsu) {
+    // stage this shift column: cell c of staged row y is window column su + c
     const int nq = WH * (cw >> 2);
     for (int q = lane; q < nq; q += 64) {
       const int y = q / (cw >> 2), c4 = (q - y * (cw >> 2)) * 4;
@@ -482,19 +499,11 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
       *reinterpret_cast<unsigned int*>(stage + (size_t)y * cs + c4) = hi;
       *reinterpret_cast<unsigned int*>(stage + (size_t)e.wh_pad * cs + (size_t)y * cs + c4) = lo;
     }
-  }
-  __syncthreads();
-  const int gg = lane >> 4, col = lane & 15;
-  double best = 0.0;
-  int bidx = INT_MAX;
-  const int64_t* mom = e.mom + 3 * li;
-  const long long N = mom[0], Sa = mom[1], Saa = mom[2];
-  const double d0 = (double)(N * Saa - Sa * Sa);
-  const double Nd = (double)N, Sad = (double)Sa;
-  double bnum = 0.0, bd1 = 1.0;  // this lane's best candidate: numerator and N C - B^2
-  const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
-  const int32_t* segs = segl;
-  if (active) {
+    // the stage is this wave's alone: a wave-level barrier orders its writes and reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  {
     const int ntile = vp >> 4;
     for (int p0 = 0; p0 < ntile; p0 += NTP) {
       v4i acc[NMT][NTP][2];
@@ -504,16 +513,18 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         for (int j = 0; j < NTP; j++)
 #pragma unroll
           for (int p = 0; p < 2; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
-      // Software pipeline: the template fragments (global) and window fragments (LDS)
-      // of chunk ch + 1 are requested before chunk ch multiplies; two named buffer sets
-      // and scheduling barriers keep the loads ahead of their use.
-      v4i fa0[NMT], fb0[NTP][2], fa1[NMT], fb1[NTP][2];
-      auto fetch = [&](int chq, v4i* fa, v4i (*fb)[2]) {
+      // Software pipeline: the template fragments (global memory) of chunk ch + 1 are
+      // requested before chunk ch multiplies (two named buffers, scheduling barriers keep
+      // the loads ahead of their use); the window fragments (LDS) are read per chunk.
+      v4i fa0[NMT], fa1[NMT], fb[NTP][2];
+      auto fetch_a = [&](int chq, v4i* fa) {
         const int chc = chq < kc ? chq : kc - 1;
 #pragma unroll
         for (int a = 0; a < NMT; a++)
           fa[a] = *reinterpret_cast<const v4i*>(tmpl + (((int64_t)a * kc + chc) * 64 + lane) * 16);
-        const int segv = segs[4 * chc + gg];
+      };
+      auto fetch_b = [&](int chq) {
+        const int segv = segs[4 * chq + gg];
         const int rowoff = (segv >> 16) * cs + (segv & 0x7fff);  // padding segments read row 0
 #pragma unroll
         for (int j = 0; j < NTP; j++) {
@@ -522,7 +533,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
           fb[j][1] = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
         }
       };
-      auto mult = [&](const v4i* fa, v4i (*fb)[2]) {
+      auto mult = [&](const v4i* fa) {
 #pragma unroll
         for (int j = 0; j < NTP; j++)
 #pragma unroll
@@ -531,15 +542,17 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
             acc[a][j][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][1], acc[a][j][1], 0, 0, 0);
           }
       };
-      fetch(0, fa0, fb0);
+      fetch_a(0, fa0);
       for (int ch = 0; ch < kc; ch += 2) {  // kc is even (host pads with empty chunks)
-        fetch(ch + 1, fa1, fb1);
+        fetch_a(ch + 1, fa1);
+        fetch_b(ch);
         __builtin_amdgcn_sched_barrier(0);
-        mult(fa0, fb0);
+        mult(fa0);
         __builtin_amdgcn_sched_barrier(0);
-        fetch(ch + 2, fa0, fb0);
+        fetch_a(ch + 2, fa0);
+        fetch_b(ch + 1);
         __builtin_amdgcn_sched_barrier(0);
-        mult(fa1, fb1);
+        mult(fa1);
         __builtin_amdgcn_sched_barrier(0);
       }
       // epilogue, one M-tile (8 steps) at a time: the cross terms go through LDS so that
@@ -602,6 +615,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
+    }
   }
   // the lane's best as the reference's quantity, NCC = num / sqrt(d0 (N C - B^2))
   if (bidx != INT_MAX) best = bnum / sqrt(d0 * bd1);
@@ -661,6 +675,23 @@ static hipError_t launch_main(const ExactArgs& e, dim3 grid, dim3 block, size_t 
   }
   k_search_exact<NMT, NTP><<<grid, block, lds, st>>>(e);
   return hipGetLastError();
+}
+
+// Vector registers of a k_search_exact instance (from its code object), for occupancy.
+static int main_vgprs(int nmt, int ntp) {
+  const void* fn = nullptr;
+#define PTG_FN(M, T) \
+  if (nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T>;
+  PTG_FN(1, 3) PTG_FN(2, 3) PTG_FN(3, 3) PTG_FN(4, 2) PTG_FN(5, 2) PTG_FN(6, 2)
+#undef PTG_FN
+  static int cache[8][4] = {};
+  if (!fn) return 256;
+  int& v = cache[nmt][ntp];
+  if (v == 0) {
+    hipFuncAttributes at;
+    v = hipFuncGetAttributes(&at, fn) == hipSuccess && at.numRegs > 0 ? at.numRegs : 256;
+  }
+  return v;
 }
 
 static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
@@ -870,7 +901,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // shift blocks: results of block b of slot li go to out_best[li * U + b], the rest stay
   // "none"
   e.out = out_best;
-  e.part_cap = U;
+  e.part_cap = U;  // at most U shift blocks per slot; unused parts stay "none"
   *nparts_out = U;
   k_fill_best<<<(nlist * U + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * U);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
@@ -920,22 +951,41 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const size_t win_bytes = ((size_t)WW * WH * 2 + 15) / 16 * 16;
     const size_t stage_bytes = ((size_t)2 * WH * cs + 15) / 16 * 16;
     const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
+    const bool alias = P->vp / 16 <= ntp && abuf_bytes <= stage_bytes;
     const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
     // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
     // 16-byte read); typical lists total about 1.5 rho entries per rotation step
     const int nls = 2 * nsteps;
     const int dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
     const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + dl_cap + 4) * 4;
-    auto need = [&](int w) { return win_bytes + w * (stage_bytes + abuf_bytes) + seg_bytes + dl_bytes; };
-    int nsu_b = std::min(8, U);
-    while (nsu_b > 1 && need(nsu_b) > 150 * 1024) nsu_b--;
+    auto need = [&](int w) {
+      return win_bytes + w * (stage_bytes + (alias ? 0 : abuf_bytes)) + seg_bytes + dl_bytes;
+    };
+    // shift columns split over nsub workgroups when the bucket has few grains; waves per
+    // workgroup as many as keep two workgroups per CU (else one)
+    const int nsub = std::max(1, std::min(U, (512 + nb - 1) / nb));
+    const int cpb = (U + nsub - 1) / nsub;
+    // waves per workgroup: the most waves resident per CU under the register limit (a
+    // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
+    const int vgpr = main_vgprs(nmt, ntp);
+    const int wave_cap = 4 * std::max(1, 512 / std::max(vgpr, 1));
+    int nsu_b = 1, best_res = 0;
+    for (int w = 1; w <= std::min(8, cpb); w++) {
+      if (need(w) > 160 * 1024) break;
+      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);
+      if (res >= best_res) {
+        best_res = res;
+        nsu_b = w;
+      }
+    }
     if (need(nsu_b) > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
     ExactArgs eb = e;
-    eb.nsub = (U + nsu_b - 1) / nsu_b;
-    eb.nsu = (U + eb.nsub - 1) / eb.nsub;  // balance the columns over the blocks
+    eb.nsub = nsub;
+    eb.nsu = nsu_b;
     eb.cs_max = cs;
     eb.dl_lds_cap = dl_cap;
     eb.seg_cap = bk.max_kc;
+    eb.abuf_alias = alias ? 1 : 0;
     const size_t lds = need(eb.nsu);
     const int groups = (nb + 7) / 8;
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
