@@ -61,7 +61,7 @@ struct Obj {
 
 // One LDS pixel: a plain 16-bit load (kept from being merged into wider, unaligned
 // loads of neighbouring taps).
-__device__ __forceinline__ double tile_px(int idx);
+__device__ __forceinline__ unsigned int tile_u16(int idx);
 
 // Tile coordinates of the 4x4 stencil at (ix, iy) when it lies inside the tile.
 __device__ __forceinline__ bool in_tile(const Obj& o, int ix, int iy, int& off) {
@@ -70,32 +70,70 @@ __device__ __forceinline__ bool in_tile(const Obj& o, int ix, int iy, int& off) 
   return o.tile && lx < (unsigned)(o.tw - 3) && ly < (unsigned)(o.th - 3);
 }
 
-// image_interpCubic::getValue (src/interpol.cpp:55-86) reading the stencil from the LDS
-// tile when it lies inside it (the tile holds the frame's values, edge-clamped as the
-// global path clamps), else from global memory.
-__device__ __forceinline__ double sample(const Obj& o, double x, double y, bool& clamped) {
-  const int ix = (int)floor(x), iy = (int)floor(y);
-  int off;
-  if (in_tile(o, ix, iy, off)) {
-    if (!(ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2)) clamped = true;
-    const double y01 = y - (double)iy, x01 = x - (double)ix;
-    double arr[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-      arr[i] = cubic1(tile_px(off + i), tile_px(off + o.tw + i), tile_px(off + 2 * o.tw + i),
-                      tile_px(off + 3 * o.tw + i), y01);
-    return cubic1(arr[0], arr[1], arr[2], arr[3], x01);
-  }
-  return interp_cubic(o.I1, o.dimx, o.dimy, o.pitch, x, y, clamped);
+// The Catmull-Rom segment of src/interpol.cpp:42-53 written as weights of its four taps:
+// cubic1(p0, p1, p2, p3, t) = sum_i w_i(t) p_i, with
+//   w0 = t (-1/2 + t (1 - t/2)),  w1 = 1 + t^2 (-5/2 + 3t/2),
+//   w2 = t (1/2 + t (2 - 3t/2)),  w3 = t^2 (-1/2 + t/2),
+// and their derivatives in t. The same interpolant as the reference's, evaluated in another
+// order (the subpixel stage is judged to a tolerance, not bit for bit).
+__device__ __forceinline__ void cr_weights(double t, double* w) {
+  const double t2 = t * t;
+  w[0] = t * fma(t, fma(-0.5, t, 1.0), -0.5);
+  w[1] = fma(t2, fma(1.5, t, -2.5), 1.0);
+  w[2] = t * fma(t, fma(-1.5, t, 2.0), 0.5);
+  w[3] = t2 * fma(0.5, t, -0.5);
 }
 
-// d/dx of cubic1 (the Catmull-Rom segment of src/interpol.cpp:42-53).
-__device__ __forceinline__ double cubic1_d(double p0, double p1, double p2, double p3, double x) {
-  const double fp0 = 0.5 * (p2 - p0);
-  const double fp1 = 0.5 * (p3 - p1);
-  const double b = -3.0 * p1 + 3.0 * p2 - 2.0 * fp0 - fp1;
-  const double a = 2.0 * p1 - 2.0 * p2 + fp0 + fp1;
-  return (3.0 * a * x + 2.0 * b) * x + fp0;
+__device__ __forceinline__ void cr_dweights(double t, double* d) {
+  d[0] = fma(t, fma(-1.5, t, 2.0), -0.5);
+  d[1] = t * fma(4.5, t, -5.0);
+  d[2] = fma(t, fma(-4.5, t, 4.0), 0.5);
+  d[3] = t * fma(1.5, t, -1.0);
+}
+
+// The 4x4 stencil of image_interpCubic::getValue (src/interpol.cpp:55-86) at (ix, iy):
+// p[j][i] = I1(ix - 1 + i, iy - 1 + j), from the LDS tile when the stencil lies inside it
+// (the tile holds the frame's values, edge-clamped as the global path clamps), else from
+// global memory with clamped taps.
+__device__ __forceinline__ void stencil(const Obj& o, int ix, int iy, unsigned int (*p)[4], bool& clamped) {
+  const bool inside = ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2;
+  if (!inside) clamped = true;
+  int off;
+  if (in_tile(o, ix, iy, off)) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) p[j][i] = tile_u16(off + j * o.tw + i);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int Y = clampi(iy - 1 + j, 0, o.dimy - 1);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int X = clampi(ix - 1 + i, 0, o.dimx - 1);
+        p[j][i] = o.I1[(int64_t)Y * o.pitch + X];
+      }
+    }
+  }
+}
+
+// The sampler's value at (x, y).
+__device__ __forceinline__ double sample(const Obj& o, double x, double y, bool& clamped) {
+  const int ix = (int)floor(x), iy = (int)floor(y);
+  const double y01 = y - (double)iy, x01 = x - (double)ix;
+  unsigned int p[4][4];
+  stencil(o, ix, iy, p, clamped);
+  double wx[4], wy[4];
+  cr_weights(x01, wx);
+  cr_weights(y01, wy);
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double r = fma(wx[3], (double)p[j][3],
+                         fma(wx[2], (double)p[j][2], fma(wx[1], (double)p[j][1], wx[0] * (double)p[j][0])));
+    v = fma(wy[j], r, v);
+  }
+  return v;
 }
 
 // The sampler's value and its x and y derivatives.
@@ -103,30 +141,25 @@ __device__ __forceinline__ void sample_grad(const Obj& o, double x, double y, do
                                             bool& clamped) {
   const int ix = (int)floor(x), iy = (int)floor(y);
   const double y01 = y - (double)iy, x01 = x - (double)ix;
-  const bool inside = ix >= 1 && iy >= 1 && ix < o.dimx - 2 && iy < o.dimy - 2;
-  if (!inside) clamped = true;
-  int off;
-  const bool t = in_tile(o, ix, iy, off);
-  double arr[4], darr[4];
+  unsigned int p[4][4];
+  stencil(o, ix, iy, p, clamped);
+  double wx[4], wy[4], dx[4], dy[4];
+  cr_weights(x01, wx);
+  cr_weights(y01, wy);
+  cr_dweights(x01, dx);
+  cr_dweights(y01, dy);
+  v = 0.0;
+  gx = 0.0;
+  gy = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    double p[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (t) {
-        p[j] = tile_px(off + j * o.tw + i);
-      } else {
-        const int X = inside ? ix - 1 + i : clampi(ix - 1 + i, 0, o.dimx - 1);
-        const int Y = inside ? iy - 1 + j : clampi(iy - 1 + j, 0, o.dimy - 1);
-        p[j] = (double)o.I1[(int64_t)Y * o.pitch + X];
-      }
-    }
-    arr[i] = cubic1(p[0], p[1], p[2], p[3], y01);
-    darr[i] = cubic1_d(p[0], p[1], p[2], p[3], y01);
+  for (int j = 0; j < 4; j++) {
+    const double q0 = (double)p[j][0], q1 = (double)p[j][1], q2 = (double)p[j][2], q3 = (double)p[j][3];
+    const double r = fma(wx[3], q3, fma(wx[2], q2, fma(wx[1], q1, wx[0] * q0)));
+    const double sd = fma(dx[3], q3, fma(dx[2], q2, fma(dx[1], q1, dx[0] * q0)));
+    v = fma(wy[j], r, v);
+    gx = fma(wy[j], sd, gx);
+    gy = fma(dy[j], r, gy);
   }
-  v = cubic1(arr[0], arr[1], arr[2], arr[3], x01);
-  gx = cubic1_d(arr[0], arr[1], arr[2], arr[3], x01);
-  gy = cubic1(darr[0], darr[1], darr[2], darr[3], x01);
 }
 
 // Calls f(x, y, d0) for this thread's share of the grain's pattern pixels.
@@ -680,9 +713,7 @@ constexpr size_t kSpOffTile = kSpOffRed + sp_align(sizeof(double) * kSpWaves * k
 // LDS-qualified: a volatile access through a generic pointer would become a flat load.
 typedef __attribute__((address_space(3))) const volatile uint16_t lds_u16;
 
-__device__ __forceinline__ double tile_px(int idx) {
-  return (double)*(lds_u16*)(sp_lds + kSpOffTile + 2 * idx);
-}
+__device__ __forceinline__ unsigned int tile_u16(int idx) { return *(lds_u16*)(sp_lds + kSpOffTile + 2 * idx); }
 
 __device__ __forceinline__ void stage_grain(Obj& o, const ptg_grain& gr, int prad, const SpLayout& lay) {
   uint16_t* tile = reinterpret_cast<uint16_t*>(sp_lds + kSpOffTile);
@@ -762,7 +793,7 @@ __device__ __noinline__ void sp_step() {
 
 // follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128: one workgroup per grain of the
 // unmasked list, taken in the order of `list` (largest patterns first).
-__global__ __launch_bounds__(kSpThreads) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy,
+__global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy,
                                                          int64_t pitch, ptg_grain* __restrict__ grains,
                                                          const int64_t* __restrict__ pbegin,
                                                          const int32_t* __restrict__ p0, const double* __restrict__ d0,
