@@ -335,6 +335,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   a.pbegin = c->d_pbegin;
   a.p0 = c->d_p0;
   a.d0 = c->d_d0;
+  a.pwords = c->d_pwords;
   a.prad = c->d_prad;
   a.list = d_list;
   a.count = d_count;
@@ -416,7 +417,7 @@ int ptg_create(void** out, int device) {
 
 static void free_grain_buffers(Ctx* c) {
   void** bufs[] = {(void**)&c->d_grains, (void**)&c->d_pattern, (void**)&c->d_pbegin, (void**)&c->d_p0,
-                   (void**)&c->d_d0,     (void**)&c->d_prad,    (void**)&c->d_list[0], (void**)&c->d_list[1],
+                   (void**)&c->d_d0,     (void**)&c->d_pwords, (void**)&c->d_prad,    (void**)&c->d_list[0], (void**)&c->d_list[1],
                    (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0,  (void**)&c->d_list_lpt};
   for (void** b : bufs) {
     if (*b) (void)hipFree(*b);
@@ -602,6 +603,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
       hipMalloc((void**)&c->d_pbegin, sizeof(int64_t) * (Gs + 1)) != hipSuccess ||
       hipMalloc((void**)&c->d_p0, sizeof(int32_t) * 2 * total) != hipSuccess ||
       hipMalloc((void**)&c->d_d0, sizeof(double) * total) != hipSuccess ||
+      hipMalloc((void**)&c->d_pwords, sizeof(uint32_t) * total) != hipSuccess ||
       hipMalloc((void**)&c->d_prad, sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[0], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[1], sizeof(int32_t) * Gs) != hipSuccess ||

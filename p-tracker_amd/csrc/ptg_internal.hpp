@@ -41,6 +41,7 @@ struct SearchArgs {
   const int64_t* pbegin;    // G + 1
   const int32_t* p0;        // pattern0_rotated pairs, same indexing as pattern
   const double* d0;         // I0 at p0 minus mean0, same indexing
+  const uint32_t* pwords;   // per pattern pixel: dx, dy as int8 and I0 at p0 (u16), same indexing
   const int32_t* prad;      // per grain: every rotated offset lies in [-prad, prad]^2
   const int32_t* list;      // grains of this stage
   const int32_t* count;     // device count of list entries
@@ -68,6 +69,7 @@ struct Context {
   int64_t* d_pbegin = nullptr;
   int32_t* d_p0 = nullptr;
   double* d_d0 = nullptr;
+  uint32_t* d_pwords = nullptr;  // per pattern pixel: (dx, dy) as int8, I0(p0) as u16
   int32_t* d_prad = nullptr;
   int64_t total_px = 0;
   std::vector<int32_t> h_pattern;

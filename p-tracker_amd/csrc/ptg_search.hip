@@ -83,7 +83,8 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
                                                    const int32_t* __restrict__ pattern,
                                                    const int64_t* __restrict__ pbegin,
                                                    const double* __restrict__ cs0, int32_t* __restrict__ p0,
-                                                   double* __restrict__ d0, double* __restrict__ sd0) {
+                                                   double* __restrict__ d0, double* __restrict__ sd0,
+                                                   uint32_t* __restrict__ pwords) {
   __shared__ double chunk[kPreChunk];
   const int g = blockIdx.x;
   const int lane = threadIdx.x;
@@ -95,15 +96,20 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
   long long s = 0;
   bool clamped = false;
   for (int i = lane; i < n; i += 64) {
+    const int px = pattern[2 * (b + i)], py = pattern[2 * (b + i) + 1];
     int x, y;
-    rotate_px(pattern[2 * (b + i)], pattern[2 * (b + i) + 1], c0, s0, x, y);
+    rotate_px(px, py, c0, s0, x, y);
     p0[2 * (b + i)] = x;
     p0[2 * (b + i) + 1] = y;
     int X = rx + x, Y = ry + y;
     if (X < 0 || Y < 0 || X >= dimx || Y >= dimy) clamped = true;
     X = clampi(X, 0, dimx - 1);
     Y = clampi(Y, 0, dimy - 1);
-    s += I0[(int64_t)Y * pitch + X];
+    const uint32_t v = I0[(int64_t)Y * pitch + X];
+    s += v;
+    // the exact search's compact copy: offsets as int8 (its patterns are well inside
+    // +-127) and the reference value
+    pwords[b + i] = ((uint32_t)px & 0xffu) | (((uint32_t)py & 0xffu) << 8) | (v << 16);
   }
   s = wave_sum_i64(s);
   const double mean0 = (double)s / (double)n;
@@ -140,7 +146,7 @@ hipError_t launch_precompute(Context& c) {
   if (c.G == 0) return hipSuccess;
   const Image& im = c.img[c.iref];
   k_precompute<<<c.G, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern, c.d_pbegin,
-                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0);
+                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords);
   return hipGetLastError();
 }
 

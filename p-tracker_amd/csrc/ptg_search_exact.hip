@@ -46,6 +46,7 @@ struct ExactSlot {
   int64_t dl_off;    // first Delta entry (nsteps * 2 * dcap entries)
   int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
   int64_t mseg_off;  // first segment of the middle-step GEMM's segment table
+  int64_t mmask_off; // first byte of the middle step's multiplicity grid (side x side)
   int32_t mkc;       // its K-chunks (4 segments each)
   int32_t pad;
 };
@@ -79,6 +80,7 @@ struct ExactArgs {
   int32_t mid_stride;          // digit-plane row stride (bytes) of the launch
   int32_t mid_mkc_cap;         // largest mkc of the launch
   int32_t abuf_alias;          // epilogue buffer in the wave's staging area (one vertical pass)
+  uint8_t* mmask;              // middle-step multiplicity grids (written by k_exact_mid)
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -97,8 +99,9 @@ __device__ __forceinline__ int win_y0(const ptg_grain& gr, int up, int rho) {
 }
 
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
-// LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2] =
-// count of the middle step. Rows beyond the last step are written as zeros.
+// LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
+// bytes = count of the middle step (from k_exact_mid, which runs first). Rows beyond the
+// last step are written as zeros.
 __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int bi = blockIdx.x;
@@ -111,28 +114,22 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   const int rho = sl.rho, side = 2 * rho + 1, ncell = side * side;
   const int nsteps = e.s.nsteps;
   int32_t* packed = reinterpret_cast<int32_t*>(smem);
-  int32_t* cmid = packed + ncell;
-  for (int t = threadIdx.x; t < 2 * ncell; t += blockDim.x) packed[t] = 0;
-  __syncthreads();
+  uint8_t* cmid = reinterpret_cast<uint8_t*>(packed + ncell);
+  for (int t = threadIdx.x; t < ncell; t += blockDim.x) packed[t] = 0;
   const bool real = k < nsteps;
-  const ptg_grain& gr = e.s.grains[g];
+  // the middle step's multiplicities (k_exact_mid)
+  if (real)
+    for (int t = threadIdx.x; t < ncell; t += blockDim.x) cmid[t] = e.mmask[sl.mmask_off + t];
+  __syncthreads();
   const int64_t b = e.s.pbegin[g];
   const int n = (int)(e.s.pbegin[g + 1] - b);
-  const int kmid = nsteps / 2;
   if (real) {
     const double c = e.s.cs[2 * ((int64_t)g * nsteps + k)], s = e.s.cs[2 * ((int64_t)g * nsteps + k) + 1];
-    const double cm = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], sm = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
-    const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int px = e.s.pattern[2 * (b + i)], py = e.s.pattern[2 * (b + i) + 1];
-      const int X = clampi(rx + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
-      const int Y = clampi(ry + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
-      const int a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
+      const uint32_t w = e.s.pwords[b + i];
       int x, y;
-      rotate_px(px, py, c, s, x, y);
-      atomicAdd(&packed[(y + rho) * side + (x + rho)], a + (1 << 24));
-      rotate_px(px, py, cm, sm, x, y);
-      atomicAdd(&cmid[(y + rho) * side + (x + rho)], 1);
+      rotate_px((int)(int8_t)(w & 0xffu), (int)(int8_t)((w >> 8) & 0xffu), c, s, x, y);
+      atomicAdd(&packed[(y + rho) * side + (x + rho)], (int)(w >> 16) + (1 << 24));
     }
   }
   __syncthreads();
@@ -169,7 +166,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
     int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * e.dcap;
     const int ww = win_width(e.s.left + e.s.right + 1, sl.cw);  // plain window width of this slot
     for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
-      const int d = ((packed[t] >> 24) & 0xff) - cmid[t];
+        const int d = ((packed[t] >> 24) & 0xff) - (int)cmid[t];
       if (d != 0) {
         const int sg = d > 0 ? 0 : 1, m = d > 0 ? d : -d;
         const int pos = atomicAdd(&cnt[sg], m);
@@ -243,17 +240,16 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   const double c = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], s = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
   long long Sa = 0, Saa = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t pw = e.s.pwords[b + i];
     int x, y;
-    rotate_px(e.s.pattern[2 * (b + i)], e.s.pattern[2 * (b + i) + 1], c, s, x, y);
+    rotate_px((int)(int8_t)(pw & 0xffu), (int)(int8_t)((pw >> 8) & 0xffu), c, s, x, y);
     // byte counters in 32-bit words; a count above 127 (not an int8 operand) raises the
     // capacity flag, which voids the search before a byte could carry into its neighbour
     const int off = (y + rho) * MLR + 16 + (x + rho);
     const unsigned int old = atomicAdd(reinterpret_cast<unsigned int*>(mrow + (off & ~3)), 1u << (8 * (off & 3)));
     if (((old >> (8 * (off & 3))) & 0xffu) >= 127u) atomicOr(&e.flags[0], 8);
     if (blockIdx.y == 0) {
-      const int X = clampi(gr.refcoord_xpix + e.s.p0[2 * (b + i)], 0, e.s.dimx - 1);
-      const int Y = clampi(gr.refcoord_ypix + e.s.p0[2 * (b + i) + 1], 0, e.s.dimy - 1);
-      const long long a = e.s.I0[(int64_t)Y * e.s.pitch0 + X];
+      const long long a = pw >> 16;
       Sa += a;
       Saa += a * a;
     }
@@ -290,6 +286,13 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
     for (int p = 0; p < kMidPlanes; p++) *reinterpret_cast<uint32_t*>(planes + p * psz + (size_t)y * stride + x4) = w[p];
   }
   __syncthreads();
+  if (blockIdx.y == 0) {
+    // the middle step's multiplicities for the template kernel's Delta lists
+    for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
+      const int qy = t / side, qx = t - qy * side;
+      e.mmask[sl.mmask_off + t] = mrow[(size_t)qy * MLR + 16 + qx];
+    }
+  }
   if (blockIdx.y == 0 && threadIdx.x == 0) {
     long long A = 0, AA = 0;
     for (int w = 0; w < nwave; w++) {
@@ -723,7 +726,7 @@ struct ExactPlan {
   std::vector<int32_t> msegs;
   std::vector<Bucket> buckets;
   std::vector<int32_t> bucket_cat;
-  size_t b_mseg = 0;
+  size_t b_mseg = 0, b_mmask = 0;
   size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
   size_t total = 0;
   void* dev_base = nullptr;  // scratch the tables were last uploaded to
@@ -747,7 +750,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   const int S = U * V;
   P.slots.assign(nlist, ExactSlot{});
   P.segs.clear();
-  int64_t tmpl_total = 0, mid_total = 0;
+  int64_t tmpl_total = 0, mid_total = 0, mmask_total = 0;
   int rho_max = 0;
   for (int li = 0; li < nlist; li++) {
     const int g = list[li];
@@ -770,6 +773,8 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     tmpl_total += (int64_t)P.nmt * s.kc * 1024;
     s.mid_off = mid_total;
     mid_total += 2 * (int64_t)S;
+    s.mmask_off = mmask_total;
+    mmask_total += (int64_t)(2 * rho + 1) * (2 * rho + 1);
     // middle-step GEMM: per mask row, 16-aligned segments covering x in
     // [rho - xm, rho + xm + 15] (the row's cells moved right by up to 15)
     s.mseg_off = (int64_t)P.msegs.size();
@@ -807,8 +812,9 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   P.b_dlen = al(sizeof(int32_t) * 2 * (size_t)nlist * nsteps);
   P.b_mid = al(sizeof(int64_t) * (size_t)mid_total);
   P.b_mom = al(sizeof(int64_t) * 3 * nlist);
+  P.b_mmask = al((size_t)mmask_total);
   P.b_flags = al(sizeof(int32_t) * 4);
-  P.total = P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
+  P.total = P.b_mmask + P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
   P.dev_base = nullptr;
 }
 
@@ -844,6 +850,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   } else {
     build_plan(c, local, list, nlist, U, V, nsteps);
   }
+  for (const Bucket& bk : P->buckets)
+    if (!bk.slots.empty() && bk.rho_max > 120)  // pattern words hold offsets as int8
+      return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", bk.rho_max);
   int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
   if (rc != PTG_OK) return rc;
   char* base = (char*)c.d_exact_scratch;
@@ -869,6 +878,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   base += P->b_mid;
   e.mom = (int64_t*)base;
   base += P->b_mom;
+  e.mmask = (uint8_t*)base;
+  base += P->b_mmask;
   e.flags = (int32_t*)base;
   e.dcap = P->dcap;
   e.nmt = nmt;
@@ -914,15 +925,6 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     off += nb;
     const int rho = bk.rho_max, side = 2 * rho + 1;
     const int cw = (side + 15) / 16 * 16, cs = ((cw / 16) % 2 == 1) ? cw : cw + 16;
-    // templates and Delta lists
-    const size_t lds_t = (size_t)2 * side * side * 4;
-    if (lds_t > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", rho);
-    if (lds_t > 65536 &&
-        (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_t)) != hipSuccess)
-      return c.check(he, "hipFuncSetAttribute");
-    k_exact_template<<<dim3(nb, 8 * nmt), 256, lds_t, c.stream>>>(e);
-    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
     // middle-step window sums B_mid, C_mid and the reference moments: as many 16-column
     // blocks per workgroup as keep two workgroups per CU (at least one block)
     const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
@@ -944,6 +946,15 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       k_exact_mid<<<dim3(nb, (ncb + cbw - 1) / cbw), 64 * nwave_m, lds_m, c.stream>>>(em);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     }
+    // templates and Delta lists (against the middle step's grid from k_exact_mid)
+    const size_t lds_t = (size_t)side * side * 5;
+    if (lds_t > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", rho);
+    if (lds_t > 65536 &&
+        (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_t)) != hipSuccess)
+      return c.check(he, "hipFuncSetAttribute");
+    k_exact_template<<<dim3(nb, 8 * nmt), 256, lds_t, c.stream>>>(e);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
     // the MFMA search
     const int WW = win_width(U, cw), WH = P->vp + 2 * rho;
     e.ww = WW;
