@@ -22,7 +22,7 @@ namespace ptg {
 
 constexpr int kSpWaves = 4;
 constexpr int kSpThreads = 64 * kSpWaves;
-constexpr int kSpSums = 18;             // per-pixel sums of the value-and-derivatives pass
+constexpr int kSpSums = 24;             // per-pixel sums of the value-and-derivatives pass
 constexpr int kSpWordsCap = 8192;       // pattern pixels staged in LDS per grain
 constexpr int kSpTileBytesCap = 65536;  // the frame tile is skipped above this size
 
@@ -36,6 +36,9 @@ extern __shared__ __align__(16) unsigned char sp_lds[];
 struct Request {
   double xpix1, ypix1;  // refcoord + dx + X[0], refcoord + dy + X[1]
   double c1, s1;        // cos, sin of drot + X[2]
+  // weights of the Hessian's second-order sums, w_i = wa d0_i - wb (v_i - wm): the ZNCC
+  // residual of the latest evaluation (wb = NCC / sig1, wm = mean of v there)
+  double wa, wb, wm;
   int32_t kind;
   int32_t pad;
 };
@@ -162,6 +165,43 @@ __device__ __forceinline__ void sample_grad(const Obj& o, double x, double y, do
   }
 }
 
+__device__ __forceinline__ void cr_d2weights(double t, double* d) {
+  d[0] = fma(-3.0, t, 2.0);
+  d[1] = fma(9.0, t, -5.0);
+  d[2] = fma(-9.0, t, 4.0);
+  d[3] = fma(3.0, t, -1.0);
+}
+
+// The sampler's value, gradient and Hessian (xx, xy, yy) at (x, y).
+__device__ __forceinline__ void sample_hess(const Obj& o, double x, double y, double& v, double& gx, double& gy,
+                                            double& gxx, double& gxy, double& gyy, bool& clamped) {
+  const int ix = (int)floor(x), iy = (int)floor(y);
+  const double y01 = y - (double)iy, x01 = x - (double)ix;
+  unsigned int p[4][4];
+  stencil(o, ix, iy, p, clamped);
+  double wx[4], wy[4], dx[4], dy[4], ex[4], ey[4];
+  cr_weights(x01, wx);
+  cr_weights(y01, wy);
+  cr_dweights(x01, dx);
+  cr_dweights(y01, dy);
+  cr_d2weights(x01, ex);
+  cr_d2weights(y01, ey);
+  v = gx = gy = gxx = gxy = gyy = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double q0 = (double)p[j][0], q1 = (double)p[j][1], q2 = (double)p[j][2], q3 = (double)p[j][3];
+    const double r = fma(wx[3], q3, fma(wx[2], q2, fma(wx[1], q1, wx[0] * q0)));
+    const double sd = fma(dx[3], q3, fma(dx[2], q2, fma(dx[1], q1, dx[0] * q0)));
+    const double se = fma(ex[3], q3, fma(ex[2], q2, fma(ex[1], q1, ex[0] * q0)));
+    v = fma(wy[j], r, v);
+    gx = fma(wy[j], sd, gx);
+    gy = fma(dy[j], r, gy);
+    gxx = fma(wy[j], se, gxx);
+    gxy = fma(dy[j], sd, gxy);
+    gyy = fma(ey[j], r, gyy);
+  }
+}
+
 // Calls f(x, y, d0) for this thread's share of the grain's pattern pixels.
 template <class F>
 __device__ __forceinline__ void for_pixels(const Obj& o, F& f) {
@@ -190,19 +230,28 @@ __device__ __forceinline__ void pixel_sums_value(const Obj& o, const Request& rq
   for_pixels(o, px);
 }
 
+// Derivative pass: with g(X) = I1(p(X)) - mean0 and its derivatives G = dg/d(u, v, theta)
+// and G2 = d2g (xx, xy, xt, yy, yt, tt), the sums g, g^2, d0 g, G, g G, d0 G, G G^T and
+// w G2 (w the residual weight of the request).
 __device__ __forceinline__ void pixel_sums_derivs(const Obj& o, const Request& rq, double* S, bool& cl) {
 #pragma unroll
   for (int k = 0; k < kSpSums; k++) S[k] = 0.0;
   const double xpix1 = rq.xpix1, ypix1 = rq.ypix1, c1 = rq.c1, s1 = rq.s1;
+  const double wa = rq.wa, wb = rq.wb, wm = rq.wm;
   auto px = [&](int xi, int yi, double dd) {
     const double x = (double)xi, y = (double)yi;
     const double xs = xpix1 + c1 * x + s1 * y;
     const double ys = ypix1 - s1 * x + c1 * y;
-    double v, gx, gy;
-    sample_grad(o, xs, ys, v, gx, gy, cl);
+    double v, gx, gy, gxx, gxy, gyy;
+    sample_hess(o, xs, ys, v, gx, gy, gxx, gxy, gyy, cl);
     v -= o.shift;
-    const double gt = gx * (-s1 * x + c1 * y) + gy * (-c1 * x - s1 * y);
+    // d(xs, ys)/dtheta and their second derivatives (xtt = yt, ytt = -xt)
+    const double xt = -s1 * x + c1 * y, yt = -c1 * x - s1 * y;
+    const double gt = gx * xt + gy * yt;
     const double G[3] = {gx, gy, gt};
+    const double gxt = fma(gxx, xt, gxy * yt), gyt = fma(gxy, xt, gyy * yt);
+    const double gtt = fma(gxt, xt, fma(gyt, yt, fma(gx, yt, -gy * xt)));
+    const double w = fma(wa, dd, -wb * (v - wm));
     S[0] += v;
     S[1] = fma(v, v, S[1]);
     S[2] = fma(dd, v, S[2]);
@@ -218,6 +267,12 @@ __device__ __forceinline__ void pixel_sums_derivs(const Obj& o, const Request& r
     S[15] = fma(G[1], G[1], S[15]);
     S[16] = fma(G[1], G[2], S[16]);
     S[17] = fma(G[2], G[2], S[17]);
+    S[18] = fma(w, gxx, S[18]);
+    S[19] = fma(w, gxy, S[19]);
+    S[20] = fma(w, gxt, S[20]);
+    S[21] = fma(w, gyy, S[21]);
+    S[22] = fma(w, gyt, S[22]);
+    S[23] = fma(w, gtt, S[23]);
   };
   for_pixels(o, px);
 }
@@ -272,36 +327,49 @@ __device__ __forceinline__ double weight(const ObjConst& o, const double* X) {
   return W;
 }
 
-__device__ __forceinline__ void make_request(const ObjConst& o, const double* X, int kind, Request* rq) {
+__device__ __forceinline__ void make_request(const ObjConst& o, const double* X, int kind, double wb, double wm,
+                                             Request* rq) {
   const double rot = o.drot + X[2];
   rq->xpix1 = o.xr + X[0];
   rq->ypix1 = o.yr + X[1];
   rq->c1 = cos(rot);
   rq->s1 = sin(rot);
+  rq->wa = 1.0 / sqrt(o.C0C0);
+  rq->wb = wb;
+  rq->wm = wm;
   rq->kind = kind;
 }
 
 // (1 - NCC) * W at X from the value sums. The reference's two passes (mean, then
 // centred products) are replaced by one pass over shifted sums:
 // C1C1 = Sgg - Sg^2/N, C0C1 = Sdg - Sg*Sd0/N.
-__device__ __forceinline__ double finish_value(const ObjConst& o, const double* S, const double* X) {
+__device__ __forceinline__ double finish_value(const ObjConst& o, const double* S, const double* X, double* wb,
+                                               double* wm) {
   const double N = o.N;
   const double C1C1 = S[1] - S[0] * S[0] / N;
   const double C0C1 = S[2] - S[0] * o.sd0 / N;
   const double NCC = C0C1 / sqrt(o.C0C0 * C1C1);
+  *wb = NCC / sqrt(C1C1);
+  *wm = S[0] / N;
   return (1.0 - NCC) * weight(o, X);
 }
 
-// Value and gradient at X; also the Gauss-Newton matrix of the ZNCC residual
-// r_i = (g_i - m1)/sqrt(C1C1) - d0_i/sqrt(C0C0), whose squared norm is 2 (1 - NCC):
-// H = (sum h h^T - beta beta^T) / C1C1 with h_i = dg_i - mean(dg), beta = sum ghat_i h_i.
+// Value, gradient and Hessian of f = (1 - NCC) W at X. With u = g - mean(g), sig1 = |u|,
+// fhat = d0 / sig0, beta_a = u.G_a / sig1, hf_a = fhat.G_a and D_ab = sum (G_a - mean G_a)
+// (G_b - mean G_b):
+//   dNCC_a = (hf_a - NCC beta_a) / sig1
+//   d2NCC_ab = T_ab / sig1 - (hf_a beta_b + hf_b beta_a + NCC (D_ab - 3 beta_a beta_b)) / sig1^2
+// where T_ab = sum (fhat - NCC u / sig1) G2_ab, taken with the residual weights of the
+// request (those of the latest evaluation, so T is exact only at a fixed point; the gradient
+// is exact everywhere, so the optimizer's fixed points are those of f).
 struct Derivs {
   double f, NCC;
   double grad[3];
-  double H[6];  // xx, xy, xt, yy, yt, tt of the Gauss-Newton matrix (W = 1)
+  double H[6];  // xx, xy, xt, yy, yt, tt
 };
 
-__device__ __forceinline__ void finish_derivs(const ObjConst& o, const double* S, const double* X, Derivs& d) {
+__device__ __forceinline__ void finish_derivs(const ObjConst& o, const double* S, const double* X, Derivs& d,
+                                              double* wb, double* wm) {
   const double N = o.N;
   const double m1 = S[0] / N;
   const double C1C1 = S[1] - S[0] * m1;
@@ -312,31 +380,41 @@ __device__ __forceinline__ void finish_derivs(const ObjConst& o, const double* S
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     mG[k] = S[3 + k] / N;
-    beta[k] = (S[6 + k] - m1 * S[3 + k]) / sig1;  // sum ghat_i h_i
-    hf[k] = (S[9 + k] - mG[k] * o.sd0) / sig0;    // sum h_i fhat_i
+    beta[k] = (S[6 + k] - m1 * S[3 + k]) / sig1;  // u . G_k / sig1
+    hf[k] = (S[9 + k] - mG[k] * o.sd0) / sig0;    // fhat . G_k
   }
-  // dNCC/dX = (sum h fhat - beta NCC) / sig1
   double dN[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) dN[k] = (hf[k] - beta[k] * NCC) / sig1;
   const double W = weight(o, X);
   double dW[3] = {0.0, 0.0, 0.0};
+  double d2W[3] = {0.0, 0.0, 0.0};  // xx, xy, yy of the penalty weight (zero in theta)
   const double disp = sqrt(X[0] * X[0] + X[1] * X[1]);
   if (disp > o.dmax && disp > 0.0) {
     dW[0] = o.stiff * X[0] / disp;
     dW[1] = o.stiff * X[1] / disp;
+    const double i3 = o.stiff / (disp * disp * disp);
+    d2W[0] = i3 * X[1] * X[1];
+    d2W[1] = -i3 * X[0] * X[1];
+    d2W[2] = i3 * X[0] * X[0];
   }
   d.NCC = NCC;
   d.f = (1.0 - NCC) * W;
 #pragma unroll
   for (int k = 0; k < 3; k++) d.grad[k] = -dN[k] * W + (1.0 - NCC) * dW[k];
   const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  const int w2[6] = {0, 1, -1, 2, -1, -1};  // index into d2W, -1: zero
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int a = ij[q][0], b = ij[q][1];
-    const double hh = S[12 + q] - N * mG[a] * mG[b];
-    d.H[q] = (hh - beta[a] * beta[b]) / C1C1;
+    const double D = S[12 + q] - N * mG[a] * mG[b];
+    const double d2N = S[18 + q] / sig1 - (hf[a] * beta[b] + hf[b] * beta[a] + NCC * (D - 3.0 * beta[a] * beta[b])) / C1C1;
+    double h = -d2N * W - (dN[a] * dW[b] + dN[b] * dW[a]);
+    if (w2[q] >= 0) h += (1.0 - NCC) * d2W[w2[q]];
+    d.H[q] = h;
   }
+  *wb = NCC / sig1;
+  *wm = m1;
 }
 
 // ---------------------------------------------------------------------------------
@@ -381,8 +459,8 @@ struct Powell {
 
 // The LM form: the first sweep of Powell's method (line minimisations along the three
 // initial directions, which is where Powell's run picks its basin when the objective has
-// several minima in theta), then Levenberg-Marquardt on (u, v, theta) with the
-// Gauss-Newton matrix of the ZNCC residual until the step is below 1e-8 px and 1e-10 rad.
+// several minima in theta), then damped Newton (Levenberg-Marquardt on the objective's
+// Hessian) on (u, v, theta) until the step is below 1e-8 px and 1e-10 rad.
 #define LM_SWEEP_TOL 1.0e-4
 #define LM_STEP_PX 1.0e-8
 #define LM_STEP_RAD 1.0e-10
@@ -404,6 +482,7 @@ struct SpState {
   int32_t lm, nevals;
   Line L;         // the line search in progress
   Derivs dq;      // derivatives at Xq, for derivative requests
+  double hwb, hwm;  // residual weights (NCC / sig1, mean of v) of the latest evaluation
   double sums[kSpSums];  // the workgroup's pixel sums at Xq
   union {
     Powell P;
@@ -612,9 +691,11 @@ __device__ __noinline__ bool pw_feed(double f) {
 }
 
 __device__ __forceinline__ bool solve3(const double* H, const double* g, double lambda, double* dx) {
-  // (H + lambda diag(H)) dx = -g, symmetric 3x3 by Cramer's rule
-  double a = H[0] * (1.0 + lambda), b = H[1], c = H[2];
-  double d = H[3] * (1.0 + lambda), e = H[4], f = H[5] * (1.0 + lambda);
+  // (H + lambda D) dx = -g with D = |diag(H)| (floored), symmetric 3x3 by Cramer's rule
+  const double dmax = fmax(fabs(H[0]), fmax(fabs(H[3]), fabs(H[5])));
+  const double floor_d = 1e-12 * dmax + 1e-300;
+  double a = H[0] + lambda * fmax(fabs(H[0]), floor_d), b = H[1], c = H[2];
+  double d = H[3] + lambda * fmax(fabs(H[3]), floor_d), e = H[4], f = H[5] + lambda * fmax(fabs(H[5]), floor_d);
   const double A = d * f - e * e, B = c * e - b * f, C = b * e - c * d;
   const double det = a * A + b * B + c * C;
   if (!(fabs(det) > 0.0) || !isfinite(det)) return false;
@@ -779,16 +860,16 @@ __device__ __noinline__ void sp_step() {
   st.nevals++;
   int next;
   if (st.rq.kind == RQ_DERIVS) {
-    finish_derivs(st.k, st.sums, X, st.dq);
+    finish_derivs(st.k, st.sums, X, st.dq, &st.hwb, &st.hwm);
     next = lm_feed(st.dq.f);
   } else {
-    const double f = finish_value(st.k, st.sums, X);
+    const double f = finish_value(st.k, st.sums, X, &st.hwb, &st.hwm);
     next = st.lm ? lm_feed(f) : (pw_feed(f) ? RQ_VALUE : RQ_DONE);
   }
   if (next == RQ_DONE)
     st.rq.kind = RQ_DONE;
   else
-    make_request(st.k, st.Xq, next, &st.rq);
+    make_request(st.k, st.Xq, next, st.hwb, st.hwm, &st.rq);
 }
 
 // follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128: one workgroup per grain of the
@@ -837,7 +918,8 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
       P.max_iter = prm.max_iter > 0 ? prm.max_iter : 200;
       P.state = PS_INIT;
     }
-    make_request(st.k, st.Xq, RQ_VALUE, &st.rq);
+    st.hwb = st.hwm = 0.0;
+    make_request(st.k, st.Xq, RQ_VALUE, 0.0, 0.0, &st.rq);
   }
   bool cl = false;
   for (;;) {
@@ -952,7 +1034,8 @@ __global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __rest
   const double X[3] = {Xs[3 * e], Xs[3 * e + 1], Xs[3 * e + 2]};
   const ObjConst k = obj_const(o);
   Request q;
-  make_request(k, X, grad ? RQ_DERIVS : RQ_VALUE, &q);
+  make_request(k, X, grad ? RQ_DERIVS : RQ_VALUE, 0.0, 0.0, &q);
+  double wb_, wm_;
   bool cl = false;
   double S[kSpSums];
   if (grad) {
@@ -962,7 +1045,7 @@ __global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __rest
     if (threadIdx.x == 0) {
       reduce_partials<kSpSums>(red, S);
       Derivs d;
-      finish_derivs(k, S, X, d);
+      finish_derivs(k, S, X, d, &wb_, &wm_);
       f[e] = d.f;
       grad[3 * e] = d.grad[0];
       grad[3 * e + 1] = d.grad[1];
@@ -974,7 +1057,7 @@ __global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __rest
     __syncthreads();
     if (threadIdx.x == 0) {
       reduce_partials<3>(red, S);
-      f[e] = finish_value(k, S, X);
+      f[e] = finish_value(k, S, X, &wb_, &wm_);
     }
   }
 }
