@@ -28,6 +28,34 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = "profiles/exact_lm_cfg2_6000x4000_2000grains_hbm_bytes.txt"
+
+
+def hbm_bytes_per_launch(kernel):
+    """HBM bytes per launch of `kernel` from the byte-counter summary, or None."""
+    try:
+        for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
+            if line.startswith("#") or kernel not in line:
+                continue
+            return float(line.split()[-1]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes: a kernel of known HBM traffic (nbytes read,
+    nbytes written) whose FETCH_SIZE / WRITE_SIZE counters scale the search kernel's."""
+    import torch
+
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,9 +65,14 @@ def main():
     ap.add_argument("--subpix", choices=["auto", "lm", "powell"], default="auto")
     ap.add_argument("--grains", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain search+Powell form (0: skip)")
-    ap.add_argument("--cpu-grains", type=int, default=480, help="grains in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-grains", type=int, default=2000, help="grains in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device to device and exit (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards grains across GPUs "
               "without collectives in the hot loop, and this build runs on a single device.", file=sys.stderr)
@@ -122,6 +155,13 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
     roof["kernel"] = "k_search_exact (MFMA)" if method == "exact" else "k_search_plain (FP64)"
+    # HBM bytes per launch of the search kernel, from the committed byte-counter summary
+    # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy of known size)
+    if method == "exact":
+        tb = hbm_bytes_per_launch("k_search_exact")
+        if tb is not None:
+            roof["traffic"] = tb
+            roof["traffic_source"] = TRAFFIC_FILE
     roof["algorithmic_flops_per_pair"] = flops
     roof["kernel_ms_per_pair"] = k_ms
     roof["launches_per_pair"] = k_launches
@@ -177,11 +217,11 @@ def main():
                     break
         except OSError:
             pass
-        # the fast form's error against the FP64 oracle (the oracle's Powell run to a tight
-        # tolerance) on the same sample, untimed
+        # the fast form's error against the FP64 oracle (the oracle's Powell at a tight
+        # tolerance, restarted until it no longer improves) on the same sample, untimed
         tp = pk.track_params(w.params.search_zone, w.params.search_zone_rescue, w.params.search_zone_super_rescue,
                              NCC_min=0.7, rescue_level=1, subpixel=1, search_method=pk.SEARCH_PLAIN,
-                             subpix=pk.default_subpix(pk.SUBPIX_POWELL, subpix_tol=1e-13))
+                             subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
         ot, _, _ = ptref.track_pair(W, tp, nthreads=threads, subset=sample)
         fs = out[sample]
         epx = np.hypot(fs["dx"] - ot["dx"], fs["dy"] - ot["dy"])

@@ -453,6 +453,22 @@ typedef struct {
   double dmax, stiff;
 } obj_ctx;
 
+double ptref_minimize_fp64(ptref_fn3 f, void* user, double* x, const double* dx, double ftol,
+                           int32_t max_iter, int32_t max_restarts, int64_t* nevals) {
+  double fret = ptref_powell(f, user, x, dx, ftol, max_iter, nevals);
+  if (fret != fret) return fret;
+  for (int r = 0; r < max_restarts; r++) {
+    double xr[3] = {x[0], x[1], x[2]};
+    const double fr = ptref_powell(f, user, xr, dx, ftol, max_iter, nevals);
+    if (!(fr < fret)) break;
+    x[0] = xr[0];
+    x[1] = xr[1];
+    x[2] = xr[2];
+    fret = fr;
+  }
+  return fret;
+}
+
 static double obj_fn(const double* X, void* user) {
   obj_ctx* o = (obj_ctx*)user;
   return ptref_ncc_objective(o->I0, o->I1, o->pitch, o->g, o->n, o->p0, X, o->dmax, o->stiff);
@@ -464,7 +480,10 @@ void ptref_subpixel(const uint16_t* I0, const uint16_t* I1, int64_t pitch, ptg_g
                     int64_t* nevals) {
   double X[3] = {0.0, 0.0, 0.0};
   obj_ctx o = {I0, I1, pitch, g, n, p0_xy, prm->subpix_displacement_max, prm->overflow_stiffness};
-  double fret = ptref_powell(obj_fn, &o, X, prm->initial_direction, prm->subpix_tol, prm->max_iter, nevals);
+  double fret = prm->method == PTG_SUBPIX_LM
+                    ? ptref_minimize_fp64(obj_fn, &o, X, prm->initial_direction, prm->subpix_tol, prm->max_iter, 20,
+                                          nevals)
+                    : ptref_powell(obj_fn, &o, X, prm->initial_direction, prm->subpix_tol, prm->max_iter, nevals);
   g->NCC_subpix = 1.0 - fret;
   g->upix += X[0];
   g->vpix += X[1];

@@ -179,9 +179,9 @@ def test_track_pair_cfg1_against_oracle(tracker, cfg1, method):
     tracker.track_pair(prm)
     got = tracker.get_grains()
     # Powell parity: the oracle's Powell at the reference tolerance; LM: the FP64 oracle,
-    # the oracle's Powell continued to a tight tolerance (same basin, converged minimum)
+    # the oracle's Powell at a tight tolerance restarted to convergence (same basin)
     oprm = cfg1.params if method == "powell" else pk.track_params(
-        cfg1.params.search_zone, rescue_level=0, subpixel=1, subpix=pk.default_subpix(subpix_tol=1e-13))
+        cfg1.params.search_zone, rescue_level=0, subpixel=1, subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
     ref, _, _ = ptref.track_pair(oracle_workload(cfg1), oprm, nthreads=4)
     assert bits_equal(got["NCC"], ref["NCC"]) and bits_equal(got["NCC_rescue"], ref["NCC_rescue"])
     assert np.abs(got["dx"] - ref["dx"]).max() < TOL_PX
@@ -201,7 +201,7 @@ def test_track_pair_cfg2_small_lm_against_fp64_oracle(tracker, cfg2s):
     tracker.track_pair(prm)
     got = tracker.get_grains()
     oprm = pk.track_params(p.search_zone, p.search_zone_rescue, p.search_zone_super_rescue, rescue_level=1,
-                           subpix=pk.default_subpix(subpix_tol=1e-13))
+                           subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
     ref, _, _ = ptref.track_pair(oracle_workload(cfg2s), oprm, nthreads=8)
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
     assert np.abs(got["dx"] - ref["dx"]).max() < TOL_PX and np.abs(got["dy"] - ref["dy"]).max() < TOL_PX

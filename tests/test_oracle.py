@@ -356,6 +356,56 @@ def test_powell_minimizes_quadratic_and_rosenbrock():
     assert np.allclose(x, [1, 1, 2], atol=1e-4)
 
 
+def test_fp64_minimizer_restarts_never_worse_and_reach_a_narrow_valley_minimum():
+    # a narrow, rotated valley (condition number 1e6) where one Powell run stops early
+    F = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+    f = F(lambda x, u: (x[0] + x[1] - 1) ** 2 + 1e-6 * (x[0] - x[1] - 0.5) ** 2 + 1e3 * (x[2] - 0.01 * x[0]) ** 2)
+    lib = ptref.lib()
+    lib.ptref_powell.argtypes = [F, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                                 ctypes.c_void_p]
+    lib.ptref_powell.restype = ctypes.c_double
+    lib.ptref_minimize_fp64.argtypes = [F, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+    lib.ptref_minimize_fp64.restype = ctypes.c_double
+    dx = np.full(3, 0.01)
+    ne = ctypes.c_int64(0)
+    x1 = np.zeros(3)
+    f1 = lib.ptref_powell(f, None, x1.ctypes.data, dx.ctypes.data, 1e-13, 200, ctypes.byref(ne))
+    x2 = np.zeros(3)
+    f2 = lib.ptref_minimize_fp64(f, None, x2.ctypes.data, dx.ctypes.data, 1e-13, 200, 20, ctypes.byref(ne))
+    xs = np.array([0.75, 0.25, 0.0075])  # the closed-form minimizer
+    assert f2 <= f1
+    assert np.abs(x2 - xs).max() < 1e-5 and f2 < 1e-14
+
+
+def test_fp64_minimizer_continues_where_one_powell_run_stalls():
+    # grain 1656 of the seeded 2000-grain 6000x4000 pair: one Powell run at a tight
+    # tolerance stops 2e-8 above the minimum in a flat valley (3.6e-5 rad off); the
+    # restarted run reaches a point where the objective's central-difference gradient
+    # vanishes
+    from ptracker_amd import synth
+    w = synth.workload(2)
+    W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
+    z = w.params
+    g = 1656
+    ti = pk.track_params(z.search_zone, z.search_zone_rescue, z.search_zone_super_rescue, NCC_min=0.7,
+                         rescue_level=1, subpixel=0)
+    oi, _, _ = ptref.track_pair(W, ti, nthreads=1, subset=np.array([g]))
+    pat = np.ascontiguousarray(W.pattern(g), dtype=np.int32)
+    sub = ptref.Workload(w.I0, w.I1, oi, pat, np.array([0, len(pat)], dtype=np.int64), w.bits)
+    gr, p0 = ptref.precompute(sub, oi)
+    one, rst = gr.copy(), gr.copy()
+    ptref.subpixel(sub, one, p0, 0, pk.default_subpix(pk.SUBPIX_POWELL, subpix_tol=1e-13))
+    ptref.subpixel(sub, rst, p0, 0, pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+    f_one, f_rst = 1 - one["NCC_subpix"][0], 1 - rst["NCC_subpix"][0]
+    assert f_rst < f_one - 1e-9
+    X = np.array([rst[c][0] - gr[c][0] for c in ("dx", "dy", "drot")])
+    h = 1e-6
+    grad = [(ptref.ncc_objective(sub, gr, p0, 0, X + h * e) - ptref.ncc_objective(sub, gr, p0, 0, X - h * e)) / (2 * h)
+            for e in np.eye(3)]
+    assert np.abs(grad).max() < 1e-6
+
+
 # ---- undistor, src/distortion_correction.cpp:20-61 -----------------------------------------
 def _params(**kw):
     X = np.zeros(14)
