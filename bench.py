@@ -43,6 +43,34 @@ def hbm_bytes_per_launch(kernel):
     return None
 
 
+def pick_mismatches(pk, tr, w, load=False):
+    """Main-stage integer picks (dx, dy, drot) of the exact and plain searches on all grains
+    of workload w; returns the number of grains whose picks differ in any bit."""
+    if load:
+        tr.set_image(0, w.I0, w.bits)
+        tr.set_image(1, w.I1, w.bits)
+        tr.set_image_indices(0, 1)
+        tr.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
+    z = w.params.search_zone
+    res = {}
+    for m in (pk.SEARCH_EXACT, pk.SEARCH_PLAIN):
+        tr.put_grains(w.grains)
+        tr.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=m))
+        res[m] = tr.get_grains()
+    a, b = res[pk.SEARCH_EXACT], res[pk.SEARCH_PLAIN]
+    diff = np.zeros(len(a), dtype=bool)
+    for c in ("dx", "dy", "drot"):
+        diff |= a[c].view(np.uint64) != b[c].view(np.uint64)
+    steps = len(ptref_steps(z))
+    return {"grains": int(len(a)), "rotation_steps": steps, "mismatches": int(diff.sum()),
+            "max_ncc_diff": float(np.abs(a["NCC"] - b["NCC"]).max())}
+
+
+def ptref_steps(z):
+    import ptref
+    return ptref.rotation_steps(z.inc_rot, z.num_rot)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes: a kernel of known HBM traffic (nbytes read,
     nbytes written) whose FETCH_SIZE / WRITE_SIZE counters scale the search kernel's."""
@@ -196,7 +224,21 @@ def main():
         result["plain_gpu"] = {"value": pv, "unit": "grains/s", "steps": args.plain_steps,
                                "speedup": value / pv, "max_px_diff_vs_plain": dev}
         log(f"plain search+Powell: {1e3 * pe / args.plain_steps:.2f} ms/pair, {pv:.0f} grains/s")
-    tr.close()
+    # precision form: integer picks of the exact MFMA search against the plain FP64 search
+    # (bit-identical to the reference's arithmetic) on every grain, main stage only
+    if method == "exact":
+        prec = {"cfg2_21_steps_12bit": pick_mismatches(pk, tr, w)}
+        tr.close()
+        w3 = synth.workload(3, n_grains=args.grains)
+        t3 = pk.Tracker(0)
+        prec["cfg3_40_steps_12bit"] = pick_mismatches(pk, t3, w3, load=True)
+        w3.I0, w3.I1, w3.bits = (w3.I0 >> 4).astype(np.uint16), (w3.I1 >> 4).astype(np.uint16), 8
+        prec["cfg3_40_steps_8bit"] = pick_mismatches(pk, t3, w3, load=True)
+        t3.close()
+        result["precision"] = prec
+        log("exact vs plain integer picks: " + ", ".join(f"{k}: {v['mismatches']} of {v['grains']}" for k, v in prec.items()))
+    else:
+        tr.close()
 
     if args.cpu_grains > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

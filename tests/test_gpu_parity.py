@@ -300,6 +300,30 @@ def test_rescue_exact_picks_equal_oracle(tracker):
     assert np.abs(got["NCC_rescue"] - ref["NCC_rescue"]).max() < 1e-12
 
 
+@pytest.mark.parametrize("bits", [12, 8])
+def test_cfg3_exact_40_steps_picks_equal_oracle(tracker, bits):
+    # config 3: +-20 px x 20 num_rot at inc 0.005, which is 40 steps in the reference's
+    # loop; 12-bit frames and their 8-bit requantisation (v >> 4). Zero argmax mismatches.
+    w = synth.workload(3, n_grains=20, size=(1000, 800), seed=3033)
+    if bits == 8:
+        w.I0 = (w.I0 >> 4).astype(np.uint16)
+        w.I1 = (w.I1 >> 4).astype(np.uint16)
+        w.bits = 8
+    z = w.params.search_zone
+    assert len(ptref.rotation_steps(z.inc_rot, z.num_rot)) == 40
+    prm = pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT)
+    say(f"config 3, 40 steps, {bits}-bit, exact")
+    load(tracker, w)
+    tracker.track_pair(prm)
+    got = tracker.get_grains()
+    oprm = pk.track_params(z, rescue_level=0, subpixel=0)
+    ref, _, _ = ptref.track_pair(oracle_workload(w), oprm, nthreads=8)
+    mism = sum(int(not bits_equal(got[c][i:i + 1], ref[c][i:i + 1])) for c in ("dx", "dy", "drot")
+               for i in range(w.G))
+    assert mism == 0
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+
+
 def test_exact_rejects_16bit_frames(tracker, cfg1):
     say("exact search refuses 16-bit frames")
     tracker.set_image(0, cfg1.I0, 16)
