@@ -6,6 +6,14 @@ search +-20 px x 21 rotation steps, rescue +-40 px for grains with NCC < 0.7, su
 refinement, and the 9 result columns copied back to the host. Frames are resident in
 HBM before timing; each step re-uploads the grains' input records (144 B each) as the
 pair's input. Prints one JSON line.
+
+--sequence runs the batching form instead: a 16-frame 6000x4000 12-bit sequence with
+10,000 disk grains of radius U(6, 60) px moved per frame by U(-5, 5) px and U(-0.03, 0.03)
+rad (accumulated), tracked pair after pair with the reference frame swapped in between
+(search +-8 px x 9 steps, subpixel). All frames are resident in HBM; a pair's latency runs
+from the reference swap to the 9 result columns on the host. It prints one JSON line with
+grains/s and the p50/p99 pair latency, and checks a sample of grains of every pair against
+the oracle.
 """
 import argparse
 import json
@@ -84,6 +92,118 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+def run_sequence(args, pk, synth, ptref):
+    import torch
+
+    t0 = time.time()
+    S = synth.sequence(n_frames=args.seq_frames, n_grains=args.seq_grains)
+    H, W = S.frames[0].shape
+    log(f"sequence generated in {time.time() - t0:.1f}s: {S.G} grains, {len(S.pattern_xy)} pattern px, "
+        f"{len(S.frames)} frames")
+    dev = [torch.from_numpy(f).to("cuda:0") for f in S.frames]  # every frame resident in HBM
+    torch.cuda.synchronize()
+    z = S.params.search_zone
+    prm = pk.track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    tr = pk.Tracker(0)
+    npairs = len(S.frames) - 1
+
+    def attach(slot, k):
+        tr.set_image_device(slot, dev[k].data_ptr(), W, H, W, S.bits)
+
+    def one_pass(lat, keep):
+        attach(0, 0)
+        attach(1, 1)
+        tr.set_image_indices(0, 1)
+        tr.set_grains(S.grains, S.pattern_xy, S.pattern_begin)
+        tr.sync()
+        outs = []
+        for k in range(npairs):
+            t = time.perf_counter()
+            if k > 0:
+                tr.change_reference()
+                attach((k + 1) % 2, k + 1)
+            tr.track_pair(prm)
+            out = tr.get_grains()
+            lat.append(time.perf_counter() - t)
+            if keep:
+                outs.append(out)
+        return outs
+
+    one_pass([], False)  # warm-up: exact-search plans, code objects
+    lat = []
+    outs = None
+    f0 = tr.exact_fallbacks()
+    tr.set_timing(True)
+    tr.reset_stage_times()
+    t1 = time.perf_counter()
+    for r in range(args.seq_repeats):
+        o = one_pass(lat, r == 0)
+        outs = outs or o
+    el = time.perf_counter() - t1
+    ms, launches = tr.stage_times()
+    tr.set_timing(False)
+    f1 = tr.exact_fallbacks()
+    stage_names = ["precompute", "search_main", "search_rescue", "search_super", "subpixel", "remap", "search_kernel"]
+    stage_ms = {n: ms[i] / len(lat) for i, n in enumerate(stage_names) if launches[i]}
+    log("stage ms/pair: " + ", ".join(f"{k} {v:.2f}" for k, v in stage_ms.items()) +
+        f"; grains handed to the plain kernel: {(f1[1] - f0[1]) / len(lat):.0f} per pair")
+    lat_ms = 1e3 * np.array(lat)
+    value = S.G * len(lat) / lat_ms.sum() * 1e3
+    log(f"sequence: {len(lat)} pairs, {value:.0f} grains/s, pair latency p50 {np.percentile(lat_ms, 50):.2f} ms "
+        f"p99 {np.percentile(lat_ms, 99):.2f} ms ({el:.1f}s with the set-up of each pass)")
+    tr.close()
+
+    # oracle check on a sample of every pair, from the GPU's own input records of that pair
+    check = {}
+    if args.seq_check_grains > 0:
+        rng = np.random.default_rng(0)
+        sample = np.sort(rng.choice(S.G, size=min(args.seq_check_grains, S.G), replace=False))
+        tp = pk.track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=pk.SEARCH_PLAIN,
+                             subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+        gin = S.grains.copy()
+        epx, erad, bad_int = [], [], 0
+        t2 = time.perf_counter()
+        for k in range(npairs):
+            Wk = ptref.Workload(S.frames[k], S.frames[k + 1], gin, S.pattern_xy, S.pattern_begin, S.bits)
+            ot, _, _ = ptref.track_pair(Wk, tp, nthreads=min(16, os.cpu_count() or 1), subset=sample)
+            g = outs[k][sample]
+            o = ot  # the oracle returns the sample's rows
+            epx.append(np.hypot(g["dx"] - o["dx"], g["dy"] - o["dy"]))
+            erad.append(np.abs(g["drot"] - o["drot"]))
+            # an integer pick that differs moves the result by about a pixel or a step
+            bad_int += int(((np.abs(g["dx"] - o["dx"]) > 0.5) | (np.abs(g["dy"] - o["dy"]) > 0.5)).sum())
+            gin = ptref.change_reference(outs[k])
+        epx, erad = np.concatenate(epx), np.concatenate(erad)
+        check = {"grains_per_pair": int(len(sample)), "pairs": npairs, "max_px": float(epx.max()),
+                 "max_rad": float(erad.max()), "over_1e-3_px": int((epx > 1e-3).sum()),
+                 "over_1e-5_rad": int((erad > 1e-5).sum()), "over_half_px": bad_int,
+                 "seconds": time.perf_counter() - t2}
+        log(f"oracle check: {len(sample)} grains x {npairs} pairs, max {epx.max():.2e} px {erad.max():.2e} rad, "
+            f"{(epx > 1e-3).sum()} over 1e-3 px")
+    P = np.diff(S.pattern_begin).astype(np.float64)
+    R = len(ptref.rotation_steps(z.inc_rot, z.num_rot))
+    Sh = (z.left + z.right + 1) * (z.up + z.down + 1)
+    result = {
+        "metric": "grains/s over a frame sequence (batching form)", "value": value, "unit": "grains/s",
+        "n_gpus": 1, "steps": len(lat), "warmup": npairs, "ms_per_step": float(lat_ms.mean()),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i8", "data": "synthetic",
+        "config": {"workload": f"{len(S.frames)}-frame 6000x4000 12-bit sequence, {S.G} disk grains r ~ U(6, 60) "
+                               "px (overlapping), per-frame motion U(-5, 5) px, U(-0.03, 0.03) rad accumulated, "
+                               "consecutive pairs with reference swap, search +-8 px x 9 steps, subpixel LM",
+                   "grains": int(S.G), "pattern_px": int(P.sum()), "rotation_steps": R, "shifts": Sh,
+                   "pairs_timed": len(lat)},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                       "max": float(lat_ms.max()), "min": float(lat_ms.min()),
+                       "per_pair_first_pass": [round(float(x), 2) for x in lat_ms[:npairs]]},
+        "stage_ms": stage_ms,
+        "plain_kernel_grains_per_pair": (f1[1] - f0[1]) / len(lat),
+        "algorithmic_search_flops_per_pair": 2.0 * R * Sh * P.sum(),
+        "oracle_check": check,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +215,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain search+Powell form (0: skip)")
     ap.add_argument("--cpu-grains", type=int, default=2000, help="grains in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
+    ap.add_argument("--seq-frames", type=int, default=16)
+    ap.add_argument("--seq-grains", type=int, default=10000)
+    ap.add_argument("--seq-repeats", type=int, default=2, help="timed passes over the sequence")
+    ap.add_argument("--seq-check-grains", type=int, default=48, help="grains per pair checked against the oracle")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device and exit (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -110,6 +235,10 @@ def main():
 
     pk = ptref.package()
     from ptracker_amd import synth
+
+    if args.sequence:
+        run_sequence(args, pk, synth, ptref)
+        return
 
     t0 = time.time()
     w = synth.workload(2, n_grains=args.grains)

@@ -210,6 +210,14 @@ int ptg_reset_stage_times(void* ctx);
  * (per_grain may be NULL; it receives G counts, 0 for masked grains). */
 int ptg_get_subpix_evals(void* ctx, int64_t* evals);
 int ptg_get_subpix_evals_per_grain(void* ctx, int32_t* per_grain);
+/* Work the exact form handed to the plain form since the context was created (the picks
+ * are the same; NCC comes from the two-pass formula). searches: PTG_SEARCH_EXACT searches
+ * run wholly by the plain kernel (frames above 12 bits, more than 48 rotation steps, a
+ * pattern radius above 120 px, or a pattern and window beyond the exact kernels' LDS).
+ * grains: grains of exact searches whose rotated templates differ from the middle step's
+ * in more cells than the exact kernel holds (large patterns at large reference rotations),
+ * searched by the plain kernel within the same stage. Synchronises. */
+int ptg_get_exact_fallbacks(void* ctx, int64_t* searches, int64_t* grains);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_reset_stage_times": ([vp], c_int),
         "ptg_get_subpix_evals": ([vp, POINTER(c_int64)], c_int),
         "ptg_get_subpix_evals_per_grain": ([vp, vp], c_int),
+        "ptg_get_exact_fallbacks": ([vp, POINTER(c_int64), POINTER(c_int64)], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -165,7 +166,7 @@ def exported_symbols():
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
-            "ptg_get_subpix_evals_per_grain"]
+            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks"]
 
 
 class PtgError(RuntimeError):
@@ -319,6 +320,12 @@ class Tracker:
         out = np.zeros(self.G, dtype=np.int32)
         self._ck(self.lib.ptg_get_subpix_evals_per_grain(self.h, _ptr(out)), "ptg_get_subpix_evals_per_grain")
         return out
+
+    def exact_fallbacks(self):
+        """(searches, grains) handed from the exact form to the plain kernel."""
+        n, g = c_int64(), c_int64()
+        self._ck(self.lib.ptg_get_exact_fallbacks(self.h, ctypes.byref(n), ctypes.byref(g)), "ptg_get_exact_fallbacks")
+        return n.value, g.value
 
     def subpix_evals(self) -> int:
         e = c_int64()

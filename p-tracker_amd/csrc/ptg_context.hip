@@ -119,14 +119,16 @@ struct PinnedBuf {
 // The context as seen through the C ABI, with the host-side staging it owns.
 struct Ctx : Context {
   PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list, pin_flags;
-  // capacity flags of exact searches copied to pin_flags[stage], read at the next
+  // capacity flags of exact searches and their count of grains served by the plain
+  // kernel, copied to pin_flags[2 stage], pin_flags[2 stage + 1] and read at the next
   // synchronisation of the stream
   bool flags_pending[3] = {false, false, false};
   int check_flags() {
     for (int st = 0; st < 3; st++) {
       if (!flags_pending[st]) continue;
       flags_pending[st] = false;
-      const int32_t f = static_cast<const int32_t*>(pin_flags.p)[st];
+      const int32_t f = static_cast<const int32_t*>(pin_flags.p)[2 * st];
+      exact_grain_fallbacks += static_cast<const int32_t*>(pin_flags.p)[2 * st + 1];
       if (f)
         return fail(PTG_ERR_LIMIT,
                     "exact search capacity exceeded in stage %d (flags %d): the stage changed nothing; use "
@@ -355,6 +357,22 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   c->mark_begin(tstage, &ev);
   int nparts = nsteps;
   const int32_t* flag_dev = nullptr;
+  if (method != PTG_SEARCH_PLAIN) {
+    // the unmasked list is the same on every main-stage call until it is rebuilt
+    const uint64_t key = stage == PTG_STAGE_MAIN ? c->unmasked_version : 0;
+    const int rc = run_search_exact(*c, a, nlist, list, key, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN);
+    if (rc == kExactUnfit) {
+      method = PTG_SEARCH_PLAIN;
+      nparts = nsteps;
+      c->exact_fallbacks++;
+    } else {
+      TRY(rc);
+      TRY(c->pin_flags.ensure(*c, 6 * sizeof(int32_t)));
+      CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + 2 * stage, flag_dev, 2 * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+      c->flags_pending[stage] = true;
+    }
+  }
   if (method == PTG_SEARCH_PLAIN) {
     const size_t lds = (size_t)(win_cap * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
     if (lds > kMaxLds)
@@ -363,14 +381,6 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     if (stage == PTG_STAGE_MAIN) c->mark_begin(PTG_T_SEARCH_KERNEL, &ek);
     CK(c, launch_search_plain(*c, a, nlist, lds));
     if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
-  } else {
-    // the unmasked list is the same on every main-stage call until it is rebuilt
-    const uint64_t key = stage == PTG_STAGE_MAIN ? c->unmasked_version : 0;
-    TRY(run_search_exact(*c, a, nlist, list, key, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN));
-    TRY(c->pin_flags.ensure(*c, 4 * sizeof(int32_t)));
-    CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + stage, flag_dev, sizeof(int32_t),
-                         hipMemcpyDeviceToHost, c->stream));
-    c->flags_pending[stage] = true;
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
   int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
@@ -944,6 +954,16 @@ int ptg_get_subpix_evals_per_grain(void* p, int32_t* per_grain) {
   }
   CK(c, hipMemcpyAsync(per_grain, c->d_evals_g, sizeof(int32_t) * c->G, hipMemcpyDeviceToHost, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_get_exact_fallbacks(void* p, int64_t* searches, int64_t* grains) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!searches || !grains) return c->fail(PTG_ERR_ARG, "null output");
+  if (c->any_flags_pending()) TRY(c->sync_and_check());
+  *searches = c->exact_fallbacks;
+  *grains = c->exact_grain_fallbacks;
   return PTG_OK;
 }
 

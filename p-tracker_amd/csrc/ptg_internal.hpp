@@ -113,6 +113,8 @@ struct Context {
   void* exact_plan = nullptr;
   uint64_t exact_dev_key = 0;       // plan whose tables the scratch holds (0: none)
   uint64_t unmasked_version = 0;    // bumped whenever the unmasked list is rebuilt
+  int64_t exact_fallbacks = 0;      // exact-form searches that ran the plain form instead
+  int64_t exact_grain_fallbacks = 0;  // grains of exact-form searches served by the plain kernel
 
   // objective / sampler batches
   void* d_tmp = nullptr;
@@ -138,6 +140,10 @@ struct Context {
 hipError_t launch_begin_pair(Context& c);
 hipError_t launch_precompute(Context& c);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
+// The plain kernel over a device list of *sub_count slots of a's list (sub[j] < list_cap),
+// step k of slot li written to a.best[li * best_stride + k].
+hipError_t launch_search_plain_sub(Context& c, const SearchArgs& a, int list_cap, const int32_t* sub,
+                                   const int32_t* sub_count, int best_stride, size_t lds_bytes);
 // Reduces nparts partial argmaxes per list slot (global candidate index k * S + u * V + v,
 // first maximum wins) and applies the stage's update rules.
 // abort_flag (device, may be null): when non-zero the commit changes nothing.
@@ -148,7 +154,14 @@ hipError_t launch_end_pair(Context& c);
 hipError_t launch_change_reference(Context& c);
 hipError_t launch_build_unmasked(Context& c);
 
-// exact (MFMA) form: returns an error code of the C ABI, writes message into c.err.
+// Returned by run_search_exact, before it launches anything, when the search is beyond
+// the exact form's limits (frames above 12 bits, more than 48 rotation steps, a pattern
+// radius above 120, or a bucket whose window and pattern exceed the CU's LDS): the caller
+// then runs the plain form, whose picks are the same.
+constexpr int kExactUnfit = 1;
+
+// exact (MFMA) form: returns an error code of the C ABI (or kExactUnfit), writes message
+// into c.err.
 // Writes nparts partial argmaxes per list slot into out_best. The capacity flag (template
 // digit or Delta list overflow) is left in *flag_dev on the device; the caller passes it
 // to the commit and checks it at its next synchronisation.

@@ -9,6 +9,8 @@
 // order and applies the reference's update rules.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "ptg_device.hpp"
 #include "ptg_internal.hpp"
 
@@ -161,12 +163,10 @@ __device__ __forceinline__ bool better(double ncc, int idx, double best, int bid
 constexpr int kPlainThreads = 256;
 constexpr int kPlainCPT = 4;  // candidates per lane per round
 
-__global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
+// One (list slot li, step k): the workgroup's best candidate to out.
+__device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int k, StepBest* out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nsteps = a.nsteps;
-  const int li = blockIdx.x / nsteps;
-  const int k = blockIdx.x - li * nsteps;
-  if (li >= *a.count) return;
   const int g = a.list[li];
   const ptg_grain& gr = a.grains[g];
   const int64_t b = a.pbegin[g];
@@ -283,7 +283,28 @@ __global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
     sb.ncc = best;
     sb.idx = bidx == INT_MAX ? INT_MAX : k * S + bidx;  // global candidate index
     sb.pad = 0;
-    a.best[(int64_t)li * nsteps + k] = sb;
+    *out = sb;
+  }
+}
+
+__global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
+  const int li = blockIdx.x / a.nsteps;
+  const int k = blockIdx.x - li * a.nsteps;
+  if (li >= *a.count) return;
+  plain_slot_step(a, li, k, &a.best[(int64_t)li * a.nsteps + k]);
+}
+
+// The slots sub[0 .. *sub_count - 1] of the list; workgroup w takes step w % nsteps of
+// entries w / nsteps, w / nsteps + gridDim.x / nsteps, ...
+__global__ __launch_bounds__(kPlainThreads) void k_search_plain_sub(SearchArgs a, const int32_t* sub,
+                                                                    const int32_t* sub_count, int best_stride) {
+  const int nsteps = a.nsteps;
+  const int k = blockIdx.x % nsteps;
+  const int n = *sub_count;
+  for (int j = blockIdx.x / nsteps; j < n; j += gridDim.x / nsteps) {
+    const int li = sub[j];
+    plain_slot_step(a, li, k, &a.best[(int64_t)li * best_stride + k]);
+    __syncthreads();  // the next entry reuses the window, offsets and reduction slots
   }
 }
 
@@ -296,6 +317,20 @@ hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, si
     if (e != hipSuccess) return e;
   }
   k_search_plain<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_plain_sub(Context& c, const SearchArgs& a, int list_cap, const int32_t* sub,
+                                   const int32_t* sub_count, int best_stride, size_t lds_bytes) {
+  if (list_cap == 0) return hipSuccess;
+  // up to 256 entries in flight; a workgroup with no entry returns at once
+  const int64_t blocks = (int64_t)std::min(list_cap, 256) * a.nsteps;
+  if (lds_bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_search_plain_sub, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  k_search_plain_sub<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a, sub, sub_count, best_stride);
   return hipGetLastError();
 }
 

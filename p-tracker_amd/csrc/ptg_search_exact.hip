@@ -48,7 +48,7 @@ struct ExactSlot {
   int64_t mseg_off;  // first segment of the middle-step GEMM's segment table
   int64_t mmask_off; // first byte of the middle step's multiplicity grid (side x side)
   int32_t mkc;       // its K-chunks (4 segments each)
-  int32_t pad;
+  int32_t dcap;      // Delta entries per (step, sign), at most 1023
 };
 
 struct ExactArgs {
@@ -58,7 +58,7 @@ struct ExactArgs {
   int32_t nbucket;
   const int32_t* seg;          // segment table: (qy + rho) << 16 | c0
   uint8_t* tmpl;               // template fragments
-  int32_t* dlist;              // Delta cell offsets: per step dcap cells gained, then dcap cells lost
+  int32_t* dlist;              // Delta cell offsets: per step dcap cells gained, then dcap cells lost (per slot)
                                // (a cell whose multiplicity changes by 2 appears twice)
   int32_t* dlen;               // per (slot, step): cells gained, cells lost
   int64_t* mid;                // B_mid, C_mid per (slot, shift)
@@ -66,7 +66,7 @@ struct ExactArgs {
   int32_t* flags;              // [0]: template or Delta overflow
   StepBest* out;               // per (slot, shift block), part_cap entries per slot
   int32_t part_cap;
-  int32_t dcap;                // per sign; at most 255 so 32-bit sums of I1^2 stay exact
+  int32_t dcap;                // largest slot dcap of the search
   int32_t nmt;                 // M-tiles (8 steps each)
   int32_t nsub;                // shift blocks per grain
   int32_t nsu;                 // shift columns per block (= waves)
@@ -81,6 +81,11 @@ struct ExactArgs {
   int32_t mid_mkc_cap;         // largest mkc of the launch
   int32_t abuf_alias;          // epilogue buffer in the wave's staging area (one vertical pass)
   uint8_t* mmask;              // middle-step multiplicity grids (written by k_exact_mid)
+  int32_t mid_npl;             // digit planes staged per pass of k_exact_mid (6 or 3)
+  int32_t pad_mid;
+  int32_t* ovf_slot;           // per slot: 1 when its template or Delta lists overflowed (or null:
+                               // an overflow then raises flags[0] bit 0 and voids the stage)
+  int32_t* ovf_list;           // overflowed slots, flags[1] of them, searched by the plain kernel
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
     __shared__ int cnt[2];
     if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
     __syncthreads();
-    int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * e.dcap;
+    int32_t* dl = e.dlist + sl.dl_off + (int64_t)k * 2 * sl.dcap;
     const int ww = win_width(e.s.left + e.s.right + 1, sl.cw);  // plain window width of this slot
     for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
         const int d = ((packed[t] >> 24) & 0xff) - (int)cmid[t];
@@ -172,17 +177,22 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
         const int pos = atomicAdd(&cnt[sg], m);
         const int qy = t / side, qx = t - qy * side;  // both + rho
         for (int r = 0; r < m; r++)
-          if (pos + r < e.dcap) dl[sg * e.dcap + pos + r] = qy * ww + qx;
+          if (pos + r < sl.dcap) dl[sg * sl.dcap + pos + r] = qy * ww + qx;
       }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (cnt[0] > e.dcap || cnt[1] > e.dcap) overflow = true;
-      e.dlen[2 * ((int64_t)li * nsteps + k)] = min(cnt[0], e.dcap);
-      e.dlen[2 * ((int64_t)li * nsteps + k) + 1] = min(cnt[1], e.dcap);
+      if (cnt[0] > sl.dcap || cnt[1] > sl.dcap) overflow = true;
+      e.dlen[2 * ((int64_t)li * nsteps + k)] = min(cnt[0], sl.dcap);
+      e.dlen[2 * ((int64_t)li * nsteps + k) + 1] = min(cnt[1], sl.dcap);
     }
   }
-  if (overflow) atomicOr(&e.flags[0], 1);
+  if (overflow) {
+    // this grain goes to the plain kernel (its MFMA search returns at once); without that
+    // fallback the stage is void
+    if (!e.ovf_slot) atomicOr(&e.flags[0], 1);
+    else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
+  }
 }
 
 // ---- B_mid, C_mid for every shift and the reference moments -------------------------
@@ -194,8 +204,9 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
 //   B_mid = 128 acc_h + acc_l,  C_mid = sum_d 128^d acc_qd,
 // every product at most 4 * 127 and every sum exact in int32.
 // One workgroup per (slot, column block); wave w takes N-tiles w, w + 4, ...
-// LDS: 6 digit planes [WHm rows x stride] | mask rows [side x MLR] bytes (16 zero bytes on
-// the left) | segment table.
+// LDS: NPL digit planes [WHm rows x stride] | mask rows [side x MLR] bytes (16 zero bytes on
+// the left) | segment table. Large patterns stage the six planes in two passes; each
+// pass adds its planes' share of B_mid and C_mid to the values the same lane wrote before.
 constexpr int kMidPlanes = 6;
 
 // Row stride of the digit planes for cbw column blocks: an odd multiple of 16 bytes that
@@ -205,12 +216,13 @@ __host__ __device__ __forceinline__ int mid_stride(int rho, int cbw) {
   return ((w / 16) % 2 == 1) ? w : w + 16;
 }
 __host__ __device__ __forceinline__ int mid_mask_row(int rho) { return (2 * rho + 1 + 50 + 15) / 16 * 16; }
-__host__ __device__ __forceinline__ size_t mid_lds_bytes(int rho, int vp, int cbw, int mkc) {
+__host__ __device__ __forceinline__ size_t mid_lds_bytes(int rho, int vp, int cbw, int mkc, int npl) {
   const size_t whm = vp + 2 * rho;
-  return (size_t)kMidPlanes * whm * mid_stride(rho, cbw) + (size_t)(2 * rho + 1) * mid_mask_row(rho) +
+  return (size_t)npl * whm * mid_stride(rho, cbw) + (size_t)(2 * rho + 1) * mid_mask_row(rho) +
          (size_t)16 * mkc + 64;
 }
 
+template <int NPL>
 __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ long long red[2][16];
@@ -228,7 +240,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   const int su0 = 16 * cbw * blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   uint8_t* planes = smem;
-  uint8_t* mrow = smem + (size_t)kMidPlanes * WHm * stride;
+  uint8_t* mrow = smem + (size_t)NPL * WHm * stride;
   int32_t* segt = reinterpret_cast<int32_t*>(mrow + (size_t)side * MLR);
   const ptg_grain& gr = e.s.grains[g];
   const int64_t b = e.s.pbegin[g];
@@ -265,90 +277,104 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   // this slot's segment table (host built)
   const int nch = sl.mkc;
   for (int q = threadIdx.x; q < 4 * nch; q += blockDim.x) segt[q] = e.mseg[sl.mseg_off + q];
-  // the window's digit planes for columns su0 .. su0 + stride - 1
   const int X0 = gr.refcoord_xpix + (int)round(gr.dx) - e.s.left - rho + su0;
   const int Y0 = gr.refcoord_ypix + (int)round(gr.dy) - e.s.up - rho;
   const size_t psz = (size_t)WHm * stride;
-  for (int t = threadIdx.x; t < WHm * (stride / 4); t += blockDim.x) {
-    const int y = t / (stride / 4), x4 = (t - y * (stride / 4)) * 4;
-    const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
-    uint32_t w[kMidPlanes] = {0u, 0u, 0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
-      const uint32_t q = v * v;
-      const uint32_t dig[kMidPlanes] = {(v >> 7) & 127u, v & 127u, q & 127u, (q >> 7) & 127u, (q >> 14) & 127u,
-                                        (q >> 21) & 127u};
-#pragma unroll
-      for (int p = 0; p < kMidPlanes; p++) w[p] |= dig[p] << (8 * j);
-    }
-#pragma unroll
-    for (int p = 0; p < kMidPlanes; p++) *reinterpret_cast<uint32_t*>(planes + p * psz + (size_t)y * stride + x4) = w[p];
-  }
-  __syncthreads();
-  if (blockIdx.y == 0) {
-    // the middle step's multiplicities for the template kernel's Delta lists
-    for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
-      const int qy = t / side, qx = t - qy * side;
-      e.mmask[sl.mmask_off + t] = mrow[(size_t)qy * MLR + 16 + qx];
-    }
-  }
-  if (blockIdx.y == 0 && threadIdx.x == 0) {
-    long long A = 0, AA = 0;
-    for (int w = 0; w < nwave; w++) {
-      A += red[0][w];
-      AA += red[1][w];
-    }
-    e.mom[3 * li] = n;
-    e.mom[3 * li + 1] = A;
-    e.mom[3 * li + 2] = AA;
-  }
   const int m = lane & 15, gq = lane >> 4;
   const int ntile = vp >> 4;
-  for (int job = wave; job < ntile * cbw; job += nwave) {
-    const int nt = job % ntile, cbl = job / ntile;  // N-tile, column block within the workgroup
-    if (su0 + 16 * cbl >= U) continue;
-    v4i acc[kMidPlanes];
+  for (int p0 = 0; p0 < kMidPlanes; p0 += NPL) {
+    // the window's digit planes p0 .. p0 + NPL - 1 for columns su0 .. su0 + stride - 1
+    if (p0 > 0) __syncthreads();  // every wave is done with the previous pass's planes
+    for (int t = threadIdx.x; t < WHm * (stride / 4); t += blockDim.x) {
+      const int y = t / (stride / 4), x4 = (t - y * (stride / 4)) * 4;
+      const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
+      uint32_t w[kMidPlanes] = {0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int p = 0; p < kMidPlanes; p++) acc[p] = v4i{0, 0, 0, 0};
-    for (int ch = 0; ch < nch; ch++) {
-      const int sv = segt[4 * ch + gq];
-      v4i af = v4i{0, 0, 0, 0};
-      int rowoff = 0;
-      if (!(sv & 0x8000)) {
-        const int qy = sv >> 16, c0 = sv & 0x7fff;
-        // A: bytes M(c0 + j - m, qy) = mask row byte 16 + c0 - m + j: five aligned words,
-        // realigned per lane
-        const int bs = 16 + c0 - m;
-        const uint32_t* wp = reinterpret_cast<const uint32_t*>(mrow + (size_t)qy * MLR + (bs & ~3));
-        const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3], w4 = wp[4];
-        const uint32_t sh = (uint32_t)(bs & 3);
-        af[0] = (int)__builtin_amdgcn_alignbyte(w1, w0, sh);
-        af[1] = (int)__builtin_amdgcn_alignbyte(w2, w1, sh);
-        af[2] = (int)__builtin_amdgcn_alignbyte(w3, w2, sh);
-        af[3] = (int)__builtin_amdgcn_alignbyte(w4, w3, sh);
-        rowoff = qy * stride + 16 * cbl + c0;
+      for (int j = 0; j < 4; j++) {
+        const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
+        const uint32_t q = v * v;
+        const uint32_t dig[kMidPlanes] = {(v >> 7) & 127u, v & 127u, q & 127u, (q >> 7) & 127u, (q >> 14) & 127u,
+                                          (q >> 21) & 127u};
+#pragma unroll
+        for (int p = 0; p < kMidPlanes; p++) w[p] |= dig[p] << (8 * j);
       }
-      // B: plane p, row (16 nt + col) + qy, bytes c0 .. c0 + 15 (empty segments read row 0)
-      const size_t bo = (size_t)(16 * nt + m) * stride + rowoff;
 #pragma unroll
-      for (int p = 0; p < kMidPlanes; p++) {
-        const v4i bf = *reinterpret_cast<const v4i*>(planes + p * psz + bo);
-        acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[p], 0, 0, 0);
+      for (int p = 0; p < NPL; p++)
+        *reinterpret_cast<uint32_t*>(planes + p * psz + (size_t)y * stride + x4) = w[p0 + p];
+    }
+    __syncthreads();
+    if (p0 == 0 && blockIdx.y == 0) {
+      // the middle step's multiplicities for the template kernel's Delta lists
+      for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
+        const int qy = t / side, qx = t - qy * side;
+        e.mmask[sl.mmask_off + t] = mrow[(size_t)qy * MLR + 16 + qx];
+      }
+      if (threadIdx.x == 0) {
+        long long A = 0, AA = 0;
+        for (int w = 0; w < nwave; w++) {
+          A += red[0][w];
+          AA += red[1][w];
+        }
+        e.mom[3 * li] = n;
+        e.mom[3 * li + 1] = A;
+        e.mom[3 * li + 2] = AA;
       }
     }
-    // lane holds rows 4 gq + i (columns su0 + 16 cbl + 4 gq + i) at vertical shift 16 nt + m
-    const int svv = 16 * nt + m;
+    for (int job = wave; job < ntile * cbw; job += nwave) {
+      const int nt = job % ntile, cbl = job / ntile;  // N-tile, column block within the workgroup
+      if (su0 + 16 * cbl >= U) continue;
+      v4i acc[NPL];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int su = su0 + 16 * cbl + 4 * gq + i;
-      if (su < U && svv < V) {
-        const long long Bv = (long long)acc[0][i] * 128 + acc[1][i];
-        const long long Cv = (long long)acc[2][i] + ((long long)acc[3][i] << 7) + ((long long)acc[4][i] << 14) +
-                             ((long long)acc[5][i] << 21);
-        const int64_t t = (int64_t)su * V + svv;
-        e.mid[sl.mid_off + 2 * t] = Bv;
-        e.mid[sl.mid_off + 2 * t + 1] = Cv;
+      for (int p = 0; p < NPL; p++) acc[p] = v4i{0, 0, 0, 0};
+      for (int ch = 0; ch < nch; ch++) {
+        const int sv = segt[4 * ch + gq];
+        v4i af = v4i{0, 0, 0, 0};
+        int rowoff = 0;
+        if (!(sv & 0x8000)) {
+          const int qy = sv >> 16, c0 = sv & 0x7fff;
+          // A: bytes M(c0 + j - m, qy) = mask row byte 16 + c0 - m + j: five aligned words,
+          // realigned per lane
+          const int bs = 16 + c0 - m;
+          const uint32_t* wp = reinterpret_cast<const uint32_t*>(mrow + (size_t)qy * MLR + (bs & ~3));
+          const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3], w4 = wp[4];
+          const uint32_t sh = (uint32_t)(bs & 3);
+          af[0] = (int)__builtin_amdgcn_alignbyte(w1, w0, sh);
+          af[1] = (int)__builtin_amdgcn_alignbyte(w2, w1, sh);
+          af[2] = (int)__builtin_amdgcn_alignbyte(w3, w2, sh);
+          af[3] = (int)__builtin_amdgcn_alignbyte(w4, w3, sh);
+          rowoff = qy * stride + 16 * cbl + c0;
+        }
+        // B: plane p, row (16 nt + col) + qy, bytes c0 .. c0 + 15 (empty segments read row 0)
+        const size_t bo = (size_t)(16 * nt + m) * stride + rowoff;
+#pragma unroll
+        for (int p = 0; p < NPL; p++) {
+          const v4i bf = *reinterpret_cast<const v4i*>(planes + p * psz + bo);
+          acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[p], 0, 0, 0);
+        }
+      }
+      // lane holds rows 4 gq + i (columns su0 + 16 cbl + 4 gq + i) at vertical shift 16 nt + m
+      const int svv = 16 * nt + m;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int su = su0 + 16 * cbl + 4 * gq + i;
+        if (su < U && svv < V) {
+          long long Bv = 0, Cv = 0;
+#pragma unroll
+          for (int p = 0; p < NPL; p++) {
+            const int pl = p0 + p;
+            const long long a = acc[p][i];
+            if (pl == 0) Bv += a * 128;
+            else if (pl == 1) Bv += a;
+            else Cv += a << (7 * (pl - 2));
+          }
+          const int64_t t = (int64_t)su * V + svv;
+          if (p0 > 0) {
+            Bv += e.mid[sl.mid_off + 2 * t];
+            Cv += e.mid[sl.mid_off + 2 * t + 1];
+          }
+          e.mid[sl.mid_off + 2 * t] = Bv;
+          e.mid[sl.mid_off + 2 * t + 1] = Cv;
+        }
       }
     }
   }
@@ -357,10 +383,12 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 // ---- the MFMA search ---------------------------------------------------------------
 // Sums of I1 and I1^2 over one Delta list (ks = 2 step + sign) at this lane's shift: the
 // entries held in LDS (4 per 16-byte read, 8 in flight), then any beyond the LDS
-// capacity from global memory. 32-bit sums stay exact: a list has at most 255 cells.
+// capacity from global memory. A list has at most 1023 cells, so the sum of I1 is exact in
+// 32 bits; I1^2 is summed in 32 bits per group of at most 8 cells, then in 64.
 __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
-                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int ks,
-                                           uint32_t wbase, int base, unsigned int& bsum, unsigned int& csum) {
+                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
+                                           int ks, uint32_t wbase, int base, unsigned int& bsum,
+                                           unsigned long long& csum) {
   const int n = dcnt[ks], o = dofs[ks];
   const int nl = max(0, min(n, e.dl_lds_cap - o));
   const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
@@ -372,7 +400,7 @@ __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* w
     const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
     const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
     bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
-    csum += ((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7));
+    csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
   }
   for (; q < nl; q += 4) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
@@ -381,9 +409,9 @@ __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* w
     const unsigned int w2 = q + 2 < nl ? px(e0.z) : 0u;
     const unsigned int w3 = q + 3 < nl ? px(e0.w) : 0u;
     bsum += (w0 + w1) + (w2 + w3);
-    csum += (w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3);
+    csum += (unsigned int)((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3));
   }
-  const int32_t* dg = e.dlist + dl_off + (int64_t)ks * e.dcap;
+  const int32_t* dg = e.dlist + dl_off + (int64_t)ks * dcap;
   for (int r = nl; r < n; r++) {
     const unsigned int w = win[base + dg[r]];
     bsum += w;
@@ -406,6 +434,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   if (bi >= e.nbucket) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
+  if (e.ovf_slot && e.ovf_slot[li]) return;  // served by the plain kernel
   const int g = e.s.list[li];
   const ExactSlot sl = e.slots[li];
   const int rho = sl.rho, cw = sl.cw, cs = sl.cs, kc = sl.kc;
@@ -468,8 +497,8 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     if (lane == 0) dofs[nls] = run;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < nls * e.dcap; t += nthreads) {
-    const int ks = t / e.dcap, q = t - ks * e.dcap;
+  for (int t = threadIdx.x; t < nls * sl.dcap; t += nthreads) {
+    const int ks = t / sl.dcap, q = t - ks * sl.dcap;
     const int pos = dofs[ks] + q;
     if (q < dcnt[ks] && pos < e.dl_lds_cap) dent[pos] = 2u * (uint32_t)e.dlist[sl.dl_off + t];
   }
@@ -590,9 +619,10 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         for (int kl = 0; kl < 8; kl++) {
           const int k = 8 * a + kl;
           if (k >= nsteps) break;
-          unsigned int bp = 0u, bn = 0u, cp = 0u, cn = 0u;
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, 2 * k, wbase, base, bp, cp);
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, 2 * k + 1, wbase, base, bn, cn);
+          unsigned int bp = 0u, bn = 0u;
+          unsigned long long cp = 0ull, cn = 0ull;
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, 2 * k, wbase, base, bp, cp);
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, 2 * k + 1, wbase, base, bn, cn);
           if (lane_ok) {
             // every quantity is an integer below 2^53, so these double operations are exact
             const double A = abuf[kl * (NTP * 16) + lane];
@@ -707,6 +737,16 @@ static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid,
 }
 
 namespace {
+// Launch shapes of one radius bucket.
+struct BucketLaunch {
+  int cbw = 0, npl = 0;          // k_exact_mid: column blocks per workgroup, planes per pass (6, 3)
+  size_t lds_m = 0, lds_t = 0;   // LDS of k_exact_mid, k_exact_template
+  int cs = 0, ww = 0, wh = 0;    // k_search_exact: staging stride, window width and height
+  int dl_cap = 0, nsub = 0, nsu = 0;
+  bool alias = false;
+  size_t lds = 0;
+};
+
 struct Bucket {
   int rho_max = 0;
   int max_kc = 0;
@@ -721,6 +761,7 @@ struct ExactPlan {
   uint64_t key = 0;
   int nlist = 0, U = 0, V = 0, nsteps = 0;
   int nmt = 0, vp = 0, dcap = 0;
+  int64_t dl_total = 0;
   std::vector<ExactSlot> slots;
   std::vector<int32_t> segs;
   std::vector<int32_t> msegs;
@@ -728,6 +769,7 @@ struct ExactPlan {
   std::vector<int32_t> bucket_cat;
   size_t b_mseg = 0, b_mmask = 0;
   size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
+  size_t b_ovf = 0;
   size_t total = 0;
   void* dev_base = nullptr;  // scratch the tables were last uploaded to
 };
@@ -786,10 +828,25 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     while ((P.msegs.size() - s.mseg_off) % 4) P.msegs.push_back(0x8000);
     s.mkc = (int)((P.msegs.size() - s.mseg_off) / 4);
   }
-  P.dcap = std::min(255, 4 * rho_max + 64);
-  for (int li = 0; li < nlist; li++) P.slots[li].dl_off = (int64_t)li * nsteps * 2 * P.dcap;
+  // Delta capacity per (step, sign): about 4 rho + 64 cells near a base rotation of zero;
+  // away from zero the rounded rotated disk has duplicate and empty cells whose positions
+  // move with the step, and the lists grow with the pattern and the rotation (a radius-60
+  // disk, steps within 0.04 rad: 344 cells at 0.1 rad, 944 at 0.2 rad). At most 1023
+  // cells; a longer list sends the grain to the plain kernel
+  P.dcap = 0;
+  int64_t dl_total = 0;
+  for (int li = 0; li < nlist; li++) {
+    ExactSlot& s = P.slots[li];
+    const int g = list[li];
+    const int64_t n = c.h_pbegin[g + 1] - c.h_pbegin[g];
+    s.dcap = (int32_t)std::min<int64_t>(1023, std::max<int64_t>(std::min(255, 4 * s.rho + 64), n / 8 + 64));
+    s.dl_off = dl_total;
+    dl_total += (int64_t)nsteps * 2 * s.dcap;
+    P.dcap = std::max(P.dcap, (int)s.dcap);
+  }
+  P.dl_total = dl_total;
   // buckets by rho so each launch reserves LDS for its own grains
-  const int bounds[] = {13, 21, 29, 37, 45, 1 << 20};
+  const int bounds[] = {13, 21, 29, 37, 45, 53, 61, 69, 1 << 20};
   P.buckets.assign(sizeof(bounds) / sizeof(bounds[0]), Bucket{});
   for (int li = 0; li < nlist; li++) {
     int bk = 0;
@@ -808,28 +865,29 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   P.b_mseg = al(sizeof(int32_t) * P.msegs.size());
   P.b_bucket = al(sizeof(int32_t) * nlist);
   P.b_tmpl = al((size_t)tmpl_total);
-  P.b_dl = al(sizeof(int32_t) * (size_t)nlist * nsteps * 2 * P.dcap);
+  P.b_dl = al(sizeof(int32_t) * (size_t)P.dl_total);
   P.b_dlen = al(sizeof(int32_t) * 2 * (size_t)nlist * nsteps);
   P.b_mid = al(sizeof(int64_t) * (size_t)mid_total);
   P.b_mom = al(sizeof(int64_t) * 3 * nlist);
   P.b_mmask = al((size_t)mmask_total);
   P.b_flags = al(sizeof(int32_t) * 4);
-  P.total = P.b_mmask + P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom + P.b_flags;
+  P.b_ovf = al(sizeof(int32_t) * 2 * (size_t)nlist);  // per-slot overflow flags, then the overflow list
+  P.total = P.b_mmask + P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom +
+            P.b_flags + P.b_ovf;
   P.dev_base = nullptr;
 }
 
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
                      uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
                      bool time_kernel) {
+  // frames of at most 12 bits (7-bit digits of I1 and I1^2), at most 48 rotation steps
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
-  if (i0.bits > 12 || i1.bits > 12)
-    return c.fail(PTG_ERR_LIMIT, "the exact search takes frames of at most 12 bits (got %d/%d); use PTG_SEARCH_PLAIN",
-                  i0.bits, i1.bits);
+  if (i0.bits > 12 || i1.bits > 12) return kExactUnfit;
   const int nsteps = sa.nsteps;
   const int nmt = (2 * nsteps + 15) / 16;
-  if (nmt > 6) return c.fail(PTG_ERR_LIMIT, "the exact search takes at most 48 rotation steps (got %d)", nsteps);
+  if (nmt > 6) return kExactUnfit;
   const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1;
-  if ((int64_t)nsteps * U * V >= INT_MAX / 2) return c.fail(PTG_ERR_LIMIT, "too many candidates");
+  if ((int64_t)nsteps * U * V >= INT_MAX / 2) return kExactUnfit;
   const int ntp = nmt <= 3 ? 3 : 2;
   // the host plan: cached for a keyed list, else built for this call
   ExactPlan local;
@@ -852,7 +910,66 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   }
   for (const Bucket& bk : P->buckets)
     if (!bk.slots.empty() && bk.rho_max > 120)  // pattern words hold offsets as int8
-      return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", bk.rho_max);
+      return kExactUnfit;
+  // launch shapes of every bucket, all checked against the CU's LDS before anything runs
+  std::vector<BucketLaunch> L(P->buckets.size());
+  for (size_t bi = 0; bi < P->buckets.size(); bi++) {
+    const Bucket& bk = P->buckets[bi];
+    const int nb = (int)bk.slots.size();
+    if (nb == 0) continue;
+    BucketLaunch& l = L[bi];
+    const int rho = bk.rho_max, side = 2 * rho + 1;
+    const int cw = (side + 15) / 16 * 16;
+    l.cs = ((cw / 16) % 2 == 1) ? cw : cw + 16;
+    // middle-step window sums B_mid, C_mid and the reference moments: as many 16-column
+    // blocks per workgroup as keep two workgroups per CU (at least one block); all six
+    // digit planes in one pass when they fit the CU's LDS, else two passes of 3 (a
+    // pattern too large for that is too large for the MFMA kernel's window as well)
+    const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
+    l.cbw = ncb;
+    l.npl = kMidPlanes;
+    while (l.cbw > 1 && (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > 80 * 1024 || ntile_m * l.cbw > 16))
+      l.cbw--;
+    if (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > 160 * 1024) l.npl = 3;
+    l.lds_m = mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl);
+    // templates and Delta lists: the step's packed template and the middle step's grid
+    l.lds_t = (size_t)side * side * 5;
+    // the MFMA search
+    l.ww = win_width(U, cw);
+    l.wh = P->vp + 2 * rho;
+    const size_t win_bytes = ((size_t)l.ww * l.wh * 2 + 15) / 16 * 16;
+    const size_t stage_bytes = ((size_t)2 * l.wh * l.cs + 15) / 16 * 16;
+    const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
+    l.alias = P->vp / 16 <= ntp && abuf_bytes <= stage_bytes;
+    const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
+    // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
+    // 16-byte read); typical lists total about 1.5 rho entries per rotation step
+    const int nls = 2 * nsteps;
+    l.dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
+    const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + l.dl_cap + 4) * 4;
+    auto need = [&](int w) {
+      return win_bytes + w * (stage_bytes + (l.alias ? 0 : abuf_bytes)) + seg_bytes + dl_bytes;
+    };
+    // shift columns split over nsub workgroups when the bucket has few grains
+    l.nsub = std::max(1, std::min(U, (512 + nb - 1) / nb));
+    const int cpb = (U + l.nsub - 1) / l.nsub;
+    // waves per workgroup: the most waves resident per CU under the register limit (a
+    // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
+    const int vgpr = main_vgprs(nmt, ntp);
+    const int wave_cap = 4 * std::max(1, 512 / std::max(vgpr, 1));
+    int best_res = 0;
+    l.nsu = 1;
+    for (int w = 1; w <= std::min(8, cpb); w++) {
+      if (need(w) > 160 * 1024) break;
+      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);
+      if (res >= best_res) {
+        best_res = res;
+        l.nsu = w;
+      }
+    }
+    l.lds = need(l.nsu);
+    if (l.lds_m > 160 * 1024 || l.lds_t > 160 * 1024 || l.lds > 160 * 1024) return kExactUnfit;
+  }
   int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
   if (rc != PTG_OK) return rc;
   char* base = (char*)c.d_exact_scratch;
@@ -881,6 +998,14 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   e.mmask = (uint8_t*)base;
   base += P->b_mmask;
   e.flags = (int32_t*)base;
+  base += P->b_flags;
+  e.ovf_slot = (int32_t*)base;
+  e.ovf_list = e.ovf_slot + nlist;
+  // grains whose lists overflow are searched by the plain kernel when its LDS takes the
+  // list's largest window and pattern (else an overflow voids the stage)
+  const size_t lds_plain = ((size_t)sa.win_cap * 2 + 15) / 16 * 16 + (size_t)sa.off_cap * 4;
+  const bool plain_ok = lds_plain <= 160 * 1024;
+  if (!plain_ok) e.ovf_slot = e.ovf_list = nullptr;
   e.dcap = P->dcap;
   e.nmt = nmt;
   e.vp = P->vp;
@@ -909,102 +1034,71 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   }
   k_zero_i32<<<1, 64, 0, c.stream>>>(e.flags, 4);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_zero_i32");
-  // shift blocks: results of block b of slot li go to out_best[li * U + b], the rest stay
-  // "none"
+  if (e.ovf_slot) {
+    k_zero_i32<<<(nlist + 255) / 256, 256, 0, c.stream>>>(e.ovf_slot, nlist);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_zero_i32");
+  }
+  // shift blocks: results of block b of slot li go to out_best[li * part_cap + b]; the
+  // plain kernel writes step k of an overflowed slot to out_best[li * part_cap + k]; the
+  // rest stay "none"
+  const int part_cap = std::max(U, nsteps);
   e.out = out_best;
-  e.part_cap = U;  // at most U shift blocks per slot; unused parts stay "none"
-  *nparts_out = U;
-  k_fill_best<<<(nlist * U + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * U);
+  e.part_cap = part_cap;
+  *nparts_out = part_cap;
+  k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
   size_t off = 0;
-  for (auto& bk : P->buckets) {
+  for (size_t bi = 0; bi < P->buckets.size(); bi++) {
+    const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
+    const BucketLaunch& l = L[bi];
     e.bucket = bucket_dev + off;
     e.nbucket = nb;
     off += nb;
-    const int rho = bk.rho_max, side = 2 * rho + 1;
-    const int cw = (side + 15) / 16 * 16, cs = ((cw / 16) % 2 == 1) ? cw : cw + 16;
-    // middle-step window sums B_mid, C_mid and the reference moments: as many 16-column
-    // blocks per workgroup as keep two workgroups per CU (at least one block)
-    const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
-    int cbw = ncb;
-    while (cbw > 1 && (mid_lds_bytes(rho, P->vp, cbw, bk.max_mkc) > 80 * 1024 || ntile_m * cbw > 16)) cbw--;
-    const size_t lds_m = mid_lds_bytes(rho, P->vp, cbw, bk.max_mkc);
-    if (lds_m > 160 * 1024)
-      return c.fail(PTG_ERR_LIMIT, "search zone and pattern too large for the exact search (%zu B LDS)", lds_m);
-    if (lds_m > 65536 &&
-        (he = hipFuncSetAttribute((const void*)k_exact_mid, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_m)) != hipSuccess)
-      return c.check(he, "hipFuncSetAttribute");
+    const int rho = bk.rho_max;
     {
+      const void* fm = l.npl == 6 ? (const void*)k_exact_mid<6> : (const void*)k_exact_mid<3>;
+      if (l.lds_m > 65536 &&
+          (he = hipFuncSetAttribute(fm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_m)) != hipSuccess)
+        return c.check(he, "hipFuncSetAttribute");
       ExactArgs em = e;
-      em.mid_cbw = cbw;
-      em.mid_stride = mid_stride(rho, cbw);
+      em.mid_cbw = l.cbw;
+      em.mid_stride = mid_stride(rho, l.cbw);
       em.mid_mkc_cap = bk.max_mkc;
-      const int nwave_m = std::min(16, ntile_m * cbw);
-      k_exact_mid<<<dim3(nb, (ncb + cbw - 1) / cbw), 64 * nwave_m, lds_m, c.stream>>>(em);
+      em.mid_npl = l.npl;
+      const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
+      const dim3 grid_m(nb, (ncb + l.cbw - 1) / l.cbw), block_m(64 * std::min(16, ntile_m * l.cbw));
+      if (l.npl == 6) k_exact_mid<6><<<grid_m, block_m, l.lds_m, c.stream>>>(em);
+      else k_exact_mid<3><<<grid_m, block_m, l.lds_m, c.stream>>>(em);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     }
-    // templates and Delta lists (against the middle step's grid from k_exact_mid)
-    const size_t lds_t = (size_t)side * side * 5;
-    if (lds_t > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "pattern radius %d too large for the exact search", rho);
-    if (lds_t > 65536 &&
+    if (l.lds_t > 65536 &&
         (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_t)) != hipSuccess)
+                                  (int)l.lds_t)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
-    k_exact_template<<<dim3(nb, 8 * nmt), 256, lds_t, c.stream>>>(e);
+    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, c.stream>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
-    // the MFMA search
-    const int WW = win_width(U, cw), WH = P->vp + 2 * rho;
-    e.ww = WW;
-    e.wh_pad = WH;
-    const size_t win_bytes = ((size_t)WW * WH * 2 + 15) / 16 * 16;
-    const size_t stage_bytes = ((size_t)2 * WH * cs + 15) / 16 * 16;
-    const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
-    const bool alias = P->vp / 16 <= ntp && abuf_bytes <= stage_bytes;
-    const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
-    // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
-    // 16-byte read); typical lists total about 1.5 rho entries per rotation step
-    const int nls = 2 * nsteps;
-    const int dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
-    const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + dl_cap + 4) * 4;
-    auto need = [&](int w) {
-      return win_bytes + w * (stage_bytes + (alias ? 0 : abuf_bytes)) + seg_bytes + dl_bytes;
-    };
-    // shift columns split over nsub workgroups when the bucket has few grains; waves per
-    // workgroup as many as keep two workgroups per CU (else one)
-    const int nsub = std::max(1, std::min(U, (512 + nb - 1) / nb));
-    const int cpb = (U + nsub - 1) / nsub;
-    // waves per workgroup: the most waves resident per CU under the register limit (a
-    // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
-    const int vgpr = main_vgprs(nmt, ntp);
-    const int wave_cap = 4 * std::max(1, 512 / std::max(vgpr, 1));
-    int nsu_b = 1, best_res = 0;
-    for (int w = 1; w <= std::min(8, cpb); w++) {
-      if (need(w) > 160 * 1024) break;
-      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);
-      if (res >= best_res) {
-        best_res = res;
-        nsu_b = w;
-      }
-    }
-    if (need(nsu_b) > 160 * 1024) return c.fail(PTG_ERR_LIMIT, "search window too large for the exact search");
     ExactArgs eb = e;
-    eb.nsub = nsub;
-    eb.nsu = nsu_b;
-    eb.cs_max = cs;
-    eb.dl_lds_cap = dl_cap;
+    eb.ww = l.ww;
+    eb.wh_pad = l.wh;
+    eb.nsub = l.nsub;
+    eb.nsu = l.nsu;
+    eb.cs_max = l.cs;
+    eb.dl_lds_cap = l.dl_cap;
     eb.seg_cap = bk.max_kc;
-    eb.abuf_alias = alias ? 1 : 0;
-    const size_t lds = need(eb.nsu);
+    eb.abuf_alias = l.alias ? 1 : 0;
     const int groups = (nb + 7) / 8;
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
     hipEvent_t ev = nullptr;
     if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev);
-    if ((he = dispatch_main(nmt, ntp, eb, grid, block, lds, c.stream)) != hipSuccess)
+    if ((he = dispatch_main(nmt, ntp, eb, grid, block, l.lds, c.stream)) != hipSuccess)
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev);
+  }
+  if (e.ovf_slot) {
+    if ((he = launch_search_plain_sub(c, sa, nlist, e.ovf_list, e.flags + 1, part_cap, lds_plain)) != hipSuccess)
+      return c.check(he, "k_search_plain_sub");
   }
   // a template digit or Delta list overflow leaves a flag: the commit then changes nothing
   // and the caller reports PTG_ERR_LIMIT at its next synchronisation

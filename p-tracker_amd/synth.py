@@ -263,3 +263,98 @@ def distort_frame(img, X, bits):
     xu, yu = xx + dx * k, yy + dy * k
     out = cubic_sample(img.astype(np.float64), xu, yu)
     return np.clip(np.round(out), 0, (1 << bits) - 1).astype(np.uint16)
+
+
+class SynthSequence:
+    """A frame sequence: frames[k] (uint16, H x W), the grain records of frame 0, the
+    patterns, and the true centre and rotation of every grain in every frame."""
+
+    def __init__(self, frames, grains, pattern_xy, pattern_begin, bits, centres, rots, params, name):
+        self.frames, self.grains = frames, grains
+        self.pattern_xy, self.pattern_begin = pattern_xy, pattern_begin
+        self.bits, self.centres, self.rots, self.params, self.name = bits, centres, rots, params, name
+
+    @property
+    def G(self):
+        return len(self.grains)
+
+
+def texture_sheet(rng, W, H, bits, sigma=1.5, mean_frac=0.5, std_frac=0.17):
+    """Gaussian-filtered white noise over the frame, scaled to the bit depth: the speckle
+    each grain carries in its own body frame (grain g shows the sheet about its frame-0
+    centre)."""
+    from scipy.ndimage import gaussian_filter
+
+    maxv = float((1 << bits) - 1)
+    s = gaussian_filter(rng.standard_normal((H, W)).astype(np.float32), sigma)
+    s *= np.float32(std_frac * maxv / max(float(s.std()), 1e-12))
+    s += np.float32(mean_frac * maxv)
+    return s
+
+
+def _bilinear(img, x, y):
+    x0 = np.floor(x).astype(np.int64)
+    y0 = np.floor(y).astype(np.int64)
+    tx, ty = (x - x0).astype(np.float32), (y - y0).astype(np.float32)
+    H, W = img.shape
+    x0 = np.clip(x0, 0, W - 2)
+    y0 = np.clip(y0, 0, H - 2)
+    a, b = img[y0, x0], img[y0, x0 + 1]
+    c, d = img[y0 + 1, x0], img[y0 + 1, x0 + 1]
+    return (a + (b - a) * tx) * (1 - ty) + (c + (d - c) * tx) * ty
+
+
+def render_sequence_frame(sheet, centres0, centres, rots, radii, order, bits, background, noise_sigma, rng):
+    """One frame: every grain drawn as an opaque disk at its centre and rotation, in
+    `order` (later disks cover earlier ones). Frame pixel x of grain g shows the sheet at
+    centres0[g] + R(rot) (x - centres[g]), R(t) = [[cos t, -sin t], [sin t, cos t]], which
+    is the motion the tracker's rotation (c x + s y, -s x + c y) undoes."""
+    H, W = sheet.shape
+    f = np.full((H, W), np.float32(background), dtype=np.float32)
+    for g in order:
+        r = radii[g]
+        cx, cy = centres[g]
+        x0, x1 = max(0, int(np.floor(cx - r))), min(W - 1, int(np.ceil(cx + r)))
+        y0, y1 = max(0, int(np.floor(cy - r))), min(H - 1, int(np.ceil(cy + r)))
+        yy, xx = np.mgrid[y0:y1 + 1, x0:x1 + 1]
+        px, py = xx - cx, yy - cy
+        m = px * px + py * py <= r * r
+        px, py = px[m], py[m]
+        ct, st = math.cos(rots[g]), math.sin(rots[g])
+        qx = centres0[g, 0] + ct * px - st * py
+        qy = centres0[g, 1] + st * px + ct * py
+        f[yy[m], xx[m]] = _bilinear(sheet, qx, qy)
+    f += rng.normal(0.0, noise_sigma, size=f.shape).astype(np.float32)
+    return np.clip(np.round(f), 0, (1 << bits) - 1).astype(np.uint16)
+
+
+def sequence(n_frames=16, n_grains=10000, size=(6000, 4000), seed=1005, radius=(6.0, 60.0), max_step=5.0,
+             max_rot=0.03, margin=160, bits=12, noise_sigma=8.0, search_method=SEARCH_PLAIN,
+             subpix_method=SUBPIX_POWELL) -> SynthSequence:
+    """Workload 5, the batching form over a sequence: n_frames frames of W x H with
+    n_grains disks of radius U(radius), each moved per frame by U(-max_step, max_step) px
+    and U(-max_rot, max_rot) rad, accumulated. At the full size the disks cover 1.74 times
+    the frame area (mean area pi E[r^2] = 4184 px^2), so they cannot be disjoint: they are
+    drawn largest first, each over the larger ones, in every frame. Tracker: consecutive
+    pairs (the reference swaps the reference frame after each pair), search +-8 px x 9
+    steps of 0.01 rad (num_rot 4), no rescue, subpixel on."""
+    rng = np.random.default_rng(seed)
+    W, H = size
+    G = n_grains
+    radii = rng.uniform(radius[0], radius[1], G)
+    m = min(margin, (min(W, H) - 1) // 4)
+    c0 = np.column_stack([rng.uniform(m, W - 1 - m, G), rng.uniform(m, H - 1 - m, G)])
+    steps = rng.uniform(-max_step, max_step, (n_frames - 1, G, 2))
+    rsteps = rng.uniform(-max_rot, max_rot, (n_frames - 1, G))
+    centres = np.concatenate([c0[None], c0[None] + np.cumsum(steps, axis=0)], axis=0)
+    rots = np.concatenate([np.zeros((1, G)), np.cumsum(rsteps, axis=0)], axis=0)
+    sheet = texture_sheet(rng, W, H, bits)
+    order = np.argsort(-radii, kind="stable")
+    frames = [render_sequence_frame(sheet, c0, centres[k], rots[k], radii, order, bits, 160.0, noise_sigma, rng)
+              for k in range(n_frames)]
+    grains = make_grains(c0, radii)
+    pxy, pb = make_patterns(radii)
+    z = zone(8, num_rot=4, inc_rot=0.01)
+    prm = track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=search_method,
+                       subpix=default_subpix(subpix_method))
+    return SynthSequence(frames, grains, pxy, pb, bits, centres, rots, prm, "cfg5")

@@ -324,11 +324,124 @@ def test_cfg3_exact_40_steps_picks_equal_oracle(tracker, bits):
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
 
 
-def test_exact_rejects_16bit_frames(tracker, cfg1):
-    say("exact search refuses 16-bit frames")
+def test_exact_falls_back_to_plain_on_16bit_frames(tracker, cfg1):
+    # 16-bit frames are beyond the exact form's 7-bit digits: the search runs the plain
+    # form, whose picks and NCC are the reference's
+    say("exact search on 16-bit frames runs the plain form")
     tracker.set_image(0, cfg1.I0, 16)
     tracker.set_image(1, cfg1.I1, 16)
     tracker.set_image_indices(0, 1)
     tracker.set_grains(cfg1.grains, cfg1.pattern_xy, cfg1.pattern_begin)
-    with pytest.raises(pk.PtgError):
-        tracker.search(pk.STAGE_MAIN, cfg1.params.search_zone, 0.7, 0.5, pk.SEARCH_EXACT)
+    n0 = tracker.exact_fallbacks()
+    z = cfg1.params.search_zone
+    tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT))
+    got = tracker.get_grains()
+    assert tracker.exact_fallbacks()[0] == n0[0] + 1
+    ref, _, _ = ptref.track_pair(oracle_workload(cfg1), pk.track_params(z, rescue_level=0, subpixel=0), nthreads=4)
+    for col in INT_COLS:
+        assert bits_equal(got[col], ref[col]), col
+
+
+# ---- frame sequences (batching form) -------------------------------------------------------
+def _sequence_vs_oracle(tracker, S, prm, oprm, pairs):
+    """Tracks pairs 0 .. pairs-1 of sequence S on the GPU (reference frame swapped after
+    each pair) and with the oracle from its own previous output; yields both per pair."""
+    tracker.set_image(0, S.frames[0], S.bits)
+    tracker.set_image(1, S.frames[1], S.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(S.grains, S.pattern_xy, S.pattern_begin)
+    gin = S.grains.copy()
+    for k in range(pairs):
+        if k > 0:
+            tracker.change_reference()
+            tracker.set_image((k + 1) % 2, S.frames[k + 1], S.bits)
+        tracker.track_pair(prm)
+        got = tracker.get_grains()
+        Wk = ptref.Workload(S.frames[k], S.frames[k + 1], gin, S.pattern_xy, S.pattern_begin, S.bits)
+        ref, _, _ = ptref.track_pair(Wk, oprm, nthreads=8)
+        yield k, got, ref
+        gin = ptref.change_reference(ref)
+
+
+@pytest.mark.parametrize("case", ["r50-64_s8", "r78-82_s2"])
+def test_sequence_large_radii_exact_picks_equal_oracle(tracker, case):
+    # patterns of radius 50-64 at +-8 px and 78-82 at +-2 px (near the largest the MFMA
+    # kernel's LDS takes): the middle-step kernel stages its six digit planes in two
+    # passes; three pairs with the reference swapped in between, so refrot != 0 from the
+    # second pair on
+    if case == "r50-64_s8":
+        S = synth.sequence(n_frames=4, n_grains=8, size=(700, 600), seed=505, radius=(50.0, 64.0), margin=150)
+        z = S.params.search_zone
+    else:
+        S = synth.sequence(n_frames=4, n_grains=4, size=(700, 600), seed=506, radius=(78.0, 82.0), margin=150,
+                           max_step=1.5, max_rot=0.01)
+        z = pk.zone(2, num_rot=1, inc_rot=0.01)
+    say(f"sequence {case}: G={S.G}, P max {np.diff(S.pattern_begin).max()}")
+    prm = pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT)
+    oprm = pk.track_params(z, rescue_level=0, subpixel=0)
+    n0 = tracker.exact_fallbacks()
+    for k, got, ref in _sequence_vs_oracle(tracker, S, prm, oprm, 3):
+        for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+            assert bits_equal(got[col], ref[col]), (k, col)
+        assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+        assert bits_equal(got["refrot"], ref["refrot"])
+    assert tracker.exact_fallbacks() == n0
+
+
+def test_sequence_beyond_exact_limits_falls_back_to_plain(tracker):
+    # radius 80.5-82 at +-8 px: the MFMA kernel's window and staging exceed the CU's LDS,
+    # so the search runs the plain form with the same picks
+    S = synth.sequence(n_frames=2, n_grains=3, size=(700, 600), seed=507, radius=(80.5, 82.0), margin=150,
+                       max_step=3.0, max_rot=0.01)
+    z = pk.zone(8, num_rot=1, inc_rot=0.01)
+    say(f"sequence beyond exact limits: G={S.G}")
+    prm = pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT)
+    oprm = pk.track_params(z, rescue_level=0, subpixel=0)
+    n0 = tracker.exact_fallbacks()
+    for k, got, ref in _sequence_vs_oracle(tracker, S, prm, oprm, 1):
+        for col in INT_COLS:
+            assert bits_equal(got[col], ref[col]), col
+    assert tracker.exact_fallbacks()[0] == n0[0] + 1
+
+
+def test_sequence_exact_lm_within_tolerance_of_fp64_oracle(tracker):
+    # the batching workload's shape at a small size: radii 6-60, +-8 px x 9 steps, LM
+    # subpixel, three pairs; every grain within 1e-3 px / 1e-5 rad of the FP64 minimizer
+    S = synth.sequence(n_frames=4, n_grains=40, size=(1000, 800), seed=508)
+    z = S.params.search_zone
+    say(f"sequence LM: G={S.G}")
+    prm = pk.track_params(z, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    oprm = pk.track_params(z, rescue_level=0, subpixel=1, subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+    for k, got, ref in _sequence_vs_oracle(tracker, S, prm, oprm, 3):
+        epx = np.hypot(got["dx"] - ref["dx"], got["dy"] - ref["dy"])
+        assert epx.max() < TOL_PX, (k, epx.max())
+        assert np.abs(got["drot"] - ref["drot"]).max() < TOL_RAD
+        assert np.abs(got["NCC_subpix"] - ref["NCC_subpix"]).max() < TOL_NCC
+
+
+def test_exact_overflowing_grains_go_to_plain_kernel(tracker):
+    # at a reference rotation of 0.35 rad the rounded rotated disks of radius 40+ differ
+    # from the middle step's in hundreds of cells per step: those grains are searched by
+    # the plain kernel within the stage, the small ones by the MFMA kernel; every pick is
+    # the reference's
+    S = synth.sequence(n_frames=2, n_grains=24, size=(900, 800), seed=509, radius=(8.0, 56.0))
+    gr = S.grains.copy()
+    gr["refrot"] = 0.35
+    z = S.params.search_zone
+    say(f"exact search at refrot 0.35: G={S.G}")
+    tracker.set_image(0, S.frames[0], S.bits)
+    tracker.set_image(1, S.frames[1], S.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(gr, S.pattern_xy, S.pattern_begin)
+    n0 = tracker.exact_fallbacks()
+    tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT))
+    got = tracker.get_grains()
+    n1 = tracker.exact_fallbacks()
+    assert n1[0] == n0[0]
+    assert 0 < n1[1] - n0[1] < S.G
+    W = ptref.Workload(S.frames[0], S.frames[1], gr, S.pattern_xy, S.pattern_begin, S.bits)
+    ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
