@@ -241,7 +241,7 @@ int do_precompute(Ctx* c) {
   return PTG_OK;
 }
 
-constexpr size_t kMaxLds = 160 * 1024;
+constexpr size_t kMaxLds = 160 * 1024 - 1024;  // dynamic LDS; the rest for the kernels' static arrays
 
 int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, double NCC_min_super, int method) {
   TRY(need_grains(c));
@@ -314,15 +314,27 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
   TRY(c->mark_staging());
 
-  // LDS: the largest window and pattern of the list
+  // LDS of the plain kernel: the largest window and pattern of the list; a window beyond
+  // the LDS is searched in square tiles of shifts
   int64_t win_cap = 0, off_cap = 0;
+  int rho_max = 0;
   const int U = z->left + z->right + 1, V = z->up + z->down + 1;
   for (int li = 0; li < nlist; li++) {
     const int g = list[li];
-    const int rho = c->h_prad[g];
-    win_cap = std::max<int64_t>(win_cap, (int64_t)(U + 2 * rho) * (V + 2 * rho));
+    rho_max = std::max(rho_max, c->h_prad[g]);
     off_cap = std::max<int64_t>(off_cap, c->h_pbegin[g + 1] - c->h_pbegin[g]);
   }
+  int tile_u = U, tile_v = V;
+  auto plain_lds = [&](int tu, int tv) {
+    return ((size_t)(tu + 2 * rho_max) * (tv + 2 * rho_max) * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
+  };
+  if (plain_lds(U, V) > kMaxLds) {
+    int t = std::max(U, V);
+    while (t > 1 && plain_lds(std::min(t, U), std::min(t, V)) > kMaxLds) t--;
+    tile_u = std::min(t, U);
+    tile_v = std::min(t, V);
+  }
+  win_cap = (int64_t)(tile_u + 2 * rho_max) * (tile_v + 2 * rho_max);
 
   SearchArgs a;
   const Image &i0 = c->img[c->iref], &i1 = c->img[c->icur];
@@ -350,6 +362,8 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   a.down = z->down;
   a.win_cap = (int32_t)win_cap;
   a.off_cap = (int32_t)off_cap;
+  a.tile_u = tile_u;
+  a.tile_v = tile_v;
   a.best = c->d_best;
 
   const int tstage = PTG_T_SEARCH_MAIN + stage;

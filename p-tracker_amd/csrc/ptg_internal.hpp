@@ -49,8 +49,9 @@ struct SearchArgs {
   const double* steps;      // rotation-step values
   int32_t nsteps;
   int32_t left, right, up, down;
-  int32_t win_cap;          // LDS elements reserved for the window
+  int32_t win_cap;          // LDS elements reserved for the window (of one shift tile)
   int32_t off_cap;          // LDS elements reserved for rotated offsets
+  int32_t tile_u, tile_v;   // plain kernel: shifts per tile (U, V when the window fits the LDS)
   StepBest* best;           // list slot x nsteps
 };
 

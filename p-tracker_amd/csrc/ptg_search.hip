@@ -174,7 +174,10 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
   const int rho = a.prad[g];
   const int gdx = (int)round(gr.dx), gdy = (int)round(gr.dy);
   const int U = a.left + a.right + 1, V = a.up + a.down + 1;
-  const int ww = U + 2 * rho, wh = V + 2 * rho;
+  // shifts in tiles of TU x TV (one tile unless the window exceeds the LDS); the window
+  // of a tile is (tu + 2 rho) x (tv + 2 rho) cells stored with row stride ww
+  const int TU = min(a.tile_u, U), TV = min(a.tile_v, V);
+  const int ww = TU + 2 * rho;
   const int X0 = gr.refcoord_xpix + gdx - a.left - rho;
   const int Y0 = gr.refcoord_ypix + gdy - a.up - rho;
   const double C0C0 = gr.C0C0;
@@ -182,15 +185,6 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
   uint16_t* win = reinterpret_cast<uint16_t*>(smem);
   int32_t* off = reinterpret_cast<int32_t*>(smem + ((size_t)a.win_cap * 2 + 15) / 16 * 16);
 
-  bool clamped = false;
-  for (int t = threadIdx.x; t < ww * wh; t += kPlainThreads) {
-    const int wy = t / ww, wx = t - wy * ww;
-    int X = X0 + wx, Y = Y0 + wy;
-    if (X < 0 || Y < 0 || X >= a.dimx || Y >= a.dimy) clamped = true;
-    X = clampi(X, 0, a.dimx - 1);
-    Y = clampi(Y, 0, a.dimy - 1);
-    win[t] = a.I1[(int64_t)Y * a.pitch1 + X];
-  }
   const double c = a.cs[2 * ((int64_t)g * nsteps + k)];
   const double s = a.cs[2 * ((int64_t)g * nsteps + k) + 1];
   for (int i = threadIdx.x; i < n; i += kPlainThreads) {
@@ -198,63 +192,79 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
     rotate_px(a.pattern[2 * (b + i)], a.pattern[2 * (b + i) + 1], c, s, x, y);
     off[i] = (y + rho) * ww + (x + rho);
   }
-  if (clamped) atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
-  __syncthreads();
 
   const double* __restrict__ d0 = a.d0 + b;
-  const int S = U * V;
   const double dn = (double)n;
   double best = 0.0;
   int bidx = INT_MAX;
-  for (int r0 = 0; r0 < S; r0 += kPlainThreads * kPlainCPT) {
-    int base[kPlainCPT], cidx[kPlainCPT];
-    bool valid[kPlainCPT];
-#pragma unroll
-    for (int q = 0; q < kPlainCPT; q++) {
-      // work item w walks u fastest so neighbouring lanes read neighbouring LDS words
-      const int w = r0 + q * kPlainThreads + threadIdx.x;
-      valid[q] = w < S;
-      const int ww_ = valid[q] ? w : 0;
-      const int v = ww_ / U, u = ww_ - v * U;
-      base[q] = v * ww + u;
-      cidx[q] = u * V + v;
-    }
-    unsigned int isum[kPlainCPT];
-#pragma unroll
-    for (int q = 0; q < kPlainCPT; q++) isum[q] = 0u;
-    for (int i = 0; i < n; i++) {
-      const int o = off[i];
-#pragma unroll
-      for (int q = 0; q < kPlainCPT; q++) isum[q] += win[o + base[q]];
-    }
-    // mean1 += (double)I1 over integers is exact in any order, then one division (:705)
-    double m1[kPlainCPT], c01[kPlainCPT], c11[kPlainCPT];
-#pragma unroll
-    for (int q = 0; q < kPlainCPT; q++) {
-      m1[q] = (double)isum[q] / dn;
-      c01[q] = 0.0;
-      c11[q] = 0.0;
-    }
-    for (int i = 0; i < n; i++) {
-      const int o = off[i];
-      const double dd = d0[i];
-#pragma unroll
-      for (int q = 0; q < kPlainCPT; q++) {
-        const double diffC1 = (double)win[o + base[q]] - m1[q];
-        c01[q] = c01[q] + dd * diffC1;
-        c11[q] = c11[q] + diffC1 * diffC1;
+  bool clamped = false;
+  for (int tu0 = 0; tu0 < U; tu0 += TU) {
+    for (int tv0 = 0; tv0 < V; tv0 += TV) {
+      const int tu = min(TU, U - tu0), tv = min(TV, V - tv0);
+      const int tw = tu + 2 * rho, th = tv + 2 * rho;
+      if (tu0 + tv0 > 0) __syncthreads();  // every lane is done with the previous tile
+      for (int t = threadIdx.x; t < tw * th; t += kPlainThreads) {
+        const int wy = t / tw, wx = t - wy * tw;
+        int X = X0 + tu0 + wx, Y = Y0 + tv0 + wy;
+        if (X < 0 || Y < 0 || X >= a.dimx || Y >= a.dimy) clamped = true;
+        X = clampi(X, 0, a.dimx - 1);
+        Y = clampi(Y, 0, a.dimy - 1);
+        win[wy * ww + wx] = a.I1[(int64_t)Y * a.pitch1 + X];
       }
-    }
+      __syncthreads();
+      const int St = tu * tv;
+      for (int r0 = 0; r0 < St; r0 += kPlainThreads * kPlainCPT) {
+        int base[kPlainCPT], cidx[kPlainCPT];
+        bool valid[kPlainCPT];
 #pragma unroll
-    for (int q = 0; q < kPlainCPT; q++) {
-      if (!valid[q]) continue;
-      const double ncc = c01[q] / sqrt(C0C0 * c11[q]);
-      if (better(ncc, cidx[q], best, bidx)) {
-        best = ncc;
-        bidx = cidx[q];
+        for (int q = 0; q < kPlainCPT; q++) {
+          // work item w walks u fastest so neighbouring lanes read neighbouring LDS words
+          const int w = r0 + q * kPlainThreads + threadIdx.x;
+          valid[q] = w < St;
+          const int ww_ = valid[q] ? w : 0;
+          const int v = ww_ / tu, u = ww_ - v * tu;
+          base[q] = v * ww + u;
+          cidx[q] = (tu0 + u) * V + (tv0 + v);
+        }
+        unsigned int isum[kPlainCPT];
+#pragma unroll
+        for (int q = 0; q < kPlainCPT; q++) isum[q] = 0u;
+        for (int i = 0; i < n; i++) {
+          const int o = off[i];
+#pragma unroll
+          for (int q = 0; q < kPlainCPT; q++) isum[q] += win[o + base[q]];
+        }
+        // mean1 += (double)I1 over integers is exact in any order, then one division (:705)
+        double m1[kPlainCPT], c01[kPlainCPT], c11[kPlainCPT];
+#pragma unroll
+        for (int q = 0; q < kPlainCPT; q++) {
+          m1[q] = (double)isum[q] / dn;
+          c01[q] = 0.0;
+          c11[q] = 0.0;
+        }
+        for (int i = 0; i < n; i++) {
+          const int o = off[i];
+          const double dd = d0[i];
+#pragma unroll
+          for (int q = 0; q < kPlainCPT; q++) {
+            const double diffC1 = (double)win[o + base[q]] - m1[q];
+            c01[q] = c01[q] + dd * diffC1;
+            c11[q] = c11[q] + diffC1 * diffC1;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPlainCPT; q++) {
+          if (!valid[q]) continue;
+          const double ncc = c01[q] / sqrt(C0C0 * c11[q]);
+          if (better(ncc, cidx[q], best, bidx)) {
+            best = ncc;
+            bidx = cidx[q];
+          }
+        }
       }
     }
   }
+  if (clamped) atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
   // workgroup argmax with the same ordering
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
@@ -281,7 +291,7 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
       }
     StepBest sb;
     sb.ncc = best;
-    sb.idx = bidx == INT_MAX ? INT_MAX : k * S + bidx;  // global candidate index
+    sb.idx = bidx == INT_MAX ? INT_MAX : k * (U * V) + bidx;  // global candidate index
     sb.pad = 0;
     *out = sb;
   }

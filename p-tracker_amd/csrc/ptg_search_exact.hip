@@ -208,6 +208,9 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
 // the left) | segment table. Large patterns stage the six planes in two passes; each
 // pass adds its planes' share of B_mid and C_mid to the values the same lane wrote before.
 constexpr int kMidPlanes = 6;
+// Dynamic LDS a workgroup may take: the CU's 160 KiB less room for the kernels' static
+// arrays.
+constexpr size_t kLdsDynMax = 159 * 1024;
 
 // Row stride of the digit planes for cbw column blocks: an odd multiple of 16 bytes that
 // holds columns su0 .. su0 + 16 cbw + 2 rho + 15.
@@ -930,7 +933,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     l.npl = kMidPlanes;
     while (l.cbw > 1 && (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > 80 * 1024 || ntile_m * l.cbw > 16))
       l.cbw--;
-    if (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > 160 * 1024) l.npl = 3;
+    if (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > kLdsDynMax) l.npl = 3;
     l.lds_m = mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl);
     // templates and Delta lists: the step's packed template and the middle step's grid
     l.lds_t = (size_t)side * side * 5;
@@ -960,15 +963,15 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     int best_res = 0;
     l.nsu = 1;
     for (int w = 1; w <= std::min(8, cpb); w++) {
-      if (need(w) > 160 * 1024) break;
-      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);
+      if (need(w) > kLdsDynMax) break;
+      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);  // workgroups per CU by LDS
       if (res >= best_res) {
         best_res = res;
         l.nsu = w;
       }
     }
     l.lds = need(l.nsu);
-    if (l.lds_m > 160 * 1024 || l.lds_t > 160 * 1024 || l.lds > 160 * 1024) return kExactUnfit;
+    if (l.lds_m > kLdsDynMax || l.lds_t > kLdsDynMax || l.lds > kLdsDynMax) return kExactUnfit;
   }
   int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
   if (rc != PTG_OK) return rc;
@@ -1004,7 +1007,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // grains whose lists overflow are searched by the plain kernel when its LDS takes the
   // list's largest window and pattern (else an overflow voids the stage)
   const size_t lds_plain = ((size_t)sa.win_cap * 2 + 15) / 16 * 16 + (size_t)sa.off_cap * 4;
-  const bool plain_ok = lds_plain <= 160 * 1024;
+  const bool plain_ok = lds_plain <= 159 * 1024;  // the plain kernel's static arrays take the rest
   if (!plain_ok) e.ovf_slot = e.ovf_list = nullptr;
   e.dcap = P->dcap;
   e.nmt = nmt;

@@ -445,3 +445,24 @@ def test_exact_overflowing_grains_go_to_plain_kernel(tracker):
     for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
         assert bits_equal(got[col], ref[col]), col
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("method", [pk.SEARCH_PLAIN, pk.SEARCH_EXACT])
+def test_huge_zone_tiled_plain_search_picks_equal_oracle(tracker, method):
+    # +-150 px: the window (341 x 341 cells) exceeds the LDS, so the plain kernel walks the
+    # shifts in tiles; the exact form hands the whole search to it
+    S = synth.sequence(n_frames=2, n_grains=3, size=(900, 900), seed=510, radius=(15.0, 20.0), margin=200,
+                       max_step=3.0, max_rot=0.01)
+    z = pk.zone(150, num_rot=0, inc_rot=1.0)
+    say(f"+-150 px zone, method {method}")
+    tracker.set_image(0, S.frames[0], S.bits)
+    tracker.set_image(1, S.frames[1], S.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(S.grains, S.pattern_xy, S.pattern_begin)
+    tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=method))
+    got = tracker.get_grains()
+    W = ptref.Workload(S.frames[0], S.frames[1], S.grains, S.pattern_xy, S.pattern_begin, S.bits)
+    ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
+    for col in INT_COLS:
+        assert bits_equal(got[col], ref[col]), col
+    assert (got["NCC"] > 0.8).all()  # found (the grains also rotate by up to 0.01 rad)
