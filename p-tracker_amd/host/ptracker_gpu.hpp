@@ -15,37 +15,12 @@
 #include <vector>
 
 #include "../../include/ptgpu.h"
+#include "ptracker_io.hpp"
+#include "tiff_reader.hpp"
 
 namespace ptgpu {
 
-// src/relative_coord_type.hpp:19-25
-struct relative_coord_type {
-  int dx{0};
-  int dy{0};
-};
-
-// src/search_zone_type.hpp:20-27; the same layout as ptg_search_zone
-struct search_zone_type {
-  int left{0}, right{0}, up{0}, down{0}, num_rot{0};
-  double inc_rot{1.0};
-};
 static_assert(sizeof(search_zone_type) == sizeof(ptg_search_zone), "search zone layouts differ");
-
-// src/grain_type_2D.hpp:23-71 (the neighbour list, dead in the reference, is left out)
-struct grain_type_2D {
-  int refcoord_xpix{0}, refcoord_ypix{0};
-  double refrot{0.0}, radius_pix{0.0};
-  double dx{0.0}, dy{0.0}, drot{0.0};
-  double dx_prev{0.0}, dy_prev{0.0}, drot_prev{0.0};
-  double upix{0.0}, vpix{0.0}, rot_inc{0.0};
-  double NCC{0.0}, NCC_rescue{0.0}, NCC_subpix{0.0};
-  double mean0{0.0}, C0C0{0.0};
-  bool masked{false};
-  int status{0};
-  std::vector<relative_coord_type> pattern;
-  void backup() { dx_prev = dx; dy_prev = dy; drot_prev = drot; }
-  void reset() { NCC = NCC_rescue = NCC_subpix = 0.0; }
-};
 
 class ptracker_gpu {
  public:
@@ -140,7 +115,61 @@ class ptracker_gpu {
 
   void* context() { return ctx_; }
 
+  // The command-file state the per-pair body reads.
+  void apply_settings(const tracker_settings& s) {
+    search_zone = s.search_zone;
+    search_zone_rescue = s.search_zone_rescue;
+    search_zone_super_rescue = s.search_zone_super_rescue;
+    rescue_level = s.rescue_level;
+    NCC_min = s.NCC_min;
+    NCC_min_super = s.NCC_min_super;
+    subpixel = s.subpixel;
+    subpix_tol = s.subpix_tol;
+    initial_direction_dx = s.initial_direction_dx;
+    initial_direction_dy = s.initial_direction_dy;
+    initial_direction_dr = s.initial_direction_dr;
+    subpix_displacement_max = s.subpix_displacement_max;
+    overflow_stiffness = s.overflow_stiffness;
+  }
+
+  // particle_tracking, src/p-tracker.cpp:423-611: images ibeg .. iend (step iinc) against
+  // the reference image iref, frames read from the TIFF files named by image_name, one
+  // dic_out_<num>.txt per image (in out_dir), and the reference swapped once
+  // num_image - iref >= iraz. Frames hold the reference's 16-bit gray levels. Returns the
+  // number of images tracked.
+  int particle_tracking(const tracker_settings& s, const std::string& out_dir = "") {
+    if (s.RawImages) throw std::runtime_error("RAW images are not read (libraw is not part of this build)");
+    if (s.image_interpolator != "cubic")
+      throw std::runtime_error("only the cubic image interpolator is implemented (got " + s.image_interpolator + ")");
+    if (s.iinc <= 0 && s.iend >= s.ibeg) throw std::runtime_error("iinc must be positive");
+    apply_settings(s);
+    int iref = s.iref;
+    std::string datetime[kFrames];
+    const tiff_frame f0 = read_tiff(numbered_name(s.image_name, iref));
+    set_image(im_index_ref, f0.gray.data(), f0.width, f0.height, 16);
+    datetime[im_index_ref] = f0.dateTime;
+    upload_grains();
+    int n = 0;
+    for (int num = s.ibeg; num <= s.iend; num += s.iinc) {
+      const tiff_frame f = read_tiff(numbered_name(s.image_name, num));
+      if (f.width != f0.width || f.height != f0.height)
+        throw std::runtime_error("image " + numbered_name(s.image_name, num) + " differs in size from the reference");
+      set_image(im_index_current, f.gray.data(), f.width, f.height, 16);
+      datetime[im_index_current] = f.dateTime;
+      track_pair();
+      save_grains(out_dir + numbered_name("dic_out_%d.txt", num), grain, false, numbered_name(s.image_name, iref),
+                  datetime[im_index_ref], numbered_name(s.image_name, num), datetime[im_index_current]);
+      n++;
+      if (num - iref >= s.iraz) {
+        change_reference();
+        iref = num;
+      }
+    }
+    return n;
+  }
+
  private:
+  static constexpr int kFrames = 2;  // image slots 0 and 1, as the reference's image[2]
   void* ctx_ = nullptr;
 
   void check(int rc, const char* what) {

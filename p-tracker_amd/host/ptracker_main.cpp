@@ -1,0 +1,50 @@
+// ptracker_main.cpp — the tracking program over libptgpu: `ptracker_gpu [options]
+// [commands.txt]` reads a p-tracker command file (default commands.txt, as the
+// reference's init, src/p-tracker.cpp:1170-1202) and runs its procedure.
+//
+// Options: --lm     subpixel by the LM form instead of Powell
+//          --plain  integer search by the plain FP64 form instead of the exact MFMA form
+//          --device N
+// particle_tracking runs once (the reference's run() runs it a second time after the
+// first for that procedure name, src/p-tracker.cpp:1802-1808); correction_distortion is
+// not part of this program.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+
+#include "ptracker_gpu.hpp"
+
+int main(int argc, char** argv) {
+  std::string cmd = "commands.txt";
+  int device = 0;
+  int search = PTG_SEARCH_EXACT, subpix = PTG_SUBPIX_POWELL;
+  for (int i = 1; i < argc; i++) {
+    if (!std::strcmp(argv[i], "--lm")) subpix = PTG_SUBPIX_LM;
+    else if (!std::strcmp(argv[i], "--plain")) search = PTG_SEARCH_PLAIN;
+    else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) device = std::atoi(argv[++i]);
+    else cmd = argv[i];
+  }
+  try {
+    ptgpu::tracker_settings s;
+    std::vector<ptgpu::grain_type_2D> grains;
+    ptgpu::read_data(cmd, s, grains);
+    ptgpu::finish_settings(s);
+    if (s.rotations == 0) std::cout << "Rotation are NOT TRACKED!" << std::endl;
+    if (s.procedure != "particle_tracking") {
+      std::cerr << "procedure " << s.procedure << " is not part of this program" << std::endl;
+      return 1;
+    }
+    ptgpu::ptracker_gpu pt(device);
+    pt.grain = grains;
+    pt.search_method = search;
+    pt.subpix_method = subpix;
+    const int n = pt.particle_tracking(s);
+    std::cout << "tracked " << pt.grain.size() << " grains over " << n << " images" << std::endl;
+  } catch (const std::exception& e) {
+    std::cerr << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}
