@@ -188,6 +188,32 @@ int ptg_undistort_points(void* ctx, const double* params14, int32_t n, const dou
  * iteration on undistor(). dst gets src's size and bits. */
 int ptg_remap(void* ctx, int src_index, int dst_index, const double* params14);
 
+/* ---- distortion calibration (procedure correction_distortion) --------------------- */
+/* The equiprojectivity criterion of src/distortion_correction.cpp:135-479: G grains at
+ * reference pixels ref_xy (G x 2, the reference's (double)refcoord_xpix / ypix), nimg
+ * images after the reference with tracked displacements disp (nimg x G x 2: dx, dy) and
+ * NCC_subpix ncc (nimg x G), and npairs grain pairs (npairs x 2: i, j; the reference's
+ * precompute_paires, :62-85). Residual (k, l) is
+ *   e . ((B2 - B1) - (A2 - A1)),  e = (B1 - A1) / |B1 - A1|,
+ * with A1, B1 the undistor() images of the reference pixels of pair l and A2, B2 those of
+ * its tracked positions in image k, or 0 when either grain has NCC_subpix < NCC_min
+ * (the reference skips the pair). Copies the data to the device. */
+int ptg_equiproj_set_data(void* ctx, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
+                          const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min);
+/* At params14 (undistor's 14 parameters, taken as given): f = the sum of squared residuals
+ * (the reference's Powell functor, :277-303); jtr = J^T r (14) and jtj = J^T J (14 x 14,
+ * row major) of the residuals' analytic Jacobian J; res = the residuals (nimg * npairs,
+ * image major as the reference's fvec, :409-475) and jac = J (nimg * npairs x 14, row
+ * major). Every output may be NULL. */
+int ptg_equiproj_eval(void* ctx, const double* params14, double* f, double* jtr, double* jtj, double* res,
+                      double* jac);
+/* Minimises the criterion over the parameters with free14[k] != 0 (the others held) by
+ * Levenberg-Marquardt on the analytic J^T J and J^T r, in place of the reference's Powell
+ * or lmdif (forward-difference Jacobian) runs, :277-475. params14 is updated; f_ini and
+ * f_end receive the criterion before and after, iters the iterations (each may be NULL). */
+int ptg_calibrate_distortion(void* ctx, double* params14, const int32_t* free14, int32_t max_iter, double* f_ini,
+                             double* f_end, int32_t* iters);
+
 /* ---- timing ---------------------------------------------------------------------- */
 /* Stage timers: HIP events around each stage on the context's stream. */
 enum {

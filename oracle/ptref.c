@@ -648,6 +648,50 @@ void ptref_undistor(const double* X, double xd, double yd, double* xu, double* y
   *yu = Ycp + S1 * ny1 + S2 * ny2;
 }
 
+int64_t ptref_precompute_pairs(int32_t G, const double* ref_xy, double dmin, double dmax, int32_t* pairs, int64_t cap) {
+  int64_t n = 0;
+  for (int32_t i = 0; i < G; i++) {
+    const double xi = ref_xy[2 * i], yi = ref_xy[2 * i + 1];
+    for (int32_t j = i + 1; j < G; j++) {
+      const double xij = ref_xy[2 * j] - xi, yij = ref_xy[2 * j + 1] - yi;
+      const double dij = sqrt(xij * xij + yij * yij);
+      if (dij >= dmin && dij <= dmax) {
+        if (n < cap) {
+          pairs[2 * n] = i;
+          pairs[2 * n + 1] = j;
+        }
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+void ptref_equiproj_residuals(const double* X, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
+                              const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min, double* res) {
+  for (int32_t k = 0; k < nimg; k++) {
+    const double* d = disp + 2 * (int64_t)k * G;
+    const double* nc = ncc + (int64_t)k * G;
+    for (int32_t l = 0; l < npairs; l++) {
+      const int32_t i = pairs[2 * l], j = pairs[2 * l + 1];
+      double* r = &res[(int64_t)k * npairs + l];
+      *r = 0.0;
+      if (nc[i] < NCC_min || nc[j] < NCC_min) continue;
+      double A1x, A1y, A2x, A2y, B1x, B1y, B2x, B2y;
+      ptref_undistor(X, ref_xy[2 * i], ref_xy[2 * i + 1], &A1x, &A1y);
+      ptref_undistor(X, ref_xy[2 * i] + d[2 * i], ref_xy[2 * i + 1] + d[2 * i + 1], &A2x, &A2y);
+      ptref_undistor(X, ref_xy[2 * j], ref_xy[2 * j + 1], &B1x, &B1y);
+      ptref_undistor(X, ref_xy[2 * j] + d[2 * j], ref_xy[2 * j + 1] + d[2 * j + 1], &B2x, &B2y);
+      double ABx = B1x - A1x, ABy = B1y - A1y;
+      const double AB = sqrt(ABx * ABx + ABy * ABy);
+      ABx /= AB;
+      ABy /= AB;
+      const double BBx = B2x - B1x, BBy = B2y - B1y, AAx = A2x - A1x, AAy = A2y - A1y;
+      *r = (ABx * (BBx - AAx)) + (ABy * (BBy - AAy));
+    }
+  }
+}
+
 /* Jacobian of undistor with respect to (xd, yd), by differentiating the expressions
  * above: J = Jparallax * Jbrown. */
 static void undistor_jac(const double* X, double xd, double yd, double J[4]) {

@@ -105,6 +105,16 @@ void ptref_change_reference(int32_t G, ptg_grain* grains);
 /* undistor, src/distortion_correction.cpp:20-61. */
 void ptref_undistor(const double* X, double xd, double yd, double* xu, double* yu);
 
+/* precompute_paires, src/distortion_correction.cpp:62-85: pairs (i < j) of reference
+ * pixels whose distance lies in [dmin, dmax], i outer, j inner; writes up to cap pairs
+ * and returns the number found. */
+int64_t ptref_precompute_pairs(int32_t G, const double* ref_xy, double dmin, double dmax, int32_t* pairs, int64_t cap);
+/* The equiprojectivity residuals of the LEVMAR functor, src/distortion_correction.cpp:409-475
+ * (the Powell functor :277-303 sums their squares): res[k * npairs + l], 0 where either
+ * grain of pair l has NCC_subpix < NCC_min in image k. */
+void ptref_equiproj_residuals(const double* X, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
+                              const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min, double* res);
+
 /* Remap of a whole frame (the library's ptg_remap): for every output pixel (x, y),
  * Newton iteration on undistor() for the distorted point, then a bicubic sample of src
  * with taps clamped to the frame, rounded and clamped to [0, 2^bits - 1]. */

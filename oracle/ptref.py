@@ -7,6 +7,7 @@ CPU-baseline leg, always as the checker of libptgpu, never as the product.
 from __future__ import annotations
 
 import ctypes
+import math
 import importlib.util
 import os
 import sys
@@ -62,6 +63,8 @@ def lib():
             "ptref_undistor": ([vp, c_double, c_double, vp, vp], None),
             "ptref_remap": ([vp, c_int32, c_int32, c_int64, c_int32, vp, vp, c_int64], None),
             "ptref_distort_point": ([vp, c_double, c_double, vp, vp], None),
+            "ptref_precompute_pairs": ([c_int32, vp, c_double, c_double, vp, c_int64], c_int64),
+            "ptref_equiproj_residuals": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double, vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -250,3 +253,42 @@ def remap(src, bits, X14):
     dst = np.zeros_like(s)
     lib().ptref_remap(_p(s), s.shape[1], s.shape[0], s.shape[1], bits, _p(X), _p(dst), s.shape[1])
     return dst
+
+
+def precompute_pairs(ref_xy, dmin, dmax):
+    """Grain pairs (i < j) at reference distance in [dmin, dmax] (npairs x 2)."""
+    ref = np.ascontiguousarray(np.asarray(ref_xy, dtype=np.float64).reshape(-1, 2))
+    n = lib().ptref_precompute_pairs(len(ref), _p(ref), dmin, dmax, None, 0)
+    out = np.zeros((n, 2), dtype=np.int32)
+    lib().ptref_precompute_pairs(len(ref), _p(ref), dmin, dmax, _p(out), n)
+    return out
+
+
+def equiproj_residuals(X14, ref_xy, disp, ncc, pairs, NCC_min=0.7):
+    """The equiprojectivity residuals (nimg * npairs, image major)."""
+    X = np.ascontiguousarray(X14, dtype=np.float64)
+    ref = np.ascontiguousarray(np.asarray(ref_xy, dtype=np.float64).reshape(-1, 2))
+    G = len(ref)
+    d = np.ascontiguousarray(np.asarray(disp, dtype=np.float64).reshape(-1, G, 2))
+    n = np.ascontiguousarray(np.asarray(ncc, dtype=np.float64).reshape(-1, G))
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    res = np.zeros(len(d) * len(pr))
+    lib().ptref_equiproj_residuals(_p(X), G, _p(ref), len(d), _p(d), _p(n), len(pr), _p(pr), NCC_min, _p(res))
+    return res
+
+
+def lmdif_jacobian(fun, x, epsfcn=0.0):
+    """MINPACK fdjac2's forward-difference Jacobian (lmdif, the reference's LEVMAR path):
+    column j steps x_j by h = sqrt(max(epsfcn, DBL_EPSILON)) |x_j| (h = that root when x_j
+    is 0)."""
+    eps = math.sqrt(max(epsfcn, np.finfo(np.float64).eps))
+    f0 = fun(x)
+    J = np.zeros((len(f0), len(x)))
+    for j in range(len(x)):
+        h = eps * abs(x[j])
+        if h == 0.0:
+            h = eps
+        xp = np.array(x, dtype=np.float64)
+        xp[j] = x[j] + h
+        J[:, j] = (fun(xp) - f0) / h
+    return J

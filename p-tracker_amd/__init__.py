@@ -148,6 +148,9 @@ def load_library(path: str = LIB_PATH):
         "ptg_get_subpix_evals": ([vp, POINTER(c_int64)], c_int),
         "ptg_get_subpix_evals_per_grain": ([vp, vp], c_int),
         "ptg_get_exact_fallbacks": ([vp, POINTER(c_int64), POINTER(c_int64)], c_int),
+        "ptg_equiproj_set_data": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double], c_int),
+        "ptg_equiproj_eval": ([vp, vp, vp, vp, vp, vp, vp], c_int),
+        "ptg_calibrate_distortion": ([vp, vp, vp, c_int32, vp, vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -166,7 +169,8 @@ def exported_symbols():
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
-            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks"]
+            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
+            "ptg_equiproj_eval", "ptg_calibrate_distortion"]
 
 
 class PtgError(RuntimeError):
@@ -299,6 +303,39 @@ class Tracker:
         self._ck(self.lib.ptg_undistort_points(self.h, _ptr(p), len(xy), _ptr(xy), _ptr(out)),
                  "ptg_undistort_points")
         return out
+
+    def equiproj_set_data(self, ref_xy, disp, ncc, pairs, NCC_min=0.7):
+        """ref_xy (G x 2), disp (nimg x G x 2), ncc (nimg x G), pairs (npairs x 2)."""
+        ref = np.ascontiguousarray(np.asarray(ref_xy, dtype=np.float64).reshape(-1, 2))
+        G = len(ref)
+        d = np.ascontiguousarray(np.asarray(disp, dtype=np.float64).reshape(-1, G, 2))
+        n = np.ascontiguousarray(np.asarray(ncc, dtype=np.float64).reshape(-1, G))
+        pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        self._eq_m = len(d) * len(pr)
+        self._ck(self.lib.ptg_equiproj_set_data(self.h, G, _ptr(ref), len(d), _ptr(d), _ptr(n), len(pr), _ptr(pr),
+                                                NCC_min), "ptg_equiproj_set_data")
+
+    def equiproj_eval(self, params14, full=False):
+        """(f, J^T r, J^T J) and, with full, (residuals, Jacobian)."""
+        p = np.ascontiguousarray(params14, dtype=np.float64)
+        f = np.zeros(1)
+        jtr = np.zeros(14)
+        jtj = np.zeros((14, 14))
+        res = np.zeros(self._eq_m) if full else None
+        jac = np.zeros((self._eq_m, 14)) if full else None
+        self._ck(self.lib.ptg_equiproj_eval(self.h, _ptr(p), _ptr(f), _ptr(jtr), _ptr(jtj),
+                                            _ptr(res) if full else None, _ptr(jac) if full else None),
+                 "ptg_equiproj_eval")
+        return (f[0], jtr, jtj, res, jac) if full else (f[0], jtr, jtj)
+
+    def calibrate_distortion(self, params14, free, max_iter=100):
+        """Returns (params14, f_ini, f_end, iterations)."""
+        p = np.ascontiguousarray(params14, dtype=np.float64).copy()
+        fr = np.ascontiguousarray(np.asarray(free, dtype=np.int32).reshape(14))
+        fi, fe, it = c_double(), c_double(), c_int32()
+        self._ck(self.lib.ptg_calibrate_distortion(self.h, _ptr(p), _ptr(fr), max_iter, ctypes.byref(fi),
+                                                   ctypes.byref(fe), ctypes.byref(it)), "ptg_calibrate_distortion")
+        return p, fi.value, fe.value, it.value
 
     def remap(self, src: int, dst: int, params14):
         p = np.ascontiguousarray(params14, dtype=np.float64)

@@ -457,7 +457,9 @@ int ptg_destroy(void* p) {
   for (int i = 0; i < kMaxImages; i++)
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
-  void* more[] = {c->d_counts, c->d_cs, c->d_steps, c->d_best, c->d_evals, c->d_evals_g, c->d_exact_scratch, c->d_tmp};
+  void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
+                  c->d_exact_scratch, c->d_tmp, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
+                  c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac};
   for (void* b : more)
     if (b) (void)hipFree(b);
   free_exact_plans(*c);
@@ -877,6 +879,187 @@ int ptg_undistort_points(void* p, const double* params14, int32_t n, const doubl
   CK(c, launch_undistort_points(*c, params14, n, d_in, d_out));
   CK(c, hipMemcpyAsync(xy_u, d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+// ---- distortion calibration ----------------------------------------------------------
+int ptg_equiproj_set_data(void* p, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
+                          const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (G < 0 || nimg < 0 || npairs < 0 || (G > 0 && !ref_xy) || (G > 0 && nimg > 0 && (!disp || !ncc)) ||
+      (npairs > 0 && !pairs))
+    return c->fail(PTG_ERR_ARG, "bad equiprojectivity data");
+  if ((int64_t)nimg * npairs > ((int64_t)1 << 31)) return c->fail(PTG_ERR_LIMIT, "too many residuals");
+  for (int32_t q = 0; q < 2 * npairs; q++)
+    if (pairs[q] < 0 || pairs[q] >= G) return c->fail(PTG_ERR_ARG, "pair %d names grain %d of %d", q / 2, pairs[q], G);
+  CK(c, hipStreamSynchronize(c->stream));
+  TRY(c->ensure((void**)&c->d_eq_pairs, &c->eq_pairs_cap, sizeof(int32_t) * 2 * (size_t)std::max(npairs, 1)));
+  TRY(c->ensure((void**)&c->d_eq_ref, &c->eq_ref_cap, sizeof(double) * 2 * (size_t)std::max(G, 1)));
+  TRY(c->ensure((void**)&c->d_eq_disp, &c->eq_disp_cap, sizeof(double) * 2 * (size_t)std::max<int64_t>((int64_t)nimg * G, 1)));
+  TRY(c->ensure((void**)&c->d_eq_ncc, &c->eq_ncc_cap, sizeof(double) * (size_t)std::max<int64_t>((int64_t)nimg * G, 1)));
+  const int64_t m = (int64_t)nimg * npairs;
+  TRY(c->ensure((void**)&c->d_eq_partial, &c->eq_partial_cap, sizeof(double) * 120 * (size_t)equiproj_partial_blocks(m)));
+  TRY(c->ensure((void**)&c->d_eq_out, &c->eq_out_cap, sizeof(double) * 120));
+  if (npairs) CK(c, hipMemcpyAsync(c->d_eq_pairs, pairs, sizeof(int32_t) * 2 * npairs, hipMemcpyHostToDevice, c->stream));
+  if (G) CK(c, hipMemcpyAsync(c->d_eq_ref, ref_xy, sizeof(double) * 2 * G, hipMemcpyHostToDevice, c->stream));
+  if (G && nimg) {
+    CK(c, hipMemcpyAsync(c->d_eq_disp, disp, sizeof(double) * 2 * (size_t)nimg * G, hipMemcpyHostToDevice, c->stream));
+    CK(c, hipMemcpyAsync(c->d_eq_ncc, ncc, sizeof(double) * (size_t)nimg * G, hipMemcpyHostToDevice, c->stream));
+  }
+  CK(c, hipStreamSynchronize(c->stream));
+  c->eq_G = G;
+  c->eq_nimg = nimg;
+  c->eq_npairs = npairs;
+  c->eq_NCC_min = NCC_min;
+  return PTG_OK;
+}
+
+// f and, when wanted, J^T r and J^T J (full 14 x 14) at X.
+static int equiproj_eval(Ctx* c, const double* X, double* f, double* jtr, double* jtj, double* res, double* jac) {
+  const int64_t m = (int64_t)c->eq_nimg * c->eq_npairs;
+  EquiprojArgs a;
+  a.G = c->eq_G;
+  a.nimg = c->eq_nimg;
+  a.npairs = c->eq_npairs;
+  a.pad = 0;
+  a.pairs = c->d_eq_pairs;
+  a.ref = c->d_eq_ref;
+  a.disp = c->d_eq_disp;
+  a.ncc = c->d_eq_ncc;
+  a.NCC_min = c->eq_NCC_min;
+  a.partial = c->d_eq_partial;
+  a.res = a.jac = nullptr;
+  if (res && m) {
+    TRY(c->ensure((void**)&c->d_eq_res, &c->eq_res_cap, sizeof(double) * (size_t)m));
+    a.res = c->d_eq_res;
+  }
+  if (jac && m) {
+    TRY(c->ensure((void**)&c->d_eq_jac, &c->eq_jac_cap, sizeof(double) * 14 * (size_t)m));
+    a.jac = c->d_eq_jac;
+  }
+  CK(c, launch_equiproj(*c, a, X, c->d_eq_out));
+  double out[120];
+  CK(c, hipMemcpyAsync(out, c->d_eq_out, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+  if (a.res) CK(c, hipMemcpyAsync(res, a.res, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+  if (a.jac) CK(c, hipMemcpyAsync(jac, a.jac, sizeof(double) * 14 * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  if (f) *f = out[0];
+  if (jtr)
+    for (int q = 0; q < 14; q++) jtr[q] = out[1 + q];
+  if (jtj) {
+    int q = 15;
+    for (int a1 = 0; a1 < 14; a1++)
+      for (int a2 = a1; a2 < 14; a2++, q++) jtj[a1 * 14 + a2] = jtj[a2 * 14 + a1] = out[q];
+  }
+  return PTG_OK;
+}
+
+int ptg_equiproj_eval(void* p, const double* params14, double* f, double* jtr, double* jtj, double* res,
+                      double* jac) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!params14) return c->fail(PTG_ERR_ARG, "null parameters");
+  if (!c->d_eq_out) return c->fail(PTG_ERR_STATE, "ptg_equiproj_set_data not called");
+  return equiproj_eval(c, params14, f, jtr, jtj, res, jac);
+}
+
+// Solves A x = b for symmetric positive definite A (n x n, row major) by Cholesky; false
+// when A is not positive definite.
+static bool cholesky_solve(int n, std::vector<double> A, std::vector<double>& x, const std::vector<double>& b) {
+  for (int j = 0; j < n; j++) {
+    double d = A[j * n + j];
+    for (int k = 0; k < j; k++) d -= A[j * n + k] * A[j * n + k];
+    if (!(d > 0.0)) return false;
+    A[j * n + j] = sqrt(d);
+    for (int i = j + 1; i < n; i++) {
+      double v = A[i * n + j];
+      for (int k = 0; k < j; k++) v -= A[i * n + k] * A[j * n + k];
+      A[i * n + j] = v / A[j * n + j];
+    }
+  }
+  x = b;
+  for (int i = 0; i < n; i++) {
+    for (int k = 0; k < i; k++) x[i] -= A[i * n + k] * x[k];
+    x[i] /= A[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    for (int k = i + 1; k < n; k++) x[i] -= A[k * n + i] * x[k];
+    x[i] /= A[i * n + i];
+  }
+  return true;
+}
+
+int ptg_calibrate_distortion(void* p, double* params14, const int32_t* free14, int32_t max_iter, double* f_ini,
+                             double* f_end, int32_t* iters) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!params14 || !free14 || max_iter < 0) return c->fail(PTG_ERR_ARG, "bad calibration arguments");
+  if (!c->d_eq_out) return c->fail(PTG_ERR_STATE, "ptg_equiproj_set_data not called");
+  std::vector<int> F;
+  for (int q = 0; q < 14; q++)
+    if (free14[q]) F.push_back(q);
+  const int n = (int)F.size();
+  double X[14], f = 0.0, jtr[14], jtj[196];
+  for (int q = 0; q < 14; q++) X[q] = params14[q];
+  TRY(equiproj_eval(c, X, &f, jtr, jtj, nullptr, nullptr));
+  if (f_ini) *f_ini = f;
+  // Levenberg-Marquardt on the free parameters, scaled by the square roots of the
+  // diagonal of J^T J (K1 ~ 1e-9 and K2 ~ 1e-16 px^-2k sit beside pixel centres)
+  double lambda = 1e-3;
+  int it = 0;
+  for (; it < max_iter && n > 0; it++) {
+    std::vector<double> D(n), A(n * n), b(n), y;
+    for (int a = 0; a < n; a++) {
+      const double d = jtj[F[a] * 14 + F[a]];
+      D[a] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    for (int a = 0; a < n; a++) {
+      b[a] = -jtr[F[a]] / D[a];
+      for (int a2 = 0; a2 < n; a2++) A[a * n + a2] = jtj[F[a] * 14 + F[a2]] / (D[a] * D[a2]);
+    }
+    bool improved = false, converged = false;
+    while (lambda < 1e16) {
+      std::vector<double> M = A;
+      for (int a = 0; a < n; a++) M[a * n + a] += lambda * (A[a * n + a] > 0.0 ? A[a * n + a] : 1.0);
+      if (!cholesky_solve(n, M, y, b)) {
+        lambda *= 10.0;
+        continue;
+      }
+      double Xn[14];
+      for (int q = 0; q < 14; q++) Xn[q] = X[q];
+      double step = 0.0;
+      for (int a = 0; a < n; a++) {
+        const double dq = y[a] / D[a];
+        Xn[F[a]] = X[F[a]] + dq;
+        step = std::max(step, fabs(dq) / (fabs(X[F[a]]) + 1e-300));
+      }
+      double fn = 0.0, jtrn[14], jtjn[196];
+      TRY(equiproj_eval(c, Xn, &fn, jtrn, jtjn, nullptr, nullptr));
+      if (fn < f) {
+        converged = (f - fn) <= 1e-15 * f || step < 1e-15;
+        for (int q = 0; q < 14; q++) X[q] = Xn[q];
+        for (int q = 0; q < 14; q++) jtr[q] = jtrn[q];
+        for (int q = 0; q < 196; q++) jtj[q] = jtjn[q];
+        f = fn;
+        lambda = std::max(lambda * 0.1, 1e-12);
+        improved = true;
+        break;
+      }
+      if (step < 1e-15) {
+        converged = true;
+        break;
+      }
+      lambda *= 10.0;
+    }
+    if (!improved || converged) {
+      it++;
+      break;
+    }
+  }
+  for (int q = 0; q < 14; q++) params14[q] = X[q];
+  if (f_end) *f_end = f;
+  if (iters) *iters = it;
   return PTG_OK;
 }
 

@@ -117,6 +117,20 @@ struct Context {
   int64_t exact_fallbacks = 0;      // exact-form searches that ran the plain form instead
   int64_t exact_grain_fallbacks = 0;  // grains of exact-form searches served by the plain kernel
 
+  // distortion calibration data (ptg_equiproj_set_data)
+  int32_t eq_G = 0, eq_nimg = 0, eq_npairs = 0;
+  double eq_NCC_min = 0.0;
+  int32_t* d_eq_pairs = nullptr;
+  double* d_eq_ref = nullptr;
+  double* d_eq_disp = nullptr;
+  double* d_eq_ncc = nullptr;
+  double* d_eq_partial = nullptr;
+  double* d_eq_out = nullptr;
+  double* d_eq_res = nullptr;
+  double* d_eq_jac = nullptr;
+  size_t eq_pairs_cap = 0, eq_ref_cap = 0, eq_disp_cap = 0, eq_ncc_cap = 0, eq_partial_cap = 0, eq_out_cap = 0,
+         eq_res_cap = 0, eq_jac_cap = 0;
+
   // objective / sampler batches
   void* d_tmp = nullptr;
   size_t tmp_cap = 0;
@@ -173,6 +187,21 @@ int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vect
                      uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
                      bool time_kernel);
 void free_exact_plans(Context& c);
+
+// Equiprojectivity criterion of the distortion calibration (device data, see ptgpu.h).
+struct EquiprojArgs {
+  int32_t G, nimg, npairs, pad;
+  const int32_t* pairs;  // npairs x (i, j)
+  const double* ref;     // G x 2: reference pixels
+  const double* disp;    // nimg x G x 2: tracked displacements
+  const double* ncc;     // nimg x G: NCC_subpix
+  double NCC_min;
+  double* res;           // nimg * npairs residuals, or null
+  double* jac;           // nimg * npairs x 14, or null
+  double* partial;       // per workgroup: sum r^2, J^T r (14), J^T J upper triangle (105)
+};
+hipError_t launch_equiproj(Context& c, const EquiprojArgs& a, const double* params14, double* d_out);
+int equiproj_partial_blocks(int64_t m);
 
 // subpixel
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);

@@ -135,7 +135,15 @@ def render_pair(rng, W, H, centres, radii, disp, rot, bits, background, base, am
         inside[y0:y1 + 1, x0:x1 + 1] |= (xx - centres[g, 0]) ** 2 + (yy - centres[g, 1]) ** 2 <= r * r
     f1 = np.where(inside, base + field, background)
     I0 = np.clip(np.round(f1), 0, maxv).astype(np.uint16)
-    # frame 2: each grain resampled from frame 1 about its centre
+    return I0, resample_grains(rng, I0, centres, radii, disp, rot, bits, background, noise_sigma)
+
+
+def resample_grains(rng, I0, centres, radii, disp, rot, bits, background, noise_sigma):
+    """The second frame: each grain of I0 moved by disp and rot (Catmull-Rom resampling
+    about its centre) on a flat background, plus Gaussian noise."""
+    H, W = I0.shape
+    G = len(radii)
+    maxv = float((1 << bits) - 1)
     f2 = np.full((H, W), float(background))
     I0f = I0.astype(np.float64)
     for g in range(G):
@@ -152,8 +160,7 @@ def render_pair(rng, W, H, centres, radii, disp, rot, bits, background, base, am
         qy = centres[g, 1] + st * px + ct * py
         f2[yy[m], xx[m]] = cubic_sample(I0f, qx, qy)
     f2 += rng.normal(0.0, noise_sigma, size=f2.shape)
-    I1 = np.clip(np.round(f2), 0, maxv).astype(np.uint16)
-    return I0, I1
+    return np.clip(np.round(f2), 0, maxv).astype(np.uint16)
 
 
 def make_grains(centres, radii):
@@ -358,3 +365,71 @@ def sequence(n_frames=16, n_grains=10000, size=(6000, 4000), seed=1005, radius=(
     prm = track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=search_method,
                        subpix=default_subpix(subpix_method))
     return SynthSequence(frames, grains, pxy, pb, bits, centres, rots, prm, "cfg5")
+
+
+class CalibrationSet:
+    """Frames of one static scene at imposed rigid shifts, seen through a lens: frames[k]
+    shows the scene moved by shifts[k] (shifts[0] = 0), each warped by the radial
+    distortion params14 (undistor of a frame pixel gives its scene position)."""
+
+    def __init__(self, frames, grains, pattern_xy, pattern_begin, bits, shifts, params14, truth):
+        self.frames, self.grains = frames, grains
+        self.pattern_xy, self.pattern_begin = pattern_xy, pattern_begin
+        self.bits, self.shifts, self.params14, self.truth = bits, shifts, params14, truth
+
+    @property
+    def G(self):
+        return len(self.grains)
+
+
+def calibration_set(n_grains=2000, size=(6000, 4000), seed=1004, shifts=((0, 0), (36, 11), (-29, 33), (22, -38)),
+                    k1=-2e-9, k2=1e-16, margin=240, noise_sigma=8.0) -> CalibrationSet:
+    """Workload 4 for the distortion calibration (the reference's correction_distortion,
+    src/distortion_correction.cpp:135-479): the config-2 grains and texture, the whole
+    scene translated by each shift (the imposed displacements), every frame warped by
+    radial distortion k1, k2 about the frame centre. Grains start with dx, dy = 0 at
+    integer reference pixels; the tracker gets the shift as its starting displacement."""
+    rng = np.random.default_rng(seed)
+    W, H = size
+    G = n_grains
+    bits = 12
+    radii = np.clip(np.exp(rng.normal(np.log(20.0), 0.4, G)), 8.0, 40.0)
+    m = min(margin, (min(W, H) - 1) // 4)
+    centres = place_grains(rng, G, W, H, radii, m, gap=4)
+    nblobs = np.maximum(1, np.round(0.1 * np.pi * radii ** 2)).astype(int)
+    zero = np.zeros((G, 2))
+    I0, _ = render_pair(rng, W, H, centres, radii, zero, np.zeros(G), bits, background=160, base=640,
+                        amp_range=(960, 3200), blobs_per_grain=nblobs, sigma_range=(1.0, 3.0), noise_sigma=noise_sigma)
+    X = np.array([0.5 * W, 0.5 * H, k1, k2, 0, 0, 0, 0, 0.5 * W, 0.5 * H, 0, 0, 0, 0], dtype=np.float64)
+    frames = []
+    for t in shifts:
+        moved = resample_grains(rng, I0, centres, radii, np.tile(np.asarray(t, dtype=np.float64), (G, 1)),
+                                np.zeros(G), bits, 160, noise_sigma)
+        frames.append(distort_frame(moved, X, bits))
+    # reference pixels: the distorted images of the grain centres, rounded
+    gr = np.zeros(G, dtype=GRAIN_DTYPE)
+    cd = np.array([_distort_point(X, c[0], c[1]) for c in centres])
+    gr["refcoord_xpix"] = np.round(cd[:, 0]).astype(np.int32)
+    gr["refcoord_ypix"] = np.round(cd[:, 1]).astype(np.int32)
+    gr["radius_pix"] = radii
+    pxy, pb = make_patterns(np.maximum(4.0, radii - 2.0))
+    return CalibrationSet(frames, gr, pxy, pb, bits, np.asarray(shifts, dtype=np.float64), X,
+                          {"centres": centres, "radii": radii})
+
+
+def _distort_point(X, xu, yu):
+    """The frame pixel whose undistor image (radial part) is (xu, yu): Newton iterations."""
+    x, y = xu, yu
+    for _ in range(30):
+        dx, dy = x - X[0], y - X[1]
+        r2 = dx * dx + dy * dy
+        k = X[2] * r2 + X[3] * r2 * r2
+        fx, fy = x + dx * k - xu, y + dy * k - yu
+        kp = X[2] + 2.0 * X[3] * r2
+        j11 = 1.0 + k + 2.0 * dx * dx * kp
+        j22 = 1.0 + k + 2.0 * dy * dy * kp
+        j12 = 2.0 * dx * dy * kp
+        det = j11 * j22 - j12 * j12
+        x -= (j22 * fx - j12 * fy) / det
+        y -= (j11 * fy - j12 * fx) / det
+    return x, y

@@ -1,0 +1,196 @@
+"""GPU: the distortion calibration (src/distortion_correction.cpp:135-479) through the C
+ABI. The equiprojectivity residuals against the oracle, their analytic Jacobian against
+central differences and against MINPACK's forward-difference Jacobian (the reference's
+lmdif path), the reduced normal equations, the Levenberg-Marquardt fit against scipy's
+MINPACK fit and the lens, and the whole differentiation form on distorted frames: grains
+tracked with the LM subpixel stage, then K1, K2 fitted."""
+import sys
+
+import numpy as np
+import pytest
+
+import ptref
+
+pk = ptref.package()
+from ptracker_amd import synth  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, __file__.rsplit("/", 1)[0])
+from test_oracle import _lens, _rigid_views  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a, dtype=np.float64).view(np.uint64), np.asarray(b, dtype=np.float64).view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def views():
+    X = _lens()
+    rng = np.random.default_rng(21)
+    ref = np.round(rng.uniform(300, 5700, (70, 2)))
+    disp = _rigid_views(X, ref, [(36.0, 11.0), (-29.0, 33.0), (22.0, -38.0)])
+    disp += rng.normal(0, 0.01, disp.shape)  # tracking noise
+    ncc = rng.uniform(0.75, 1.0, (3, 70))
+    ncc[2, 5] = 0.3  # drops the pairs of grain 5 in image 2
+    pairs = ptref.precompute_pairs(ref, 0.0, 1000.0)
+    return X, ref, disp, ncc, pairs
+
+
+def every_term():
+    # all 14 parameters away from zero, so every path of the derivative is exercised
+    return np.array([2950.0, 2040.0, -2e-9, 1e-16, 3e-24, 2e-8, -1e-8, 3e-8, 3010.0, 1985.0, 0.02, 1e-3, -0.015,
+                     -2e-3])
+
+
+def test_residuals_equal_oracle(tracker, views):
+    X, ref, disp, ncc, pairs = views
+    say(f"equiproj residuals: {len(pairs)} pairs x 3 images")
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    for P in (X, every_term()):
+        f, jtr, jtj, res, jac = tracker.equiproj_eval(P, full=True)
+        exp = ptref.equiproj_residuals(P, ref, disp, ncc, pairs, NCC_min=0.7)
+        assert bits_equal(res, exp)
+        assert abs(f - (exp ** 2).sum()) <= 1e-12 * (exp ** 2).sum()
+        # the reduced sums are those of the returned residuals and Jacobian
+        assert np.allclose(jtr, jac.T @ res, rtol=1e-11, atol=0.0)
+        assert np.allclose(jtj, jac.T @ jac, rtol=1e-11, atol=1e-300)
+    assert (res.reshape(3, -1)[2, (pairs[:, 0] == 5) | (pairs[:, 1] == 5)] == 0.0).all()
+
+
+def test_jacobian_against_central_differences(tracker, views):
+    _, ref, disp, ncc, pairs = views
+    P = every_term()
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    _, _, _, res, jac = tracker.equiproj_eval(P, full=True)
+    # steps between truncation and rounding error: 0.1 px for the distortion centre, 1e-4
+    # rad for the parallax angles, about 1e-3 relative for the coefficients
+    h = np.array([0.1, 0.1, 2e-12, 1e-19, 3e-27, 2e-10, 1e-10, 3e-10, 1e-3, 1e-3, 1e-4, 1e-5, 1e-4, 2e-5])
+    for j in range(14):
+        Pp, Pm = P.copy(), P.copy()
+        Pp[j] += h[j]
+        Pm[j] -= h[j]
+        cd = (ptref.equiproj_residuals(Pp, ref, disp, ncc, pairs, 0.7) -
+              ptref.equiproj_residuals(Pm, ref, disp, ncc, pairs, 0.7)) / (2 * h[j])
+        if j in (8, 9):
+            # the parallax centre moves every undistorted point alike: residuals do not
+            # depend on it (central differences are rounding noise)
+            assert np.abs(jac[:, j]).max() <= 1e-9 * np.abs(jac).max() and np.abs(cd).max() < 1e-6
+            continue
+        scale = np.abs(cd).max()
+        assert np.abs(jac[:, j] - cd).max() < 1e-6 * scale, (j, np.abs(jac[:, j] - cd).max() / scale)
+
+
+def test_jacobian_against_minpack_forward_differences(tracker, views):
+    _, ref, disp, ncc, pairs = views
+    P = every_term()
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    _, _, _, _, jac = tracker.equiproj_eval(P, full=True)
+    fun = lambda x: ptref.equiproj_residuals(x, ref, disp, ncc, pairs, 0.7)  # noqa: E731
+    Jf = ptref.lmdif_jacobian(fun, P)
+    h = np.array([0.1, 0.1, 2e-12, 1e-19, 3e-27, 2e-10, 1e-10, 3e-10, 1e-3, 1e-3, 1e-4, 1e-5, 1e-4, 2e-5])
+    for j in range(14):
+        if j in (8, 9):  # columns that are zero (see the central-difference test)
+            continue
+        scale = np.abs(Jf[:, j]).max()
+        err = np.abs(jac[:, j] - Jf[:, j]).max() / scale
+        if j <= 4:
+            # centre and radial terms: MINPACK's columns hold to about 1e-4
+            assert err < 1e-3, (j, err)
+        else:
+            # tangential and parallax terms: MINPACK's step sqrt(eps) |x| is below the
+            # residuals' rounding, its columns are off by 1e-3 to 1e-1; the analytic column
+            # is the one that agrees with well-stepped central differences
+            Pp, Pm = P.copy(), P.copy()
+            Pp[j] += h[j]
+            Pm[j] -= h[j]
+            cd = (fun(Pp) - fun(Pm)) / (2 * h[j])
+            assert np.abs(jac[:, j] - cd).max() < 1e-2 * np.abs(Jf[:, j] - cd).max(), j
+
+
+def test_calibration_matches_minpack_fit_and_lens(tracker, views):
+    from scipy.optimize import leastsq
+
+    X, ref, disp, ncc, pairs = views
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    start = X.copy()
+    start[2] = start[3] = 0.0
+    free = np.zeros(14, dtype=np.int32)
+    free[[2, 3]] = 1
+    got, f0, f1, it = tracker.calibrate_distortion(start, free, max_iter=100)
+    say(f"LM: {it} iterations, criterion {f0:.3e} -> {f1:.3e}, K1 {got[2]:.6e}, K2 {got[3]:.6e}")
+    assert f1 < 1e-2 * f0  # down to the tracking-noise floor
+    assert (got[[0, 1] + list(range(4, 14))] == start[[0, 1] + list(range(4, 14))]).all()
+
+    def fun(q):
+        Y = start.copy()
+        Y[2], Y[3] = q[0] * 1e-9, q[1] * 1e-16
+        return ptref.equiproj_residuals(Y, ref, disp, ncc, pairs, 0.7)
+
+    q, _ = leastsq(fun, [0.0, 0.0], ftol=np.finfo(float).eps, xtol=np.finfo(float).eps, gtol=0.0)
+    assert f1 <= (fun(q) ** 2).sum() * (1.0 + 1e-9)
+    assert abs(got[2] - q[0] * 1e-9) < 1e-6 * abs(got[2]) and abs(got[3] - q[1] * 1e-16) < 1e-5 * abs(got[3])
+    # 0.01 px of tracking noise on 70 grains leaves the fit about 0.5% from the lens
+    assert abs(got[2] - X[2]) < 2e-2 * abs(X[2]) and abs(got[3] - X[3]) < 2e-2 * abs(X[3])
+
+
+def test_differentiation_form_on_distorted_frames(tracker):
+    # the config-4 lens on a small scene seen at four imposed shifts: track each image from
+    # the first (search +-8 px, LM subpixel), then fit K1, K2 by the equiprojectivity
+    # criterion; compared with the lens and with MINPACK on the same tracked positions
+    from scipy.optimize import leastsq
+
+    C = synth.calibration_set(n_grains=90, size=(1600, 1200), seed=1044, k1=-2e-8, k2=1e-14)
+    say(f"calibration frames: {C.G} grains, {len(C.frames)} images")
+    z = pk.zone(8, num_rot=0, inc_rot=1.0)
+    prm = pk.track_params(z, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    tracker.set_image(0, C.frames[0], C.bits)
+    disp, ncc = [], []
+    for k in range(1, len(C.frames)):
+        gr = C.grains.copy()
+        gr["dx"], gr["dy"] = C.shifts[k]
+        tracker.set_image(1, C.frames[k], C.bits)
+        tracker.set_image_indices(0, 1)
+        tracker.set_grains(gr, C.pattern_xy, C.pattern_begin)
+        tracker.track_pair(prm)
+        out = tracker.get_grains()
+        disp.append(np.stack([out["dx"], out["dy"]], axis=1))
+        ncc.append(out["NCC_subpix"])
+    disp, ncc = np.array(disp), np.array(ncc)
+    assert (ncc > 0.9).mean() > 0.9
+    ref = np.stack([C.grains["refcoord_xpix"], C.grains["refcoord_ypix"]], axis=1).astype(np.float64)
+    pairs = ptref.precompute_pairs(ref, 0.0, 1000.0)
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    start = C.params14.copy()
+    start[2] = start[3] = 0.0
+    free = np.zeros(14, dtype=np.int32)
+    free[[2, 3]] = 1
+    got, f0, f1, it = tracker.calibrate_distortion(start, free, max_iter=100)
+    say(f"K1 {got[2]:.5e} (lens {C.params14[2]:.1e}), K2 {got[3]:.5e} (lens {C.params14[3]:.1e}), {it} iterations")
+
+    def fun(q):
+        Y = start.copy()
+        Y[2], Y[3] = q[0] * 1e-8, q[1] * 1e-14
+        return ptref.equiproj_residuals(Y, ref, disp, ncc, pairs, 0.7)
+
+    q, _ = leastsq(fun, [0.0, 0.0], ftol=np.finfo(float).eps, xtol=np.finfo(float).eps, gtol=0.0)
+    assert abs(got[2] - q[0] * 1e-8) < 1e-6 * abs(got[2]) and abs(got[3] - q[1] * 1e-14) < 1e-5 * abs(got[3])
+    # K1 and K2 trade off over a small field; the fitted correction itself matches the lens
+    # wherever there are grains (the lens moves those points by up to 5 px)
+    xs = np.linspace(ref[:, 0].min(), ref[:, 0].max(), 9)
+    ys = np.linspace(ref[:, 1].min(), ref[:, 1].max(), 9)
+    diff, size = 0.0, 0.0
+    for x in xs:
+        for y in ys:
+            fx, fy = ptref.undistor(got, x, y)
+            lx, ly = ptref.undistor(C.params14, x, y)
+            diff = max(diff, np.hypot(fx - lx, fy - ly))
+            size = max(size, np.hypot(lx - x, ly - y))
+    say(f"fitted vs lens correction: max {diff:.3f} px over the grains' region (lens: up to {size:.2f} px)")
+    assert size > 3.0 and diff < 0.15
+    assert abs(got[2] - C.params14[2]) < 0.1 * abs(C.params14[2])

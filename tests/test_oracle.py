@@ -493,3 +493,78 @@ def test_change_reference_matches_closed_form():
     h = ptref.change_reference(g)
     assert h["refcoord_xpix"].tolist() == [13, 20]
     assert np.allclose(h["dx"], [-0.4, -0.4]) and np.allclose(h["refrot"], [0.15, -0.02]) and np.all(h["drot"] == 0)
+
+
+# ---- distortion calibration: the equiprojectivity criterion ---------------------------------
+def _lens(k1=-2e-9, k2=1e-16, W=6000, H=4000):
+    return np.array([W / 2, H / 2, k1, k2, 0, 0, 0, 0, W / 2, H / 2, 0, 0, 0, 0], dtype=np.float64)
+
+
+def _rigid_views(X, ref, shifts):
+    """Tracked displacements of reference pixels ref when the scene moves rigidly by each
+    shift behind the lens X: the pixel showing undistor(ref) + t, minus ref."""
+    disp = np.zeros((len(shifts), len(ref), 2))
+    for k, t in enumerate(shifts):
+        for g, (x, y) in enumerate(ref):
+            ux, uy = ptref.undistor(X, x, y)
+            dx, dy = ptref.distort_point(X, ux + t[0], uy + t[1])
+            disp[k, g] = (dx - x, dy - y)
+    return disp
+
+
+def test_precompute_pairs_matches_brute_force():
+    ref = np.random.default_rng(3).integers(0, 800, (40, 2)).astype(np.float64)
+    exp = [(i, j) for i in range(40) for j in range(i + 1, 40)
+           if 100.0 <= math.sqrt((ref[j, 0] - ref[i, 0]) ** 2 + (ref[j, 1] - ref[i, 1]) ** 2) <= 400.0]
+    assert [tuple(p) for p in ptref.precompute_pairs(ref, 100.0, 400.0)] == exp
+
+
+def test_equiproj_uniform_scale_closed_form():
+    # no correction (identity parameters) and a uniform scale s about c: every residual is
+    # (s - 1) |B1 - A1|
+    X = _lens(0.0, 0.0)
+    rng = np.random.default_rng(4)
+    ref = rng.uniform(500, 5500, (30, 2))
+    s, c = 1.0 + 3e-4, np.array([2500.0, 1800.0])
+    disp = ((s - 1.0) * (ref - c))[None]
+    pairs = ptref.precompute_pairs(ref, 0.0, 1e9)
+    r = ptref.equiproj_residuals(X, ref, disp, np.ones((1, 30)), pairs)
+    exp = (s - 1.0) * np.hypot(*(ref[pairs[:, 1]] - ref[pairs[:, 0]]).T)
+    assert np.abs(r - exp).max() < 1e-9
+
+
+def test_equiproj_vanishes_for_rigid_motion_at_the_true_lens_only():
+    X = _lens()
+    rng = np.random.default_rng(5)
+    ref = np.round(rng.uniform(300, 5700, (25, 2)))
+    disp = _rigid_views(X, ref, [(36.0, 11.0), (-29.0, 33.0)])
+    pairs = ptref.precompute_pairs(ref, 0.0, 1000.0)
+    ncc = np.ones((2, 25))
+    assert np.abs(ptref.equiproj_residuals(X, ref, disp, ncc, pairs)).max() < 1e-8
+    assert np.abs(ptref.equiproj_residuals(_lens(0.0, 0.0), ref, disp, ncc, pairs)).max() > 1e-2
+    # a grain below NCC_min drops its pairs
+    ncc[1, 3] = 0.5
+    r = ptref.equiproj_residuals(_lens(0.0, 0.0), ref, disp, ncc, pairs, NCC_min=0.7).reshape(2, -1)
+    drop = (pairs[:, 0] == 3) | (pairs[:, 1] == 3)
+    assert (r[1, drop] == 0.0).all() and (r[0, drop] != 0.0).all()
+
+
+def test_minpack_least_squares_recovers_the_lens():
+    # the reference's LEVMAR path is MINPACK lmdif (forward-difference Jacobian); scipy's
+    # leastsq is that routine. On scaled K1, K2 it recovers the lens from rigid views.
+    from scipy.optimize import leastsq
+
+    X = _lens()
+    rng = np.random.default_rng(6)
+    ref = np.round(rng.uniform(300, 5700, (60, 2)))
+    disp = _rigid_views(X, ref, [(36.0, 11.0), (-29.0, 33.0), (22.0, -38.0)])
+    pairs = ptref.precompute_pairs(ref, 0.0, 1000.0)
+    ncc = np.ones((3, 60))
+
+    def fun(q):
+        Y = X.copy()
+        Y[2], Y[3] = q[0] * 1e-9, q[1] * 1e-16
+        return ptref.equiproj_residuals(Y, ref, disp, ncc, pairs)
+
+    q, _ = leastsq(fun, [0.0, 0.0], ftol=np.finfo(float).eps, xtol=np.finfo(float).eps, gtol=0.0)
+    assert abs(q[0] * 1e-9 - X[2]) < 1e-6 * abs(X[2]) and abs(q[1] * 1e-16 - X[3]) < 1e-5 * abs(X[3])
