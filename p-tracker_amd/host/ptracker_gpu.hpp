@@ -10,6 +10,8 @@
 
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -166,6 +168,82 @@ class ptracker_gpu {
       }
     }
     return n;
+  }
+
+  // correction_distortion, src/distortion_correction.cpp:135-479: the images
+  // image_numbers_corrDisto[1..] tracked from image_numbers_corrDisto[0] (every grain
+  // starting at the imposed displacement of the image; search and subpixel, no rescue),
+  // the grain pairs between equiProj_dist_min and equiProj_dist_max apart, then the
+  // equiprojectivity criterion minimised on the GPU over the parameters whose
+  // perturbation (d*_corrDistor) is non-zero, as those the reference's Powell moves.
+  // Writes dic_out_<num>.txt per image and corrDisto_log.txt (the reference's log lines)
+  // into out_dir; returns the 14 parameters.
+  std::vector<double> correction_distortion(tracker_settings& s, const std::string& out_dir = "") {
+    const std::vector<int>& im = s.image_numbers_corrDisto;
+    if (im.size() < 2) throw std::runtime_error("image_numbers_corrDisto needs at least two images");
+    if (s.RawImages) throw std::runtime_error("RAW images are not read (libraw is not part of this build)");
+    apply_settings(s);
+    const tiff_frame f0 = read_tiff(numbered_name(s.image_name, im[0]));
+    set_image(im_index_ref, f0.gray.data(), f0.width, f0.height, 16);
+    const int G = (int)grain.size();
+    const int nimg = (int)im.size() - 1;
+    std::vector<double> ref(2 * (size_t)G), disp(2 * (size_t)nimg * G), ncc((size_t)nimg * G);
+    for (int g = 0; g < G; g++) {
+      ref[2 * g] = grain[g].refcoord_xpix;
+      ref[2 * g + 1] = grain[g].refcoord_ypix;
+    }
+    const int rescue = rescue_level;
+    rescue_level = 0;  // no rescue is attempted (:193)
+    for (int k = 1; k <= nimg; k++) {
+      const tiff_frame f = read_tiff(numbered_name(s.image_name, im[k]));
+      set_image(im_index_current, f.gray.data(), f.width, f.height, 16);
+      for (auto& g : grain) {
+        g.dx = k < (int)s.imposed_displ_x_pix.size() ? s.imposed_displ_x_pix[k] : 0.0;
+        g.dy = k < (int)s.imposed_displ_y_pix.size() ? s.imposed_displ_y_pix[k] : 0.0;
+        g.drot = 0.0;
+        g.reset();
+      }
+      upload_grains();
+      track_pair();
+      for (int g = 0; g < G; g++) {
+        disp[2 * ((size_t)(k - 1) * G + g)] = grain[g].dx;
+        disp[2 * ((size_t)(k - 1) * G + g) + 1] = grain[g].dy;
+        ncc[(size_t)(k - 1) * G + g] = grain[g].NCC_subpix;
+      }
+      save_grains(out_dir + numbered_name("dic_out_%d.txt", im[k]), grain, false, numbered_name(s.image_name, im[0]),
+                  f0.dateTime, numbered_name(s.image_name, im[k]), f.dateTime);
+    }
+    rescue_level = rescue;
+    // centres left negative default to the image centre (:230-233)
+    std::vector<double> X(s.disto_parameters, s.disto_parameters + 14);
+    if (X[0] < 0.0) X[0] = 0.5 * f0.width;
+    if (X[1] < 0.0) X[1] = 0.5 * f0.height;
+    if (X[8] < 0.0) X[8] = 0.5 * f0.width;
+    if (X[9] < 0.0) X[9] = 0.5 * f0.height;
+    const std::vector<int32_t> pairs = precompute_pairs(grain, s.equiProj_dist_min, s.equiProj_dist_max);
+    check(ptg_equiproj_set_data(ctx_, G, ref.data(), nimg, disp.data(), ncc.data(), (int32_t)(pairs.size() / 2),
+                                pairs.data(), s.NCC_min),
+          "ptg_equiproj_set_data");
+    int32_t free14[14];
+    for (int q = 0; q < 14; q++) free14[q] = s.disto_parameters_perturb[q] != 0.0 ? 1 : 0;
+    double f_ini = 0.0, f_end = 0.0;
+    int32_t iters = 0;
+    check(ptg_calibrate_distortion(ctx_, X.data(), free14, 200, &f_ini, &f_end, &iters), "ptg_calibrate_distortion");
+    // the reference reports the criterion divided by the number of image pairs (:234-236)
+    const double index_ini = f_ini / nimg, index_end = f_end / nimg;
+    std::ofstream log(out_dir + "corrDisto_log.txt");
+    log << std::scientific << std::setprecision(15);
+    log << "Initial distorsion index " << index_ini << '\n';
+    log << "Final distorsion index " << index_end << '\n';
+    log << "Index gain (>1 is good) " << index_ini / index_end << '\n';
+    static const char* const names[14] = {"xc_corrDistor ", "yc_corrDistor ", "K1_corrDistor ", "K2_corrDistor ",
+                                          "K3_corrDistor ", "P1_corrDistor ", "P2_corrDistor ", "P3_corrDistor ",
+                                          "xc_corrParallax     ", "yc_corrParallax     ", "angle1_corrParallax ",
+                                          "A1_corrParallax     ", "angle2_corrParallax ", "A2_corrParallax     "};
+    for (int q = 0; q < 14; q++) log << names[q] << X[q] << (q == 13 ? "" : "\n");
+    log << std::endl;
+    for (int q = 0; q < 14; q++) s.disto_parameters[q] = X[q];
+    return X;
   }
 
  private:

@@ -421,6 +421,24 @@ inline void read_data(const std::string& name, tracker_settings& s, std::vector<
   }
 }
 
+// precompute_paires, src/distortion_correction.cpp:62-85: pairs (i < j) whose reference
+// pixels lie between dmin and dmax apart, i outer, j inner.
+inline std::vector<int32_t> precompute_pairs(const std::vector<grain_type_2D>& grain, double dmin, double dmax) {
+  std::vector<int32_t> pairs;
+  for (size_t i = 0; i < grain.size(); i++) {
+    const double xi = grain[i].refcoord_xpix, yi = grain[i].refcoord_ypix;
+    for (size_t j = i + 1; j < grain.size(); j++) {
+      const double xij = grain[j].refcoord_xpix - xi, yij = grain[j].refcoord_ypix - yi;
+      const double dij = std::sqrt(xij * xij + yij * yij);
+      if (dij >= dmin && dij <= dmax) {
+        pairs.push_back((int32_t)i);
+        pairs.push_back((int32_t)j);
+      }
+    }
+  }
+  return pairs;
+}
+
 // What the reference's init does after reading the command file: without rotation
 // tracking every zone searches one step (inc_rot 1, num_rot 0), src/p-tracker.cpp:1191-1201.
 inline void finish_settings(tracker_settings& s) {

@@ -5,12 +5,13 @@
 // Options: --lm     subpixel by the LM form instead of Powell
 //          --plain  integer search by the plain FP64 form instead of the exact MFMA form
 //          --device N
-// particle_tracking runs once (the reference's run() runs it a second time after the
-// first for that procedure name, src/p-tracker.cpp:1802-1808); correction_distortion is
-// not part of this program.
+// Procedures: particle_tracking (run once; the reference's run() runs it a second time
+// for that procedure name, src/p-tracker.cpp:1802-1808) and correction_distortion (the
+// equiprojectivity fit on the GPU by Levenberg-Marquardt in place of Powell / lmdif).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <string>
 
@@ -32,16 +33,21 @@ int main(int argc, char** argv) {
     ptgpu::read_data(cmd, s, grains);
     ptgpu::finish_settings(s);
     if (s.rotations == 0) std::cout << "Rotation are NOT TRACKED!" << std::endl;
-    if (s.procedure != "particle_tracking") {
-      std::cerr << "procedure " << s.procedure << " is not part of this program" << std::endl;
-      return 1;
-    }
     ptgpu::ptracker_gpu pt(device);
     pt.grain = grains;
     pt.search_method = search;
     pt.subpix_method = subpix;
-    const int n = pt.particle_tracking(s);
-    std::cout << "tracked " << pt.grain.size() << " grains over " << n << " images" << std::endl;
+    if (s.procedure == "correction_distortion") {
+      const std::vector<double> X = pt.correction_distortion(s);
+      std::cout << std::scientific << std::setprecision(15) << "K1_corrDistor " << X[2] << "\nK2_corrDistor " << X[3]
+                << std::endl;
+    } else if (s.procedure == "particle_tracking") {
+      const int n = pt.particle_tracking(s);
+      std::cout << "tracked " << pt.grain.size() << " grains over " << n << " images" << std::endl;
+    } else {
+      std::cerr << "procedure " << s.procedure << " is not part of this program" << std::endl;
+      return 1;
+    }
   } catch (const std::exception& e) {
     std::cerr << e.what() << std::endl;
     return 1;

@@ -194,3 +194,90 @@ def test_differentiation_form_on_distorted_frames(tracker):
     say(f"fitted vs lens correction: max {diff:.3f} px over the grains' region (lens: up to {size:.2f} px)")
     assert size > 3.0 and diff < 0.15
     assert abs(got[2] - C.params14[2]) < 0.1 * abs(C.params14[2])
+
+
+def test_program_correction_distortion(tracker, tmp_path):
+    # procedure correction_distortion of the tracking program on TIFF frames: the same fit
+    # as through the C ABI on the same decoded frames
+    import os
+    import subprocess
+
+    from test_host_io import write_tiff
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "p-tracker_amd", "host", "ptracker_gpu")
+    io = os.path.join(root, "p-tracker_amd", "host", "ptracker_io_check")
+    C = synth.calibration_set(n_grains=60, size=(1400, 1000), seed=1045, k1=-2e-8, k2=1e-14)
+    for k, f in enumerate(C.frames):
+        write_tiff(tmp_path / f"cal{k}.tif", f.astype(np.int64) << 4, 1, 16)  # 12-bit values in 16-bit samples
+    with open(tmp_path / "start.txt", "w") as fs:
+        fs.write(f"{C.G}\n")
+        for g in C.grains:
+            fs.write(f"{int(g['refcoord_xpix'])} {int(g['refcoord_ypix'])} 0 {float(g['radius_pix'])!r}\n")
+    shifts = " ".join(f"{x:g} {y:g}" for x, y in C.shifts[1:])
+    (tmp_path / "commands.txt").write_text(f"""procedure correction_distortion
+image_name cal%d.tif
+grains_to_follow start.txt
+rotations 0
+subpixel 1
+rescue_level 0
+search_zone.left 8
+search_zone.right 8
+search_zone.up 8
+search_zone.down 8
+NCC_min 0.7
+pattern grain_diameter 1.0 4
+image_numbers_corrDisto {len(C.frames)} {' '.join(str(k) for k in range(len(C.frames)))}
+imposedDisplApprox_corrDisto {shifts}
+equiProj_dist_range 0 1000
+K1_corrDistor 0
+K2_corrDistor 0
+""" + "".join(f"d{n} 0\n" for n in ("xc_corrDistor", "yc_corrDistor", "K3_corrDistor", "P1_corrDistor",
+                                     "P2_corrDistor", "P3_corrDistor", "xc_corrParallax", "yc_corrParallax",
+                                     "angle1_corrParallax", "A1_corrParallax", "angle2_corrParallax",
+                                     "A2_corrParallax")))
+    say("program: correction_distortion")
+    r = subprocess.run([exe, "--lm", "commands.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    log = dict(l.rsplit(" ", 1) for l in (tmp_path / "corrDisto_log.txt").read_text().splitlines() if l.strip())
+    K1, K2 = float(log["K1_corrDistor"]), float(log["K2_corrDistor"])
+    assert float(log["Final distorsion index"]) < float(log["Initial distorsion index"])
+    assert os.path.exists(tmp_path / "dic_out_3.txt")
+    # the same steps through the C ABI on the frames as the program decodes them
+    frames = []
+    for k in range(len(C.frames)):
+        subprocess.run([io, "tiff", str(tmp_path / f"cal{k}.tif"), str(tmp_path / "f.raw")], check=True)
+        b = (tmp_path / "f.raw").read_bytes()
+        l1 = b.index(b"\n")
+        l2 = b.index(b"\n", l1 + 1)
+        w, h = map(int, b[:l1].split())
+        frames.append(np.frombuffer(b[l2 + 1:], dtype="<u2").reshape(h, w).copy())
+    pats = [synth.circ_pattern(max(int(r), 4)) for r in C.grains["radius_pix"]]
+    pxy = np.ascontiguousarray(np.concatenate(pats), dtype=np.int32)
+    pb = np.zeros(C.G + 1, dtype=np.int64)
+    pb[1:] = np.cumsum([len(p) for p in pats])
+    z = pk.zone(8, num_rot=0, inc_rot=1.0)
+    prm = pk.track_params(z, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    tracker.set_image(0, frames[0], 16)
+    disp, ncc = [], []
+    for k in range(1, len(frames)):
+        gr = C.grains.copy()
+        gr["dx"], gr["dy"] = C.shifts[k]
+        tracker.set_image(1, frames[k], 16)
+        tracker.set_image_indices(0, 1)
+        tracker.set_grains(gr, pxy, pb)
+        tracker.track_pair(prm)
+        out = tracker.get_grains()
+        disp.append(np.stack([out["dx"], out["dy"]], axis=1))
+        ncc.append(out["NCC_subpix"])
+    ref = np.stack([C.grains["refcoord_xpix"], C.grains["refcoord_ypix"]], axis=1).astype(np.float64)
+    tracker.equiproj_set_data(ref, np.array(disp), np.array(ncc), ptref.precompute_pairs(ref, 0.0, 1000.0), 0.7)
+    free = np.zeros(14, dtype=np.int32)
+    free[[2, 3]] = 1
+    start = C.params14.copy()
+    start[2] = start[3] = 0.0
+    got, _, _, _ = tracker.calibrate_distortion(start, free, max_iter=200)
+    say(f"program K1 {K1:.6e} K2 {K2:.6e}; C ABI K1 {got[2]:.6e} K2 {got[3]:.6e}")
+    assert abs(K1 - got[2]) <= 1e-9 * abs(got[2]) and abs(K2 - got[3]) <= 1e-9 * abs(got[3])
+    assert abs(K1 - C.params14[2]) < 0.2 * abs(C.params14[2])
