@@ -500,10 +500,11 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     if (lane == 0) dofs[nls] = run;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < nls * sl.dcap; t += nthreads) {
-    const int ks = t / sl.dcap, q = t - ks * sl.dcap;
-    const int pos = dofs[ks] + q;
-    if (q < dcnt[ks] && pos < e.dl_lds_cap) dent[pos] = 2u * (uint32_t)e.dlist[sl.dl_off + t];
+  // each wave copies whole lists (their actual length, up to the LDS capacity)
+  for (int ks = wave; ks < nls; ks += nthreads >> 6) {
+    const int n = min(dcnt[ks], e.dl_lds_cap - dofs[ks]), o = dofs[ks];
+    const int32_t* src = e.dlist + sl.dl_off + (int64_t)ks * sl.dcap;
+    for (int q = lane; q < n; q += 64) dent[o + q] = 2u * (uint32_t)src[q];
   }
   __syncthreads();
   const int gg = lane >> 4, col = lane & 15;
