@@ -471,7 +471,9 @@ int ptg_destroy(void* p) {
     (void)hipEventDestroy(pe.b);
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->aux_events) (void)hipEventDestroy(e);
   if (c->staging_done) (void)hipEventDestroy(c->staging_done);
+  if (c->aux_stream) (void)hipStreamDestroy(c->aux_stream);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return PTG_OK;

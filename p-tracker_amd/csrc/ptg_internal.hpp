@@ -58,6 +58,12 @@ struct SearchArgs {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of the exact search (each radius bucket's template and middle-step
+  // kernels run ahead of the MFMA kernels of the previous buckets), default priority, and
+  // its fork/join events (created on first use)
+  hipStream_t aux_stream = nullptr;
+  std::vector<hipEvent_t> aux_events;
+  int ensure_aux(size_t nevents);
   std::string err;
 
   Image img[kMaxImages];

@@ -779,6 +779,18 @@ struct ExactPlan {
 };
 }  // namespace
 
+int Context::ensure_aux(size_t nevents) {
+  hipError_t e;
+  if (!aux_stream && (e = hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking)) != hipSuccess)
+    return check(e, "hipStreamCreateWithFlags");
+  while (aux_events.size() < nevents) {
+    hipEvent_t ev;
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return check(e, "hipEventCreate");
+    aux_events.push_back(ev);
+  }
+  return PTG_OK;
+}
+
 void free_exact_plans(Context& c) {
   delete static_cast<ExactPlan*>(c.exact_plan);
   c.exact_plan = nullptr;
@@ -1051,8 +1063,16 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   *nparts_out = part_cap;
   k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
+  // the template and middle-step kernels of every bucket run in order on the second
+  // stream, each bucket's MFMA kernel on the library stream after its bucket's event, so
+  // the next bucket's preparation overlaps the current search
+  const size_t nb_all = P->buckets.size();
+  if (int rca = c.ensure_aux(nb_all + 1); rca != PTG_OK) return rca;
+  hipEvent_t fork = c.aux_events[nb_all];
+  if ((he = hipEventRecord(fork, c.stream)) != hipSuccess || (he = hipStreamWaitEvent(c.aux_stream, fork, 0)) != hipSuccess)
+    return c.check(he, "exact search fork");
   size_t off = 0;
-  for (size_t bi = 0; bi < P->buckets.size(); bi++) {
+  for (size_t bi = 0; bi < nb_all; bi++) {
     const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
@@ -1073,16 +1093,28 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       em.mid_npl = l.npl;
       const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
       const dim3 grid_m(nb, (ncb + l.cbw - 1) / l.cbw), block_m(64 * std::min(16, ntile_m * l.cbw));
-      if (l.npl == 6) k_exact_mid<6><<<grid_m, block_m, l.lds_m, c.stream>>>(em);
-      else k_exact_mid<3><<<grid_m, block_m, l.lds_m, c.stream>>>(em);
+      if (l.npl == 6) k_exact_mid<6><<<grid_m, block_m, l.lds_m, c.aux_stream>>>(em);
+      else k_exact_mid<3><<<grid_m, block_m, l.lds_m, c.aux_stream>>>(em);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     }
     if (l.lds_t > 65536 &&
         (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)l.lds_t)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
-    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, c.stream>>>(e);
+    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, c.aux_stream>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
+    if ((he = hipEventRecord(c.aux_events[bi], c.aux_stream)) != hipSuccess) return c.check(he, "hipEventRecord");
+  }
+  off = 0;
+  for (size_t bi = 0; bi < nb_all; bi++) {
+    const Bucket& bk = P->buckets[bi];
+    const int nb = (int)bk.slots.size();
+    if (nb == 0) continue;
+    const BucketLaunch& l = L[bi];
+    e.bucket = bucket_dev + off;
+    e.nbucket = nb;
+    off += nb;
+    if ((he = hipStreamWaitEvent(c.stream, c.aux_events[bi], 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
     ExactArgs eb = e;
     eb.ww = l.ww;
     eb.wh_pad = l.wh;
