@@ -476,9 +476,10 @@ struct Powell {
 // several minima in theta), then damped Newton (Levenberg-Marquardt on the objective's
 // Hessian) on (u, v, theta) until the step is below 1e-8 px and 1e-10 rad. The sweep's
 // line searches only have to land in the same basin as Powell's (Brent tolerance 3e-8),
-// but that needs more than a coarse bracket: on 2000 grains of a 6000x4000 frame pair 1e-4
-// reaches the oracle's minimum everywhere while 1e-2 sends one grain to another minimum.
-#define LM_SWEEP_TOL 1.0e-4
+// but that needs more than a coarse bracket: on 2000 grains of a 6000x4000 frame pair 1e-3
+// and 1e-4 reach the oracle's minimum everywhere while 1e-2 sends one grain to another
+// minimum.
+#define LM_SWEEP_TOL 1.0e-3
 #define LM_STEP_PX 1.0e-8
 #define LM_STEP_RAD 1.0e-10
 
