@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain search+Powell form (0: skip)")
     ap.add_argument("--cpu-grains", type=int, default=2000, help="grains in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the exact-vs-plain pick comparison (configs 2 and 3) after the timed run")
     ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
     ap.add_argument("--seq-frames", type=int, default=16)
     ap.add_argument("--seq-grains", type=int, default=10000)
@@ -355,7 +357,7 @@ def main():
         log(f"plain search+Powell: {1e3 * pe / args.plain_steps:.2f} ms/pair, {pv:.0f} grains/s")
     # precision form: integer picks of the exact MFMA search against the plain FP64 search
     # (bit-identical to the reference's arithmetic) on every grain, main stage only
-    if method == "exact":
+    if method == "exact" and not args.no_precision:
         prec = {"cfg2_21_steps_12bit": pick_mismatches(pk, tr, w)}
         tr.close()
         w3 = synth.workload(3, n_grains=args.grains)
