@@ -385,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 
 // ---- the MFMA search ---------------------------------------------------------------
 // Sums of I1 and I1^2 over one Delta list (ks = 2 step + sign) at this lane's shift: the
-// entries held in LDS (4 per 16-byte read, 8 in flight), then any beyond the LDS
+// entries held in LDS (4 per 16-byte read, 16 window reads in flight), then any beyond the LDS
 // capacity from global memory. A list has at most 1023 cells, so the sum of I1 is exact in
 // 32 bits; I1^2 is summed in 32 bits per group of at most 8 cells, then in 64.
 __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
@@ -397,6 +397,19 @@ __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* w
   const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
   auto px = [&](uint32_t off) { return (unsigned int)*reinterpret_cast<const uint16_t*>(wb + off); };
   int q = 0;
+  for (; q + 16 <= nl; q += 16) {
+    const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
+    const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
+    const uint4 e2 = *reinterpret_cast<const uint4*>(dent + o + q + 8);
+    const uint4 e3 = *reinterpret_cast<const uint4*>(dent + o + q + 12);
+    const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
+    const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
+    const unsigned int w8 = px(e2.x), w9 = px(e2.y), w10 = px(e2.z), w11 = px(e2.w);
+    const unsigned int w12 = px(e3.x), w13 = px(e3.y), w14 = px(e3.z), w15 = px(e3.w);
+    bsum += (((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))) + (((w8 + w9) + (w10 + w11)) + ((w12 + w13) + (w14 + w15)));
+    csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
+    csum += (unsigned int)(((w8 * w8 + w9 * w9) + (w10 * w10 + w11 * w11)) + ((w12 * w12 + w13 * w13) + (w14 * w14 + w15 * w15)));
+  }
   for (; q + 8 <= nl; q += 8) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
     const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
