@@ -442,7 +442,8 @@ int ptg_create(void** out, int device) {
 static void free_grain_buffers(Ctx* c) {
   void** bufs[] = {(void**)&c->d_grains, (void**)&c->d_pattern, (void**)&c->d_pbegin, (void**)&c->d_p0,
                    (void**)&c->d_d0,     (void**)&c->d_pwords, (void**)&c->d_prad,    (void**)&c->d_list[0], (void**)&c->d_list[1],
-                   (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0,  (void**)&c->d_list_lpt};
+                   (void**)&c->d_list[2], (void**)&c->d_cs0,    (void**)&c->d_sd0,  (void**)&c->d_list_lpt,
+                   (void**)&c->d_spwords, (void**)&c->d_m0};
   for (void** b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -579,7 +580,6 @@ static int upload_unmasked(Ctx* c, bool force) {
     lpt = ul;
     auto size_of = [c](int32_t g) { return c->h_pbegin[g + 1] - c->h_pbegin[g]; };
     std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
-    c->max_pattern_unmasked = lpt.empty() ? 0 : (int32_t)size_of(lpt[0]);
     c->max_prad_unmasked = 0;
     for (int32_t g : ul) c->max_prad_unmasked = std::max(c->max_prad_unmasked, c->h_prad[g]);
     if (!ul.empty()) {
@@ -632,13 +632,15 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
       hipMalloc((void**)&c->d_p0, sizeof(int32_t) * 2 * total) != hipSuccess ||
       hipMalloc((void**)&c->d_d0, sizeof(double) * total) != hipSuccess ||
       hipMalloc((void**)&c->d_pwords, sizeof(uint32_t) * total) != hipSuccess ||
+      hipMalloc((void**)&c->d_spwords, sizeof(uint32_t) * total) != hipSuccess ||
       hipMalloc((void**)&c->d_prad, sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[0], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[1], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list[2], sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_list_lpt, sizeof(int32_t) * Gs) != hipSuccess ||
       hipMalloc((void**)&c->d_cs0, sizeof(double) * 2 * Gs) != hipSuccess ||
-      hipMalloc((void**)&c->d_sd0, sizeof(double) * Gs) != hipSuccess) {
+      hipMalloc((void**)&c->d_sd0, sizeof(double) * Gs) != hipSuccess ||
+      hipMalloc((void**)&c->d_m0, sizeof(double) * Gs) != hipSuccess) {
     free_grain_buffers(c);
     c->G = 0;
     return c->fail(PTG_ERR_NOMEM, "grain buffers for %d grains / %lld pixels", G, (long long)total);

@@ -77,6 +77,7 @@ struct Context {
   int32_t* d_p0 = nullptr;
   double* d_d0 = nullptr;
   uint32_t* d_pwords = nullptr;  // per pattern pixel: (dx, dy) as int8, I0(p0) as u16
+  uint32_t* d_spwords = nullptr; // per pattern pixel: rotated p0 offsets as int8, I0(p0) as u16
   int32_t* d_prad = nullptr;
   int64_t total_px = 0;
   std::vector<int32_t> h_pattern;
@@ -95,7 +96,6 @@ struct Context {
   // schedules its longest grains first
   int32_t* d_list_lpt = nullptr;
   std::vector<int32_t> h_list_lpt;
-  int32_t max_pattern_unmasked = 0;
   int32_t max_prad_unmasked = 0;
   const std::vector<int32_t>& h_unmasked_ref() const { return h_unmasked; }
 
@@ -113,6 +113,7 @@ struct Context {
   int32_t* d_evals_g = nullptr;  // per grain, last subpixel stage
   size_t evals_g_cap = 0;
   double* d_sd0 = nullptr;  // per grain: sum of d0 (used by the one-pass objective)
+  double* d_m0 = nullptr;   // per grain: the mean0 d0 was formed with (d0 = I0(p0) - m0)
 
   // exact-form scratch and the cached host plan of the last keyed list
   void* d_exact_scratch = nullptr;

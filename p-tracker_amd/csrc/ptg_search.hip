@@ -76,8 +76,9 @@ hipError_t launch_change_reference(Context& c) {
 // a sum of integers, exact in any order, so mean0 matches the reference's bits; C0C0
 // is accumulated by one lane in the reference's pixel order from an LDS chunk that the
 // whole wave stages. Also stores d0[i] = I0(p0_i) - mean0 (the reference's
-// `(double)image[...] - mean0`, :708) and the per-grain sum of d0 for the one-pass
-// subpixel objective.
+// `(double)image[...] - mean0`, :708), the per-grain sum of d0 for the one-pass
+// subpixel objective, and two packed words per pattern pixel (offsets as int8, I0 as
+// u16): unrotated for the exact search, rotated (p0) for the subpixel stage.
 constexpr int kPreChunk = 1024;
 
 __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ I0, int dimx, int dimy,
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
                                                    const int64_t* __restrict__ pbegin,
                                                    const double* __restrict__ cs0, int32_t* __restrict__ p0,
                                                    double* __restrict__ d0, double* __restrict__ sd0,
-                                                   uint32_t* __restrict__ pwords) {
+                                                   uint32_t* __restrict__ pwords, uint32_t* __restrict__ spwords,
+                                                   double* __restrict__ m0) {
   __shared__ double chunk[kPreChunk];
   const int g = blockIdx.x;
   const int lane = threadIdx.x;
@@ -112,6 +114,10 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
     // the exact search's compact copy: offsets as int8 (its patterns are well inside
     // +-127) and the reference value
     pwords[b + i] = ((uint32_t)px & 0xffu) | (((uint32_t)py & 0xffu) << 8) | (v << 16);
+    // the subpixel stage's: the rotated offsets (int8 when the pattern radius is at most
+    // 127, the only case it reads them) and the same value, from which d0 = v - m0 is
+    // re-formed bit for bit
+    spwords[b + i] = ((uint32_t)x & 0xffu) | (((uint32_t)y & 0xffu) << 8) | (v << 16);
   }
   s = wave_sum_i64(s);
   const double mean0 = (double)s / (double)n;
@@ -141,6 +147,7 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
     gr.mean0 = mean0;
     gr.C0C0 = C0C0;
     sd0[g] = sd;
+    m0[g] = mean0;
   }
 }
 
@@ -148,7 +155,7 @@ hipError_t launch_precompute(Context& c) {
   if (c.G == 0) return hipSuccess;
   const Image& im = c.img[c.iref];
   k_precompute<<<c.G, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern, c.d_pbegin,
-                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords);
+                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords, c.d_spwords, c.d_m0);
   return hipGetLastError();
 }
 

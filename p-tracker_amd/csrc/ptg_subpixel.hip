@@ -4,10 +4,11 @@
 // and a Levenberg-Marquardt form driven by the analytic derivative of the ZNCC residual
 // through the bicubic sampler.
 //
-// One workgroup of kSpWaves waves owns one grain. At its start the workgroup stages the
-// grain in LDS: a tile of the current frame around the grain (16-bit pixels) and the
-// pattern as one 32-bit word per pixel (x, y offsets as int8 and the reference image
-// value I0, from which d0 = I0 - mean0 is recomputed bit-exactly). The optimizers are
+// One workgroup of kSpWaves waves owns one grain. At its start the workgroup stages a
+// tile of the current frame around the grain (16-bit pixels) in LDS; the pattern is read
+// from global memory as one 32-bit word per pixel, written by the precomputation (x, y
+// offsets as int8 and the reference image value I0, from which d0 = I0 - mean0 is
+// re-formed bit-exactly). The optimizers are
 // written in reverse-communication form: wave 0 holds their state in LDS, consumes the
 // objective at the last requested point and publishes the next point (or "done"); every
 // wave then sums its share of the pattern pixels for that point. So the objective is
@@ -23,13 +24,12 @@ namespace ptg {
 constexpr int kSpWaves = 4;
 constexpr int kSpThreads = 64 * kSpWaves;
 constexpr int kSpSums = 24;             // per-pixel sums of the value-and-derivatives pass
-constexpr int kSpWordsCap = 8192;       // pattern pixels staged in LDS per grain
 constexpr int kSpTileBytesCap = 65536;  // the frame tile is skipped above this size
 
 enum { RQ_VALUE = 0, RQ_DERIVS = 1, RQ_DONE = 2 };
 
-// The workgroup's dynamic LDS. Tile and pattern words are addressed as offsets into it
-// (not through generic pointers) so that their loads are LDS loads.
+// The workgroup's dynamic LDS. The tile is addressed as an offset into it (not through a
+// generic pointer) so that its loads are LDS loads.
 extern __shared__ __align__(16) unsigned char sp_lds[];
 
 // The point an evaluation is requested at, as the pixel loop needs it.
@@ -50,9 +50,10 @@ struct Obj {
   int64_t pitch;
   bool tile;           // LDS tile of I1 present: 16-bit pixels, row stride tw, at kSpOffTile
   int tx0, ty0, tw, th;
-  int words_off;        // byte offset of the packed pattern words [0, nw) in LDS
+  const uint32_t* __restrict__ words;  // packed pattern words of this grain (used when nw == n)
+  double m0;                            // the mean0 of d0 = I0 - m0
   int nw;
-  const int32_t* __restrict__ p0;  // pattern0_rotated pairs of this grain (pixels [nw, n))
+  const int32_t* __restrict__ p0;  // pattern0_rotated pairs of this grain (used when nw == 0)
   const double* __restrict__ d0;   // I0(p0) - mean0 of this grain
   int n;
   double xr, yr;  // refcoord + dx, refcoord + dy (:1031-1032 before adding X)
@@ -219,10 +220,9 @@ __device__ __forceinline__ void sample_hess(const Obj& o, double x, double y, do
 // Calls f(x, y, d0) for this thread's share of the grain's pattern pixels.
 template <class F>
 __device__ __forceinline__ void for_pixels(const Obj& o, F& f) {
-  const uint32_t* words = reinterpret_cast<const uint32_t*>(sp_lds + o.words_off);
   for (int i = threadIdx.x; i < o.nw; i += kSpThreads) {
-    const uint32_t w = words[i];
-    f((int)(int8_t)(w & 0xffu), (int)(int8_t)((w >> 8) & 0xffu), (double)(w >> 16) - o.shift);
+    const uint32_t w = o.words[i];
+    f((int)(int8_t)(w & 0xffu), (int)(int8_t)((w >> 8) & 0xffu), (double)(w >> 16) - o.m0);
   }
   for (int i = o.nw + (int)threadIdx.x; i < o.n; i += kSpThreads) f(o.p0[2 * i], o.p0[2 * i + 1], o.d0[i]);
 }
@@ -797,14 +797,13 @@ __device__ __noinline__ int lm_feed(double f) {
 }
 
 // ---------------------------------------------------------------------------------
-// Workgroup staging of one grain: the frame tile and the packed pattern words in LDS.
+// Workgroup staging of one grain: the frame tile in LDS.
 struct SpLayout {
   int32_t tile_elems;  // 16-bit elements reserved for the tile (0: no tile)
-  int32_t words_cap;   // 32-bit pattern words reserved
   int32_t margin;      // tile pixels beyond the pattern's extent
 };
 
-// LDS: [SpState][kSpWaves x kSpSums doubles][tile][words]
+// LDS: [SpState][kSpWaves x kSpSums doubles][tile]
 constexpr size_t sp_align(size_t b) { return (b + 15) & ~(size_t)15; }
 constexpr size_t kSpOffRed = sp_align(sizeof(SpState));
 constexpr size_t kSpOffTile = kSpOffRed + sp_align(sizeof(double) * kSpWaves * kSpSums);
@@ -816,8 +815,6 @@ __device__ __forceinline__ unsigned int tile_u16(int idx) { return *(lds_u16*)(s
 
 __device__ __forceinline__ void stage_grain(Obj& o, const ptg_grain& gr, int prad, const SpLayout& lay) {
   uint16_t* tile = reinterpret_cast<uint16_t*>(sp_lds + kSpOffTile);
-  const int words_off = (int)(kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems));
-  uint32_t* words = reinterpret_cast<uint32_t*>(sp_lds + words_off);
   // the tile: the grain's pattern extent plus margin pixels, clamped to the frame edge
   const int h = prad + lay.margin;
   const int tw = 2 * h + 2;
@@ -833,29 +830,20 @@ __device__ __forceinline__ void stage_grain(Obj& o, const ptg_grain& gr, int pra
     }
     o.tile = true;
   }
-  // the pattern words: x, y as int8 and I0 = d0 + mean0 (recovered exactly: the pixel's
-  // word is used only when (double)I0 - mean0 reproduces its d0 bit for bit)
-  int nw = prad <= 127 ? min(o.n, lay.words_cap) : 0;
-  int bad = 0;
-  for (int i = threadIdx.x; i < nw; i += kSpThreads) {
-    const double dd = o.d0[i];
-    const double iv = rint(dd + o.shift);
-    const uint32_t I0 = (iv >= 0.0 && iv <= 65535.0) ? (uint32_t)iv : 0u;
-    if (!((double)I0 - o.shift == dd)) bad = 1;
-    words[i] = ((uint32_t)o.p0[2 * i] & 0xffu) | (((uint32_t)o.p0[2 * i + 1] & 0xffu) << 8) | (I0 << 16);
-  }
-  if (__syncthreads_or(bad)) nw = 0;
-  o.words_off = words_off;
-  o.nw = nw;
+  // the packed words hold the rotated offsets as int8: exact up to a pattern radius of 127
+  o.nw = prad <= 127 ? o.n : 0;
+  __syncthreads();  // the tile is written
 }
 
 __device__ __forceinline__ void load_obj(Obj& o, const ptg_grain& g, const uint16_t* I1, int dimx, int dimy,
-                                         int64_t pitch, const int32_t* p0, const double* d0, int n, double sd0,
-                                         double dmax, double stiff) {
+                                         int64_t pitch, const uint32_t* words, double m0, const int32_t* p0,
+                                         const double* d0, int n, double sd0, double dmax, double stiff) {
   o.I1 = I1;
   o.dimx = dimx;
   o.dimy = dimy;
   o.pitch = pitch;
+  o.words = words;
+  o.m0 = m0;
   o.p0 = p0;
   o.d0 = d0;
   o.n = n;
@@ -895,6 +883,8 @@ __device__ __noinline__ void sp_step() {
 __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy,
                                                          int64_t pitch, ptg_grain* __restrict__ grains,
                                                          const int64_t* __restrict__ pbegin,
+                                                         const uint32_t* __restrict__ words,
+                                                         const double* __restrict__ m0,
                                                          const int32_t* __restrict__ p0, const double* __restrict__ d0,
                                                          const double* __restrict__ sd0,
                                                          const int32_t* __restrict__ list, int nlist,
@@ -909,8 +899,8 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   Obj o;
-  load_obj(o, gr, I1, dimx, dimy, pitch, p0 + 2 * b, d0 + b, n, sd0[g], prm.subpix_displacement_max,
-           prm.overflow_stiffness);
+  load_obj(o, gr, I1, dimx, dimy, pitch, words + b, m0[g], p0 + 2 * b, d0 + b, n, sd0[g],
+           prm.subpix_displacement_max, prm.overflow_stiffness);
   stage_grain(o, gr, prad[g], lay);
   SpState& st = sp_state();
   if (threadIdx.x == 0) {
@@ -990,7 +980,7 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
-static SpLayout sp_layout(const Context& c, int max_prad, int max_n, double dmax) {
+static SpLayout sp_layout(int max_prad, double dmax) {
   SpLayout lay;
   // the tile covers the pattern's extent, the optimizer's excursion up to the displacement
   // bound, and the 4x4 stencil; samples outside it read global memory
@@ -999,13 +989,11 @@ static SpLayout sp_layout(const Context& c, int max_prad, int max_n, double dmax
   const int tw = 2 * (max_prad + lay.margin) + 2;
   const size_t tile_bytes = (size_t)tw * tw * sizeof(uint16_t);
   lay.tile_elems = tile_bytes <= (size_t)kSpTileBytesCap ? tw * tw : 0;
-  lay.words_cap = std::min(max_n, kSpWordsCap);
-  (void)c;
   return lay;
 }
 
 static size_t sp_lds_bytes(const SpLayout& lay) {
-  return kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems) + sizeof(uint32_t) * lay.words_cap;
+  return kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems);
 }
 
 static hipError_t sp_set_lds(const void* fn, size_t lds) {
@@ -1016,12 +1004,13 @@ static hipError_t sp_set_lds(const void* fn, size_t lds) {
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
   if (c.n_unmasked == 0) return hipSuccess;
   const Image& im = c.img[c.icur];
-  const SpLayout lay = sp_layout(c, c.max_prad_unmasked, c.max_pattern_unmasked, p.subpix_displacement_max);
+  const SpLayout lay = sp_layout(c.max_prad_unmasked, p.subpix_displacement_max);
   const size_t lds = sp_lds_bytes(lay);
   hipError_t e = sp_set_lds((const void*)k_subpixel, lds);
   if (e != hipSuccess) return e;
   k_subpixel<<<c.n_unmasked, kSpThreads, lds, c.stream>>>(
-      im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0, c.d_d0, c.d_sd0, c.d_list_lpt, c.n_unmasked,
+      im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0,
+      c.d_list_lpt, c.n_unmasked,
       p, reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, lay, c.d_evals_g);
   return hipGetLastError();
 }
@@ -1032,6 +1021,8 @@ hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
 __global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __restrict__ I1, int dimx, int dimy,
                                                           int64_t pitch, const ptg_grain* __restrict__ grains,
                                                           const int64_t* __restrict__ pbegin,
+                                                          const uint32_t* __restrict__ words,
+                                                          const double* __restrict__ m0,
                                                           const int32_t* __restrict__ p0,
                                                           const double* __restrict__ d0,
                                                           const double* __restrict__ sd0, int n_eval,
@@ -1047,7 +1038,7 @@ __global__ __launch_bounds__(kSpThreads) void k_objective(const uint16_t* __rest
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   Obj o;
-  load_obj(o, gr, I1, dimx, dimy, pitch, p0 + 2 * b, d0 + b, n, sd0[g], dmax, stiff);
+  load_obj(o, gr, I1, dimx, dimy, pitch, words + b, m0[g], p0 + 2 * b, d0 + b, n, sd0[g], dmax, stiff);
   stage_grain(o, gr, prad[g], lay);
   const double X[3] = {Xs[3 * e], Xs[3 * e + 1], Xs[3 * e + 2]};
   const ObjConst k = obj_const(o);
@@ -1086,15 +1077,13 @@ hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, co
   const Image& im = c.img[c.icur];
   int max_prad = 0;
   for (int32_t r : c.h_prad) max_prad = std::max(max_prad, r);
-  int64_t max_n = 0;
-  for (int g = 0; g < c.G; g++) max_n = std::max(max_n, c.h_pbegin[g + 1] - c.h_pbegin[g]);
-  const SpLayout lay = sp_layout(c, max_prad, (int)std::min<int64_t>(max_n, kSpWordsCap), dmax);
+  const SpLayout lay = sp_layout(max_prad, dmax);
   const size_t lds = sp_lds_bytes(lay);
   hipError_t e = sp_set_lds((const void*)k_objective, lds);
   if (e != hipSuccess) return e;
-  k_objective<<<n, kSpThreads, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_p0,
-                                                c.d_d0, c.d_sd0, n, d_ig, d_X, dmax, stiff, c.d_prad, lay, d_f,
-                                                d_grad);
+  k_objective<<<n, kSpThreads, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin,
+                                                c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0, n, d_ig, d_X, dmax,
+                                                stiff, c.d_prad, lay, d_f, d_grad);
   return hipGetLastError();
 }
 
