@@ -474,12 +474,18 @@ struct Powell {
 // The LM form: the first sweep of Powell's method (line minimisations along the three
 // initial directions, which is where Powell's run picks its basin when the objective has
 // several minima in theta), then damped Newton (Levenberg-Marquardt on the objective's
-// Hessian) on (u, v, theta) until the step is below 1e-8 px and 1e-10 rad. The sweep's
-// line searches only have to land in the same basin as Powell's (Brent tolerance 3e-8),
-// but that needs more than a coarse bracket: on 2000 grains of a 6000x4000 frame pair 1e-3
-// and 1e-4 reach the oracle's minimum everywhere while 1e-2 sends one grain to another
-// minimum.
+// Hessian) on (u, v, theta). It stops at the current point, without evaluating the next,
+// once the undamped Newton step from it (the damped step times 1 + lambda, a bound for the
+// diagonal damping used here) is below 1e-6 px and 1e-8 rad: the point is then that close
+// to the minimum (Newton converges quadratically there), far inside the stage's 1e-3 px
+// tolerance, and its objective is already known. A step below 1e-8 px and 1e-10 rad (by
+// damping) also ends the run after its evaluation. The sweep's line searches only have to
+// land in the same basin as Powell's (Brent tolerance 3e-8), but that needs more than a
+// coarse bracket: on 2000 grains of a 6000x4000 frame pair 1e-3 and 1e-4 reach the
+// oracle's minimum everywhere while 1e-2 sends one grain to another minimum.
 #define LM_SWEEP_TOL 1.0e-3
+#define LM_DONE_PX 1.0e-6
+#define LM_DONE_RAD 1.0e-8
 #define LM_STEP_PX 1.0e-8
 #define LM_STEP_RAD 1.0e-10
 
@@ -734,6 +740,9 @@ __device__ __forceinline__ int lm_step(SpState& st) {
 #pragma unroll
   for (int j = 0; j < 3; j++) g[j] = M.d.grad[j];
   if (!solve3(H, g, M.lambda, step)) return RQ_DONE;
+  const double sc = 1.0 + M.lambda;
+  if (fabs(step[0]) * sc < LM_DONE_PX && fabs(step[1]) * sc < LM_DONE_PX && fabs(step[2]) * sc < LM_DONE_RAD)
+    return RQ_DONE;
   M.small = fabs(step[0]) < LM_STEP_PX && fabs(step[1]) < LM_STEP_PX && fabs(step[2]) < LM_STEP_RAD;
 #pragma unroll
   for (int j = 0; j < 3; j++) st.Xq[j] = M.Xn[j] = M.X[j] + step[j];
