@@ -319,6 +319,7 @@ __device__ __forceinline__ void reduce_partials(const double* red, double* S) {
 struct ObjConst {
   double N, sd0, C0C0, dmax, stiff;
   double xr, yr, drot;
+  double wa;  // 1 / sqrt(C0C0), the residual weight of d0
 };
 
 __device__ __forceinline__ ObjConst obj_const(const Obj& o) {
@@ -331,6 +332,7 @@ __device__ __forceinline__ ObjConst obj_const(const Obj& o) {
   k.xr = o.xr;
   k.yr = o.yr;
   k.drot = o.drot;
+  k.wa = 1.0 / sqrt(o.C0C0);
   return k;
 }
 
@@ -341,14 +343,26 @@ __device__ __forceinline__ double weight(const ObjConst& o, const double* X) {
   return W;
 }
 
+// rc (optional): the last angle and its cosine and sine, reused while the angle is
+// unchanged (the line searches along u and v keep it).
 __device__ __forceinline__ void make_request(const ObjConst& o, const double* X, int kind, double wb, double wm,
-                                             Request* rq) {
+                                             Request* rq, double* rc = nullptr) {
   const double rot = o.drot + X[2];
   rq->xpix1 = o.xr + X[0];
   rq->ypix1 = o.yr + X[1];
-  rq->c1 = cos(rot);
-  rq->s1 = sin(rot);
-  rq->wa = 1.0 / sqrt(o.C0C0);
+  if (rc && rot == rc[0]) {
+    rq->c1 = rc[1];
+    rq->s1 = rc[2];
+  } else {
+    rq->c1 = cos(rot);
+    rq->s1 = sin(rot);
+    if (rc) {
+      rc[0] = rot;
+      rc[1] = rq->c1;
+      rc[2] = rq->s1;
+    }
+  }
+  rq->wa = o.wa;
   rq->wb = wb;
   rq->wm = wm;
   rq->kind = kind;
@@ -507,6 +521,7 @@ struct SpState {
   Line L;         // the line search in progress
   Derivs dq;      // derivatives at Xq, for derivative requests
   double hwb, hwm;  // residual weights (NCC / sig1, mean of v) of the latest evaluation
+  double rc[3];     // the last requested angle, its cosine and sine
   double sums[kSpSums];  // the workgroup's pixel sums at Xq
   union {
     Powell P;
@@ -884,7 +899,7 @@ __device__ __noinline__ void sp_step() {
   if (next == RQ_DONE)
     st.rq.kind = RQ_DONE;
   else
-    make_request(st.k, st.Xq, next, st.hwb, st.hwm, &st.rq);
+    make_request(st.k, st.Xq, next, st.hwb, st.hwm, &st.rq, st.rc);
 }
 
 // follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128: one workgroup per grain of the
@@ -936,7 +951,8 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
       P.state = PS_INIT;
     }
     st.hwb = st.hwm = 0.0;
-    make_request(st.k, st.Xq, RQ_VALUE, 0.0, 0.0, &st.rq);
+    st.rc[0] = __builtin_nan("");
+    make_request(st.k, st.Xq, RQ_VALUE, 0.0, 0.0, &st.rq, st.rc);
   }
   bool cl = false;
   for (;;) {
