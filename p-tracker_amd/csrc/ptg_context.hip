@@ -59,19 +59,20 @@ hipEvent_t Context::get_event() {
   return e;
 }
 
-void Context::mark_begin(int stage, hipEvent_t* ev) {
+// Events around work on stream `on` (the library stream when null).
+void Context::mark_begin(int stage, hipEvent_t* ev, hipStream_t on) {
   *ev = nullptr;
   if (!timing) return;
   *ev = get_event();
-  if (*ev) (void)hipEventRecord(*ev, stream);
+  if (*ev) (void)hipEventRecord(*ev, on ? on : stream);
   (void)stage;
 }
 
-void Context::mark_end(int stage, hipEvent_t ev) {
+void Context::mark_end(int stage, hipEvent_t ev, hipStream_t on) {
   if (!timing || !ev) return;
   hipEvent_t e2 = get_event();
   if (!e2) return;
-  (void)hipEventRecord(e2, stream);
+  (void)hipEventRecord(e2, on ? on : stream);
   pending.push_back({stage, ev, e2});
 }
 
@@ -473,6 +474,8 @@ int ptg_destroy(void* p) {
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->aux_events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->search_joins) (void)hipEventDestroy(e);
+  for (hipStream_t st : c->search_streams) (void)hipStreamDestroy(st);
   if (c->staging_done) (void)hipEventDestroy(c->staging_done);
   if (c->aux_stream) (void)hipStreamDestroy(c->aux_stream);
   (void)hipStreamDestroy(c->stream);

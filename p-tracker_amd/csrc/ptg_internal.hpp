@@ -63,6 +63,10 @@ struct Context {
   // its fork/join events (created on first use)
   hipStream_t aux_stream = nullptr;
   std::vector<hipEvent_t> aux_events;
+  // further streams the radius buckets' MFMA searches alternate over (with the library
+  // stream), and the events that join them back into it
+  std::vector<hipStream_t> search_streams;
+  std::vector<hipEvent_t> search_joins;
   int ensure_aux(size_t nevents);
   std::string err;
 
@@ -154,8 +158,8 @@ struct Context {
   int check(hipError_t e, const char* what);
   int ensure(void** p, size_t* cap, size_t bytes);
   hipEvent_t get_event();
-  void mark_begin(int stage, hipEvent_t* ev);
-  void mark_end(int stage, hipEvent_t ev);
+  void mark_begin(int stage, hipEvent_t* ev, hipStream_t on = nullptr);
+  void mark_end(int stage, hipEvent_t ev, hipStream_t on = nullptr);
 };
 
 // launchers (each returns hipError_t of the launch)

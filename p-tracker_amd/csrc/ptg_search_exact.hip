@@ -792,10 +792,25 @@ struct ExactPlan {
 };
 }  // namespace
 
+// Streams the radius buckets' MFMA searches alternate over: the library stream and
+// kSearchLanes - 1 more. A bucket's launch ends with a tail of its last workgroups; the
+// next bucket, on another stream, fills the CUs it leaves (measured on 2,000 grains in five
+// buckets: 3.10 ms for the main stage on one stream, 2.93 ms on two).
+constexpr int kSearchLanes = 2;
+
 int Context::ensure_aux(size_t nevents) {
   hipError_t e;
   if (!aux_stream && (e = hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking)) != hipSuccess)
     return check(e, "hipStreamCreateWithFlags");
+  while (search_streams.size() < (size_t)(kSearchLanes - 1)) {
+    hipStream_t st;
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+      return check(e, "hipStreamCreateWithFlags");
+    search_streams.push_back(st);
+    hipEvent_t ev;
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return check(e, "hipEventCreate");
+    search_joins.push_back(ev);
+  }
   while (aux_events.size() < nevents) {
     hipEvent_t ev;
     if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return check(e, "hipEventCreate");
@@ -1077,8 +1092,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
   // the template and middle-step kernels of every bucket run in order on the second
-  // stream, each bucket's MFMA kernel on the library stream after its bucket's event, so
-  // the next bucket's preparation overlaps the current search
+  // stream; each bucket's MFMA kernel runs after its bucket's event, on the library stream
+  // and the search streams in turn, so the next bucket's preparation overlaps the current
+  // search and consecutive searches overlap their tails
   const size_t nb_all = P->buckets.size();
   if (int rca = c.ensure_aux(nb_all + 1); rca != PTG_OK) return rca;
   hipEvent_t fork = c.aux_events[nb_all];
@@ -1119,6 +1135,14 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     if ((he = hipEventRecord(c.aux_events[bi], c.aux_stream)) != hipSuccess) return c.check(he, "hipEventRecord");
   }
   off = 0;
+  hipStream_t lanes[kSearchLanes];
+  lanes[0] = c.stream;
+  for (int q = 1; q < kSearchLanes; q++) {
+    lanes[q] = c.search_streams[q - 1];
+    // the tables and cleared outputs above come first
+    if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
+  }
+  int nlaunch = 0;
   for (size_t bi = 0; bi < nb_all; bi++) {
     const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
@@ -1127,7 +1151,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     e.bucket = bucket_dev + off;
     e.nbucket = nb;
     off += nb;
-    if ((he = hipStreamWaitEvent(c.stream, c.aux_events[bi], 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
+    hipStream_t sx = lanes[nlaunch++ % kSearchLanes];
+    if ((he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
     ExactArgs eb = e;
     eb.ww = l.ww;
     eb.wh_pad = l.wh;
@@ -1140,10 +1165,15 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int groups = (nb + 7) / 8;
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
     hipEvent_t ev = nullptr;
-    if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev);
-    if ((he = dispatch_main(nmt, ntp, eb, grid, block, l.lds, c.stream)) != hipSuccess)
+    if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev, sx);
+    if ((he = dispatch_main(nmt, ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
       return c.check(he, "k_search_exact");
-    if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev);
+    if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev, sx);
+  }
+  for (int q = 1; q < kSearchLanes; q++) {
+    if ((he = hipEventRecord(c.search_joins[q - 1], lanes[q])) != hipSuccess ||
+        (he = hipStreamWaitEvent(c.stream, c.search_joins[q - 1], 0)) != hipSuccess)
+      return c.check(he, "exact search join");
   }
   if (e.ovf_slot) {
     if ((he = launch_search_plain_sub(c, sa, nlist, e.ovf_list, e.flags + 1, part_cap, lds_plain)) != hipSuccess)
