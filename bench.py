@@ -324,6 +324,13 @@ def main():
     roof["algorithmic_flops_per_pair"] = flops
     roof["kernel_ms_per_pair"] = k_ms
     roof["launches_per_pair"] = k_launches
+    if method == "exact":
+        # the radius buckets' launches run two at a time on two streams, so their summed
+        # durations exceed the time the main stage takes: the same flops over the stage's
+        # own span (HIP events on the library stream around the whole search)
+        st_ms = ms[pk.T_SEARCH_MAIN] / args.steps
+        roof["search_stage_ms_per_pair"] = st_ms
+        roof["achieved_over_stage_span"] = flops / (st_ms * 1e-3) / 1e12
 
     result = {
         "metric": METRIC, "value": value, "unit": "grains/s", "n_gpus": 1, "steps": args.steps,

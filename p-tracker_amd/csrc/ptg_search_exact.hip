@@ -20,8 +20,9 @@
 //    16. For each shift column the window digit planes are copied into LDS rows whose
 //    cell c is window column su + c, so every B-fragment is one aligned ds_read_b128.
 // B and C use the multiset mask M_mid of the middle rotation step: B_mid(s), C_mid(s)
-// come from row prefix sums, and each step adds the sparse difference M_k - M_mid
-// (a few tens of cells) on the vector units.
+// come from a second GEMM whose rows are copies of M_mid moved by 0..15 cells
+// (k_exact_mid), and each step adds the sparse difference M_k - M_mid (a few tens of
+// cells) on the vector units.
 #include <limits.h>
 #include <math.h>
 
