@@ -618,6 +618,8 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
   c->h_pattern.assign(pattern_xy, pattern_xy + 2 * total);
   c->h_pbegin.assign(pattern_begin, pattern_begin + G + 1);
   c->h_prad.resize(G);
+  c->h_disk.assign(G, 0);
+  std::vector<uint8_t> seen;
   for (int g = 0; g < G; g++) {
     int64_t m2 = 0;
     for (int64_t i = pattern_begin[g]; i < pattern_begin[g + 1]; i++) {
@@ -626,6 +628,29 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
     }
     // every rotated offset round(c dx + s dy) lies within ceil(|(dx, dy)|) + 1
     c->h_prad[g] = (int32_t)ceil(sqrt((double)m2)) + 1;
+    // a full digital disk: every cell with x^2 + y^2 <= m2 exactly once
+    const int64_t n = pattern_begin[g + 1] - pattern_begin[g];
+    int64_t r = (int64_t)sqrt((double)m2);
+    while (r * r > m2) r--;
+    while ((r + 1) * (r + 1) <= m2) r++;
+    int64_t cells = 0;
+    for (int64_t y = -r; y <= r; y++) {
+      int64_t xm = (int64_t)sqrt((double)(m2 - y * y));
+      while (xm * xm > m2 - y * y) xm--;
+      while ((xm + 1) * (xm + 1) <= m2 - y * y) xm++;
+      cells += 2 * xm + 1;
+    }
+    if (n > 0 && cells == n && r <= 1024) {
+      const int64_t side = 2 * r + 1;
+      seen.assign((size_t)(side * side), 0);
+      bool dup = false;
+      for (int64_t i = pattern_begin[g]; i < pattern_begin[g + 1] && !dup; i++) {
+        uint8_t& f = seen[(size_t)((pattern_xy[2 * i + 1] + r) * side + pattern_xy[2 * i] + r)];
+        dup = f != 0;
+        f = 1;
+      }
+      c->h_disk[g] = dup ? 0 : 1;
+    }
   }
   if (G == 0) return PTG_OK;
   const size_t Gs = (size_t)G;

@@ -19,10 +19,13 @@
 //  - K-chunks are 4 segments of 16 template cells of one row, segment starts aligned to
 //    16. For each shift column the window digit planes are copied into LDS rows whose
 //    cell c is window column su + c, so every B-fragment is one aligned ds_read_b128.
-// B and C use the multiset mask M_mid of the middle rotation step: B_mid(s), C_mid(s)
-// come from a second GEMM whose rows are copies of M_mid moved by 0..15 cells
-// (k_exact_mid), and each step adds the sparse difference M_k - M_mid (a few tens of
-// cells) on the vector units.
+// B and C use a base multiset mask M_mid: the middle rotation step's, or for a pattern that
+// is a full digital disk the pattern itself (a rotation maps the disk onto itself up to a
+// few rim cells, so the rounded rotations differ from it only by their own duplicate and
+// empty cells: 30-40% fewer Delta cells than against the middle step at base rotations of
+// 0.05-0.3 rad, the same at zero). B_mid(s), C_mid(s) come from a second GEMM whose rows
+// are copies of M_mid moved by 0..15 cells (k_exact_mid), and each step adds the sparse
+// difference M_k - M_mid (a few tens of cells) on the vector units.
 #include <limits.h>
 #include <math.h>
 
@@ -47,9 +50,11 @@ struct ExactSlot {
   int64_t dl_off;    // first Delta entry (nsteps * 2 * dcap entries)
   int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
   int64_t mseg_off;  // first segment of the middle-step GEMM's segment table
-  int64_t mmask_off; // first byte of the middle step's multiplicity grid (side x side)
+  int64_t mmask_off; // first byte of the base mask's multiplicity grid (side x side)
   int32_t mkc;       // its K-chunks (4 segments each)
   int32_t dcap;      // Delta entries per (step, sign), at most 1023
+  int32_t base_pat;  // the base mask is the pattern itself (a full digital disk), not the middle step
+  int32_t pad;
 };
 
 struct ExactArgs {
@@ -106,7 +111,7 @@ __device__ __forceinline__ int win_y0(const ptg_grain& gr, int up, int rho) {
 
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
 // LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
-// bytes = count of the middle step (from k_exact_mid, which runs first). Rows beyond the
+// bytes = count of the base mask (from k_exact_mid, which runs first). Rows beyond the
 // last step are written as zeros.
 __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -123,7 +128,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   uint8_t* cmid = reinterpret_cast<uint8_t*>(packed + ncell);
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) packed[t] = 0;
   const bool real = k < nsteps;
-  // the middle step's multiplicities (k_exact_mid)
+  // the base mask's multiplicities (k_exact_mid)
   if (real)
     for (int t = threadIdx.x; t < ncell; t += blockDim.x) cmid[t] = e.mmask[sl.mmask_off + t];
   __syncthreads();
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
     *reinterpret_cast<uint4*>(e.tmpl + sl.tmpl_off + (((int64_t)mt * kc + ch) * 64 + lane) * 16) = val;
   }
   if (real) {
-    // Delta lists: cells whose multiplicity differs from the middle step's, split by sign
+    // Delta lists: cells whose multiplicity differs from the base mask's, split by sign
     __shared__ int cnt[2];
     if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
 
 // ---- B_mid, C_mid for every shift and the reference moments -------------------------
 // For a block of 16 shift columns su0 + m (m < 16), B_mid and C_mid of every vertical shift
-// come from one GEMM on v_mfma_i32_16x16x64_i8: row m of A is the middle step's multiset
+// come from one GEMM on v_mfma_i32_16x16x64_i8: row m of A is the base multiset
 // mask moved right by m cells (A_m[qy][x] = M_mid(x - m, qy)), columns are vertical shifts,
 // K runs over (qy, x) in 16-cell segments, and the B operands are the window's digit planes
 // starting at column su0: I1 = 128 h + l and I1^2 = sum_d 128^d q_d (7-bit digits), so
@@ -250,15 +255,16 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   const int64_t b = e.s.pbegin[g];
   const int n = (int)(e.s.pbegin[g + 1] - b);
   const int nsteps = e.s.nsteps, kmid = nsteps / 2;
-  // the middle step's mask (bytes) and, in the first column block, the reference moments
+  // the base mask (bytes) and, in the first column block, the reference moments
   for (int t = threadIdx.x; t < side * MLR / 4; t += blockDim.x) reinterpret_cast<uint32_t*>(mrow)[t] = 0u;
   __syncthreads();
   const double c = e.s.cs[2 * ((int64_t)g * nsteps + kmid)], s = e.s.cs[2 * ((int64_t)g * nsteps + kmid) + 1];
   long long Sa = 0, Saa = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const uint32_t pw = e.s.pwords[b + i];
-    int x, y;
-    rotate_px((int)(int8_t)(pw & 0xffu), (int)(int8_t)((pw >> 8) & 0xffu), c, s, x, y);
+    const int px = (int)(int8_t)(pw & 0xffu), py = (int)(int8_t)((pw >> 8) & 0xffu);
+    int x = px, y = py;
+    if (!sl.base_pat) rotate_px(px, py, c, s, x, y);
     // byte counters in 32-bit words; a count above 127 (not an int8 operand) raises the
     // capacity flag, which voids the search before a byte could carry into its neighbour
     const int off = (y + rho) * MLR + 16 + (x + rho);
@@ -308,7 +314,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
     }
     __syncthreads();
     if (p0 == 0 && blockIdx.y == 0) {
-      // the middle step's multiplicities for the template kernel's Delta lists
+      // the base mask's multiplicities for the template kernel's Delta lists
       for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
         const int qy = t / side, qx = t - qy * side;
         e.mmask[sl.mmask_off + t] = mrow[(size_t)qy * MLR + 16 + qx];
@@ -885,6 +891,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     const int g = list[li];
     const int64_t n = c.h_pbegin[g + 1] - c.h_pbegin[g];
     s.dcap = (int32_t)std::min<int64_t>(1023, std::max<int64_t>(std::min(255, 4 * s.rho + 64), n / 8 + 64));
+    s.base_pat = c.h_disk[g] ? 1 : 0;
     s.dl_off = dl_total;
     dl_total += (int64_t)nsteps * 2 * s.dcap;
     P.dcap = std::max(P.dcap, (int)s.dcap);
@@ -977,7 +984,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       l.cbw--;
     if (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > kLdsDynMax) l.npl = 3;
     l.lds_m = mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl);
-    // templates and Delta lists: the step's packed template and the middle step's grid
+    // templates and Delta lists: the step's packed template and the base mask's grid
     l.lds_t = (size_t)side * side * 5;
     // the MFMA search
     l.ww = win_width(U, cw);

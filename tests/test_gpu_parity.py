@@ -466,3 +466,55 @@ def test_huge_zone_tiled_plain_search_picks_equal_oracle(tracker, method):
     for col in INT_COLS:
         assert bits_equal(got[col], ref[col]), col
     assert (got["NCC"] > 0.8).all()  # found (the grains also rotate by up to 0.01 rad)
+
+
+def _full_disk(P):
+    """True when the offsets are every cell of x^2 + y^2 <= max once (the library then takes
+    the pattern itself as the base mask of the Delta lists)."""
+    d2 = (P.astype(np.int64) ** 2).sum(1)
+    m2 = int(d2.max())
+    r = math.isqrt(m2)
+    cells = sum(2 * math.isqrt(m2 - y * y) + 1 for y in range(-r, r + 1))
+    return cells == len(P) and len({(int(x), int(y)) for x, y in P}) == len(P)
+
+
+def test_exact_base_mask_for_disks_and_other_patterns(tracker):
+    # at a reference rotation of 0.2 rad: full digital disks (base mask = the pattern),
+    # squares, rings, a disk with a cell missing and one with a cell counted twice (base mask
+    # = the middle step's rounded rotation); every pick is the reference's
+    S = synth.sequence(n_frames=2, n_grains=20, size=(800, 700), seed=511, radius=(10.0, 24.0))
+    pats, kinds = [], []
+    for g in range(S.G):
+        P = S.pattern_xy[S.pattern_begin[g]:S.pattern_begin[g + 1]]
+        r = math.isqrt(int((P.astype(np.int64) ** 2).sum(1).max()))
+        k = g % 5
+        if k == 1:
+            ys, xs = np.mgrid[-(r // 2):r // 2 + 1, -(r // 2):r // 2 + 1]
+            P = np.stack([xs.ravel(), ys.ravel()], 1)
+        elif k == 2:
+            d2 = (P.astype(np.int64) ** 2).sum(1)
+            P = P[d2 > (r // 2) ** 2]
+        elif k == 3:
+            P = P[1:]
+        elif k == 4:
+            P = np.concatenate([P, P[len(P) // 2:len(P) // 2 + 1]])
+        pats.append(P.astype(np.int32))
+        kinds.append(_full_disk(P))
+    assert any(kinds) and not all(kinds)
+    pxy = np.concatenate(pats).astype(np.int32)
+    pb = np.concatenate([[0], np.cumsum([len(p) for p in pats])]).astype(np.int64)
+    gr = S.grains.copy()
+    gr["refrot"] = 0.2
+    z = S.params.search_zone
+    say(f"exact search, base masks, refrot 0.2: G={S.G}")
+    tracker.set_image(0, S.frames[0], S.bits)
+    tracker.set_image(1, S.frames[1], S.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(gr, pxy, pb)
+    tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT))
+    got = tracker.get_grains()
+    W = ptref.Workload(S.frames[0], S.frames[1], gr, pxy, pb, S.bits)
+    ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
