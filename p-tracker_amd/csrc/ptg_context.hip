@@ -619,6 +619,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
   c->h_pbegin.assign(pattern_begin, pattern_begin + G + 1);
   c->h_prad.resize(G);
   c->h_disk.assign(G, 0);
+  c->h_diskm2.assign(G, 0);
   std::vector<uint8_t> seen;
   for (int g = 0; g < G; g++) {
     int64_t m2 = 0;
@@ -641,6 +642,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
       cells += 2 * xm + 1;
     }
     if (n > 0 && cells == n && r <= 1024) {
+      c->h_diskm2[g] = (int32_t)m2;
       const int64_t side = 2 * r + 1;
       seen.assign((size_t)(side * side), 0);
       bool dup = false;

@@ -88,6 +88,7 @@ struct Context {
   std::vector<int64_t> h_pbegin;
   std::vector<int32_t> h_prad;
   std::vector<uint8_t> h_disk;  // per grain: the pattern is a full digital disk (each cell of x^2 + y^2 <= max once)
+  std::vector<int32_t> h_diskm2; // for such a pattern: the largest x^2 + y^2
   std::vector<ptg_grain> h_grains;  // host snapshot, valid when h_valid
   bool h_valid = false;
   bool require_precomputations = true;

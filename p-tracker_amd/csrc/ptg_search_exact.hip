@@ -54,7 +54,7 @@ struct ExactSlot {
   int32_t mkc;       // its K-chunks (4 segments each)
   int32_t dcap;      // Delta entries per (step, sign), at most 1023
   int32_t base_pat;  // the base mask is the pattern itself (a full digital disk), not the middle step
-  int32_t pad;
+  int32_t disk_m2;   // for such a disk: the largest x^2 + y^2 of its cells
 };
 
 struct ExactArgs {
@@ -88,10 +88,12 @@ struct ExactArgs {
   int32_t abuf_alias;          // epilogue buffer in the wave's staging area (one vertical pass)
   uint8_t* mmask;              // middle-step multiplicity grids (written by k_exact_mid)
   int32_t mid_npl;             // digit planes staged per pass of k_exact_mid (6 or 3)
-  int32_t pad_mid;
+  int32_t mid_skip_disk;       // k_exact_mid leaves the disk-based slots to k_exact_base_disk
   int32_t* ovf_slot;           // per slot: 1 when its template or Delta lists overflowed (or null:
                                // an overflow then raises flags[0] bit 0 and voids the stage)
   int32_t* ovf_list;           // overflowed slots, flags[1] of them, searched by the plain kernel
+  int32_t base_vb;             // k_exact_base_disk: vertical shifts per workgroup
+  int32_t pad_b;
 };
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
@@ -241,6 +243,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   if (li >= *e.s.count) return;
   const int g = e.s.list[li];
   const ExactSlot sl = e.slots[li];
+  if (sl.base_pat && e.mid_skip_disk) return;  // k_exact_base_disk's
   const int rho = sl.rho, side = 2 * rho + 1;
   const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1;
   const int vp = e.vp, WHm = vp + 2 * rho;
@@ -388,6 +391,144 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
       }
     }
   }
+}
+
+// ---- B_mid, C_mid of a full digital disk from row prefix sums ---------------------------
+// Every row qy of a disk base mask is one run of cells, |qx| <= xm(qy) = isqrt(m2 - qy^2),
+// so over a window row the run's sums of I1 and I1^2 are differences of two prefix sums:
+//   B_mid(s) = sum_qy P(row, hi) - P(row, lo),  C_mid likewise on I1^2,
+// 2 (2 r + 1) LDS reads per shift instead of a GEMM over the disk's cells. The prefix of I1^2
+// is kept modulo 2^32: a run has at most 2 rho - 1 <= 255 cells of at most 4095^2, so its
+// true sum is below 2^32 and the modular difference is exact (a window value above 4095,
+// beyond the frames' declared 12 bits, sends the grain to the plain kernel). Also writes the
+// disk's multiplicity grid and, in the first block, the reference moments. One workgroup per
+// (slot, block of base_vb vertical shifts); LDS: the window rows of the block (base_vb +
+// 2 rho) x (U + 2 rho + 1) prefix pairs, then the run half-widths.
+__global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ long long red[2][8];
+  __shared__ int big;
+  const int bi = blockIdx.x;
+  if (bi >= e.nbucket) return;
+  const int li = e.bucket[bi];
+  if (li >= *e.s.count) return;
+  const ExactSlot sl = e.slots[li];
+  if (!sl.base_pat) return;
+  const int g = e.s.list[li];
+  const int rho = sl.rho, side = 2 * rho + 1;
+  const int U = e.s.left + e.s.right + 1, V = e.s.up + e.s.down + 1;
+  const int sv0 = blockIdx.y * e.base_vb;
+  if (sv0 >= V) return;
+  const int nv = min(e.base_vb, V - sv0);
+  const int nrow = nv + 2 * rho, ncol = U + 2 * rho, pst = ncol + 1;
+  uint2* pre = reinterpret_cast<uint2*>(smem);
+  int32_t* xm = reinterpret_cast<int32_t*>(smem + sizeof(uint2) * (size_t)nrow * pst);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const ptg_grain& gr = e.s.grains[g];
+  const int X0 = win_x0(gr, e.s.left, rho), Y0 = win_y0(gr, e.s.up, rho) + sv0;
+  const int m2 = sl.disk_m2;
+  int r = (int)sqrt((double)m2);
+  while (r * r > m2) r--;
+  while ((r + 1) * (r + 1) <= m2) r++;
+  if (threadIdx.x == 0) big = 0;
+  for (int q = threadIdx.x; q < 2 * r + 1; q += blockDim.x) {
+    const int qy = q - r, t2 = m2 - qy * qy;
+    int w = (int)sqrt((double)t2);
+    while (w * w > t2) w--;
+    while ((w + 1) * (w + 1) <= t2) w++;
+    xm[q] = w;
+  }
+  __syncthreads();
+  // exclusive row prefix sums of I1 and I1^2 (mod 2^32), one wave per row; the window
+  // reads clamp to the frame as the MFMA kernels' do
+  bool over = false;
+  for (int row = wave; row < nrow; row += nwave) {
+    const int Y = clampi(Y0 + row, 0, e.s.dimy - 1);
+    const uint16_t* src = e.s.I1 + (int64_t)Y * e.s.pitch1;
+    uint2* dst = pre + (size_t)row * pst;
+    unsigned int c1 = 0u, c2 = 0u;
+    if (lane == 0) dst[0] = make_uint2(0u, 0u);
+    for (int x0 = 0; x0 < ncol; x0 += 64) {
+      const int x = x0 + lane;
+      unsigned int v = 0u;
+      if (x < ncol) v = src[clampi(X0 + x, 0, e.s.dimx - 1)];
+      over |= v > 4095u;
+      unsigned int a = v, b = v * v;
+#pragma unroll
+      for (int m = 1; m < 64; m <<= 1) {
+        const unsigned int oa = __shfl_up(a, m, 64), ob = __shfl_up(b, m, 64);
+        if (lane >= m) {
+          a += oa;
+          b += ob;
+        }
+      }
+      if (x < ncol) dst[x + 1] = make_uint2(c1 + a, c2 + b);
+      c1 += __shfl(a, 63, 64);
+      c2 += __shfl(b, 63, 64);
+    }
+  }
+  if (over) big = 1;
+  __syncthreads();
+  if (big) {
+    // beyond 12 bits: this grain goes to the plain kernel (its MFMA search returns at once)
+    if (threadIdx.x == 0) {
+      if (!e.ovf_slot) atomicOr(&e.flags[0], 1);
+      else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
+    }
+    return;
+  }
+  // one shift per thread: (su, sv0 + v)
+  for (int t = threadIdx.x; t < U * nv; t += blockDim.x) {
+    const int v = t / U, su = t - v * U;
+    unsigned int B = 0u;
+    unsigned long long C = 0ull;
+    const uint2* rowp = pre + (size_t)(v + rho - r) * pst + su + rho;
+    for (int q = 0; q <= 2 * r; q++, rowp += pst) {
+      const int w = xm[q];
+      const uint2 lo = rowp[-w], hi = rowp[w + 1];
+      B += hi.x - lo.x;
+      C += (unsigned long long)(hi.y - lo.y);
+    }
+    const int64_t idx = (int64_t)su * V + sv0 + v;
+    e.mid[sl.mid_off + 2 * idx] = (long long)B;
+    e.mid[sl.mid_off + 2 * idx + 1] = (long long)C;
+  }
+  if (blockIdx.y == 0) {
+    // the disk's multiplicities (each cell once) for the template kernel's Delta lists
+    for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
+      const int qy = t / side - rho, qx = t - (t / side) * side - rho;
+      e.mmask[sl.mmask_off + t] = (uint8_t)(qx * qx + qy * qy <= m2 ? 1 : 0);
+    }
+    const int64_t b = e.s.pbegin[g];
+    const int n = (int)(e.s.pbegin[g + 1] - b);
+    long long Sa = 0, Saa = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const long long a = e.s.pwords[b + i] >> 16;
+      Sa += a;
+      Saa += a * a;
+    }
+    Sa = wave_sum_i64(Sa);
+    Saa = wave_sum_i64(Saa);
+    if (lane == 0) {
+      red[0][wave] = Sa;
+      red[1][wave] = Saa;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long A = 0, AA = 0;
+      for (int w = 0; w < nwave; w++) {
+        A += red[0][w];
+        AA += red[1][w];
+      }
+      e.mom[3 * li] = n;
+      e.mom[3 * li + 1] = A;
+      e.mom[3 * li + 2] = AA;
+    }
+  }
+}
+
+__host__ __forceinline__ size_t base_disk_lds_bytes(int rho, int U, int vb) {
+  return sizeof(uint2) * (size_t)(vb + 2 * rho) * (U + 2 * rho + 1) + sizeof(int32_t) * (size_t)(2 * rho + 1) + 16;
 }
 
 // ---- the MFMA search ---------------------------------------------------------------
@@ -764,6 +905,8 @@ namespace {
 // Launch shapes of one radius bucket.
 struct BucketLaunch {
   int cbw = 0, npl = 0;          // k_exact_mid: column blocks per workgroup, planes per pass (6, 3)
+  int vb = 0;                    // k_exact_base_disk: vertical shifts per workgroup (0: k_exact_mid's)
+  size_t lds_d = 0;              // its LDS
   size_t lds_m = 0, lds_t = 0;   // LDS of k_exact_mid, k_exact_template
   int cs = 0, ww = 0, wh = 0;    // k_search_exact: staging stride, window width and height
   int dl_cap = 0, nsub = 0, nsu = 0;
@@ -773,6 +916,7 @@ struct BucketLaunch {
 
 struct Bucket {
   int rho_max = 0;
+  int ndisk = 0;  // slots whose base mask is their (disk) pattern
   int max_kc = 0;
   int max_mkc = 0;
   std::vector<int32_t> slots;
@@ -892,6 +1036,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     const int64_t n = c.h_pbegin[g + 1] - c.h_pbegin[g];
     s.dcap = (int32_t)std::min<int64_t>(1023, std::max<int64_t>(std::min(255, 4 * s.rho + 64), n / 8 + 64));
     s.base_pat = c.h_disk[g] ? 1 : 0;
+    s.disk_m2 = c.h_disk[g] ? c.h_diskm2[g] : 0;
     s.dl_off = dl_total;
     dl_total += (int64_t)nsteps * 2 * s.dcap;
     P.dcap = std::max(P.dcap, (int)s.dcap);
@@ -908,6 +1053,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     b.rho_max = std::max(b.rho_max, P.slots[li].rho);
     b.max_kc = std::max(b.max_kc, P.slots[li].kc);
     b.max_mkc = std::max(b.max_mkc, P.slots[li].mkc);
+    if (P.slots[li].base_pat) b.ndisk++;
   }
   P.bucket_cat.clear();
   for (auto& bk : P.buckets) P.bucket_cat.insert(P.bucket_cat.end(), bk.slots.begin(), bk.slots.end());
@@ -984,6 +1130,21 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       l.cbw--;
     if (mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl) > kLdsDynMax) l.npl = 3;
     l.lds_m = mid_lds_bytes(rho, P->vp, l.cbw, bk.max_mkc, l.npl);
+    // disk base masks from row prefix sums: all vertical shifts per workgroup when that
+    // keeps two workgroups per CU, else as many as the LDS holds (none: k_exact_mid's)
+    l.vb = 0;
+    if (bk.ndisk > 0) {
+      int vb = V;
+      while (vb > 1 && base_disk_lds_bytes(rho, U, vb) > 80 * 1024) vb--;
+      if (vb < 8 && vb < V) {
+        vb = V;
+        while (vb > 1 && base_disk_lds_bytes(rho, U, vb) > kLdsDynMax) vb--;
+      }
+      if (base_disk_lds_bytes(rho, U, vb) <= kLdsDynMax) {
+        l.vb = vb;
+        l.lds_d = base_disk_lds_bytes(rho, U, vb);
+      }
+    }
     // templates and Delta lists: the step's packed template and the base mask's grid
     l.lds_t = (size_t)side * side * 5;
     // the MFMA search
@@ -1118,12 +1279,23 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     e.nbucket = nb;
     off += nb;
     const int rho = bk.rho_max;
-    {
+    if (l.vb > 0) {
+      if (l.lds_d > 65536 && (he = hipFuncSetAttribute((const void*)k_exact_base_disk,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)l.lds_d)) != hipSuccess)
+        return c.check(he, "hipFuncSetAttribute");
+      ExactArgs ed = e;
+      ed.base_vb = l.vb;
+      k_exact_base_disk<<<dim3(nb, (V + l.vb - 1) / l.vb), 512, l.lds_d, c.aux_stream>>>(ed);
+      if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_base_disk");
+    }
+    if (l.vb == 0 || bk.ndisk < nb) {
       const void* fm = l.npl == 6 ? (const void*)k_exact_mid<6> : (const void*)k_exact_mid<3>;
       if (l.lds_m > 65536 &&
           (he = hipFuncSetAttribute(fm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_m)) != hipSuccess)
         return c.check(he, "hipFuncSetAttribute");
       ExactArgs em = e;
+      em.mid_skip_disk = l.vb > 0 ? 1 : 0;
       em.mid_cbw = l.cbw;
       em.mid_stride = mid_stride(rho, l.cbw);
       em.mid_mkc_cap = bk.max_mkc;
