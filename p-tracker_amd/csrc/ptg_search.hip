@@ -72,26 +72,29 @@ hipError_t launch_change_reference(Context& c) {
   return hipGetLastError();
 }
 
-// do_precomputations, src/p-tracker.cpp:624-656: one wave per grain. The sum of I0 is
-// a sum of integers, exact in any order, so mean0 matches the reference's bits; C0C0
-// is accumulated by one lane in the reference's pixel order from an LDS chunk that the
-// whole wave stages. Also stores d0[i] = I0(p0_i) - mean0 (the reference's
-// `(double)image[...] - mean0`, :708), the per-grain sum of d0 for the one-pass
-// subpixel objective, and two packed words per pattern pixel (offsets as int8, I0 as
-// u16): unrotated for the exact search, rotated (p0) for the subpixel stage.
+// do_precomputations, src/p-tracker.cpp:624-656: one workgroup of 4 waves per grain. The
+// sum of I0 is a sum of integers, exact in any order, so mean0 matches the reference's
+// bits. C0C0 is accumulated by one lane in the reference's pixel order: waves 1-3 stage
+// chunk c of the differences in LDS while lane 0 of wave 0 sums chunk c - 1 (double
+// buffered, 16 values per read batch). Also stores d0[i] = I0(p0_i) - mean0 (the
+// reference's `(double)image[...] - mean0`, :708), the per-grain sum of d0 for the one-pass
+// subpixel objective, and two packed words per pattern pixel (offsets as int8, I0 as u16):
+// unrotated for the exact search, rotated (p0) for the subpixel stage.
+constexpr int kPreThreads = 256;
 constexpr int kPreChunk = 1024;
 
-__global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ I0, int dimx, int dimy,
-                                                   int64_t pitch, ptg_grain* __restrict__ grains,
-                                                   const int32_t* __restrict__ pattern,
-                                                   const int64_t* __restrict__ pbegin,
-                                                   const double* __restrict__ cs0, int32_t* __restrict__ p0,
-                                                   double* __restrict__ d0, double* __restrict__ sd0,
-                                                   uint32_t* __restrict__ pwords, uint32_t* __restrict__ spwords,
-                                                   double* __restrict__ m0) {
-  __shared__ double chunk[kPreChunk];
+__global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __restrict__ I0, int dimx, int dimy,
+                                                            int64_t pitch, ptg_grain* __restrict__ grains,
+                                                            const int32_t* __restrict__ pattern,
+                                                            const int64_t* __restrict__ pbegin,
+                                                            const double* __restrict__ cs0, int32_t* __restrict__ p0,
+                                                            double* __restrict__ d0, double* __restrict__ sd0,
+                                                            uint32_t* __restrict__ pwords,
+                                                            uint32_t* __restrict__ spwords, double* __restrict__ m0) {
+  __shared__ __align__(16) double chunk[2][kPreChunk];
+  __shared__ long long part[kPreThreads / 64];
   const int g = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   ptg_grain& gr = grains[g];
@@ -99,51 +102,91 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
   const double c0 = cs0[2 * g], s0 = cs0[2 * g + 1];
   long long s = 0;
   bool clamped = false;
-  for (int i = lane; i < n; i += 64) {
-    const int px = pattern[2 * (b + i)], py = pattern[2 * (b + i) + 1];
-    int x, y;
-    rotate_px(px, py, c0, s0, x, y);
-    p0[2 * (b + i)] = x;
-    p0[2 * (b + i) + 1] = y;
-    int X = rx + x, Y = ry + y;
-    if (X < 0 || Y < 0 || X >= dimx || Y >= dimy) clamped = true;
-    X = clampi(X, 0, dimx - 1);
-    Y = clampi(Y, 0, dimy - 1);
-    const uint32_t v = I0[(int64_t)Y * pitch + X];
-    s += v;
-    // the exact search's compact copy: offsets as int8 (its patterns are well inside
-    // +-127) and the reference value
-    pwords[b + i] = ((uint32_t)px & 0xffu) | (((uint32_t)py & 0xffu) << 8) | (v << 16);
-    // the subpixel stage's: the rotated offsets (int8 when the pattern radius is at most
-    // 127, the only case it reads them) and the same value, from which d0 = v - m0 is
-    // re-formed bit for bit
-    spwords[b + i] = ((uint32_t)x & 0xffu) | (((uint32_t)y & 0xffu) << 8) | (v << 16);
-  }
-  s = wave_sum_i64(s);
-  const double mean0 = (double)s / (double)n;
-  double C0C0 = 0.0, sd = 0.0;
-  for (int base = 0; base < n; base += kPreChunk) {
-    const int m = min(kPreChunk, n - base);
-    for (int t = lane; t < m; t += 64) {
-      const int i = base + t;
-      const int X = clampi(rx + p0[2 * (b + i)], 0, dimx - 1);
-      const int Y = clampi(ry + p0[2 * (b + i) + 1], 0, dimy - 1);
-      const double diff = (double)I0[(int64_t)Y * pitch + X] - mean0;
-      chunk[t] = diff;
-      d0[b + i] = diff;
+  // four pixels per thread and pass, their loads issued together
+  constexpr int kU = 4;
+  for (int i0 = tid; i0 < n; i0 += kU * kPreThreads) {
+    int px[kU], py[kU], x[kU], y[kU];
+    uint32_t v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int i = min(i0 + u * kPreThreads, n - 1);
+      px[u] = pattern[2 * (b + i)];
+      py[u] = pattern[2 * (b + i) + 1];
     }
-    __syncthreads();
-    if (lane == 0) {
-      for (int t = 0; t < m; t++) {
-        const double diff = chunk[t];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      rotate_px(px[u], py[u], c0, s0, x[u], y[u]);
+      int X = rx + x[u], Y = ry + y[u];
+      if (i0 + u * kPreThreads < n && (X < 0 || Y < 0 || X >= dimx || Y >= dimy)) clamped = true;
+      X = clampi(X, 0, dimx - 1);
+      Y = clampi(Y, 0, dimy - 1);
+      v[u] = I0[(int64_t)Y * pitch + X];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int i = i0 + u * kPreThreads;
+      if (i < n) {
+        p0[2 * (b + i)] = x[u];
+        p0[2 * (b + i) + 1] = y[u];
+        s += v[u];
+        // the exact search's compact copy: offsets as int8 (its patterns are well inside
+        // +-127) and the reference value
+        pwords[b + i] = ((uint32_t)px[u] & 0xffu) | (((uint32_t)py[u] & 0xffu) << 8) | (v[u] << 16);
+        // the subpixel stage's: the rotated offsets (int8 when the pattern radius is at
+        // most 127, the only case it reads them) and the same value, from which
+        // d0 = v - m0 is re-formed bit for bit
+        spwords[b + i] = ((uint32_t)x[u] & 0xffu) | (((uint32_t)y[u] & 0xffu) << 8) | (v[u] << 16);
+      }
+    }
+  }
+  if (clamped) atomicOr(&gr.status, (int)PTG_STATUS_CLAMPED);
+  s = wave_sum_i64(s);
+  if (lane == 0) part[wave] = s;
+  __syncthreads();  // also makes every thread's packed words visible to the whole workgroup
+  long long tot = 0;
+#pragma unroll
+  for (int w = 0; w < kPreThreads / 64; w++) tot += part[w];
+  const double mean0 = (double)tot / (double)n;
+  double C0C0 = 0.0, sd = 0.0;
+  const int nch = (n + kPreChunk - 1) / kPreChunk;
+  for (int ch = 0; ch <= nch; ch++) {
+    if (wave > 0 && ch < nch) {
+      // stage chunk ch: d0 from the packed words' I0
+      const int base = ch * kPreChunk, m = min(kPreChunk, n - base);
+      double* dst = chunk[ch & 1];
+      for (int t = tid - 64; t < m; t += kPreThreads - 64) {
+        const double diff = (double)(spwords[b + base + t] >> 16) - mean0;
+        dst[t] = diff;
+        d0[b + base + t] = diff;
+      }
+    }
+    if (tid == 0 && ch > 0) {
+      // sum chunk ch - 1 in pixel order
+      const int base = (ch - 1) * kPreChunk, m = min(kPreChunk, n - base);
+      const double2* src = reinterpret_cast<const double2*>(chunk[(ch - 1) & 1]);
+      int t = 0;
+      for (; t + 16 <= m; t += 16) {
+        double2 q[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) q[j] = src[t / 2 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          C0C0 += q[j].x * q[j].x;
+          sd += q[j].x;
+          C0C0 += q[j].y * q[j].y;
+          sd += q[j].y;
+        }
+      }
+      const double* srcd = chunk[(ch - 1) & 1];
+      for (; t < m; t++) {
+        const double diff = srcd[t];
         C0C0 += diff * diff;
         sd += diff;
       }
     }
     __syncthreads();
   }
-  if (clamped) atomicOr(&gr.status, (int)PTG_STATUS_CLAMPED);
-  if (lane == 0) {
+  if (tid == 0) {
     gr.mean0 = mean0;
     gr.C0C0 = C0C0;
     sd0[g] = sd;
@@ -154,8 +197,9 @@ __global__ __launch_bounds__(64) void k_precompute(const uint16_t* __restrict__ 
 hipError_t launch_precompute(Context& c) {
   if (c.G == 0) return hipSuccess;
   const Image& im = c.img[c.iref];
-  k_precompute<<<c.G, 64, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern, c.d_pbegin,
-                                         c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords, c.d_spwords, c.d_m0);
+  k_precompute<<<c.G, kPreThreads, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern,
+                                                  c.d_pbegin, c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords,
+                                                  c.d_spwords, c.d_m0);
   return hipGetLastError();
 }
 
