@@ -262,6 +262,8 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   std::vector<int32_t> list;
   int32_t* d_list = c->d_list[stage];
   int32_t* d_count = c->d_counts + stage;
+  // next queue counter starts at zero (enqueued ahead of the wait for this stage's list)
+  if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
   if (stage == PTG_STAGE_MAIN) {
     if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
     list = c->h_unmasked;  // built at set/put time
@@ -276,8 +278,6 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     CK(c, hipStreamSynchronize(c->stream));
   }
   const int nlist = (int)list.size();
-  // next queue counter starts at zero
-  if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
   if (nlist == 0) return PTG_OK;
 
   // rotation tables: cos/sin(refrot + drot + step) per (grain, step) from the host's
@@ -735,25 +735,48 @@ int ptg_search(void* p, int stage, const ptg_search_zone* zone, double NCC_min, 
   return PTG_OK;
 }
 
-int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
-  Ctx* c = as_ctx(p);
-  if (!c) return PTG_ERR_ARG;
+// Zeroes the subpixel stage's evaluation counters (the total and the per-grain counts).
+__global__ void k_subpixel_reset(long long* evals, int32_t* evals_g, int G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *evals = 0;
+  if (i < G) evals_g[i] = 0;
+}
+
+static int subpixel_reset(Ctx* c) {
+  TRY(c->ensure((void**)&c->d_evals_g, &c->evals_g_cap, sizeof(int32_t) * c->G));
+  k_subpixel_reset<<<std::max(1, (c->G + 255) / 256), 256, 0, c->stream>>>((long long*)c->d_evals, c->d_evals_g,
+                                                                           c->G);
+  CK(c, hipGetLastError());
+  return PTG_OK;
+}
+
+static int subpixel_check(Ctx* c, const ptg_subpix_params* prm) {
   if (!prm) return c->fail(PTG_ERR_ARG, "null subpixel params");
-  TRY(need_grains(c));
-  TRY(need_images(c, true, true));
   if (prm->method != PTG_SUBPIX_POWELL && prm->method != PTG_SUBPIX_LM)
     return c->fail(PTG_ERR_ARG, "bad subpixel method %d", prm->method);
+  return PTG_OK;
+}
+
+// reset: the counters were already zeroed in this pair (track_pair does it before the
+// rescue stages' host round trip, so that only the kernel launch follows it)
+static int subpixel_run(Ctx* c, const ptg_subpix_params* prm, bool reset) {
   if (c->require_precomputations) TRY(do_precompute(c));
+  if (reset) TRY(subpixel_reset(c));
   hipEvent_t ev;
   c->mark_begin(PTG_T_SUBPIXEL, &ev);
-  k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
-  CK(c, hipGetLastError());
-  TRY(c->ensure((void**)&c->d_evals_g, &c->evals_g_cap, sizeof(int32_t) * c->G));
-  CK(c, fill_i32(*c, c->d_evals_g, c->G, 0));
   CK(c, launch_subpixel(*c, *prm));
   c->mark_end(PTG_T_SUBPIXEL, ev);
   c->snap = Ctx::SNAP_INVALID;
   return PTG_OK;
+}
+
+int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(subpixel_check(c, prm));
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  return subpixel_run(c, prm, true);
 }
 
 int ptg_end_pair(void* p) {
@@ -799,18 +822,20 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
   if (!prm) return c->fail(PTG_ERR_ARG, "null track params");
   TRY(need_grains(c));
   TRY(need_images(c, true, true));
+  if (prm->subpixel) TRY(subpixel_check(c, &prm->subpix));
   if (c->require_precomputations) TRY(do_precompute(c));
   if (prm->rescue_level >= 0) {
     TRY(ptg_begin_pair(p));
     TRY(do_search(c, PTG_STAGE_MAIN, &prm->search_zone, prm->NCC_min, prm->NCC_min_super, prm->search_method));
   }
+  if (prm->subpixel) TRY(subpixel_reset(c));
   if (prm->rescue_level >= 1)
     TRY(do_search(c, PTG_STAGE_RESCUE, &prm->search_zone_rescue, prm->NCC_min, prm->NCC_min_super,
                   prm->search_method));
   if (prm->rescue_level >= 2)
     TRY(do_search(c, PTG_STAGE_SUPER, &prm->search_zone_super_rescue, prm->NCC_min, prm->NCC_min_super,
                   prm->search_method));
-  if (prm->subpixel) TRY(ptg_subpixel(p, &prm->subpix));
+  if (prm->subpixel) TRY(subpixel_run(c, &prm->subpix, false));
   TRY(ptg_end_pair(p));
   // a stage that could not be served exactly is reported here, after the pair's work
   if (c->any_flags_pending()) TRY(c->sync_and_check());
