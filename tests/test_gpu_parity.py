@@ -518,3 +518,52 @@ def test_exact_base_mask_for_disks_and_other_patterns(tracker):
     for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
         assert bits_equal(got[col], ref[col]), col
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+
+
+def test_exact_and_plain_agree_on_clamped_windows(tracker):
+    # grains moved against the frame edges: their windows (and patterns) read beyond the
+    # frame, which both forms clamp to the edge pixel and report as PTG_STATUS_CLAMPED; the
+    # exact form (disk and square base masks) picks as the plain one
+    w = synth.workload(2, n_grains=24, size=(700, 600), seed=2077)
+    gr = w.grains.copy()
+    H, W = w.I0.shape
+    edge = np.zeros(w.G, dtype=bool)
+    for g in range(0, w.G, 3):
+        r = int(gr["radius_pix"][g])
+        side = (g // 3) % 4
+        if side == 0:
+            gr["refcoord_xpix"][g] = r // 2
+        elif side == 1:
+            gr["refcoord_ypix"][g] = H - 1 - r // 2
+        elif side == 2:
+            gr["refcoord_xpix"][g] = W - 1 - r // 3
+        else:
+            gr["refcoord_ypix"][g] = 2
+        edge[g] = True
+    pats = []
+    for g in range(w.G):
+        P = w.pattern_xy[w.pattern_begin[g]:w.pattern_begin[g + 1]]
+        if g % 2 == 1:
+            h = math.isqrt(int((P.astype(np.int64) ** 2).sum(1).max())) // 2
+            ys, xs = np.mgrid[-h:h + 1, -h:h + 1]
+            P = np.stack([xs.ravel(), ys.ravel()], 1)
+        pats.append(P.astype(np.int32))
+    pxy = np.concatenate(pats).astype(np.int32)
+    pb = np.concatenate([[0], np.cumsum([len(p) for p in pats])]).astype(np.int64)
+    z = pk.zone(12, num_rot=3, inc_rot=0.02)
+    say(f"clamped windows, exact vs plain: G={w.G}")
+    res = {}
+    for m in (pk.SEARCH_PLAIN, pk.SEARCH_EXACT):
+        tracker.set_image(0, w.I0, w.bits)
+        tracker.set_image(1, w.I1, w.bits)
+        tracker.set_image_indices(0, 1)
+        tracker.set_grains(gr, pxy, pb)
+        tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=m))
+        res[m] = tracker.get_grains()
+    a, b = res[pk.SEARCH_EXACT], res[pk.SEARCH_PLAIN]
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        assert bits_equal(a[col], b[col]), col
+    assert np.abs(a["NCC"] - b["NCC"]).max() < 1e-12
+    clamped = (a["status"] & 1) != 0
+    assert clamped[edge].all() and not clamped[~edge].any()
+    assert ((b["status"] & 1) != 0).tolist() == clamped.tolist()
