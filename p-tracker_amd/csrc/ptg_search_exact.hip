@@ -881,6 +881,7 @@ static int main_vgprs(int nmt, int ntp) {
 #define PTG_FN(M, T) \
   if (nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T>;
   PTG_FN(1, 3) PTG_FN(2, 3) PTG_FN(3, 3) PTG_FN(4, 2) PTG_FN(5, 2) PTG_FN(6, 2)
+  PTG_FN(1, 2) PTG_FN(2, 2) PTG_FN(3, 2) PTG_FN(1, 1) PTG_FN(2, 1) PTG_FN(3, 1) PTG_FN(4, 1) PTG_FN(5, 1) PTG_FN(6, 1)
 #undef PTG_FN
   static int cache[8][4] = {};
   if (!fn) return 256;
@@ -897,6 +898,8 @@ static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid,
 #define PTG_CASE(M, T) \
   if (nmt == M && ntp == T) return launch_main<M, T>(e, grid, block, lds, st);
   PTG_CASE(1, 3) PTG_CASE(2, 3) PTG_CASE(3, 3) PTG_CASE(4, 2) PTG_CASE(5, 2) PTG_CASE(6, 2)
+  PTG_CASE(1, 2) PTG_CASE(2, 2) PTG_CASE(3, 2) PTG_CASE(1, 1) PTG_CASE(2, 1) PTG_CASE(3, 1) PTG_CASE(4, 1)
+  PTG_CASE(5, 1) PTG_CASE(6, 1)
 #undef PTG_CASE
   return hipErrorInvalidValue;
 }
@@ -1086,7 +1089,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   if (nmt > 6) return kExactUnfit;
   const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1;
   if ((int64_t)nsteps * U * V >= INT_MAX / 2) return kExactUnfit;
-  const int ntp = nmt <= 3 ? 3 : 2;
+  // N-tiles per pass: 3 (2 for more than 3 M-tiles, the accumulators' register budget), and
+  // no more than the vertical shifts need (a zone of +-8 has 17: 2 tiles, not 3)
+  const int ntp = std::min(nmt <= 3 ? 3 : 2, (V + 15) / 16);
   // the host plan: cached for a keyed list, else built for this call
   ExactPlan local;
   ExactPlan* P = &local;
