@@ -535,12 +535,13 @@ __host__ __forceinline__ size_t base_disk_lds_bytes(int rho, int U, int vb) {
 // Sums of I1 and I1^2 over one Delta list (ks = 2 step + sign) at this lane's shift: the
 // entries held in LDS (4 per 16-byte read, 16 window reads in flight), then any beyond the LDS
 // capacity from global memory. A list has at most 1023 cells, so the sum of I1 is exact in
-// 32 bits; I1^2 is summed in 32 bits per group of at most 8 cells, then in 64.
+// 32 bits; I1^2 is summed in 32 bits per group of at most 8 cells, then in 64. An inactive
+// lane (act false) sums nothing.
 __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
                                            const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
-                                           int ks, uint32_t wbase, int base, unsigned int& bsum,
+                                           bool act, int ks, uint32_t wbase, int base, unsigned int& bsum,
                                            unsigned long long& csum) {
-  const int n = dcnt[ks], o = dofs[ks];
+  const int n = act ? dcnt[ks] : 0, o = act ? dofs[ks] : 0;
   const int nl = max(0, min(n, e.dl_lds_cap - o));
   const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
   auto px = [&](uint32_t off) { return (unsigned int)*reinterpret_cast<const uint16_t*>(wb + off); };
@@ -753,9 +754,13 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_sched_barrier(0);
       }
       // epilogue, one M-tile (8 steps) at a time: the cross terms go through LDS so that
-      // lanes become shifts and the step (and its correction list) is wave-uniform
-      const int svl = p0 * 16 + lane;  // this lane's shift in the epilogue
-      const bool lane_ok = lane < NTP * 16 && svl < V;
+      // lanes become (step, shift) pairs, R = 64 / (16 NTP) steps side by side over the
+      // pass's 16 NTP vertical shifts (R > 1 keeps the lanes busy when a zone has few
+      // vertical shifts: 17 of them fill 34 lanes, not 17)
+      constexpr int TW = NTP * 16, R = 64 / TW;
+      const int sub = R == 1 ? 0 : lane / TW, lsh = R == 1 ? lane : lane - sub * TW;
+      const int svl = p0 * 16 + lsh;  // this lane's shift in the epilogue
+      const bool lane_ok = (R == 1 ? lane < TW : sub < R) && svl < V;
       double Bm = 0.0, Cm = 0.0;
       int base = 0, sidx = 0;
       if (lane_ok) {
@@ -781,16 +786,20 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int kl = 0; kl < 8; kl++) {
-          const int k = 8 * a + kl;
-          if (k >= nsteps) break;
+        for (int kl0 = 0; kl0 < 8; kl0 += R) {
+          if (8 * a + kl0 >= nsteps) break;
+          // one step per round (R = 1): the step and its lists stay wave-uniform
+          const int kl = R == 1 ? kl0 : kl0 + sub, k = 8 * a + kl;
+          // the lane's candidate exists; its lists are summed by every lane when R = 1
+          const bool act = lane_ok && (R == 1 || k < nsteps);
+          const bool dact = R == 1 || act;
           unsigned int bp = 0u, bn = 0u;
           unsigned long long cp = 0ull, cn = 0ull;
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, 2 * k, wbase, base, bp, cp);
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, 2 * k + 1, wbase, base, bn, cn);
-          if (lane_ok) {
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, dact, 2 * k, wbase, base, bp, cp);
+          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, dact, 2 * k + 1, wbase, base, bn, cn);
+          if (act) {
             // every quantity is an integer below 2^53, so these double operations are exact
-            const double A = abuf[kl * (NTP * 16) + lane];
+            const double A = abuf[kl * TW + lsh];
             const double B = Bm + ((double)bp - (double)bn);
             const double C = Cm + ((double)cp - (double)cn);
             const double num = Nd * A - Sad * B;
