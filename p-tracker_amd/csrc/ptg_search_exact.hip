@@ -754,22 +754,25 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_sched_barrier(0);
       }
       // epilogue, one M-tile (8 steps) at a time: the cross terms go through LDS so that
-      // lanes become (step, shift) pairs, R = 64 / (16 NTP) steps side by side over the
-      // pass's 16 NTP vertical shifts (R > 1 keeps the lanes busy when a zone has few
-      // vertical shifts: 17 of them fill 34 lanes, not 17)
-      constexpr int TW = NTP * 16, R = 64 / TW;
-      const int sub = R == 1 ? 0 : lane / TW, lsh = R == 1 ? lane : lane - sub * TW;
-      const int svl = p0 * 16 + lsh;  // this lane's shift in the epilogue
-      const bool lane_ok = (R == 1 ? lane < TW : sub < R) && svl < V;
-      double Bm = 0.0, Cm = 0.0;
-      int base = 0, sidx = 0;
+      // lanes become (step, shift) pairs. Three N-tiles per pass (48 shifts): lane = shift,
+      // one step per round, so the step and its correction lists are wave-uniform. One or
+      // two N-tiles (few vertical shifts, or many steps): the M-tile's (step, shift) pairs
+      // laid out 64 per round, several steps side by side, so the lanes stay busy (17
+      // shifts x 8 steps take 3 rounds, not 8; measured 7.28 -> 5.27 ms for 40 steps x
+      // 41 x 41 shifts, while the uniform form stays faster with 48 shifts per pass)
+      constexpr int TW = NTP * 16;
+      constexpr bool kUniform = NTP == 3;
+      const int vpass = min(TW, V - p0 * 16);  // vertical shifts of this pass
+      const int svl0 = p0 * 16 + lane;  // the uniform form's shift of this lane
+      const bool lane_ok = kUniform && lane < TW && svl0 < V;
+      double Bm0 = 0.0, Cm0 = 0.0;
+      int base0 = 0, sidx0 = 0;
       if (lane_ok) {
-        sidx = su * V + svl;
-        Bm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx];
-        Cm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
-        base = svl * WW + su;
+        sidx0 = su * V + svl0;
+        Bm0 = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx0];
+        Cm0 = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx0 + 1];
+        base0 = svl0 * WW + su;
       }
-      const uint32_t wbase = 2u * (uint32_t)base;  // byte offset of this lane's window origin
 #pragma unroll
       for (int a = 0; a < NMT; a++) {
         // lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg + h, digits (1, 0) per step
@@ -786,35 +789,57 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int kl0 = 0; kl0 < 8; kl0 += R) {
-          if (8 * a + kl0 >= nsteps) break;
-          // one step per round (R = 1): the step and its lists stay wave-uniform
-          const int kl = R == 1 ? kl0 : kl0 + sub, k = 8 * a + kl;
-          // the lane's candidate exists; its lists are summed by every lane when R = 1
-          const bool act = lane_ok && (R == 1 || k < nsteps);
-          const bool dact = R == 1 || act;
-          unsigned int bp = 0u, bn = 0u;
-          unsigned long long cp = 0ull, cn = 0ull;
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, dact, 2 * k, wbase, base, bp, cp);
-          delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, dact, 2 * k + 1, wbase, base, bn, cn);
-          if (act) {
-            // every quantity is an integer below 2^53, so these double operations are exact
-            const double A = abuf[kl * TW + lsh];
-            const double B = Bm + ((double)bp - (double)bn);
-            const double C = Cm + ((double)cp - (double)cn);
-            const double num = Nd * A - Sad * B;
-            const double d1 = Nd * C - B * B;
-            const int gidx = k * S + sidx;
-            // rank by NCC |NCC| * d0 = num |num| / d1 by cross-multiplication (no divide);
-            // only num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680)
-            if (num > 0.0 && d1 > 0.0) {
-              const double lhs = num * num * bd1, rhs = bnum * bnum * d1;
-              if (bidx == INT_MAX || lhs > rhs || (lhs == rhs && gidx < bidx)) {
-                bnum = num;
-                bd1 = d1;
-                bidx = gidx;
-              }
+        // one candidate (step k, shift sidx) with its Delta sums
+        auto consider = [&](int k, int sidx, double A, double Bm, double Cm, unsigned int bp, unsigned int bn,
+                            unsigned long long cp, unsigned long long cn) {
+          // every quantity is an integer below 2^53, so these double operations are exact
+          const double B = Bm + ((double)bp - (double)bn);
+          const double C = Cm + ((double)cp - (double)cn);
+          const double num = Nd * A - Sad * B;
+          const double d1 = Nd * C - B * B;
+          const int gidx = k * S + sidx;
+          // rank by NCC |NCC| * d0 = num |num| / d1 by cross-multiplication (no divide);
+          // only num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680)
+          if (num > 0.0 && d1 > 0.0) {
+            const double lhs = num * num * bd1, rhs = bnum * bnum * d1;
+            if (bidx == INT_MAX || lhs > rhs || (lhs == rhs && gidx < bidx)) {
+              bnum = num;
+              bd1 = d1;
+              bidx = gidx;
             }
+          }
+        };
+        if constexpr (kUniform) {
+          const uint32_t wbase = 2u * (uint32_t)base0;  // byte offset of this lane's window origin
+          for (int kl = 0; kl < 8; kl++) {
+            const int k = 8 * a + kl;
+            if (k >= nsteps) break;
+            unsigned int bp = 0u, bn = 0u;
+            unsigned long long cp = 0ull, cn = 0ull;
+            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, true, 2 * k, wbase, base0, bp, cp);
+            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, true, 2 * k + 1, wbase, base0, bn, cn);
+            if (lane_ok) consider(k, sidx0, abuf[kl * TW + lane], Bm0, Cm0, bp, bn, cp, cn);
+          }
+        } else {
+          const int nk = min(8, nsteps - 8 * a);
+          const int nitem = nk > 0 ? nk * vpass : 0;
+          for (int it0 = 0; it0 < nitem; it0 += 64) {
+            const int it = it0 + lane;
+            const bool act = it < nitem;
+            const int kl = act ? it / vpass : 0, vl = act ? it - kl * vpass : 0;
+            const int k = 8 * a + kl, svl = p0 * 16 + vl;
+            const int sidx = su * V + svl, base = svl * WW + su;
+            double Bm = 0.0, Cm = 0.0;
+            if (act) {
+              Bm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx];
+              Cm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
+            }
+            const uint32_t wbase = 2u * (uint32_t)base;
+            unsigned int bp = 0u, bn = 0u;
+            unsigned long long cp = 0ull, cn = 0ull;
+            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k, wbase, base, bp, cp);
+            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k + 1, wbase, base, bn, cn);
+            if (act) consider(k, sidx, abuf[kl * TW + vl], Bm, Cm, bp, bn, cp, cn);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
