@@ -477,21 +477,43 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
     }
     return;
   }
-  // one shift per thread: (su, sv0 + v)
-  for (int t = threadIdx.x; t < U * nv; t += blockDim.x) {
-    const int v = t / U, su = t - v * U;
-    unsigned int B = 0u;
-    unsigned long long C = 0ull;
-    const uint2* rowp = pre + (size_t)(v + rho - r) * pst + su + rho;
-    for (int q = 0; q <= 2 * r; q++, rowp += pst) {
-      const int w = xm[q];
-      const uint2 lo = rowp[-w], hi = rowp[w + 1];
-      B += hi.x - lo.x;
-      C += (unsigned long long)(hi.y - lo.y);
+  // kSh shifts per thread and pass, (su, sv0 + v) for t = threadIdx.x + j blockDim.x: one
+  // half-width read serves all of them, and their 2 kSh prefix reads per row are in flight
+  // together
+  constexpr int kSh = 4;
+  const int nsh = U * nv;
+  for (int t0 = threadIdx.x; t0 < nsh; t0 += kSh * blockDim.x) {
+    const uint2* rowp[kSh];
+    unsigned int B[kSh];
+    unsigned long long C[kSh];
+#pragma unroll
+    for (int j = 0; j < kSh; j++) {
+      const int t = min(t0 + j * (int)blockDim.x, nsh - 1);
+      const int v = t / U, su = t - v * U;
+      rowp[j] = pre + (size_t)(v + rho - r) * pst + su + rho;
+      B[j] = 0u;
+      C[j] = 0ull;
     }
-    const int64_t idx = (int64_t)su * V + sv0 + v;
-    e.mid[sl.mid_off + 2 * idx] = (long long)B;
-    e.mid[sl.mid_off + 2 * idx + 1] = (long long)C;
+    for (int q = 0; q <= 2 * r; q++) {
+      const int w = xm[q];
+      const int ro = q * pst;  // row q's offset (the window's prefix table is below 2^31 entries)
+#pragma unroll
+      for (int j = 0; j < kSh; j++) {
+        const uint2 lo = rowp[j][ro - w], hi = rowp[j][ro + w + 1];
+        B[j] += hi.x - lo.x;
+        C[j] += (unsigned long long)(hi.y - lo.y);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSh; j++) {
+      const int t = t0 + j * (int)blockDim.x;
+      if (t < nsh) {
+        const int v = t / U, su = t - v * U;
+        const int64_t idx = (int64_t)su * V + sv0 + v;
+        e.mid[sl.mid_off + 2 * idx] = (long long)B[j];
+        e.mid[sl.mid_off + 2 * idx + 1] = (long long)C[j];
+      }
+    }
   }
   if (blockIdx.y == 0) {
     // the disk's multiplicities (each cell once) for the template kernel's Delta lists
