@@ -58,7 +58,7 @@ struct SearchArgs {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
-  // second stream of the exact search (each radius bucket's template and middle-step
+  // second stream of the exact search (each radius bucket's template and base-mask
   // kernels run ahead of the MFMA kernels of the previous buckets), default priority, and
   // its fork/join events (created on first use)
   hipStream_t aux_stream = nullptr;

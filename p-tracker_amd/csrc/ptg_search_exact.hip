@@ -49,7 +49,7 @@ struct ExactSlot {
   int64_t tmpl_off;  // first byte of this slot's template fragments
   int64_t dl_off;    // first Delta entry (nsteps * 2 * dcap entries)
   int64_t mid_off;   // first int64 of (B_mid, C_mid) pairs (S pairs)
-  int64_t mseg_off;  // first segment of the middle-step GEMM's segment table
+  int64_t mseg_off;  // first segment of the base-mask GEMM's segment table (k_exact_mid)
   int64_t mmask_off; // first byte of the base mask's multiplicity grid (side x side)
   int32_t mkc;       // its K-chunks (4 segments each)
   int32_t dcap;      // Delta entries per (step, sign), at most 1023
@@ -81,12 +81,12 @@ struct ExactArgs {
   int32_t cs_max;              // staging row stride of the largest slot of the launch
   int32_t dl_lds_cap;          // Delta entries staged in LDS per slot (the rest is read from global)
   int32_t seg_cap;             // K-chunks of the largest slot of the launch (segment table in LDS)
-  const int32_t* mseg;         // middle-step GEMM segments: (qy << 16) | x0, 0x8000 = empty
-  int32_t mid_cbw;             // 16-column blocks per middle-step workgroup
+  const int32_t* mseg;         // base-mask GEMM segments: (qy << 16) | x0, 0x8000 = empty
+  int32_t mid_cbw;             // 16-column blocks per k_exact_mid workgroup
   int32_t mid_stride;          // digit-plane row stride (bytes) of the launch
   int32_t mid_mkc_cap;         // largest mkc of the launch
   int32_t abuf_alias;          // epilogue buffer in the wave's staging area (one vertical pass)
-  uint8_t* mmask;              // middle-step multiplicity grids (written by k_exact_mid)
+  uint8_t* mmask;              // base-mask multiplicity grids (written by k_exact_mid / k_exact_base_disk)
   int32_t mid_npl;             // digit planes staged per pass of k_exact_mid (6 or 3)
   int32_t mid_skip_disk;       // k_exact_mid leaves the disk-based slots to k_exact_base_disk
   int32_t* ovf_slot;           // per slot: 1 when its template or Delta lists overflowed (or null:
@@ -113,7 +113,7 @@ __device__ __forceinline__ int win_y0(const ptg_grain& gr, int up, int rho) {
 
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
 // LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
-// bytes = count of the base mask (from k_exact_mid, which runs first). Rows beyond the
+// bytes = count of the base mask (from k_exact_mid or k_exact_base_disk, which run first). Rows beyond the
 // last step are written as zeros.
 __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   uint8_t* cmid = reinterpret_cast<uint8_t*>(packed + ncell);
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) packed[t] = 0;
   const bool real = k < nsteps;
-  // the base mask's multiplicities (k_exact_mid)
+  // the base mask's multiplicities (k_exact_mid or k_exact_base_disk)
   if (real)
     for (int t = threadIdx.x; t < ncell; t += blockDim.x) cmid[t] = e.mmask[sl.mmask_off + t];
   __syncthreads();
@@ -1071,7 +1071,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     mid_total += 2 * (int64_t)S;
     s.mmask_off = mmask_total;
     mmask_total += (int64_t)(2 * rho + 1) * (2 * rho + 1);
-    // middle-step GEMM: per mask row, 16-aligned segments covering x in
+    // base-mask GEMM: per mask row, 16-aligned segments covering x in
     // [rho - xm, rho + xm + 15] (the row's cells moved right by up to 15)
     s.mseg_off = (int64_t)P.msegs.size();
     for (int qy = -rho; qy <= rho; qy++) {
@@ -1180,7 +1180,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int rho = bk.rho_max, side = 2 * rho + 1;
     const int cw = (side + 15) / 16 * 16;
     l.cs = ((cw / 16) % 2 == 1) ? cw : cw + 16;
-    // middle-step window sums B_mid, C_mid and the reference moments: as many 16-column
+    // base-mask window sums B_mid, C_mid and the reference moments: as many 16-column
     // blocks per workgroup as keep two workgroups per CU (at least one block); all six
     // digit planes in one pass when they fit the CU's LDS, else two passes of 3 (a
     // pattern too large for that is too large for the MFMA kernel's window as well)
@@ -1321,7 +1321,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   *nparts_out = part_cap;
   k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
-  // the template and middle-step kernels of every bucket run in order on the second
+  // the template and base-mask kernels of every bucket run in order on the second
   // stream; each bucket's MFMA kernel runs after its bucket's event, on the library stream
   // and the search streams in turn, so the next bucket's preparation overlaps the current
   // search and consecutive searches overlap their tails
