@@ -205,8 +205,10 @@ int sync_snapshot(Ctx* c) {
   return c->check_flags();
 }
 
+// Grains were set (ptg_set_grains); zero grains is a valid set, on which the stage calls
+// do nothing, as the reference's loops over an empty grain list do.
 int need_grains(Ctx* c) {
-  if (c->G <= 0) return c->fail(PTG_ERR_STATE, "no grains set (ptg_set_grains)");
+  if (!c->grains_set) return c->fail(PTG_ERR_STATE, "no grains set (ptg_set_grains)");
   return PTG_OK;
 }
 
@@ -223,6 +225,7 @@ int need_images(Ctx* c, bool ref, bool cur) {
 
 int do_precompute(Ctx* c) {
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, false));
   // cos/sin(refrot) per grain from the host's libm, as do_precomputations (:636-637)
   TRY(c->wait_staging());
@@ -246,6 +249,7 @@ constexpr size_t kMaxLds = 160 * 1024 - 1024;  // dynamic LDS; the rest for the 
 
 int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, double NCC_min_super, int method) {
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, true));
   if (!z) return c->fail(PTG_ERR_ARG, "null search zone");
   if (stage < PTG_STAGE_MAIN || stage > PTG_STAGE_SUPER) return c->fail(PTG_ERR_ARG, "bad stage %d", stage);
@@ -613,6 +617,7 @@ int ptg_set_grains(void* p, int32_t G, const ptg_grain* grains, const int32_t* p
   CK(c, hipStreamSynchronize(c->stream));
   free_grain_buffers(c);
   c->G = G;
+  c->grains_set = true;
   c->total_px = total;
   c->h_grains.assign(grains, grains + G);
   c->h_pattern.assign(pattern_xy, pattern_xy + 2 * total);
@@ -690,6 +695,7 @@ int ptg_put_grains(void* p, const ptg_grain* grains) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   if (!grains) return c->fail(PTG_ERR_ARG, "null grains");
   CK(c, hipStreamSynchronize(c->stream));
   c->h_grains.assign(grains, grains + c->G);
@@ -714,6 +720,7 @@ int ptg_begin_pair(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   CK(c, launch_begin_pair(*c));
   CK(c, fill_i32(*c, c->d_counts + 1, 2, 0));
   // NCC fields and backups changed; dx/drot did not
@@ -775,6 +782,7 @@ int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
   if (!c) return PTG_ERR_ARG;
   TRY(subpixel_check(c, prm));
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, true));
   return subpixel_run(c, prm, true);
 }
@@ -783,6 +791,7 @@ int ptg_end_pair(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   TRY(need_grains(c));
+  if (c->G == 0) return PTG_OK;
   CK(c, launch_end_pair(*c));
   if (c->snap == Ctx::SNAP_VALID) {
     for (auto& g : c->h_grains) {
@@ -823,6 +832,7 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
   TRY(need_grains(c));
   TRY(need_images(c, true, true));
   if (prm->subpixel) TRY(subpixel_check(c, &prm->subpix));
+  if (c->G == 0) return PTG_OK;
   if (c->require_precomputations) TRY(do_precompute(c));
   if (prm->rescue_level >= 0) {
     TRY(ptg_begin_pair(p));

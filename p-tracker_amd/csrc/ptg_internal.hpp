@@ -75,6 +75,7 @@ struct Context {
 
   // grains
   int32_t G = 0;
+  bool grains_set = false;  // ptg_set_grains was called (possibly with zero grains)
   ptg_grain* d_grains = nullptr;
   int32_t* d_pattern = nullptr;
   int64_t* d_pbegin = nullptr;

@@ -368,6 +368,13 @@ __device__ __forceinline__ void make_request(const ObjConst& o, const double* X,
   rq->kind = kind;
 }
 
+// A sampled window whose one-pass variance C1C1 = Sgg - Sg^2 / N is zero up to the
+// rounding of Sgg: the samples are all equal, where the reference's two passes give
+// C1C1 = C0C1 = 0 exactly and NCC = 0 / 0, not a number (with integer samples, as at an
+// integer position, any other window has C1C1 >= 1/2, far above this bound). The optimizers
+// stop at such a value as the reference's do.
+__device__ __forceinline__ bool flat_window(double C1C1, double Sgg) { return !(C1C1 > 64.0 * 2.220446049250313e-16 * Sgg); }
+
 // (1 - NCC) * W at X from the value sums. The reference's two passes (mean, then
 // centred products) are replaced by one pass over shifted sums:
 // C1C1 = Sgg - Sg^2/N, C0C1 = Sdg - Sg*Sd0/N.
@@ -376,7 +383,7 @@ __device__ __forceinline__ double finish_value(const ObjConst& o, const double* 
   const double N = o.N;
   const double C1C1 = S[1] - S[0] * S[0] / N;
   const double C0C1 = S[2] - S[0] * o.sd0 / N;
-  const double NCC = C0C1 / sqrt(o.C0C0 * C1C1);
+  const double NCC = flat_window(C1C1, S[1]) ? __builtin_nan("") : C0C1 / sqrt(o.C0C0 * C1C1);
   *wb = NCC / sqrt(C1C1);
   *wm = S[0] / N;
   return (1.0 - NCC) * weight(o, X);
@@ -403,7 +410,7 @@ __device__ __forceinline__ void finish_derivs(const ObjConst& o, const double* S
   const double C1C1 = S[1] - S[0] * m1;
   const double C0C1 = S[2] - m1 * o.sd0;
   const double sig1 = sqrt(C1C1), sig0 = sqrt(o.C0C0);
-  const double NCC = C0C1 / (sig0 * sig1);
+  const double NCC = flat_window(C1C1, S[1]) ? __builtin_nan("") : C0C1 / (sig0 * sig1);
   double mG[3], beta[3], hf[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {

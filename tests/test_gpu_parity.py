@@ -567,3 +567,83 @@ def test_exact_and_plain_agree_on_clamped_windows(tracker):
     clamped = (a["status"] & 1) != 0
     assert clamped[edge].all() and not clamped[~edge].any()
     assert ((b["status"] & 1) != 0).tolist() == clamped.tolist()
+
+
+def _same(a, b):
+    """Bit-equal, or both NaN (the NaN's sign differs between the host's and the GPU's 0/0)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all(both_nan | (a.view(np.uint64) == b.view(np.uint64))))
+
+
+def test_zero_grains_is_a_no_op(tracker, cfg2s):
+    # the reference's loops over an empty grain list do nothing; so does every stage here
+    say("zero grains")
+    tracker.set_image(0, cfg2s.I0, cfg2s.bits)
+    tracker.set_image(1, cfg2s.I1, cfg2s.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(cfg2s.grains[:0], np.zeros((0, 2), np.int32), np.zeros(1, np.int64))
+    for m in (pk.SEARCH_PLAIN, pk.SEARCH_EXACT):
+        tracker.track_pair(pk.track_params(cfg2s.params.search_zone, rescue_level=2, subpixel=1, search_method=m,
+                                           subpix=pk.default_subpix(pk.SUBPIX_LM)))
+    assert len(tracker.get_grains()) == 0
+    tracker.change_reference()
+
+
+@pytest.mark.parametrize("method", [pk.SEARCH_PLAIN, pk.SEARCH_EXACT])
+def test_degenerate_grains_follow_the_reference(tracker, method):
+    # a flat reference patch (C0C0 = 0), a flat current window (C1C1 = 0 for every
+    # candidate), a one-pixel pattern and a masked grain beside ordinary ones: an undefined
+    # ZNCC never wins (best_NCC stays 0, the grain is queued for the rescues and keeps its
+    # position) and the subpixel stage leaves an undefined objective alone, as in the reference
+    w = synth.workload(2, n_grains=12, size=(900, 800), seed=2090)
+    I0, I1 = w.I0.copy(), w.I1.copy()
+    z, zr = pk.zone(8, num_rot=2, inc_rot=0.02), pk.zone(12, num_rot=2, inc_rot=0.02)
+    gr = w.grains.copy()
+    x, y, r = int(gr["refcoord_xpix"][0]), int(gr["refcoord_ypix"][0]), int(gr["radius_pix"][0]) + 2
+    I0[y - r:y + r + 1, x - r:x + r + 1] = 1000
+    x, y, r = int(gr["refcoord_xpix"][1]), int(gr["refcoord_ypix"][1]), int(gr["radius_pix"][1]) + 35
+    I1[y - r:y + r + 1, x - r:x + r + 1] = 1500
+    gr["masked"][3] = 1
+    pats = [w.pattern_xy[w.pattern_begin[g]:w.pattern_begin[g + 1]] for g in range(w.G)]
+    pats[2] = np.zeros((1, 2), np.int32)
+    pxy = np.concatenate(pats).astype(np.int32)
+    pb = np.concatenate([[0], np.cumsum([len(p) for p in pats])]).astype(np.int64)
+    say(f"degenerate grains, method {method}")
+    tracker.set_image(0, I0, w.bits)
+    tracker.set_image(1, I1, w.bits)
+    tracker.set_image_indices(0, 1)
+    for sub in (pk.SUBPIX_POWELL, pk.SUBPIX_LM):
+        tracker.set_grains(gr, pxy, pb)
+        prm = pk.track_params(z, zr, zr, NCC_min=0.7, NCC_min_super=0.5, rescue_level=2, subpixel=1,
+                              search_method=method, subpix=pk.default_subpix(sub))
+        tracker.track_pair(prm)
+        got = tracker.get_grains()
+        oprm = prm if sub == pk.SUBPIX_POWELL else pk.track_params(
+            z, zr, zr, NCC_min=0.7, NCC_min_super=0.5, rescue_level=2, subpixel=1,
+            subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+        ref, counts, _ = ptref.track_pair(ptref.Workload(I0, I1, gr, pxy, pb, w.bits), oprm, nthreads=4)
+        for g in (0, 1, 2, 3):
+            for col in ("dx", "dy", "drot", "NCC", "NCC_rescue", "NCC_subpix"):
+                assert _same(got[col][g], ref[col][g]), (g, col, got[col][g], ref[col][g])
+        assert got["NCC"][0] == 0.0 and got["NCC"][2] == 0.0  # best_NCC never left its start
+        # every grain's integer stages pick as the reference; the subpixel results of the
+        # well-tracked grains away from the flattened patches agree to the tolerance (a
+        # window partly flat, or a grain lost beyond the rescue zones, leaves a landscape
+        # where the two optimizers may settle in different minima)
+        for col in ("NCC", "NCC_rescue"):
+            if method == pk.SEARCH_PLAIN:  # the reference's arithmetic: bit for bit
+                assert _same(got[col], ref[col]), col
+            else:  # exact integer moments: the same picks, NCC within FP64 rounding
+                assert np.abs(got[col] - ref[col]).max() < 1e-12, col
+        cx, cy = gr["refcoord_xpix"].astype(float), gr["refcoord_ypix"].astype(float)
+        far = np.ones(w.G, dtype=bool)
+        for g, pad in ((0, 2), (1, 35)):
+            reach = gr["radius_pix"] + gr["radius_pix"][g] + pad + 12 + 16
+            far &= np.maximum(np.abs(cx - cx[g]), np.abs(cy - cy[g])) > reach
+        found = (ref["NCC"] >= 0.7) | (ref["NCC_rescue"] > 0.5)  # the integer search placed the grain
+        ok = np.nonzero(far & found & (ref["NCC_subpix"] > 0.9))[0]
+        assert len(ok) > 0
+        assert np.abs(got["dx"][ok] - ref["dx"][ok]).max() < TOL_PX
+        assert np.abs(got["dy"][ok] - ref["dy"][ok]).max() < TOL_PX
