@@ -568,3 +568,35 @@ def test_minpack_least_squares_recovers_the_lens():
 
     q, _ = leastsq(fun, [0.0, 0.0], ftol=np.finfo(float).eps, xtol=np.finfo(float).eps, gtol=0.0)
     assert abs(q[0] * 1e-9 - X[2]) < 1e-6 * abs(X[2]) and abs(q[1] * 1e-16 - X[3]) < 1e-5 * abs(X[3])
+
+
+def test_full_size_fixture_is_the_oracle_on_the_generated_pair():
+    # the stored full-size results (tests/golden/cfg2_full_oracle.npz) belong to the pair the
+    # generator makes now, and the oracle reproduces them bit for bit on a sample of grains
+    import os
+    import sys
+
+    from ptracker_amd import synth
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    from make_cfg2_full_fixture import SEARCH_COLS, SUB_COLS, digest, params
+
+    fx = np.load(os.path.join(here, "golden", "cfg2_full_oracle.npz"))
+    w = synth.workload(2)
+    assert digest(w) == str(fx["digest"])
+    W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
+    sample = np.arange(0, w.G, 250)
+    gs, _, _ = ptref.track_pair(W, params(w, 0), nthreads=8, subset=sample)
+    for c in SEARCH_COLS:
+        assert np.array_equal(np.asarray(gs[c], dtype=np.float64).view(np.uint64),
+                              fx["search_" + c][sample].view(np.uint64)), c
+    gl, _, _ = ptref.track_pair(W, params(w, 1, pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13)), nthreads=8,
+                                subset=sample)
+    for c in SUB_COLS:
+        assert np.array_equal(np.asarray(gl[c], dtype=np.float64).view(np.uint64),
+                              fx["sub_" + c][sample].view(np.uint64)), c
+    gp, _, _ = ptref.track_pair(W, params(w, 1, pk.default_subpix(pk.SUBPIX_POWELL)), nthreads=8, subset=sample)
+    for c in SUB_COLS:
+        assert np.array_equal(np.asarray(gp[c], dtype=np.float64).view(np.uint64),
+                              fx["powell_" + c][sample].view(np.uint64)), c
