@@ -554,21 +554,19 @@ __host__ __forceinline__ size_t base_disk_lds_bytes(int rho, int U, int vb) {
 }
 
 // ---- the MFMA search ---------------------------------------------------------------
-// Sums of I1 and I1^2 over one Delta list (ks = 2 step + sign) at this lane's shift: the
-// entries held in LDS (4 per 16-byte read, 16 window reads in flight), then any beyond the LDS
-// capacity from global memory. A list has at most 1023 cells, so the sum of I1 is exact in
-// 32 bits; I1^2 is summed in 32 bits per group of at most 8 cells, then in 64. An inactive
-// lane (act false) sums nothing.
-__device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
-                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
-                                           bool act, int ks, uint32_t wbase, int base, unsigned int& bsum,
-                                           unsigned long long& csum) {
-  const int n = act ? dcnt[ks] : 0, o = act ? dofs[ks] : 0;
-  const int nl = max(0, min(n, e.dl_lds_cap - o));
+// Sums of I1 and I1^2 at this lane's shift over the entries [lo, lo + len) of one Delta list
+// held in LDS from dent[o] (lo a multiple of 4; 4 entries per 16-byte read, 16 window reads
+// in flight) and, with tail set, over the list's entries [nl, n) beyond the LDS capacity from
+// global memory (dg). A list has at most 1023 cells, so the sum of I1 is exact in 32 bits;
+// I1^2 is summed in 32 bits per group of at most 8 cells, then in 64.
+__device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* dent, const int32_t* dg, int o,
+                                          int lo, int len, bool tail, int nl, int n, uint32_t wbase, int base,
+                                          unsigned int& bsum, unsigned long long& csum) {
+  o += lo;
   const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
   auto px = [&](uint32_t off) { return (unsigned int)*reinterpret_cast<const uint16_t*>(wb + off); };
   int q = 0;
-  for (; q + 16 <= nl; q += 16) {
+  for (; q + 16 <= len; q += 16) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
     const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
     const uint4 e2 = *reinterpret_cast<const uint4*>(dent + o + q + 8);
@@ -581,7 +579,7 @@ __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* w
     csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
     csum += (unsigned int)(((w8 * w8 + w9 * w9) + (w10 * w10 + w11 * w11)) + ((w12 * w12 + w13 * w13) + (w14 * w14 + w15 * w15)));
   }
-  for (; q + 8 <= nl; q += 8) {
+  for (; q + 8 <= len; q += 8) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
     const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
     const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
@@ -589,21 +587,64 @@ __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* w
     bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
     csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
   }
-  for (; q < nl; q += 4) {
+  for (; q < len; q += 4) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
     const unsigned int w0 = px(e0.x);
-    const unsigned int w1 = q + 1 < nl ? px(e0.y) : 0u;
-    const unsigned int w2 = q + 2 < nl ? px(e0.z) : 0u;
-    const unsigned int w3 = q + 3 < nl ? px(e0.w) : 0u;
+    const unsigned int w1 = q + 1 < len ? px(e0.y) : 0u;
+    const unsigned int w2 = q + 2 < len ? px(e0.z) : 0u;
+    const unsigned int w3 = q + 3 < len ? px(e0.w) : 0u;
     bsum += (w0 + w1) + (w2 + w3);
     csum += (unsigned int)((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3));
   }
-  const int32_t* dg = e.dlist + dl_off + (int64_t)ks * dcap;
-  for (int r = nl; r < n; r++) {
-    const unsigned int w = win[base + dg[r]];
-    bsum += w;
-    csum += w * w;
+  if (tail)
+    for (int r = nl; r < n; r++) {
+      const unsigned int w = win[base + dg[r]];
+      bsum += w;
+      csum += w * w;
+    }
+}
+
+// The whole list ks = 2 step + sign at this lane's shift; an inactive lane (act false) sums
+// nothing.
+__device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
+                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
+                                           bool act, int ks, uint32_t wbase, int base, unsigned int& bsum,
+                                           unsigned long long& csum) {
+  const int n = act ? dcnt[ks] : 0, o = act ? dofs[ks] : 0;
+  const int nl = max(0, min(n, e.dl_lds_cap - o));
+  delta_seg(win, dent, e.dlist + dl_off + (int64_t)ks * dcap, o, 0, nl, true, nl, n, wbase, base, bsum, csum);
+}
+
+// Both lists of step k (cells gained, cells lost) at this lane's shift, for a wave whose lanes
+// share the step: their common length is walked side by side, 8 entries of each list per
+// iteration, so that the two lists' LDS latencies overlap; then the rest of each.
+__device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
+                                           const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
+                                           int k, uint32_t wbase, int base, unsigned int& bp,
+                                           unsigned long long& cp, unsigned int& bn, unsigned long long& cn) {
+  const int n0 = dcnt[2 * k], n1 = dcnt[2 * k + 1], o0 = dofs[2 * k], o1 = dofs[2 * k + 1];
+  const int nl0 = max(0, min(n0, e.dl_lds_cap - o0)), nl1 = max(0, min(n1, e.dl_lds_cap - o1));
+  const unsigned char* wb = reinterpret_cast<const unsigned char*>(win) + wbase;
+  auto px = [&](uint32_t off) { return (unsigned int)*reinterpret_cast<const uint16_t*>(wb + off); };
+  const int nc = min(nl0, nl1) & ~7;
+  for (int q = 0; q < nc; q += 8) {
+    const uint4 a0 = *reinterpret_cast<const uint4*>(dent + o0 + q);
+    const uint4 a1 = *reinterpret_cast<const uint4*>(dent + o0 + q + 4);
+    const uint4 b0 = *reinterpret_cast<const uint4*>(dent + o1 + q);
+    const uint4 b1 = *reinterpret_cast<const uint4*>(dent + o1 + q + 4);
+    const unsigned int w0 = px(a0.x), w1 = px(a0.y), w2 = px(a0.z), w3 = px(a0.w);
+    const unsigned int w4 = px(a1.x), w5 = px(a1.y), w6 = px(a1.z), w7 = px(a1.w);
+    const unsigned int v0 = px(b0.x), v1 = px(b0.y), v2 = px(b0.z), v3 = px(b0.w);
+    const unsigned int v4 = px(b1.x), v5 = px(b1.y), v6 = px(b1.z), v7 = px(b1.w);
+    bp += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+    cp += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
+    bn += ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7));
+    cn += (unsigned int)(((v0 * v0 + v1 * v1) + (v2 * v2 + v3 * v3)) + ((v4 * v4 + v5 * v5) + (v6 * v6 + v7 * v7)));
   }
+  delta_seg(win, dent, e.dlist + dl_off + (int64_t)(2 * k) * dcap, o0, nc, nl0 - nc, true, nl0, n0, wbase, base,
+            bp, cp);
+  delta_seg(win, dent, e.dlist + dl_off + (int64_t)(2 * k + 1) * dcap, o1, nc, nl1 - nc, true, nl1, n1, wbase,
+            base, bn, cn);
 }
 
 __device__ __forceinline__ bool better_g(double ncc, int idx, double best, int bidx) {
@@ -838,8 +879,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
             if (k >= nsteps) break;
             unsigned int bp = 0u, bn = 0u;
             unsigned long long cp = 0ull, cn = 0ull;
-            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, true, 2 * k, wbase, base0, bp, cp);
-            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, true, 2 * k + 1, wbase, base0, bn, cn);
+            delta_pair(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, k, wbase, base0, bp, cp, bn, cn);
             if (lane_ok) consider(k, sidx0, abuf[kl * TW + lane], Bm0, Cm0, bp, bn, cp, cn);
           }
         } else {
