@@ -89,3 +89,31 @@ def test_full_size_subpixel_within_tolerance_of_oracle(tracker, full, form):
     dn = np.abs(got["NCC_subpix"] - fx[key + "NCC_subpix"]).max()
     say(f"  max |du| {du:.3g} px, |dv| {dv:.3g} px, |dtheta| {dr:.3g} rad, |dNCC| {dn:.3g}")
     assert du < TOL_PX and dv < TOL_PX and dr < TOL_RAD and dn < TOL_NCC
+
+
+@pytest.mark.parametrize("bits", [12, 8])
+@pytest.mark.parametrize("method", ["plain", "exact"])
+def test_full_size_40_steps_search_equal_oracle(tracker, method, bits):
+    # the 2,000-grain pair searched over +-20 px x 40 rotation steps (num_rot 20 at inc 0.005),
+    # on 12-bit frames and on their 8-bit requantisation
+    from make_cfg3_full_fixture import COLS, params, workload
+
+    fx = np.load(os.path.join(os.path.dirname(FIXTURE), "cfg3_full_oracle.npz"))
+    say(f"full size, 40 steps, {bits}-bit, {method} search")
+    w = workload(bits)
+    assert digest(w) == str(fx[f"digest_{bits}"]), "the generated pair differs from the one the fixture was made from"
+    prm = params(w)
+    prm.search_method = pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN
+    tracker.set_image(0, w.I0, w.bits)
+    tracker.set_image(1, w.I1, w.bits)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
+    tracker.track_pair(prm)
+    got = tracker.get_grains()
+    for c in COLS[:3]:
+        bad = np.flatnonzero(np.ascontiguousarray(got[c]).view(np.uint64) != fx[f"{c}_{bits}"].view(np.uint64))
+        assert bad.size == 0, (c, bad[:8])
+    if method == "plain":
+        assert bits_equal(got["NCC"], fx[f"NCC_{bits}"])
+    else:
+        assert np.abs(got["NCC"] - fx[f"NCC_{bits}"]).max() < 1e-12

@@ -600,3 +600,24 @@ def test_full_size_fixture_is_the_oracle_on_the_generated_pair():
     for c in SUB_COLS:
         assert np.array_equal(np.asarray(gp[c], dtype=np.float64).view(np.uint64),
                               fx["powell_" + c][sample].view(np.uint64)), c
+
+
+@pytest.mark.parametrize("bits", [12, 8])
+def test_full_size_40_step_fixture_is_the_oracle_on_the_generated_pair(bits):
+    import os
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    from make_cfg2_full_fixture import digest
+    from make_cfg3_full_fixture import COLS, params, workload
+
+    fx = np.load(os.path.join(here, "golden", "cfg3_full_oracle.npz"))
+    w = workload(bits)
+    assert digest(w) == str(fx[f"digest_{bits}"])
+    W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
+    sample = np.arange(7, w.G, 250)
+    g, _, _ = ptref.track_pair(W, params(w), nthreads=8, subset=sample)
+    for c in COLS:
+        assert np.array_equal(np.asarray(g[c], dtype=np.float64).view(np.uint64),
+                              fx[f"{c}_{bits}"][sample].view(np.uint64)), c
