@@ -116,6 +116,8 @@ typedef struct ptg_track_params {
 /* ---- context ------------------------------------------------------------------ */
 int ptg_create(void** ctx, int device);
 int ptg_destroy(void* ctx);
+/* The message of the context's last failed call; with ctx == NULL, that of the last
+ * ptg_create that failed on the calling thread (which then released everything it made). */
 const char* ptg_last_error(const void* ctx);
 int ptg_abi_version(void);
 
@@ -126,6 +128,10 @@ int ptg_set_image(void* ctx, int index, const uint16_t* host, int32_t dimx, int3
 /* Attaches a caller-owned device frame (no copy); it must outlive its use. */
 int ptg_set_image_device(void* ctx, int index, const uint16_t* dev, int32_t dimx, int32_t dimy,
                          int64_t pitch, int32_t bits);
+/* Copies image[index] to a host buffer of row pitch `pitch` elements (synchronises): the
+ * counterpart of the reference's direct access to image[i][x][y] (src/p-tracker.hpp:80),
+ * e.g. for the frame ptg_remap wrote. */
+int ptg_get_image(void* ctx, int index, uint16_t* host, int64_t pitch);
 /* im_index_ref / im_index_current, src/p-tracker.hpp:82-83. */
 int ptg_set_image_indices(void* ctx, int im_index_ref, int im_index_current);
 

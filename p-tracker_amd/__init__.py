@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_set_image": ([vp, c_int, vp, c_int32, c_int32, c_int64, c_int32], c_int),
         "ptg_set_image_device": ([vp, c_int, vp, c_int32, c_int32, c_int64, c_int32], c_int),
         "ptg_set_image_indices": ([vp, c_int, c_int], c_int),
+        "ptg_get_image": ([vp, c_int, vp, c_int64], c_int),
         "ptg_set_grains": ([vp, c_int32, vp, vp, vp], c_int),
         "ptg_put_grains": ([vp, vp], c_int),
         "ptg_get_grains": ([vp, vp], c_int),
@@ -163,7 +164,7 @@ def load_library(path: str = LIB_PATH):
 def exported_symbols():
     """The ptg_* names the binding expects; tests compare them with include/ptgpu.h."""
     return ["ptg_abi_version", "ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_image",
-            "ptg_set_image_device", "ptg_set_image_indices", "ptg_set_grains", "ptg_put_grains",
+            "ptg_set_image_device", "ptg_set_image_indices", "ptg_get_image", "ptg_set_grains", "ptg_put_grains",
             "ptg_get_grains", "ptg_begin_pair", "ptg_precompute", "ptg_search", "ptg_subpixel",
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch",
@@ -189,7 +190,8 @@ class Tracker:
         h = c_void_p()
         rc = self.lib.ptg_create(ctypes.byref(h), device)
         if rc != PTG_OK:
-            raise PtgError(f"ptg_create(device={device}) failed with {rc}")
+            msg = self.lib.ptg_last_error(None)
+            raise PtgError(f"ptg_create(device={device}) failed with {rc}: {msg.decode() if msg else ''}")
         self.h = h
         self.G = 0
 
@@ -336,6 +338,11 @@ class Tracker:
         self._ck(self.lib.ptg_calibrate_distortion(self.h, _ptr(p), _ptr(fr), max_iter, ctypes.byref(fi),
                                                    ctypes.byref(fe), ctypes.byref(it)), "ptg_calibrate_distortion")
         return p, fi.value, fe.value, it.value
+
+    def get_image(self, index: int, dimx: int, dimy: int) -> np.ndarray:
+        out = np.zeros((dimy, dimx), dtype=np.uint16)
+        self._ck(self.lib.ptg_get_image(self.h, index, _ptr(out), dimx), "ptg_get_image")
+        return out
 
     def remap(self, src: int, dst: int, params14):
         p = np.ascontiguousarray(params14, dtype=np.float64)

@@ -418,28 +418,50 @@ extern "C" {
 
 int ptg_abi_version(void) { return PTG_ABI_VERSION; }
 
+// The message of the last ptg_create that failed on this thread (no context exists to
+// hold it); ptg_last_error(NULL) returns it.
+static thread_local std::string g_create_err;
+
+static int create_fail(int code, const char* what, hipError_t e) {
+  char buf[256];
+  snprintf(buf, sizeof(buf), "ptg_create: %s: %s", what, e == hipSuccess ? "invalid argument" : hipGetErrorString(e));
+  g_create_err = buf;
+  return code;
+}
+
 int ptg_create(void** out, int device) {
-  if (!out) return PTG_ERR_ARG;
+  if (!out) return create_fail(PTG_ERR_ARG, "null output pointer", hipSuccess);
   *out = nullptr;
+  g_create_err.clear();
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PTG_ERR_HIP;
-  if (device < 0 || device >= n) return PTG_ERR_ARG;
-  if (hipSetDevice(device) != hipSuccess) return PTG_ERR_HIP;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return create_fail(PTG_ERR_HIP, "hipGetDeviceCount", e == hipSuccess ? hipErrorNoDevice : e);
+  if (device < 0 || device >= n) return create_fail(PTG_ERR_ARG, "device index out of range", hipSuccess);
+  if ((e = hipSetDevice(device)) != hipSuccess) return create_fail(PTG_ERR_HIP, "hipSetDevice", e);
   Ctx* c = new Ctx();
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
     delete c;
-    return PTG_ERR_HIP;
+    return create_fail(PTG_ERR_HIP, "hipStreamCreateWithFlags", e);
   }
-  if (hipMalloc((void**)&c->d_counts, sizeof(int32_t) * 4) != hipSuccess ||
-      hipMalloc((void**)&c->d_evals, sizeof(int64_t)) != hipSuccess) {
+  // every failure below releases what was made before it
+  auto undo = [c]() {
+    if (c->d_counts) (void)hipFree(c->d_counts);
+    if (c->d_evals) (void)hipFree(c->d_evals);
     (void)hipStreamDestroy(c->stream);
     delete c;
-    return PTG_ERR_NOMEM;
+  };
+  if ((e = hipMalloc((void**)&c->d_counts, sizeof(int32_t) * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&c->d_evals, sizeof(int64_t))) != hipSuccess) {
+    undo();
+    return create_fail(PTG_ERR_NOMEM, "hipMalloc", e);
   }
   k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_counts, 4, 0);
   k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return PTG_ERR_HIP;
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+    undo();
+    return create_fail(PTG_ERR_HIP, "first kernels on the new stream", e);
+  }
   *out = c;
   return PTG_OK;
 }
@@ -489,7 +511,7 @@ int ptg_destroy(void* p) {
 
 const char* ptg_last_error(const void* p) {
   const Ctx* c = reinterpret_cast<const Ctx*>(p);
-  if (!c) return "null context";
+  if (!c) return g_create_err.empty() ? "null context" : g_create_err.c_str();
   return c->err.c_str();
 }
 
@@ -556,6 +578,20 @@ int ptg_set_image_device(void* p, int index, const uint16_t* dev, int32_t dimx, 
   im.dimy = dimy;
   im.pitch = pitch;
   im.bits = bits;
+  return PTG_OK;
+}
+
+int ptg_get_image(void* p, int index, uint16_t* host, int64_t pitch) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(valid_index(c, index));
+  const Image& im = c->img[index];
+  if (!im.d) return c->fail(PTG_ERR_STATE, "image[%d] not set", index);
+  if (!host) return c->fail(PTG_ERR_ARG, "null host buffer");
+  if (pitch < im.dimx) return c->fail(PTG_ERR_ARG, "pitch %lld < dimx %d", (long long)pitch, im.dimx);
+  CK(c, hipMemcpy2DAsync(host, sizeof(uint16_t) * pitch, im.d, sizeof(uint16_t) * im.pitch, sizeof(uint16_t) * im.dimx,
+                         im.dimy, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
   return PTG_OK;
 }
 

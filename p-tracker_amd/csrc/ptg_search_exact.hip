@@ -111,6 +111,13 @@ __device__ __forceinline__ int win_y0(const ptg_grain& gr, int up, int rho) {
   return gr.refcoord_ypix + (int)round(gr.dy) - up - rho;
 }
 
+// Hands list slot li to the plain kernel (its MFMA search returns at once); without that
+// fallback (the plain kernel's LDS cannot take the list) the stage is void.
+__device__ __forceinline__ void overflow_slot(const ExactArgs& e, int li) {
+  if (!e.ovf_slot) atomicOr(&e.flags[0], 1);
+  else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
+}
+
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
 // LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
 // bytes = count of the base mask (from k_exact_mid or k_exact_base_disk, which run first). Rows beyond the
@@ -195,12 +202,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
       e.dlen[2 * ((int64_t)li * nsteps + k) + 1] = min(cnt[1], sl.dcap);
     }
   }
-  if (overflow) {
-    // this grain goes to the plain kernel (its MFMA search returns at once); without that
-    // fallback the stage is void
-    if (!e.ovf_slot) atomicOr(&e.flags[0], 1);
-    else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
-  }
+  if (overflow) overflow_slot(e, li);
 }
 
 // ---- B_mid, C_mid for every shift and the reference moments -------------------------
@@ -295,9 +297,12 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   const size_t psz = (size_t)WHm * stride;
   const int m = lane & 15, gq = lane >> 4;
   const int ntile = vp >> 4;
+  __shared__ int big;
+  if (threadIdx.x == 0) big = 0;
   for (int p0 = 0; p0 < kMidPlanes; p0 += NPL) {
     // the window's digit planes p0 .. p0 + NPL - 1 for columns su0 .. su0 + stride - 1
     if (p0 > 0) __syncthreads();  // every wave is done with the previous pass's planes
+    bool over = false;
     for (int t = threadIdx.x; t < WHm * (stride / 4); t += blockDim.x) {
       const int y = t / (stride / 4), x4 = (t - y * (stride / 4)) * 4;
       const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
@@ -305,6 +310,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
+        over |= v > 4095u;
         const uint32_t q = v * v;
         const uint32_t dig[kMidPlanes] = {(v >> 7) & 127u, v & 127u, q & 127u, (q >> 7) & 127u, (q >> 14) & 127u,
                                           (q >> 21) & 127u};
@@ -315,7 +321,14 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
       for (int p = 0; p < NPL; p++)
         *reinterpret_cast<uint32_t*>(planes + p * psz + (size_t)y * stride + x4) = w[p0 + p];
     }
+    if (over) big = 1;
     __syncthreads();
+    if (big) {
+      // a window value beyond 12 bits (the frame's declared depth is not checked against
+      // its data): the 7-bit digits would drop bits, so the plain kernel takes this grain
+      if (threadIdx.x == 0) overflow_slot(e, li);
+      return;
+    }
     if (p0 == 0 && blockIdx.y == 0) {
       // the base mask's multiplicities for the template kernel's Delta lists
       for (int t = threadIdx.x; t < side * side; t += blockDim.x) {
@@ -471,10 +484,7 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
   __syncthreads();
   if (big) {
     // beyond 12 bits: this grain goes to the plain kernel (its MFMA search returns at once)
-    if (threadIdx.x == 0) {
-      if (!e.ovf_slot) atomicOr(&e.flags[0], 1);
-      else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
-    }
+    if (threadIdx.x == 0) overflow_slot(e, li);
     return;
   }
   // kSh shifts per thread and pass, (su, sv0 + v) for t = threadIdx.x + j blockDim.x: one
@@ -696,17 +706,28 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   uint32_t* dent = reinterpret_cast<uint32_t*>(dofs + ((nls + 4) & ~3));
   for (int q = threadIdx.x; q < nls; q += nthreads) dcnt[q] = e.dlen[(int64_t)li * nls + q];
 
-  bool clamped = false;
+  bool clamped = false, over = false;
+  __shared__ int big;
+  if (threadIdx.x == 0) big = 0;
+  __syncthreads();
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
     const int y = q / WW, x = q - y * WW;
     int X = X0 + x, Y = Y0 + y;
     if (X < 0 || Y < 0 || X >= e.s.dimx || Y >= e.s.dimy) clamped = true;
     X = clampi(X, 0, e.s.dimx - 1);
     Y = clampi(Y, 0, e.s.dimy - 1);
-    win[q] = e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+    const uint16_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+    over |= v > 4095;
+    win[q] = v;
   }
   if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
+  if (over) big = 1;
   __syncthreads();
+  if (big) {
+    // a window value beyond 12 bits: the plain kernel takes this grain
+    if (threadIdx.x == 0) overflow_slot(e, li);
+    return;
+  }
   if (wave == 0) {
     // exclusive prefix of the list lengths rounded up to 4 entries
     int run = 0;

@@ -78,3 +78,19 @@ def test_search_zone_layout_is_search_zone_type():
     # same members, order and padding as search_zone_type (src/search_zone_type.hpp:20-27)
     assert [f[0] for f in pk.SearchZone._fields_] == ["left", "right", "up", "down", "num_rot", "inc_rot"]
     assert pk.SearchZone.inc_rot.offset == 24 and ctypes.sizeof(pk.SearchZone) == 32
+
+
+def test_create_failure_reports_its_cause():
+    # without a visible device ptg_create fails, releases what it made and leaves the
+    # failing call's message for ptg_last_error(NULL); with a device it succeeds
+    lib = pk.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.ptg_create(ctypes.byref(h), 0)
+    if rc == pk.PTG_OK:
+        lib.ptg_destroy(h)
+        pytest.skip("a device is visible: ptg_create succeeded")
+    assert h.value is None
+    msg = lib.ptg_last_error(None).decode()
+    assert msg.startswith("ptg_create: ") and len(msg) > len("ptg_create: ")
+    assert lib.ptg_create(None, 0) == pk.PTG_ERR_ARG
+    assert lib.ptg_last_error(None).decode() == "ptg_create: null output pointer: invalid argument"

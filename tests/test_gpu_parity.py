@@ -59,9 +59,8 @@ def test_sampler_bit_exact(tracker):
     tracker.set_image(2, f, 12)
     xy = np.stack([rng.uniform(1, 396.99, 20000), rng.uniform(1, 296.99, 20000)], axis=1)
     g = tracker.interp_cubic(2, xy)
+    # the oracle's sampler is pinned to the reference's compiled interpol.cpp by the CPU tests
     assert bits_equal(g, ptref.interp_cubic(f, xy))
-    if ptref.ref_interp_available():
-        assert bits_equal(g, ptref.ref_interp_cubic(f, xy))
 
 
 def test_precompute_bit_exact(tracker, cfg2s):
@@ -243,6 +242,31 @@ def test_undistort_points_and_remap_bit_exact(tracker):
     u = tracker.undistort_points(Xp, pts)
     uo = np.array([ptref.undistor(Xp, x, y) for x, y in pts])
     assert bits_equal(u, uo)
+
+
+def test_remap_config4_lens_full_frame_bit_exact(tracker):
+    # the config-4 lens (k1 = -2e-9, k2 = 1e-16 about the centre) on a whole 6000x4000 12-bit
+    # frame: every pixel of the GPU remap equals the oracle's (itself pinned by the quadratic
+    # closed form and the round trip of the CPU tests)
+    w = synth.workload(2, n_grains=200, seed=1002)
+    H, W = w.I0.shape
+    X = np.array([0.5 * W, 0.5 * H, -2e-9, 1e-16, 0, 0, 0, 0, 0.5 * W, 0.5 * H, 0, 0, 0, 0], dtype=np.float64)
+    say("remap 6000x4000, config-4 lens")
+    tracker.set_image(2, w.I0, 12)
+    tracker.remap(2, 3, X)
+    got = tracker.get_image(3, W, H)
+    ref = ptref.remap(w.I0, 12, X)
+    assert np.array_equal(got, ref)
+    # and the remap undoes the lens: the config-4 frame remapped is the config-2 frame
+    # within the round trip of two bicubic resamplings
+    seen = synth.distort_frame(w.I0, X, 12)
+    tracker.set_image(2, seen, 12)
+    tracker.remap(2, 3, X)
+    back = tracker.get_image(3, W, H).astype(np.float64)
+    err = np.abs(back - w.I0)[50:-50, 50:-50].mean()
+    uncorrected = np.abs(seen.astype(np.float64) - w.I0)[50:-50, 50:-50].mean()
+    say(f"round trip: mean |error| {err:.3f}, uncorrected {uncorrected:.1f} gray levels")
+    assert err < 2.0 and err < uncorrected / 20
 
 
 def test_change_reference_sequence(tracker):
@@ -441,6 +465,48 @@ def test_exact_overflowing_grains_go_to_plain_kernel(tracker):
     assert n1[0] == n0[0]
     assert 0 < n1[1] - n0[1] < S.G
     W = ptref.Workload(S.frames[0], S.frames[1], gr, S.pattern_xy, S.pattern_begin, S.bits)
+    ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+
+
+def test_exact_hands_values_beyond_12_bits_to_plain_kernel(tracker):
+    # frames declared 12-bit whose right half holds 16x larger values (12-bit data stored
+    # shifted up in 16-bit words): the exact kernels find window values above 4095 while
+    # staging and send those grains to the plain kernel (disks through the prefix-sum
+    # base mask, squares through the base-mask GEMM); every pick is the reference's
+    w = synth.workload(2, n_grains=24, size=(900, 700), seed=313)
+    I0, I1 = w.I0.copy(), w.I1.copy()
+    half = I0.shape[1] // 2
+    I0[:, half:] *= 16
+    I1[:, half:] *= 16
+    pats = []
+    for g in range(w.G):
+        P = w.pattern_xy[w.pattern_begin[g]:w.pattern_begin[g + 1]]
+        if g % 2:
+            r = math.isqrt(int((P.astype(np.int64) ** 2).sum(1).max())) // 2
+            ys, xs = np.mgrid[-r:r + 1, -r:r + 1]
+            P = np.stack([xs.ravel(), ys.ravel()], 1)
+        pats.append(P.astype(np.int32))
+    pxy = np.concatenate(pats).astype(np.int32)
+    pb = np.concatenate([[0], np.cumsum([len(p) for p in pats])]).astype(np.int64)
+    right = w.grains["refcoord_xpix"] - 20 - w.truth["radii"] > half
+    left = w.grains["refcoord_xpix"] + 80 < half
+    assert right.any() and left.any()
+    z = w.params.search_zone
+    say(f"exact search, 12-bit frames with 16-bit values in the right half: G={w.G}")
+    tracker.set_image(0, I0, 12)
+    tracker.set_image(1, I1, 12)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(w.grains, pxy, pb)
+    n0 = tracker.exact_fallbacks()
+    tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT))
+    got = tracker.get_grains()
+    n1 = tracker.exact_fallbacks()
+    assert n1[0] == n0[0]
+    assert int(right.sum()) <= n1[1] - n0[1] <= w.G - int(left.sum())
+    W = ptref.Workload(I0, I1, w.grains, pxy, pb, 16)
     ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
     for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
         assert bits_equal(got[col], ref[col]), col
