@@ -379,7 +379,9 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   if (method != PTG_SEARCH_PLAIN) {
     // the unmasked list is the same on every main-stage call until it is rebuilt
     const uint64_t key = stage == PTG_STAGE_MAIN ? c->unmasked_version : 0;
-    const int rc = run_search_exact(*c, a, nlist, list, key, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN);
+    const double thr = stage == PTG_STAGE_MAIN ? NCC_min : NCC_min_super;
+    const int rc =
+        run_search_exact(*c, a, nlist, list, key, thr, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN);
     if (rc == kExactUnfit) {
       method = PTG_SEARCH_PLAIN;
       nparts = nsteps;

@@ -30,6 +30,14 @@ struct StepBest {
   int32_t pad;
 };
 
+// A block's near-tie list of the exact search: its best candidate by s = num^2 / d1 and the
+// next one within the window, index INT32_MAX when none; ovf when a third lay within it.
+struct ExactBest {
+  double s1, s2;
+  int32_t i1, i2;
+  int32_t ovf, pad;
+};
+
 // Arguments of the integer-search kernels (plain and exact forms).
 struct SearchArgs {
   const uint16_t* I0;
@@ -195,11 +203,14 @@ constexpr int kExactUnfit = 1;
 // digit or Delta list overflow) is left in *flag_dev on the device; the caller passes it
 // to the commit and checks it at its next synchronisation.
 // plan_key != 0 names a list that is the same on every call with that key (the unmasked
-// list of one grain set): its host tables are then built once and reused.
+// list of one grain set): its host tables are then built once and reused. stage_thr is the
+// threshold the commit compares the stage's NCC with (NCC_min for the main stage, else
+// NCC_min_super): a grain whose NCC lies within rounding of it is re-scored in the
+// reference's arithmetic.
 // time_kernel: bracket each MFMA kernel launch with the PTG_T_SEARCH_KERNEL timer.
 int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vector<int32_t>& list,
-                     uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
-                     bool time_kernel);
+                     uint64_t plan_key, double stage_thr, StepBest* out_best, int* nparts_out,
+                     const int32_t** flag_dev, bool time_kernel);
 void free_exact_plans(Context& c);
 
 // Equiprojectivity criterion of the distortion calibration (device data, see ptgpu.h).

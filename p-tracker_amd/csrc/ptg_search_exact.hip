@@ -70,7 +70,10 @@ struct ExactArgs {
   int64_t* mid;                // B_mid, C_mid per (slot, shift)
   int64_t* mom;                // per slot: N, Sa, Saa
   int32_t* flags;              // [0]: template or Delta overflow
-  StepBest* out;               // per (slot, shift block), part_cap entries per slot
+  ExactBest* xout;             // per (slot, shift block): the block's near-tie list, part_cap entries per slot
+  StepBest* out;               // per slot: the pick (entry 0 of part_cap)
+  int32_t* cand;               // per slot: the candidates re-scored in the reference's arithmetic (2)
+  double thr;                  // the stage's queueing threshold (NCC_min, or NCC_min_super)
   int32_t part_cap;
   int32_t dcap;                // largest slot dcap of the search
   int32_t nmt;                 // M-tiles (8 steps each)
@@ -95,6 +98,133 @@ struct ExactArgs {
   int32_t base_vb;             // k_exact_base_disk: vertical shifts per workgroup
   int32_t pad_b;
 };
+
+// ---- the near-tie window -------------------------------------------------------------
+// The reference's NCC of a candidate (two-pass FP64 sums in pattern order, unfused,
+// src/p-tracker.cpp:698-716) differs from its exact value rho = num / sqrt(d0 d1), with
+// num = N A - Sa B, d0 = N Saa - Sa^2, d1 = N C - B^2, by at most
+//   tau = 2 (2u (R0 + R1) + 7u + 2 g_N) + 8 u^2 (R0 + sqrt N)(R1 + sqrt N),
+// u = 2^-53, g_N = N u / (1 - N u), R0 = |mean0| / sigma0 = Sa / sqrt(d0) and
+// R1 = |mean1| / sigma1 = B / sqrt(d1): the roundings of the two means (u |mean|), of the
+// differences, products and sequential sums (g_N relative to the sums of absolute terms,
+// which Cauchy-Schwarz bounds by sqrt(C0C0 C1C1) and sqrt(N C) |mean|), and of the final
+// product, square root and division (3u), to first order and doubled, plus a bound of the
+// second-order terms. For windows of integers at most vmax, R1 <= vmax N / sqrt(N - 1)
+// (the least non-zero N C - B^2 is N - 1). q, formed here in FP64 from the exact
+// integers, is within 4u of rho. So the reference orders two candidates whose q differ by
+// more than W = 2 (tau + 4u) as q does, and its pick lies among the candidates with
+// q >= q* - W. The search keeps that set, per grain, and re-scores it in the reference's
+// own arithmetic.
+__host__ __device__ inline double ncc_error_bound(double N, double R0, double R1) {
+  const double u = 1.1102230246251565e-16;  // 2^-53
+  const double gN = N * u / (1.0 - N * u);
+  const double sn = sqrt(N);
+  return 2.0 * (2.0 * u * (R0 + R1) + 7.0 * u + 2.0 * gN) + 8.0 * u * u * (R0 + sn) * (R1 + sn);
+}
+
+// Grains whose best q is below 2 kQmin go to the plain kernel: the window on s = num^2 / d1
+// (= q^2 d0) below is relative and holds for q* >= kQmin.
+constexpr double kQmin = 1.0 / 64.0;
+constexpr int kExactVmax = 4095;  // window values the exact kernels take (larger: plain kernel)
+// Patterns of at most this many pixels have N A, Sa B, N C and B^2 below 2^53 (N^2 4095^2 < 2^53),
+// so the MFMA kernel forms num and d1 exactly in FP64; larger ones (radius above about 85 px)
+// go to the plain kernel.
+constexpr long long kDoubleExactN = 23170;
+
+// The relative window on s = num^2 / d1: a candidate with s < s* lo cannot be the reference's
+// pick when q* >= kQmin, since (q* - W)^2 / q*^2 >= (1 - W / kQmin)^2; 16u more cover the
+// roundings of s (5u each) and of the comparisons.
+__host__ __device__ inline double near_window_q(long long N, long long Sa, long long d0) {
+  const double u = 1.1102230246251565e-16;
+  const double Nd = (double)N;
+  const double R0 = (double)Sa / sqrt((double)d0);
+  const double R1 = N > 1 ? (double)kExactVmax * Nd / sqrt(Nd - 1.0) : 0.0;
+  return 2.0 * (ncc_error_bound(Nd, R0, R1) + 4.0 * u);
+}
+
+__host__ __device__ inline double near_window_lo(long long N, long long Sa, long long d0) {
+  const double u = 1.1102230246251565e-16;
+  const double r = 1.0 - near_window_q(N, Sa, d0) / kQmin;
+  return r * r * (1.0 - 16.0 * u);
+}
+
+// A grain's (or a block's, or a lane's) best candidate by s and, when it lies within the
+// window of the best, the next one; ovf: a third candidate lay within the window of the
+// best of its time. Ranking: larger s first, the earlier candidate (lower index) among
+// equal s. An empty entry has s = -1 and index INT_MAX.
+__device__ __forceinline__ bool rank_gt(double s, int i, double t, int j) { return s > t || (s == t && i < j); }
+
+struct Near2 {
+  double s1, s2;
+  int i1, i2, ovf;
+};
+
+__device__ __forceinline__ Near2 near2_none() { return Near2{-1.0, -1.0, INT_MAX, INT_MAX, 0}; }
+
+// One candidate into a lane's list.
+__device__ __forceinline__ void near2_add(Near2& a, double s, int idx, double lo) {
+  if (rank_gt(s, idx, a.s1, a.i1)) {
+    if (a.s1 >= s * lo) {
+      a.ovf |= a.s2 >= s * lo ? 1 : 0;
+      a.s2 = a.s1;
+      a.i2 = a.i1;
+    } else {
+      a.s2 = -1.0;
+      a.i2 = INT_MAX;
+    }
+    a.s1 = s;
+    a.i1 = idx;
+  } else if (s >= a.s1 * lo) {
+    if (a.s2 >= a.s1 * lo) a.ovf = 1;
+    if (rank_gt(s, idx, a.s2, a.i2)) {
+      a.s2 = s;
+      a.i2 = idx;
+    }
+  }
+}
+
+// near2_add for a candidate with numerator nd and d1 = dd, kept out of line (few candidates
+// per lane reach it); the list goes in and out by value, in registers.
+__device__ __attribute__((noinline)) Near2 near2_insert(Near2 a, double nd, double dd, int idx, double lo) {
+  near2_add(a, nd * nd / dd, idx, lo);
+  return a;
+}
+
+// Two lists into one: the best two of the four entries; a third entry within the window of
+// the merged best, or a list that overflowed with its best within that window, overflows.
+__device__ __forceinline__ Near2 near2_merge(const Near2& a, const Near2& b, double lo) {
+  const double cs[4] = {a.s1, a.s2, b.s1, b.s2};
+  const int ci[4] = {a.i1, a.i2, b.i1, b.i2};
+  int bi = 0;
+#pragma unroll
+  for (int q = 1; q < 4; q++)
+    if (rank_gt(cs[q], ci[q], cs[bi], ci[bi])) bi = q;
+  int si = bi == 0 ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (q != bi && rank_gt(cs[q], ci[q], cs[si], ci[si])) si = q;
+  const double thr = cs[bi] * lo;
+  int near = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) near += (ci[q] != INT_MAX && cs[q] >= thr) ? 1 : 0;
+  Near2 m;
+  m.s1 = cs[bi];
+  m.i1 = ci[bi];
+  m.s2 = cs[si];
+  m.i2 = ci[si];
+  m.ovf = (near > 2 || (a.ovf && a.s1 >= thr) || (b.ovf && b.s1 >= thr)) ? 1 : 0;
+  return m;
+}
+
+__device__ __forceinline__ Near2 near2_shfl_xor(const Near2& a, int m) {
+  Near2 b;
+  b.s1 = __shfl_xor(a.s1, m, 64);
+  b.s2 = __shfl_xor(a.s2, m, 64);
+  b.i1 = __shfl_xor(a.i1, m, 64);
+  b.i2 = __shfl_xor(a.i2, m, 64);
+  b.ovf = __shfl_xor(a.ovf, m, 64);
+  return b;
+}
 
 // Plain-window row length in uint16 elements: U - 1 + cw rounded up to 2 (mod 4), so
 // that a row is an odd number of dwords and lanes reading consecutive rows hit
@@ -135,7 +265,9 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   const int nsteps = e.s.nsteps;
   int32_t* packed = reinterpret_cast<int32_t*>(smem);
   uint8_t* cmid = reinterpret_cast<uint8_t*>(packed + ncell);
+  int32_t* dflag = reinterpret_cast<int32_t*>(cmid + ((ncell + 15) & ~15));
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) packed[t] = 0;
+  if (threadIdx.x == 0) *dflag = 0;
   const bool real = k < nsteps;
   // the base mask's multiplicities (k_exact_mid or k_exact_base_disk)
   if (real)
@@ -143,16 +275,37 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   __syncthreads();
   const int64_t b = e.s.pbegin[g];
   const int n = (int)(e.s.pbegin[g + 1] - b);
+  // differs: this step's rotated offsets differ, in pattern order, from the previous
+  // step's (step 0: always). A step whose list equals the previous one's scores every
+  // shift with the reference's NCC bit for bit as that step did, earlier in the reference's
+  // loop order, so none of its candidates can win (`best_NCC < NCC_test`,
+  // src/p-tracker.cpp:718): the search skips it.
+  int differs = k == 0 ? 1 : 0;
   if (real) {
     const double c = e.s.cs[2 * ((int64_t)g * nsteps + k)], s = e.s.cs[2 * ((int64_t)g * nsteps + k) + 1];
+    double cp = 0.0, sp = 0.0;
+    if (k > 0) {
+      cp = e.s.cs[2 * ((int64_t)g * nsteps + k - 1)];
+      sp = e.s.cs[2 * ((int64_t)g * nsteps + k - 1) + 1];
+    }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const uint32_t w = e.s.pwords[b + i];
+      const int px = (int)(int8_t)(w & 0xffu), py = (int)(int8_t)((w >> 8) & 0xffu);
       int x, y;
-      rotate_px((int)(int8_t)(w & 0xffu), (int)(int8_t)((w >> 8) & 0xffu), c, s, x, y);
+      rotate_px(px, py, c, s, x, y);
       atomicAdd(&packed[(y + rho) * side + (x + rho)], (int)(w >> 16) + (1 << 24));
+      if (k > 0 && !differs) {
+        int xp, yp;
+        rotate_px(px, py, cp, sp, xp, yp);
+        differs = (xp != x || yp != y) ? 1 : 0;
+      }
     }
   }
+  // the block-wide OR through a word after the two grids (__syncthreads_or would reserve
+  // static LDS)
+  if (differs) *dflag = 1;
   __syncthreads();
+  differs = *dflag;
   // fragments: rows 2k (digit 1) and 2k + 1 (digit 0) of M-tile k / 8
   const int mt = k >> 3, m = (2 * k) & 15;
   const int kc = sl.kc;
@@ -178,7 +331,13 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
     uint4 val = make_uint4(w[0], w[1], w[2], w[3]);
     *reinterpret_cast<uint4*>(e.tmpl + sl.tmpl_off + (((int64_t)mt * kc + ch) * 64 + lane) * 16) = val;
   }
-  if (real) {
+  if (real && !differs) {
+    // a repeated step: its Delta lists are marked absent (-1) and the search skips it
+    if (threadIdx.x == 0) {
+      e.dlen[2 * ((int64_t)li * nsteps + k)] = -1;
+      e.dlen[2 * ((int64_t)li * nsteps + k) + 1] = -1;
+    }
+  } else if (real) {
     // Delta lists: cells whose multiplicity differs from the base mask's, split by sign
     __shared__ int cnt[2];
     if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
@@ -657,10 +816,6 @@ __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* w
             base, bn, cn);
 }
 
-__device__ __forceinline__ bool better_g(double ncc, int idx, double best, int bidx) {
-  return ncc > best || (ncc == best && ncc > 0.0 && idx < bidx);
-}
-
 template <int NMT, int NTP>
 __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -695,6 +850,8 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   double* abuf = e.abuf_alias ? reinterpret_cast<double*>(stage)
                               : reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
   if (!e.abuf_alias) tail += (size_t)e.nsu * 8 * (NTP * 16) * 8;
+  int32_t* bflag = reinterpret_cast<int32_t*>(tail);  // the block's hand-over flag
+  tail += 16;
   int32_t* segl = reinterpret_cast<int32_t*>(tail);
   for (int q = threadIdx.x; q < 4 * kc; q += nthreads) segl[q] = e.seg[sl.seg_off + q];
   // Delta lists of this slot in LDS: counts, offsets of the lists in a compact table
@@ -706,10 +863,9 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   uint32_t* dent = reinterpret_cast<uint32_t*>(dofs + ((nls + 4) & ~3));
   for (int q = threadIdx.x; q < nls; q += nthreads) dcnt[q] = e.dlen[(int64_t)li * nls + q];
 
-  bool clamped = false, over = false;
-  __shared__ int big;
-  if (threadIdx.x == 0) big = 0;
+  if (threadIdx.x == 0) *bflag = 0;
   __syncthreads();
+  bool clamped = false, over = false;
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
     const int y = q / WW, x = q - y * WW;
     int X = X0 + x, Y = Y0 + y;
@@ -721,10 +877,11 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     win[q] = v;
   }
   if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
-  if (over) big = 1;
+  // a window value beyond 12 bits, or a pattern beyond FP64's exact integers: the plain
+  // kernel takes this grain (the flag word was cleared before the staging's barrier)
+  if (over || (threadIdx.x == 0 && e.mom[3 * li] > kDoubleExactN)) *bflag = 1;
   __syncthreads();
-  if (big) {
-    // a window value beyond 12 bits: the plain kernel takes this grain
+  if (*bflag) {
     if (threadIdx.x == 0) overflow_slot(e, li);
     return;
   }
@@ -733,7 +890,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     int run = 0;
     for (int q0 = 0; q0 < nls; q0 += 64) {
       const int q = q0 + lane;
-      const int len = q < nls ? ((dcnt[q] + 3) & ~3) : 0;
+      const int len = q < nls ? ((max(dcnt[q], 0) + 3) & ~3) : 0;  // a repeated step has no lists
       int inc = len;
 #pragma unroll
       for (int m = 1; m < 64; m <<= 1) {
@@ -748,19 +905,25 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   __syncthreads();
   // each wave copies whole lists (their actual length, up to the LDS capacity)
   for (int ks = wave; ks < nls; ks += nthreads >> 6) {
-    const int n = min(dcnt[ks], e.dl_lds_cap - dofs[ks]), o = dofs[ks];
+    const int n = min(max(dcnt[ks], 0), e.dl_lds_cap - dofs[ks]), o = dofs[ks];
     const int32_t* src = e.dlist + sl.dl_off + (int64_t)ks * sl.dcap;
     for (int q = lane; q < n; q += 64) dent[o + q] = 2u * (uint32_t)src[q];
   }
   __syncthreads();
   const int gg = lane >> 4, col = lane & 15;
-  double best = 0.0;
-  int bidx = INT_MAX;
+  // repeated steps (no Delta lists) as a bit mask, the same in every lane
+  uint64_t rep = 0;
+  for (int k = 0; k < nsteps; k++)
+    if (dcnt[2 * k] < 0) rep |= 1ull << k;
   const int64_t* mom = e.mom + 3 * li;
   const long long N = mom[0], Sa = mom[1], Saa = mom[2];
-  const double d0 = (double)(N * Saa - Sa * Sa);
+  const long long d0i = N * Saa - Sa * Sa;
+  // the relative near-tie window (none for a flat pattern, whose reference NCC is NaN at
+  // every candidate: the list stays empty)
+  const double lo = d0i > 0 ? near_window_lo(N, Sa, d0i) : 2.0;
+  Near2 nl = near2_none();  // this lane's list
+  double s1lo = -1.0;       // its best's s times lo: candidates below are rejected at once
   const double Nd = (double)N, Sad = (double)Sa;
-  double bnum = 0.0, bd1 = 1.0;  // this lane's best candidate: numerator and N C - B^2
   const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
   const int32_t* segs = segl;
   // this workgroup's shift columns [c_lo, c_hi); wave w takes c_lo + w, c_lo + w + nsu, ...
@@ -849,12 +1012,12 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
       const int vpass = min(TW, V - p0 * 16);  // vertical shifts of this pass
       const int svl0 = p0 * 16 + lane;  // the uniform form's shift of this lane
       const bool lane_ok = kUniform && lane < TW && svl0 < V;
-      double Bm0 = 0.0, Cm0 = 0.0;
+      long long Bm0 = 0, Cm0 = 0;
       int base0 = 0, sidx0 = 0;
       if (lane_ok) {
         sidx0 = su * V + svl0;
-        Bm0 = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx0];
-        Cm0 = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx0 + 1];
+        Bm0 = e.mid[sl.mid_off + 2 * (int64_t)sidx0];
+        Cm0 = e.mid[sl.mid_off + 2 * (int64_t)sidx0 + 1];
         base0 = svl0 * WW + su;
       }
 #pragma unroll
@@ -873,24 +1036,21 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // one candidate (step k, shift sidx) with its Delta sums
-        auto consider = [&](int k, int sidx, double A, double Bm, double Cm, unsigned int bp, unsigned int bn,
+        // one candidate (step k, shift sidx) with its Delta sums: num = N A - Sa B and
+        // d1 = N C - B^2 exactly (every term an integer below 2^53, patterns of at most
+        // kDoubleExactN pixels), then s = num^2 / d1 = q^2 d0 in FP64 (5 roundings). Only
+        // num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680); d1 = 0 is a flat
+        // window, whose reference NCC is NaN. A candidate below the window of the lane's best
+        // is dropped here; the rest (a few per lane) go through the out-of-line insertion
+        auto consider = [&](int k, int sidx, double A, long long Bm, long long Cm, unsigned int bp, unsigned int bn,
                             unsigned long long cp, unsigned long long cn) {
-          // every quantity is an integer below 2^53, so these double operations are exact
-          const double B = Bm + ((double)bp - (double)bn);
-          const double C = Cm + ((double)cp - (double)cn);
-          const double num = Nd * A - Sad * B;
-          const double d1 = Nd * C - B * B;
-          const int gidx = k * S + sidx;
-          // rank by NCC |NCC| * d0 = num |num| / d1 by cross-multiplication (no divide);
-          // only num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680)
-          if (num > 0.0 && d1 > 0.0) {
-            const double lhs = num * num * bd1, rhs = bnum * bnum * d1;
-            if (bidx == INT_MAX || lhs > rhs || (lhs == rhs && gidx < bidx)) {
-              bnum = num;
-              bd1 = d1;
-              bidx = gidx;
-            }
+          const double B = (double)Bm + ((double)bp - (double)bn);
+          const double C = (double)Cm + ((double)cp - (double)cn);
+          const double nd = Nd * A - Sad * B;
+          const double dd = Nd * C - B * B;
+          if (nd > 0.0 && dd > 0.0 && nd * nd >= s1lo * dd) {
+            nl = near2_insert(nl, nd, dd, k * S + sidx, lo);
+            s1lo = nl.s1 * lo;
           }
         };
         if constexpr (kUniform) {
@@ -898,6 +1058,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
           for (int kl = 0; kl < 8; kl++) {
             const int k = 8 * a + kl;
             if (k >= nsteps) break;
+            if ((rep >> k) & 1) continue;  // a repeated step
             unsigned int bp = 0u, bn = 0u;
             unsigned long long cp = 0ull, cn = 0ull;
             delta_pair(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, k, wbase, base0, bp, cp, bn, cn);
@@ -908,14 +1069,14 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
           const int nitem = nk > 0 ? nk * vpass : 0;
           for (int it0 = 0; it0 < nitem; it0 += 64) {
             const int it = it0 + lane;
-            const bool act = it < nitem;
-            const int kl = act ? it / vpass : 0, vl = act ? it - kl * vpass : 0;
+            const int kl = it < nitem ? it / vpass : 0, vl = it < nitem ? it - kl * vpass : 0;
             const int k = 8 * a + kl, svl = p0 * 16 + vl;
+            const bool act = it < nitem && !((rep >> k) & 1);  // a repeated step is skipped
             const int sidx = su * V + svl, base = svl * WW + su;
-            double Bm = 0.0, Cm = 0.0;
+            long long Bm = 0, Cm = 0;
             if (act) {
-              Bm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx];
-              Cm = (double)e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
+              Bm = e.mid[sl.mid_off + 2 * (int64_t)sidx];
+              Cm = e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
             }
             const uint32_t wbase = 2u * (uint32_t)base;
             unsigned int bp = 0u, bn = 0u;
@@ -932,35 +1093,58 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     }
     }
   }
-  // the lane's best as the reference's quantity, NCC = num / sqrt(d0 (N C - B^2))
-  if (bidx != INT_MAX) best = bnum / sqrt(d0 * bd1);
+  // the block's list: its best (lanes, then waves, as (s, index) pairs), then the entries
+  // within the window of that best other than itself, gathered in LDS (they are few: none
+  // for most blocks); more than one such entry, or a lane that overflowed with its best in the
+  // window, overflows the block. The LDS is the window's, once every wave is done with it
+  // (static arrays would add to the LDS every workgroup reserves).
+  double bs = nl.s1;
+  int bix = nl.i1;
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
-    const double ob = __shfl_xor(best, m, 64);
-    const int oi = __shfl_xor(bidx, m, 64);
-    if (better_g(ob, oi, best, bidx)) {
-      best = ob;
-      bidx = oi;
+    const double os = __shfl_xor(bs, m, 64);
+    const int oi = __shfl_xor(bix, m, 64);
+    if (rank_gt(os, oi, bs, bix)) {
+      bs = os;
+      bix = oi;
     }
   }
-  __shared__ double wb[8];
-  __shared__ int wi[8];
+  double* red_s = reinterpret_cast<double*>(smem);  // 8 waves' best s, then the near entry
+  int32_t* red_i = reinterpret_cast<int32_t*>(red_s + 10);  // their indices, the near count, ovf
+  __syncthreads();
   if (lane == 0) {
-    wb[wave] = best;
-    wi[wave] = bidx;
+    red_s[wave] = bs;
+    red_i[wave] = bix;
   }
+  if (threadIdx.x == 0) red_i[9] = red_i[10] = 0;
+  __syncthreads();
+  for (int w = 0; w < (nthreads >> 6); w++)
+    if (rank_gt(red_s[w], red_i[w], bs, bix)) {
+      bs = red_s[w];
+      bix = red_i[w];
+    }
+  const double thr = bs * lo;
+  const bool n1 = nl.i1 != INT_MAX && nl.i1 != bix && nl.s1 >= thr;
+  const bool n2 = nl.i2 != INT_MAX && nl.i2 != bix && nl.s2 >= thr;
+  if (n1 || n2) {
+    const int at = atomicAdd(&red_i[9], (n1 ? 1 : 0) + (n2 ? 1 : 0));
+    if (at == 0) {
+      red_s[8] = n1 ? nl.s1 : nl.s2;
+      red_i[8] = n1 ? nl.i1 : nl.i2;
+    }
+  }
+  if (nl.ovf && nl.s1 >= thr) red_i[10] = 1;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (nthreads >> 6); w++)
-      if (better_g(wb[w], wi[w], best, bidx)) {
-        best = wb[w];
-        bidx = wi[w];
-      }
-    StepBest sb;
-    sb.ncc = best;
-    sb.idx = bidx;
-    sb.pad = 0;
-    e.out[(int64_t)li * e.part_cap + blk] = sb;
+    const int nn = red_i[9];
+    ExactBest xb;
+    xb.s1 = bs;
+    xb.i1 = bix;
+    xb.s2 = nn == 1 ? red_s[8] : -1.0;
+    xb.i2 = nn == 1 ? red_i[8] : INT_MAX;
+    xb.ovf = (red_i[10] || nn > 1) ? 1 : 0;
+    xb.pad = 0;
+    e.xout[(int64_t)li * e.part_cap + blk] = xb;
   }
 }
 
@@ -972,6 +1156,198 @@ __global__ void k_fill_best(StepBest* p, int n) {
     sb.idx = INT_MAX;
     sb.pad = 0;
     p[i] = sb;
+  }
+}
+
+__global__ void k_fill_exact_best(ExactBest* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    ExactBest xb;
+    xb.s1 = xb.s2 = -1.0;
+    xb.i1 = xb.i2 = INT_MAX;
+    xb.ovf = xb.pad = 0;
+    p[i] = xb;
+  }
+}
+
+// ---- the grain's near-tie set and its re-scoring --------------------------------------
+// One wave per list slot: the blocks' lists merged into the grain's. A grain whose set could
+// hold more than two candidates (ovf), whose best q is below 2 kQmin, or which has no
+// candidate with num > 0 at all (the reference could then pick a candidate whose exact NCC
+// is 0 or below, by rounding) goes to the plain kernel; a flat pattern has none and keeps
+// none (the reference's NCC is NaN everywhere). A set of one candidate is the reference's
+// pick: its NCC is taken as q = sqrt(s / d0) (within a few ulps of the reference's) unless q
+// lies within the window of the stage's threshold, where the reference's last bits decide
+// the queueing. Such grains and sets of two go to cand[2 li], cand[2 li + 1] (INT_MAX:
+// none) for re-scoring.
+constexpr int kMergeWaves = 4;
+
+__global__ __launch_bounds__(64 * kMergeWaves) void k_exact_merge(ExactArgs e) {
+  const int lane = threadIdx.x & 63;
+  const int bi = blockIdx.x * kMergeWaves + (threadIdx.x >> 6);
+  if (bi >= e.nbucket) return;
+  const int li = e.bucket[bi];
+  if (li >= *e.s.count) return;
+  if (lane == 0) e.cand[2 * li] = e.cand[2 * li + 1] = INT_MAX;
+  if (e.ovf_slot && e.ovf_slot[li]) return;  // already the plain kernel's
+  const int64_t* mom = e.mom + 3 * li;
+  const long long N = mom[0], Sa = mom[1], Saa = mom[2];
+  const long long d0 = N * Saa - Sa * Sa;
+  if (d0 <= 0) return;
+  const double lo = near_window_lo(N, Sa, d0);
+  Near2 m = near2_none();
+  const ExactBest* xb = e.xout + (int64_t)li * e.part_cap;
+  for (int b = lane; b < e.part_cap; b += 64) {
+    const ExactBest x = xb[b];
+    m = near2_merge(m, Near2{x.s1, x.s2, x.i1, x.i2, x.ovf}, lo);
+  }
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) m = near2_merge(m, near2_shfl_xor(m, k), lo);
+  if (lane != 0) return;
+  const double q = m.i1 == INT_MAX ? 0.0 : sqrt(m.s1 / (double)d0);
+  if (m.ovf || !(q >= 2.0 * kQmin)) {
+    overflow_slot(e, li);
+    return;
+  }
+  const bool two = m.i2 != INT_MAX && m.s2 >= m.s1 * lo;
+  if (!two && fabs(q - e.thr) > near_window_q(N, Sa, d0)) {
+    StepBest sb;
+    sb.ncc = q;
+    sb.idx = m.i1;
+    sb.pad = 0;
+    e.out[(int64_t)li * e.part_cap] = sb;
+    return;
+  }
+  e.cand[2 * li] = m.i1;
+  if (two) e.cand[2 * li + 1] = m.i2;
+}
+
+// One workgroup per list slot of a radius bucket: each candidate of the grain's near-tie set
+// scored with the reference's arithmetic (src/p-tracker.cpp:698-716; the plain kernel's):
+// the integer sum of I1 over the rotated pattern, one division for the mean; then, chunk by
+// chunk, every lane forms products (I0 - mean0)(I1 - m1) and squares (I1 - m1)^2, each
+// rounded as the reference rounds it, and one lane per sum adds them in pattern order,
+// unfused. The larger NCC wins, the earlier candidate among equal values. The result is the
+// slot's only partial, so the commit sees the reference's pick and NCC.
+constexpr int kRescoreThreads = 256;
+constexpr int kRescoreChunk = 1024;
+
+__global__ __launch_bounds__(kRescoreThreads) void k_exact_rescore(ExactArgs e, StepBest* out) {
+  __shared__ __align__(16) double terms[4][kRescoreChunk];  // (candidate, C0C1 | C1C1) x chunk
+  __shared__ unsigned long long part[2][kRescoreThreads / 64];
+  __shared__ double sums[4];
+  if ((int)blockIdx.x >= e.nbucket) return;
+  const int li = e.bucket[blockIdx.x];
+  if (li >= *e.s.count) return;
+  if (e.ovf_slot && e.ovf_slot[li]) return;
+  const int ca[2] = {e.cand[2 * li], e.cand[2 * li + 1]};
+  if (ca[0] == INT_MAX) return;
+  const int nc = ca[1] == INT_MAX ? 1 : 2;
+  const int g = e.s.list[li];
+  const ptg_grain& gr = e.s.grains[g];
+  const int64_t b = e.s.pbegin[g];
+  const int n = (int)(e.s.pbegin[g + 1] - b);
+  const int nsteps = e.s.nsteps;
+  const int V = e.s.up + e.s.down + 1, S = (e.s.left + e.s.right + 1) * V;
+  const int gdx = (int)round(gr.dx), gdy = (int)round(gr.dy);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int bx[2], by[2];
+  double c[2], sn[2];
+  for (int q = 0; q < 2; q++) {
+    const int idx = q < nc ? ca[q] : ca[0];
+    const int k = idx / S, rem = idx - k * S;
+    bx[q] = gr.refcoord_xpix + gdx + rem / V - e.s.left;
+    by[q] = gr.refcoord_ypix + gdy + rem % V - e.s.up;
+    c[q] = e.s.cs[2 * ((int64_t)g * nsteps + k)];
+    sn[q] = e.s.cs[2 * ((int64_t)g * nsteps + k) + 1];
+  }
+  auto pixel = [&](int q, int i) -> unsigned int {
+    int x, y;
+    rotate_px(e.s.pattern[2 * (b + i)], e.s.pattern[2 * (b + i) + 1], c[q], sn[q], x, y);
+    const int X = clampi(bx[q] + x, 0, e.s.dimx - 1), Y = clampi(by[q] + y, 0, e.s.dimy - 1);
+    return e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+  };
+  // mean1 (:698-705): the sum of integers is exact in any order, then one division
+  unsigned long long s1[2] = {0ull, 0ull};
+  for (int i = threadIdx.x; i < n; i += kRescoreThreads)
+    for (int q = 0; q < nc; q++) s1[q] += pixel(q, i);
+  for (int q = 0; q < nc; q++) {
+    const unsigned long long t = (unsigned long long)wave_sum_i64((long long)s1[q]);
+    if (lane == 0) part[q][wave] = t;
+  }
+  __syncthreads();
+  double m1[2] = {0.0, 0.0};
+  for (int q = 0; q < nc; q++) {
+    unsigned long long t = 0ull;
+    for (int w = 0; w < kRescoreThreads / 64; w++) t += part[q][w];
+    m1[q] = (double)t / (double)n;
+  }
+  // C0C1 and C1C1 (:706-714): lane 2q + j of wave 0 adds the terms of sum j of candidate q
+  const double* d0 = e.s.d0 + b;
+  const int adder = threadIdx.x < 2 * nc ? threadIdx.x : -1;
+  double acc = 0.0;
+  for (int i0 = 0; i0 < n; i0 += kRescoreChunk) {
+    const int m = min(kRescoreChunk, n - i0);
+    __syncthreads();  // the previous chunk is summed
+    for (int t = threadIdx.x; t < m; t += kRescoreThreads) {
+      const double dd = d0[i0 + t];
+      for (int q = 0; q < nc; q++) {
+        const double diffC1 = (double)pixel(q, i0 + t) - m1[q];
+        terms[2 * q][t] = dd * diffC1;
+        terms[2 * q + 1][t] = diffC1 * diffC1;
+      }
+    }
+    __syncthreads();
+    if (adder >= 0) {
+      // the next 8 terms are read while the current 8 are added, so the chain of dependent
+      // additions does not wait on LDS
+      const double2* tp = reinterpret_cast<const double2*>(terms[adder]);
+      const int m8 = m & ~7;
+      double2 a0, a1, a2, a3;
+      if (m8 > 0) {
+        a0 = tp[0];
+        a1 = tp[1];
+        a2 = tp[2];
+        a3 = tp[3];
+      }
+      for (int t = 0; t < m8; t += 8) {
+        const int nx = t + 8 < m8 ? (t + 8) / 2 : t / 2;
+        const double2 b0 = tp[nx], b1 = tp[nx + 1], b2 = tp[nx + 2], b3 = tp[nx + 3];
+        acc = acc + a0.x;
+        acc = acc + a0.y;
+        acc = acc + a1.x;
+        acc = acc + a1.y;
+        acc = acc + a2.x;
+        acc = acc + a2.y;
+        acc = acc + a3.x;
+        acc = acc + a3.y;
+        a0 = b0;
+        a1 = b1;
+        a2 = b2;
+        a3 = b3;
+      }
+      for (int t = m8; t < m; t++) acc = acc + terms[adder][t];
+    }
+  }
+  if (adder >= 0) sums[adder] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // src/p-tracker.cpp:716-718: NCC = C0C1 / sqrt(C0C0 C1C1); strictly larger wins, from
+    // best_NCC = 0; the earlier among equals
+    double best = 0.0;
+    int bidx = INT_MAX;
+    for (int q = 0; q < nc; q++) {
+      const double v = sums[2 * q] / sqrt(gr.C0C0 * sums[2 * q + 1]);
+      if (v > best || (v == best && v > 0.0 && ca[q] < bidx)) {
+        best = v;
+        bidx = ca[q];
+      }
+    }
+    StepBest sb;
+    sb.ncc = best;
+    sb.idx = bidx;
+    sb.pad = 0;
+    out[(int64_t)li * e.part_cap] = sb;
   }
 }
 
@@ -992,22 +1368,31 @@ static hipError_t launch_main(const ExactArgs& e, dim3 grid, dim3 block, size_t 
   return hipGetLastError();
 }
 
-// Vector registers of a k_search_exact instance (from its code object), for occupancy.
-static int main_vgprs(int nmt, int ntp) {
+// Vector registers and static LDS bytes of a k_search_exact instance (from its code object),
+// for occupancy.
+struct MainAttr {
+  int vgprs = 256;
+  size_t static_lds = 1024;
+};
+static MainAttr main_attr(int nmt, int ntp) {
   const void* fn = nullptr;
 #define PTG_FN(M, T) \
   if (nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T>;
   PTG_FN(1, 3) PTG_FN(2, 3) PTG_FN(3, 3) PTG_FN(4, 2) PTG_FN(5, 2) PTG_FN(6, 2)
   PTG_FN(1, 2) PTG_FN(2, 2) PTG_FN(3, 2) PTG_FN(1, 1) PTG_FN(2, 1) PTG_FN(3, 1) PTG_FN(4, 1) PTG_FN(5, 1) PTG_FN(6, 1)
 #undef PTG_FN
-  static int cache[8][4] = {};
-  if (!fn) return 256;
-  int& v = cache[nmt][ntp];
-  if (v == 0) {
+  static MainAttr cache[8][4] = {};
+  static bool known[8][4] = {};
+  if (!fn) return MainAttr{};
+  if (!known[nmt][ntp]) {
     hipFuncAttributes at;
-    v = hipFuncGetAttributes(&at, fn) == hipSuccess && at.numRegs > 0 ? at.numRegs : 256;
+    if (hipFuncGetAttributes(&at, fn) == hipSuccess && at.numRegs > 0) {
+      cache[nmt][ntp].vgprs = at.numRegs;
+      cache[nmt][ntp].static_lds = at.sharedSizeBytes;
+    }
+    known[nmt][ntp] = true;
   }
-  return v;
+  return cache[nmt][ntp];
 }
 
 static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
@@ -1057,7 +1442,7 @@ struct ExactPlan {
   std::vector<int32_t> bucket_cat;
   size_t b_mseg = 0, b_mmask = 0;
   size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
-  size_t b_ovf = 0;
+  size_t b_ovf = 0, b_xout = 0, b_cand = 0;
   size_t total = 0;
   void* dev_base = nullptr;  // scratch the tables were last uploaded to
 };
@@ -1190,14 +1575,16 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   P.b_mmask = al((size_t)mmask_total);
   P.b_flags = al(sizeof(int32_t) * 4);
   P.b_ovf = al(sizeof(int32_t) * 2 * (size_t)nlist);  // per-slot overflow flags, then the overflow list
+  P.b_xout = al(sizeof(ExactBest) * (size_t)nlist * std::max(U, nsteps));  // the blocks' near-tie lists
+  P.b_cand = al(sizeof(int32_t) * 2 * (size_t)nlist);                   // the grains' sets to re-score
   P.total = P.b_mmask + P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom +
-            P.b_flags + P.b_ovf;
+            P.b_flags + P.b_ovf + P.b_xout + P.b_cand;
   P.dev_base = nullptr;
 }
 
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
-                     uint64_t plan_key, StepBest* out_best, int* nparts_out, const int32_t** flag_dev,
-                     bool time_kernel) {
+                     uint64_t plan_key, double stage_thr, StepBest* out_best, int* nparts_out,
+                     const int32_t** flag_dev, bool time_kernel) {
   // frames of at most 12 bits (7-bit digits of I1 and I1^2), at most 48 rotation steps
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
   if (i0.bits > 12 || i1.bits > 12) return kExactUnfit;
@@ -1267,8 +1654,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
         l.lds_d = base_disk_lds_bytes(rho, U, vb);
       }
     }
-    // templates and Delta lists: the step's packed template and the base mask's grid
-    l.lds_t = (size_t)side * side * 5;
+    // templates and Delta lists: the step's packed template, the base mask's grid and a flag word
+    l.lds_t = (size_t)side * side * 5 + 32;
     // the MFMA search
     l.ww = win_width(U, cw);
     l.wh = P->vp + 2 * rho;
@@ -1283,20 +1670,20 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     l.dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
     const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + l.dl_cap + 4) * 4;
     auto need = [&](int w) {
-      return win_bytes + w * (stage_bytes + (l.alias ? 0 : abuf_bytes)) + seg_bytes + dl_bytes;
+      return win_bytes + w * (stage_bytes + (l.alias ? 0 : abuf_bytes)) + 16 + seg_bytes + dl_bytes;
     };
     // shift columns split over nsub workgroups when the bucket has few grains
     l.nsub = std::max(1, std::min(U, (512 + nb - 1) / nb));
     const int cpb = (U + l.nsub - 1) / l.nsub;
     // waves per workgroup: the most waves resident per CU under the register limit (a
     // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
-    const int vgpr = main_vgprs(nmt, ntp);
-    const int wave_cap = 4 * std::max(1, 512 / std::max(vgpr, 1));
+    const MainAttr ma = main_attr(nmt, ntp);
+    const int wave_cap = 4 * std::max(1, 512 / std::max(ma.vgprs, 1));
     int best_res = 0;
     l.nsu = 1;
     for (int w = 1; w <= std::min(8, cpb); w++) {
       if (need(w) > kLdsDynMax) break;
-      const int res = std::min(wave_cap, (int)(160 * 1024 / need(w)) * w);  // workgroups per CU by LDS
+      const int res = std::min(wave_cap, (int)(160 * 1024 / (need(w) + ma.static_lds)) * w);  // workgroups per CU by LDS
       if (res >= best_res) {
         best_res = res;
         l.nsu = w;
@@ -1336,6 +1723,10 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   base += P->b_flags;
   e.ovf_slot = (int32_t*)base;
   e.ovf_list = e.ovf_slot + nlist;
+  base += P->b_ovf;
+  e.xout = (ExactBest*)base;
+  base += P->b_xout;
+  e.cand = (int32_t*)base;
   // grains whose lists overflow are searched by the plain kernel when its LDS takes the
   // list's largest window and pattern (else an overflow voids the stage)
   const size_t lds_plain = ((size_t)sa.win_cap * 2 + 15) / 16 * 16 + (size_t)sa.off_cap * 4;
@@ -1377,11 +1768,14 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // plain kernel writes step k of an overflowed slot to out_best[li * part_cap + k]; the
   // rest stay "none"
   const int part_cap = std::max(U, nsteps);
-  e.out = out_best;
   e.part_cap = part_cap;
+  e.out = out_best;
+  e.thr = stage_thr;
   *nparts_out = part_cap;
   k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
   if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
+  k_fill_exact_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(e.xout, nlist * part_cap);
+  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_exact_best");
   // the template and base-mask kernels of every bucket run in order on the second
   // stream; each bucket's MFMA kernel runs after its bucket's event, on the library stream
   // and the search streams in turn, so the next bucket's preparation overlaps the current
@@ -1476,6 +1870,17 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     if ((he = hipEventRecord(c.search_joins[q - 1], lanes[q])) != hipSuccess ||
         (he = hipStreamWaitEvent(c.stream, c.search_joins[q - 1], 0)) != hipSuccess)
       return c.check(he, "exact search join");
+  }
+  // every grain's near-tie set, re-scored in the reference's arithmetic into out_best (the
+  // grains the merge hands over join the plain kernel's list)
+  {
+    ExactArgs ea = e;
+    ea.bucket = bucket_dev;  // the buckets' slot lists, concatenated
+    ea.nbucket = nlist;
+    k_exact_merge<<<(nlist + kMergeWaves - 1) / kMergeWaves, 64 * kMergeWaves, 0, c.stream>>>(ea);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_merge");
+    k_exact_rescore<<<nlist, kRescoreThreads, 0, c.stream>>>(ea, out_best);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_rescore");
   }
   if (e.ovf_slot) {
     if ((he = launch_search_plain_sub(c, sa, nlist, e.ovf_list, e.flags + 1, part_cap, lds_plain)) != hipSuccess)
