@@ -183,6 +183,17 @@ int ptg_ncc_objective(void* ctx, int32_t igrain, const double* X, double subpix_
                       double overflow_stiffness, double* f);
 int ptg_ncc_objective_batch(void* ctx, int32_t n, const int32_t* igrain, const double* X,
                             double subpix_displacement_max, double overflow_stiffness, double* f);
+/* ptg_ncc_objective_batch without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (host optimizers calling the objective in a
+ * tight loop, code under HIP graph capture or in serving loops). The grain indices, points
+ * and values are staged in a library-owned static device array of 576 KiB per GPU: at most
+ * 16384 evaluations per call (PTG_ERR_LIMIT above). The reference moments must have been
+ * computed for the current frames (PTG_ERR_STATE otherwise: call ptg_precompute first).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (several
+ * contexts or threads at once) race. */
+int ptg_ncc_objective_batch_noalloc(void* ctx, int32_t n, const int32_t* igrain, const double* X,
+                                    double subpix_displacement_max, double overflow_stiffness,
+                                    double* f);
 /* Gradient of the same objective with respect to X (analytic, through the sampler). */
 int ptg_ncc_objective_grad_batch(void* ctx, int32_t n, const int32_t* igrain, const double* X,
                                  double* f, double* grad);

@@ -139,6 +139,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_sync": ([vp], c_int),
         "ptg_ncc_objective": ([vp, c_int32, vp, c_double, c_double, POINTER(c_double)], c_int),
         "ptg_ncc_objective_batch": ([vp, c_int32, vp, vp, c_double, c_double, vp], c_int),
+        "ptg_ncc_objective_batch_noalloc": ([vp, c_int32, vp, vp, c_double, c_double, vp], c_int),
         "ptg_ncc_objective_grad_batch": ([vp, c_int32, vp, vp, vp, vp], c_int),
         "ptg_interp_cubic": ([vp, c_int, c_int32, vp, vp], c_int),
         "ptg_undistort_points": ([vp, vp, c_int32, vp, vp], c_int),
@@ -167,7 +168,7 @@ def exported_symbols():
             "ptg_set_image_device", "ptg_set_image_indices", "ptg_get_image", "ptg_set_grains", "ptg_put_grains",
             "ptg_get_grains", "ptg_begin_pair", "ptg_precompute", "ptg_search", "ptg_subpixel",
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
-            "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch",
+            "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
             "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
@@ -281,6 +282,16 @@ class Tracker:
         f = np.zeros(len(ig), dtype=np.float64)
         self._ck(self.lib.ptg_ncc_objective_batch(self.h, len(ig), _ptr(ig), _ptr(Xa), dmax, stiffness, _ptr(f)),
                  "ptg_ncc_objective_batch")
+        return f
+
+    def objective_noalloc(self, igrain, X, dmax=3.0, stiffness=0.0):
+        """As objective(), staged in the library's static device array (at most 16384 points;
+        the reference moments must be computed: precompute())."""
+        ig = np.ascontiguousarray(np.atleast_1d(igrain), dtype=np.int32)
+        Xa = np.ascontiguousarray(np.asarray(X, dtype=np.float64).reshape(-1, 3))
+        f = np.zeros(len(ig), dtype=np.float64)
+        self._ck(self.lib.ptg_ncc_objective_batch_noalloc(self.h, len(ig), _ptr(ig), _ptr(Xa), dmax, stiffness,
+                                                          _ptr(f)), "ptg_ncc_objective_batch_noalloc")
         return f
 
     def objective_grad(self, igrain, X):

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ptg_internal.hpp"
@@ -95,7 +96,37 @@ __global__ void k_fill_i64(long long* p, int n, long long v) {
 
 }  // namespace ptg
 
+// ---- staging of the allocation-free objective entry point ---------------------------------
+// ptg_ncc_objective_batch_noalloc stages its grain indices, points and values in this
+// file-scope array instead of memory the context allocates: at most kNoAllocMaxEvals
+// evaluations per call. Each device holds its own copy until the process ends; host code
+// uses only the per-device address that hipGetSymbolAddress returns (the array's name is a
+// host-side stand-in, not a device address).
+constexpr int32_t kNoAllocMaxEvals = 16384;
+constexpr size_t kNoAllocIgBytes = sizeof(int32_t) * kNoAllocMaxEvals;  // a multiple of 256
+constexpr size_t kNoAllocXBytes = sizeof(double) * 3 * kNoAllocMaxEvals;
+constexpr size_t kNoAllocFBytes = sizeof(double) * kNoAllocMaxEvals;
+static __device__ __align__(256) unsigned char scratch[kNoAllocIgBytes + kNoAllocXBytes + kNoAllocFBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on `device` (current), looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 struct PinnedBuf {
   void* p = nullptr;
@@ -463,6 +494,11 @@ int ptg_create(void** out, int device) {
   if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     undo();
     return create_fail(PTG_ERR_HIP, "first kernels on the new stream", e);
+  }
+  // the allocation-free entry point's staging array on this device (device current)
+  if ((e = scratch_address(device, &c->d_noalloc)) != hipSuccess) {
+    undo();
+    return create_fail(PTG_ERR_HIP, "hipGetSymbolAddress(scratch)", e);
   }
   *out = c;
   return PTG_OK;
@@ -945,6 +981,34 @@ int ptg_ncc_objective_batch(void* p, int32_t n, const int32_t* igrain, const dou
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   return objective_common(c, n, igrain, X, dmax, stiff, f, nullptr);
+}
+
+int ptg_ncc_objective_batch_noalloc(void* p, int32_t n, const int32_t* igrain, const double* X, double dmax,
+                                    double stiff, double* f) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (n < 0 || (n > 0 && (!igrain || !X || !f))) return c->fail(PTG_ERR_ARG, "bad objective arguments");
+  if (n > kNoAllocMaxEvals)
+    return c->fail(PTG_ERR_LIMIT, "%d evaluations in one allocation-free call (at most %d)", n, kNoAllocMaxEvals);
+  for (int k = 0; k < n; k++)
+    if (igrain[k] < 0 || igrain[k] >= c->G) return c->fail(PTG_ERR_ARG, "grain index %d out of range", igrain[k]);
+  if (n == 0) return PTG_OK;
+  // the reference moments must exist already: computing them stages host tables
+  if (c->require_precomputations)
+    return c->fail(PTG_ERR_STATE, "reference moments not computed for the current frames (call ptg_precompute)");
+  if (!c->d_noalloc) return c->fail(PTG_ERR_STATE, "no staging array on this device");
+  char* base = (char*)c->d_noalloc;
+  int32_t* d_ig = (int32_t*)base;
+  double* d_X = (double*)(base + kNoAllocIgBytes);
+  double* d_f = (double*)(base + kNoAllocIgBytes + kNoAllocXBytes);
+  CK(c, hipMemcpyAsync(d_ig, igrain, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(d_X, X, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, launch_objective_batch(*c, n, d_ig, d_X, dmax, stiff, d_f, nullptr));
+  CK(c, hipMemcpyAsync(f, d_f, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
 }
 
 int ptg_ncc_objective_grad_batch(void* p, int32_t n, const int32_t* igrain, const double* X, double* f,

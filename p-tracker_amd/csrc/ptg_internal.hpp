@@ -156,6 +156,9 @@ struct Context {
   // objective / sampler batches
   void* d_tmp = nullptr;
   size_t tmp_cap = 0;
+  // this device's staging array of the allocation-free objective entry point (file-scope
+  // device memory, not owned by the context)
+  void* d_noalloc = nullptr;
 
   // timing
   bool timing = false;
