@@ -7,6 +7,13 @@ refinement, and the 9 result columns copied back to the host. Frames are residen
 HBM before timing; each step re-uploads the grains' input records (144 B each) as the
 pair's input. Prints one JSON line.
 
+After the timed steps the same pairs run again with every kernel of the exact search on one
+stream, so that the search kernel's launch durations (HIP events) do not overlap: the roofline
+is computed from those. Then the plain form, the exact-vs-plain pick comparison on configs 2 and
+3, the rescue-zone variant of the pair (300 grains moved 21-35 px, beyond the main zone: the
+rescue stage at its own +-40 px geometry) and the oracle's CPU baseline on a sample of grains
+(one warm-up, median of three).
+
 --sequence runs the batching form instead: a 16-frame 6000x4000 12-bit sequence with
 10,000 disk grains of radius U(6, 60) px moved per frame by U(-5, 5) px and U(-0.03, 0.03)
 rad (accumulated), tracked pair after pair with the reference frame swapped in between
@@ -204,6 +211,72 @@ def run_sequence(args, pk, synth, ptref):
     print(json.dumps(result), flush=True)
 
 
+def rescue_variant(pk, synth, args):
+    """The pair with 300 of its 2,000 grains moved 21-35 px along one axis (beyond the +-20 px
+    main zone, inside the +-40 px rescue zone): the fast form's rescue stage at its own
+    geometry, timed, and its picks against the plain form's."""
+    w = synth.workload(2, n_grains=args.grains, far=(max(1, 300 * args.grains // 2000), 21.0, 35.0))
+    tr = pk.Tracker(0)
+    tr.set_image(0, w.I0, w.bits)
+    tr.set_image(1, w.I1, w.bits)
+    tr.set_image_indices(0, 1)
+    tr.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
+    z = w.params
+    out = {}
+    for name, sm, sp in (("exact_lm", pk.SEARCH_EXACT, pk.SUBPIX_LM), ("plain_powell", pk.SEARCH_PLAIN, pk.SUBPIX_POWELL)):
+        prm = pk.track_params(z.search_zone, z.search_zone_rescue, z.search_zone_super_rescue, NCC_min=0.7,
+                              rescue_level=1, subpixel=1, search_method=sm, subpix=pk.default_subpix(sp))
+        steps = 3 if sm == pk.SEARCH_EXACT else 1
+        tr.put_grains(w.grains)
+        tr.track_pair(prm)
+        tr.sync()
+        f0 = tr.exact_fallbacks()
+        tr.set_timing(True)
+        tr.reset_stage_times()
+        t = time.perf_counter()
+        for _ in range(steps):
+            tr.put_grains(w.grains)
+            tr.track_pair(prm)
+            g = tr.get_grains()
+        el = time.perf_counter() - t
+        ms, _ = tr.stage_times()
+        tr.set_timing(False)
+        nres, _ = tr.queue_counts()
+        f1 = tr.exact_fallbacks()
+        out[name] = {"ms_per_pair": 1e3 * el / steps, "grains_per_s": w.G * steps / el, "rescued": int(nres),
+                     "stage_ms": {"search_main": ms[pk.T_SEARCH_MAIN] / steps,
+                                  "search_rescue": ms[pk.T_SEARCH_RESCUE] / steps,
+                                  "subpixel": ms[pk.T_SUBPIXEL] / steps},
+                     "exact_fallbacks": {"searches": int(f1[0] - f0[0]), "grains": int(f1[1] - f0[1])},
+                     "_g": g}
+    a, b = out["exact_lm"].pop("_g"), out["plain_powell"].pop("_g")
+    tr.close()
+    # integer picks after the rescue: the subpixel stage moves them, so compare the searches alone
+    tr = pk.Tracker(0)
+    tr.set_image(0, w.I0, w.bits)
+    tr.set_image(1, w.I1, w.bits)
+    tr.set_image_indices(0, 1)
+    tr.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
+    picks = {}
+    for sm in (pk.SEARCH_EXACT, pk.SEARCH_PLAIN):
+        tr.put_grains(w.grains)
+        tr.track_pair(pk.track_params(z.search_zone, z.search_zone_rescue, z.search_zone_super_rescue, NCC_min=0.7,
+                                      rescue_level=1, subpixel=0, search_method=sm))
+        picks[sm] = tr.get_grains()
+    tr.close()
+    diff = np.zeros(w.G, dtype=bool)
+    for c in ("dx", "dy", "drot"):
+        diff |= picks[pk.SEARCH_EXACT][c].view(np.uint64) != picks[pk.SEARCH_PLAIN][c].view(np.uint64)
+    out["pick_mismatches_exact_vs_plain"] = int(diff.sum())
+    out["grains"] = int(w.G)
+    out["moved_beyond_main_zone"] = int(len(w.truth["far"]))
+    log(f"rescue variant: exact+lm {out['exact_lm']['ms_per_pair']:.2f} ms/pair (rescue stage "
+        f"{out['exact_lm']['stage_ms']['search_rescue']:.2f} ms, {out['exact_lm']['rescued']} grains), plain+powell "
+        f"{out['plain_powell']['ms_per_pair']:.2f} ms/pair (rescue {out['plain_powell']['stage_ms']['search_rescue']:.2f}"
+        f" ms); pick mismatches {out['pick_mismatches_exact_vs_plain']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,8 +286,11 @@ def main():
     ap.add_argument("--subpix", choices=["auto", "lm", "powell"], default="auto")
     ap.add_argument("--grains", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain search+Powell form (0: skip)")
-    ap.add_argument("--cpu-grains", type=int, default=2000, help="grains in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-grains", type=int, default=500, help="grains in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU the process may use)")
+    ap.add_argument("--no-rescue-variant", action="store_true",
+                    help="skip the rescue-zone variant of the pair (300 grains moved beyond the main zone)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the exact-vs-plain pick comparison (configs 2 and 3) after the timed run")
     ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
@@ -314,6 +390,33 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
     roof["kernel"] = "k_search_exact (MFMA)" if method == "exact" else "k_search_plain (FP64)"
+    if method == "exact":
+        # the launches' own durations without overlap: the same pairs with every kernel of the
+        # exact search in order on the library stream (the radius buckets' launches otherwise
+        # run two at a time, and each event-timed duration then includes CU time shared with
+        # the other); the achieved rate is taken from these
+        tr.set_option(pk.OPT_SEARCH_STREAMS, 1)
+        step(prm)
+        tr.sync()
+        tr.set_timing(True)
+        tr.reset_stage_times()
+        t4 = time.perf_counter()
+        for _ in range(args.steps):
+            step(prm)
+        tr.sync()
+        ser_el = time.perf_counter() - t4
+        ms_s, launches_s = tr.stage_times()
+        tr.set_timing(False)
+        tr.set_option(pk.OPT_SEARCH_STREAMS, 2)
+        k_ms_overlapped = k_ms
+        k_ms = ms_s[pk.T_SEARCH_KERNEL] / args.steps
+        achieved = flops / (k_ms * 1e-3) / 1e12
+        roof["achieved"] = achieved
+        roof["frac"] = achieved / INT8_PEAK_TOPS
+        roof["timing"] = ("HIP events around each k_search_exact launch with every kernel of the search "
+                          "on one stream (no overlap)")
+        roof["serial_ms_per_pair"] = 1e3 * ser_el / args.steps
+        roof["kernel_ms_per_pair_overlapped"] = k_ms_overlapped
     # HBM bytes per launch of the search kernel, from the committed byte-counter summary
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy of known size)
     if method == "exact":
@@ -378,17 +481,26 @@ def main():
     else:
         tr.close()
 
+    if not args.no_rescue_variant:
+        result["rescue_variant"] = rescue_variant(pk, synth, args)
+
     if args.cpu_grains > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+            len(os.sched_getaffinity(0))
         rng = np.random.default_rng(0)
         sample = np.sort(rng.choice(w.G, size=min(args.cpu_grains, w.G), replace=False))
         W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
         cp = pk.track_params(w.params.search_zone, w.params.search_zone_rescue, w.params.search_zone_super_rescue,
                              NCC_min=0.7, rescue_level=1, subpixel=1, search_method=pk.SEARCH_PLAIN,
                              subpix=pk.default_subpix(pk.SUBPIX_POWELL))
-        t3 = time.perf_counter()
-        ptref.track_pair(W, cp, nthreads=threads, subset=sample)
-        ce = time.perf_counter() - t3
+        # one warm-up run, then the median of three
+        runs = []
+        for r in range(4):
+            t3 = time.perf_counter()
+            ptref.track_pair(W, cp, nthreads=threads, subset=sample)
+            if r > 0:
+                runs.append(time.perf_counter() - t3)
+        ce = float(np.median(runs))
         cpu_model = ""
         try:
             for line in open("/proc/cpuinfo"):
@@ -413,11 +525,14 @@ def main():
             "over_1e-5_rad": int((erad > 1e-5).sum())}
         log(f"fast form vs FP64 oracle on {len(sample)} grains: max {epx.max():.2e} px, {erad.max():.2e} rad, "
             f"{(epx > 1e-3).sum()} grains over 1e-3 px, {(erad > 1e-5).sum()} over 1e-5 rad")
-        result["cpu_baseline"] = {"value": len(sample) / ce, "unit": "grains/s", "cores": threads, "kind": "port",
-                                  "sample": f"{len(sample)} grains drawn from the same pair (seed 0), whole "
-                                            f"per-pair path (precompute, search, rescue, Powell) through the "
-                                            f"oracle with OpenMP schedule(dynamic); {ce:.1f} s; {cpu_model}"}
-        log(f"cpu baseline: {len(sample) / ce:.1f} grains/s on {threads} threads ({ce:.1f}s)")
+        result["cpu_baseline"] = {
+            "value": len(sample) / ce, "unit": "grains/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} grains drawn from the same pair (seed 0), whole per-pair path (precompute, "
+                      f"search, rescue, Powell) through the oracle (gcc -O3 -fopenmp -ffp-contract=off) with OpenMP "
+                      f"schedule(dynamic) on {threads} threads (the CPUs this process may use: "
+                      f"{len(os.sched_getaffinity(0))} of {os.cpu_count()} online); one warm-up, median of 3 runs "
+                      f"{', '.join(f'{x:.2f}' for x in runs)} s; {cpu_model}"}
+        log(f"cpu baseline: {len(sample) / ce:.1f} grains/s on {threads} threads (median {ce:.2f}s)")
     print(json.dumps(result), flush=True)
 
 

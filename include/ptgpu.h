@@ -252,6 +252,13 @@ enum {
   PTG_T_COUNT = 7
 };
 int ptg_set_timing(void* ctx, int enable);
+/* Options of a context (defaults in brackets); no counterpart in the reference.
+ *  PTG_OPT_SEARCH_STREAMS [2]: 2 overlaps an exact search's radius buckets over three
+ *  streams (each bucket's preparation kernels ahead of the searches, consecutive searches
+ *  on two streams); 1 runs every kernel of the search in order on the context's stream, so
+ *  that per-launch timings (PTG_T_SEARCH_KERNEL, a profiler's trace) do not overlap. */
+enum { PTG_OPT_SEARCH_STREAMS = 0 };
+int ptg_set_option(void* ctx, int option, int64_t value);
 /* Accumulated milliseconds and launch counts since the last reset (synchronises). */
 int ptg_get_stage_times(void* ctx, double* ms, int64_t* launches, int n);
 int ptg_reset_stage_times(void* ctx);

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libptgpu.so")
 
 PTG_OK = 0
+OPT_SEARCH_STREAMS = 0
 PTG_ERR_ARG, PTG_ERR_HIP, PTG_ERR_STATE, PTG_ERR_NOMEM, PTG_ERR_LIMIT = -1, -2, -3, -4, -5
 PTG_STATUS_CLAMPED = 1
 STAGE_MAIN, STAGE_RESCUE, STAGE_SUPER = 0, 1, 2
@@ -145,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_undistort_points": ([vp, vp, c_int32, vp, vp], c_int),
         "ptg_remap": ([vp, c_int, c_int, vp], c_int),
         "ptg_set_timing": ([vp, c_int], c_int),
+        "ptg_set_option": ([vp, c_int, c_int64], c_int),
         "ptg_get_stage_times": ([vp, vp, vp, c_int], c_int),
         "ptg_reset_stage_times": ([vp], c_int),
         "ptg_get_subpix_evals": ([vp, POINTER(c_int64)], c_int),
@@ -170,7 +172,7 @@ def exported_symbols():
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
-            "ptg_set_timing", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
+            "ptg_set_timing", "ptg_set_option", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
             "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
             "ptg_equiproj_eval", "ptg_calibrate_distortion"]
 
@@ -358,6 +360,9 @@ class Tracker:
     def remap(self, src: int, dst: int, params14):
         p = np.ascontiguousarray(params14, dtype=np.float64)
         self._ck(self.lib.ptg_remap(self.h, src, dst, _ptr(p)), "ptg_remap")
+
+    def set_option(self, option: int, value: int):
+        self._ck(self.lib.ptg_set_option(self.h, option, value), "ptg_set_option")
 
     def set_timing(self, on: bool):
         self._ck(self.lib.ptg_set_timing(self.h, 1 if on else 0), "ptg_set_timing")

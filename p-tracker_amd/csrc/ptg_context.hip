@@ -1268,6 +1268,18 @@ int ptg_remap(void* p, int src, int dst, const double* params14) {
   return PTG_OK;
 }
 
+int ptg_set_option(void* p, int option, int64_t value) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (option == PTG_OPT_SEARCH_STREAMS) {
+    if (value != 1 && value != 2) return c->fail(PTG_ERR_ARG, "PTG_OPT_SEARCH_STREAMS takes 1 or 2, not %lld",
+                                                 (long long)value);
+    c->serial_search = value == 1;
+    return PTG_OK;
+  }
+  return c->fail(PTG_ERR_ARG, "unknown option %d", option);
+}
+
 int ptg_set_timing(void* p, int enable) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;

@@ -76,6 +76,8 @@ struct Context {
   std::vector<hipStream_t> search_streams;
   std::vector<hipEvent_t> search_joins;
   int ensure_aux(size_t nevents);
+  // PTG_OPT_SEARCH_STREAMS = 1: every kernel of an exact search in order on `stream`
+  bool serial_search = false;
   std::string err;
 
   Image img[kMaxImages];

@@ -1780,10 +1780,15 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // stream; each bucket's MFMA kernel runs after its bucket's event, on the library stream
   // and the search streams in turn, so the next bucket's preparation overlaps the current
   // search and consecutive searches overlap their tails
+  // (with the serial option every kernel runs in order on the library stream instead, so
+  // that per-launch timings do not overlap)
   const size_t nb_all = P->buckets.size();
+  const bool serial = c.serial_search;
   if (int rca = c.ensure_aux(nb_all + 1); rca != PTG_OK) return rca;
+  hipStream_t prep = serial ? c.stream : c.aux_stream;
   hipEvent_t fork = c.aux_events[nb_all];
-  if ((he = hipEventRecord(fork, c.stream)) != hipSuccess || (he = hipStreamWaitEvent(c.aux_stream, fork, 0)) != hipSuccess)
+  if (!serial && ((he = hipEventRecord(fork, c.stream)) != hipSuccess ||
+                  (he = hipStreamWaitEvent(c.aux_stream, fork, 0)) != hipSuccess))
     return c.check(he, "exact search fork");
   size_t off = 0;
   for (size_t bi = 0; bi < nb_all; bi++) {
@@ -1802,7 +1807,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
         return c.check(he, "hipFuncSetAttribute");
       ExactArgs ed = e;
       ed.base_vb = l.vb;
-      k_exact_base_disk<<<dim3(nb, (V + l.vb - 1) / l.vb), 512, l.lds_d, c.aux_stream>>>(ed);
+      k_exact_base_disk<<<dim3(nb, (V + l.vb - 1) / l.vb), 512, l.lds_d, prep>>>(ed);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_base_disk");
     }
     if (l.vb == 0 || bk.ndisk < nb) {
@@ -1818,22 +1823,23 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       em.mid_npl = l.npl;
       const int ncb = (U + 15) / 16, ntile_m = P->vp / 16;
       const dim3 grid_m(nb, (ncb + l.cbw - 1) / l.cbw), block_m(64 * std::min(16, ntile_m * l.cbw));
-      if (l.npl == 6) k_exact_mid<6><<<grid_m, block_m, l.lds_m, c.aux_stream>>>(em);
-      else k_exact_mid<3><<<grid_m, block_m, l.lds_m, c.aux_stream>>>(em);
+      if (l.npl == 6) k_exact_mid<6><<<grid_m, block_m, l.lds_m, prep>>>(em);
+      else k_exact_mid<3><<<grid_m, block_m, l.lds_m, prep>>>(em);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     }
     if (l.lds_t > 65536 &&
         (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)l.lds_t)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
-    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, c.aux_stream>>>(e);
+    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, prep>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
-    if ((he = hipEventRecord(c.aux_events[bi], c.aux_stream)) != hipSuccess) return c.check(he, "hipEventRecord");
+    if ((he = hipEventRecord(c.aux_events[bi], prep)) != hipSuccess) return c.check(he, "hipEventRecord");
   }
   off = 0;
   hipStream_t lanes[kSearchLanes];
+  const int nlanes = serial ? 1 : kSearchLanes;
   lanes[0] = c.stream;
-  for (int q = 1; q < kSearchLanes; q++) {
+  for (int q = 1; q < nlanes; q++) {
     lanes[q] = c.search_streams[q - 1];
     // the tables and cleared outputs above come first
     if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
@@ -1847,8 +1853,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     e.bucket = bucket_dev + off;
     e.nbucket = nb;
     off += nb;
-    hipStream_t sx = lanes[nlaunch++ % kSearchLanes];
-    if ((he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
+    hipStream_t sx = lanes[nlaunch++ % nlanes];
+    if (!serial && (he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess)
+      return c.check(he, "hipStreamWaitEvent");
     ExactArgs eb = e;
     eb.ww = l.ww;
     eb.wh_pad = l.wh;
@@ -1866,7 +1873,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev, sx);
   }
-  for (int q = 1; q < kSearchLanes; q++) {
+  for (int q = 1; q < nlanes; q++) {
     if ((he = hipEventRecord(c.search_joins[q - 1], lanes[q])) != hipSuccess ||
         (he = hipStreamWaitEvent(c.stream, c.search_joins[q - 1], 0)) != hipSuccess)
       return c.check(he, "exact search join");

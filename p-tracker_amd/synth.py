@@ -203,9 +203,12 @@ class SynthWorkload:
 
 
 def workload(cfg: int, seed: int | None = None, n_grains: int | None = None, size=None,
-             search_method=SEARCH_PLAIN, subpix_method=SUBPIX_POWELL, max_disp=None) -> SynthWorkload:
+             search_method=SEARCH_PLAIN, subpix_method=SUBPIX_POWELL, max_disp=None, far=None) -> SynthWorkload:
     """Builds workload `cfg` (1-4). n_grains / size shrink it for tests; max_disp overrides
-    the displacement range (to force rescues, for example)."""
+    the displacement range (to force rescues, for example). far = (n, lo, hi), configs 2-4:
+    n grains drawn at random move by U(lo, hi) px, either sign, along one axis drawn at
+    random (the other keeps its U(-15, 15)), e.g. beyond the main search zone but inside
+    the rescue zone; truth["far"] lists them."""
     seed = 1000 + cfg if seed is None else seed
     rng = np.random.default_rng(seed)
     if cfg == 1:
@@ -234,6 +237,14 @@ def workload(cfg: int, seed: int | None = None, n_grains: int | None = None, siz
         centres = place_grains(rng, G, W, H, radii, margin, gap=2 * md + 4)
         disp = rng.uniform(-md, md, (G, 2))
         rot = rng.uniform(-0.1, 0.1, G)
+        far_idx = np.zeros(0, dtype=np.int64)
+        if far is not None:
+            nf, lo, hi = far
+            frng = np.random.default_rng(seed + 7919)  # leaves the main stream's draws unchanged
+            far_idx = np.sort(frng.choice(G, size=nf, replace=False))
+            axis = frng.integers(0, 2, nf)
+            mag = frng.uniform(lo, hi, nf) * np.where(frng.uniform(0, 1, nf) < 0.5, -1.0, 1.0)
+            disp[far_idx, axis] = mag
         nblobs = np.maximum(1, np.round(0.1 * np.pi * radii ** 2)).astype(int)
         I0, I1 = render_pair(rng, W, H, centres, radii, disp, rot, bits, background=160, base=640,
                              amp_range=(960, 3200), blobs_per_grain=nblobs, sigma_range=(1.0, 3.0),
@@ -256,6 +267,8 @@ def workload(cfg: int, seed: int | None = None, n_grains: int | None = None, siz
     grains = make_grains(centres, radii)
     pxy, pb = make_patterns(radii)
     truth = {"centres": centres, "disp": disp, "rot": rot, "radii": radii}
+    if cfg in (2, 3, 4):
+        truth["far"] = far_idx
     return SynthWorkload(I0, I1, grains, pxy, pb, bits, truth, prm, f"cfg{cfg}")
 
 

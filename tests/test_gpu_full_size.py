@@ -117,3 +117,39 @@ def test_full_size_40_steps_search_equal_oracle(tracker, method, bits):
         assert bits_equal(got["NCC"], fx[f"NCC_{bits}"])
     else:
         assert np.abs(got["NCC"] - fx[f"NCC_{bits}"]).max() < 1e-12
+
+
+@pytest.fixture(scope="module")
+def rescue_variant():
+    from make_cfg2_rescue_fixture import FAR
+
+    say("generating the rescue variant of the 2,000-grain pair (300 grains moved 21-35 px)")
+    fx = dict(np.load(os.path.join(os.path.dirname(FIXTURE), "cfg2_rescue_oracle.npz")))
+    w = synth.workload(2, far=FAR)
+    assert digest(w) == str(fx["digest"]), "the generated pair differs from the one the fixture was made from"
+    return w, fx
+
+
+@pytest.mark.parametrize("method", ["plain", "exact"])
+def test_full_size_rescue_zone_equals_oracle(tracker, rescue_variant, method):
+    # the rescue stage at its own geometry: +-40 px x 21 steps (137,781 candidates per grain)
+    # for the grains the +-20 px main search misses; picks bit-identical to the oracle's, NCC
+    # and NCC_rescue bit-identical for the plain form and within 1e-12 for the exact one
+    w, fx = rescue_variant
+    n0 = tracker.exact_fallbacks()
+    got = run(tracker, w, pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN, 0)
+    n1 = tracker.exact_fallbacks()
+    counts = list(tracker.queue_counts())
+    say(f"full size rescue variant, {method}: queues {counts}, exact-form fallbacks (searches, grains) "
+        f"{n1[0] - n0[0]}, {n1[1] - n0[1]}")
+    assert counts == list(fx["counts"]) and counts[0] >= 250
+    for c in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
+        bad = np.flatnonzero(np.ascontiguousarray(got[c]).view(np.uint64) != fx[c].view(np.uint64))
+        assert bad.size == 0, (c, bad[:8])
+    for c in ("NCC", "NCC_rescue"):
+        if method == "plain":
+            assert bits_equal(got[c], fx[c]), c
+        else:
+            assert np.abs(got[c] - fx[c]).max() < 1e-12, c
+    if method == "exact":
+        assert n1[0] == n0[0]  # neither stage left the exact form

@@ -674,6 +674,35 @@ def test_full_size_fixture_is_the_oracle_on_the_generated_pair():
                               fx["powell_" + c][sample].view(np.uint64)), c
 
 
+def test_rescue_variant_fixture_is_the_oracle_on_the_generated_pair():
+    # tests/golden/cfg2_rescue_oracle.npz: the pair with 300 grains moved beyond the main
+    # zone; the oracle reproduces the stored columns on a sample of rescued and other grains,
+    # and the rescued grains' picks lie beyond +-20 px (the rescue zone's own geometry)
+    import os
+    import sys
+
+    from ptracker_amd import synth
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    from make_cfg2_full_fixture import SEARCH_COLS, digest, params
+    from make_cfg2_rescue_fixture import FAR
+
+    fx = np.load(os.path.join(here, "golden", "cfg2_rescue_oracle.npz"))
+    w = synth.workload(2, far=FAR)
+    assert digest(w) == str(fx["digest"])
+    assert np.array_equal(fx["far"], w.truth["far"])
+    rescued = np.flatnonzero(fx["NCC"] < 0.7)
+    assert len(rescued) == fx["counts"][0] >= 250
+    moved = np.maximum(np.abs(fx["upix"][rescued]), np.abs(fx["vpix"][rescued]))
+    assert np.mean(moved > 20.5) > 0.9
+    W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
+    sample = np.union1d(rescued[::40], np.arange(1, w.G, 400))
+    gs, _, _ = ptref.track_pair(W, params(w, 0), nthreads=8, subset=sample)
+    for c in SEARCH_COLS:
+        assert np.array_equal(np.asarray(gs[c], dtype=np.float64).view(np.uint64), fx[c][sample].view(np.uint64)), c
+
+
 @pytest.mark.parametrize("bits", [12, 8])
 def test_full_size_40_step_fixture_is_the_oracle_on_the_generated_pair(bits):
     import os
