@@ -211,6 +211,41 @@ def run_sequence(args, pk, synth, ptref):
     print(json.dumps(result), flush=True)
 
 
+def run_remap(args):
+    """The distortion-correction remap of a 6000x4000 12-bit frame through the config-4 lens
+    (k1 = -2e-9, k2 = 1e-16 about the centre), frame resident in HBM: one JSON line with the
+    kernel's time per frame and its HBM rate against the 4 bytes per pixel it must move."""
+    import ptref
+
+    pk = ptref.package()
+    from ptracker_amd import synth
+
+    w = synth.workload(2, n_grains=200)
+    H, W = w.I0.shape
+    X = np.array([0.5 * W, 0.5 * H, -2e-9, 1e-16, 0, 0, 0, 0, 0.5 * W, 0.5 * H, 0, 0, 0, 0], dtype=np.float64)
+    tr = pk.Tracker(0)
+    tr.set_image(2, w.I0, w.bits)
+    for _ in range(args.warmup):
+        tr.remap(2, 3, X)
+    tr.sync()
+    tr.set_timing(True)
+    tr.reset_stage_times()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        tr.remap(2, 3, X)
+    tr.sync()
+    el = time.perf_counter() - t
+    ms, _ = tr.stage_times()
+    tr.close()
+    k_ms = ms[pk.T_REMAP] / args.steps
+    nbytes = 4.0 * W * H
+    achieved = nbytes / (k_ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "remap frames/s", "value": args.steps / el, "unit": "frames/s", "frame": f"{W}x{H}",
+                      "kernel_ms": k_ms, "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                                      "algorithmic_bytes": nbytes}}), flush=True)
+
+
 def rescue_variant(pk, synth, args):
     """The pair with 300 of its 2,000 grains moved 21-35 px along one axis (beyond the +-20 px
     main zone, inside the +-40 px rescue zone): the fast form's rescue stage at its own
@@ -298,11 +333,20 @@ def main():
     ap.add_argument("--seq-grains", type=int, default=10000)
     ap.add_argument("--seq-repeats", type=int, default=2, help="timed passes over the sequence")
     ap.add_argument("--seq-check-grains", type=int, default=48, help="grains per pair checked against the oracle")
+    ap.add_argument("--serial-search", action="store_true",
+                    help="run every kernel of the exact search on one stream throughout (for profiler traces "
+                         "and counters whose per-kernel figures must not overlap)")
+    ap.add_argument("--remap", action="store_true",
+                    help="only time the distortion-correction remap of a 6000x4000 12-bit frame (the lens of "
+                         "config 4) and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device and exit (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.remap:
+        run_remap(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards grains across GPUs "
@@ -326,6 +370,8 @@ def main():
     method = args.method
     subpix = args.subpix
     tr = pk.Tracker(0)
+    if args.serial_search:
+        tr.set_option(pk.OPT_SEARCH_STREAMS, 1)
     tr.set_image(0, w.I0, w.bits)
     tr.set_image(1, w.I1, w.bits)
     tr.set_image_indices(0, 1)
@@ -407,7 +453,7 @@ def main():
         ser_el = time.perf_counter() - t4
         ms_s, launches_s = tr.stage_times()
         tr.set_timing(False)
-        tr.set_option(pk.OPT_SEARCH_STREAMS, 2)
+        tr.set_option(pk.OPT_SEARCH_STREAMS, 1 if args.serial_search else 2)
         k_ms_overlapped = k_ms
         k_ms = ms_s[pk.T_SEARCH_KERNEL] / args.steps
         achieved = flops / (k_ms * 1e-3) / 1e12
