@@ -211,6 +211,32 @@ int ptg_undistort_points(void* ctx, const double* params14, int32_t n, const dou
  * iteration on undistor(). dst gets src's size and bits. */
 int ptg_remap(void* ctx, int src_index, int dst_index, const double* params14);
 
+/* ---- joint refinement of the grains' motion and a radial lens (differentiation form) --
+ * No counterpart in the reference, which fits the lens only through the equiprojectivity
+ * criterion (src/distortion_correction.cpp:135-479). undistor's radial part
+ * U(p) = p + (p - c)(K1 r^2 + K2 r^4) (src/distortion_correction.cpp:20-39) maps frame pixels
+ * to scene coordinates; grain g's pattern pixel q = ref + p0_i moves rigidly in the scene by
+ * (Tx, Ty) and the rotation theta about U(ref), with NCC_to_minimize_xyR's rotation
+ * convention, and is sampled in image[current] at U^-1 of the result. Its objective is
+ * 1 - NCC (NCC_to_minimize_xyR with the lens; with K1 = K2 = 0 exactly that function at
+ * X = (Tx - dx, Ty - dy, theta - drot)).
+ * ptg_joint_eval: for grains igrain[k] at P[3k..3k+2] = (Tx, Ty, theta) and lens =
+ * (K1, K2, xc, yc): f[k]; grad[5k..5k+4] = df/d(Tx, Ty, theta, K1, K2) (may be NULL); jtj[15k..]
+ * = the upper triangle, row by row, of the Gauss-Newton matrix of the normalised ZNCC
+ * residual over the same five (may be NULL). Analytic, through the sampler and the inverse
+ * lens. */
+int ptg_joint_eval(void* ctx, int32_t n, const int32_t* igrain, const double* P, const double* lens, double* f,
+                   double* grad, double* jtj);
+/* Minimises the sum of the objective over the unmasked grains with NCC_subpix >= ncc_min,
+ * jointly over each grain's (Tx, Ty, theta), started from its (dx, dy, drot), and the lens's
+ * (K1, K2) about the fixed centre (lens[2], lens[3]), started from lens[0..1]: Levenberg-
+ * Marquardt on the Gauss-Newton matrix, the per-grain blocks eliminated (Schur complement)
+ * for the lens step. On return lens[0..1] hold K1, K2, those grains' (dx, dy, drot) hold their
+ * scene motion (Tx, Ty, theta) and NCC_subpix = 1 - f; f_ini, f_end (the sums), iters and
+ * ngrains (the grains used) may be NULL. */
+int ptg_refine_joint_distortion(void* ctx, double* lens, double ncc_min, int32_t max_iter, double* f_ini,
+                                double* f_end, int32_t* iters, int32_t* ngrains);
+
 /* ---- distortion calibration (procedure correction_distortion) --------------------- */
 /* The equiprojectivity criterion of src/distortion_correction.cpp:135-479: G grains at
  * reference pixels ref_xy (G x 2, the reference's (double)refcoord_xpix / ypix), nimg

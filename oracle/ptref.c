@@ -775,3 +775,63 @@ void ptref_remap(const uint16_t* src, int32_t dimx, int32_t dimy, int64_t pitch,
       dst[(int64_t)y * dst_pitch + x] = (uint16_t)v;
     }
 }
+
+/* ---------------------------------------------------------------------------------
+ * The differentiation form's joint objective (test infrastructure; the reference has no
+ * counterpart: it fits the lens only through the equiprojectivity criterion). Pattern pixel
+ * q = ref + p0_i of grain g maps to undistorted coordinates by undistor()
+ * (src/distortion_correction.cpp:20-61, radial K1 = lens[0], K2 = lens[1] about
+ * (lens[2], lens[3])), moves rigidly there by T = (P[0], P[1]) and the rotation P[2] about
+ * undistor(ref), with the rotation convention of NCC_to_minimize_xyR ((x, y) -> (c x + s y,
+ * -s x + c y), src/p-tracker.cpp:1040-1045), and is sampled in I1 at the distorted position
+ * of the result (ptref_distort_point: Newton on the general undistor). The value is
+ * 1 - NCC with the reference's two-pass sums and sampler (:1018-1088). With K1 = K2 = 0 it is
+ * NCC_to_minimize_xyR at X = (P[0] - dx, P[1] - dy, P[2] - drot). */
+double ptref_joint_objective(const uint16_t* I0, const uint16_t* I1, int64_t pitch, const ptg_grain* g,
+                             int32_t n, const int32_t* p0_xy, const double* P, const double* lens) {
+  double X[14] = {lens[2], lens[3], lens[0], lens[1], 0.0, 0.0, 0.0, 0.0, lens[2], lens[3], 0.0, 0.0, 0.0, 0.0};
+  int xr = g->refcoord_xpix, yr = g->refcoord_ypix;
+  double wrx, wry;
+  ptref_undistor(X, (double)xr, (double)yr, &wrx, &wry);
+  double c1 = cos(P[2]), s1 = sin(P[2]);
+  double* vals = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+  double inv_num_pix = 1.0 / (double)n, mean1 = 0.0;
+  for (int i = 0; i < n; i++) {
+    double wx, wy;
+    ptref_undistor(X, (double)(xr + p0_xy[2 * i]), (double)(yr + p0_xy[2 * i + 1]), &wx, &wy);
+    double ax = wx - wrx, ay = wy - wry;
+    double tx = wrx + P[0] + c1 * ax + s1 * ay;
+    double ty = wry + P[1] - s1 * ax + c1 * ay;
+    double xs, ys;
+    ptref_distort_point(X, tx, ty, &xs, &ys);
+    double v = ptref_interp_cubic(I1, pitch, xs, ys);
+    vals[i] = v;
+    mean1 += v;
+  }
+  mean1 *= inv_num_pix;
+  double C0C1 = 0.0, C1C1 = 0.0;
+  for (int i = 0; i < n; i++) {
+    double diffC1 = vals[i] - mean1;
+    double diffC0 = (double)PX(I0, pitch, xr + p0_xy[2 * i], yr + p0_xy[2 * i + 1]) - g->mean0;
+    C0C1 += diffC0 * diffC1;
+    C1C1 += diffC1 * diffC1;
+  }
+  free(vals);
+  return 1.0 - C0C1 / sqrt(g->C0C0 * C1C1);
+}
+
+/* ptref_joint_objective for grains list[0..n-1] (OpenMP over grains): f[k] for grain list[k] at
+ * P[3k..3k+2]; pattern0_rotated offsets p0_xy with pattern_begin as the tracker's CSR layout. */
+void ptref_joint_objective_batch(const uint16_t* I0, const uint16_t* I1, int64_t pitch, const ptg_grain* grains,
+                                 const int32_t* p0_xy, const int64_t* pattern_begin, int32_t n,
+                                 const int32_t* list, const double* P, const double* lens, double* f,
+                                 int32_t nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; k++) {
+    const int g = list[k];
+    const int64_t b = pattern_begin[g];
+    f[k] = ptref_joint_objective(I0, I1, pitch, &grains[g], (int32_t)(pattern_begin[g + 1] - b), p0_xy + 2 * b,
+                                 P + 3 * k, lens);
+  }
+}

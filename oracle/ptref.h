@@ -123,6 +123,16 @@ void ptref_remap(const uint16_t* src, int32_t dimx, int32_t dimy, int64_t pitch,
 /* The distorted point for one undistorted point (the remap's inverse). */
 void ptref_distort_point(const double* X, double xu, double yu, double* xd, double* yd);
 
+/* The differentiation form's joint objective (no reference counterpart): 1 - NCC of grain g
+ * whose pattern pixels move rigidly by P = (Tx, Ty, theta) in the undistorted coordinates of
+ * the radial lens lens = (K1, K2, xc, yc), sampled in I1 through the inverse lens. */
+double ptref_joint_objective(const uint16_t* I0, const uint16_t* I1, int64_t pitch, const ptg_grain* g,
+                             int32_t n, const int32_t* p0_xy, const double* P, const double* lens);
+void ptref_joint_objective_batch(const uint16_t* I0, const uint16_t* I1, int64_t pitch, const ptg_grain* grains,
+                                 const int32_t* p0_xy, const int64_t* pattern_begin, int32_t n,
+                                 const int32_t* list, const double* P, const double* lens, double* f,
+                                 int32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,8 @@ def lib():
             "ptref_interp_cubic": ([vp, c_int64, c_double, c_double], c_double),
             "ptref_interp_cubic_batch": ([vp, c_int64, c_int32, vp, vp], None),
             "ptref_ncc_objective": ([vp, vp, c_int64, vp, c_int32, vp, vp, c_double, c_double], c_double),
+            "ptref_joint_objective": ([vp, vp, c_int64, vp, c_int32, vp, vp, vp], c_double),
+            "ptref_joint_objective_batch": ([vp, vp, c_int64, vp, vp, vp, c_int32, vp, vp, vp, vp, c_int32], None),
             "ptref_subpixel": ([vp, vp, c_int64, vp, c_int32, vp, vp, vp], None),
             "ptref_track_pair": ([vp, vp, c_int64, c_int32, vp, vp, vp, vp, c_int32, vp, vp], ctypes.c_int),
             "ptref_change_reference": ([c_int32, vp], None),
@@ -198,6 +200,28 @@ def ncc_objective(w: Workload, grains, p0, g, X, dmax=3.0, stiffness=0.0):
     X = np.ascontiguousarray(X, dtype=np.float64)
     return lib().ptref_ncc_objective(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g),
                                      int(e - b), _p(p0[b:e]), _p(X), dmax, stiffness)
+
+
+def joint_objective(w: Workload, grains, p0, g, P, lens):
+    """The differentiation form's joint objective of grain g: P = (Tx, Ty, theta) in undistorted
+    coordinates, lens = (K1, K2, xc, yc); see ptref_joint_objective."""
+    b, e = w.pattern_begin[g], w.pattern_begin[g + 1]
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    L = np.ascontiguousarray(lens, dtype=np.float64)
+    return lib().ptref_joint_objective(_p(w.I0), _p(w.I1), w.pitch, _rec(grains, g), int(e - b), _p(p0[b:e]),
+                                       _p(P), _p(L))
+
+
+def joint_objective_batch(w: Workload, grains, p0, glist, P, lens, nthreads=0):
+    """joint_objective for grains glist at P (len(glist) x 3), OpenMP over grains."""
+    gl = np.ascontiguousarray(glist, dtype=np.int32)
+    Pa = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(-1, 3))
+    L = np.ascontiguousarray(lens, dtype=np.float64)
+    gr = np.ascontiguousarray(grains)
+    f = np.zeros(len(gl))
+    lib().ptref_joint_objective_batch(_p(w.I0), _p(w.I1), w.pitch, _p(gr), _p(p0), _p(w.pattern_begin), len(gl),
+                                      _p(gl), _p(Pa), _p(L), _p(f), nthreads)
+    return f
 
 
 def subpixel(w: Workload, grains, p0, g, prm):

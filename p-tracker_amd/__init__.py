@@ -155,6 +155,8 @@ def load_library(path: str = LIB_PATH):
         "ptg_equiproj_set_data": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double], c_int),
         "ptg_equiproj_eval": ([vp, vp, vp, vp, vp, vp, vp], c_int),
         "ptg_calibrate_distortion": ([vp, vp, vp, c_int32, vp, vp, vp], c_int),
+        "ptg_joint_eval": ([vp, c_int32, vp, vp, vp, vp, vp, vp], c_int),
+        "ptg_refine_joint_distortion": ([vp, vp, c_double, c_int32, vp, vp, vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -174,7 +176,7 @@ def exported_symbols():
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_set_option", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
             "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
-            "ptg_equiproj_eval", "ptg_calibrate_distortion"]
+            "ptg_equiproj_eval", "ptg_calibrate_distortion", "ptg_joint_eval", "ptg_refine_joint_distortion"]
 
 
 class PtgError(RuntimeError):
@@ -356,6 +358,30 @@ class Tracker:
         out = np.zeros((dimy, dimx), dtype=np.uint16)
         self._ck(self.lib.ptg_get_image(self.h, index, _ptr(out), dimx), "ptg_get_image")
         return out
+
+    def joint_eval(self, igrain, P, lens, derivs=True):
+        """The joint objective of grains igrain at P (n x 3: Tx, Ty, theta) with the radial lens
+        (k1, k2, xc, yc): f, and with derivs df/d(Tx, Ty, theta, k1, k2) (n x 5) and the
+        Gauss-Newton matrices' upper triangles (n x 15)."""
+        ig = np.ascontiguousarray(np.atleast_1d(igrain), dtype=np.int32)
+        Pa = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(-1, 3))
+        L = np.ascontiguousarray(lens, dtype=np.float64)
+        f = np.zeros(len(ig))
+        g = np.zeros((len(ig), 5)) if derivs else None
+        h = np.zeros((len(ig), 15)) if derivs else None
+        self._ck(self.lib.ptg_joint_eval(self.h, len(ig), _ptr(ig), _ptr(Pa), _ptr(L), _ptr(f),
+                                         _ptr(g) if derivs else None, _ptr(h) if derivs else None), "ptg_joint_eval")
+        return (f, g, h) if derivs else f
+
+    def refine_joint_distortion(self, k1, k2, xc, yc, ncc_min=0.7, max_iter=50):
+        """Joint LM over the grains' motion and (k1, k2); returns (k1, k2, f_ini, f_end, iters, ngrains)."""
+        L = np.array([k1, k2, xc, yc], dtype=np.float64)
+        fi, fe = c_double(), c_double()
+        it, ng = c_int32(), c_int32()
+        self._ck(self.lib.ptg_refine_joint_distortion(self.h, _ptr(L), ncc_min, max_iter, ctypes.byref(fi),
+                                                      ctypes.byref(fe), ctypes.byref(it), ctypes.byref(ng)),
+                 "ptg_refine_joint_distortion")
+        return float(L[0]), float(L[1]), fi.value, fe.value, it.value, ng.value
 
     def remap(self, src: int, dst: int, params14):
         p = np.ascontiguousarray(params14, dtype=np.float64)

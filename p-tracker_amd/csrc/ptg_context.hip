@@ -524,7 +524,7 @@ int ptg_destroy(void* p) {
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
-                  c->d_exact_scratch, c->d_tmp, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
+                  c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
                   c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac};
   for (void* b : more)
     if (b) (void)hipFree(b);
@@ -1230,6 +1230,262 @@ int ptg_calibrate_distortion(void* p, double* params14, const int32_t* free14, i
   }
   for (int q = 0; q < 14; q++) params14[q] = X[q];
   if (f_end) *f_end = f;
+  if (iters) *iters = it;
+  return PTG_OK;
+}
+
+// ---- joint refinement of the grains' motion and a radial lens --------------------------
+// Device layout of one call: the grain list (n ints, 256-aligned), the parameters (3n
+// doubles), the per-grain outputs (joint_out_stride() n doubles).
+static int joint_buffers(Ctx* c, int32_t n, int32_t** d_list, double** d_P, double** d_out) {
+  const size_t bl = ((size_t)n * 4 + 255) / 256 * 256, bp = ((size_t)n * 24 + 255) / 256 * 256;
+  const size_t bo = (size_t)n * 8 * joint_out_stride();
+  TRY(c->ensure(&c->d_joint, &c->joint_cap, bl + bp + bo));
+  *d_list = (int32_t*)c->d_joint;
+  *d_P = (double*)((char*)c->d_joint + bl);
+  *d_out = (double*)((char*)c->d_joint + bl + bp);
+  return PTG_OK;
+}
+
+int ptg_joint_eval(void* p, int32_t n, const int32_t* igrain, const double* P, const double* lens, double* f,
+                   double* grad, double* jtj) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (n < 0 || (n > 0 && (!igrain || !P || !lens || !f))) return c->fail(PTG_ERR_ARG, "bad joint arguments");
+  for (int k = 0; k < n; k++)
+    if (igrain[k] < 0 || igrain[k] >= c->G) return c->fail(PTG_ERR_ARG, "grain index %d out of range", igrain[k]);
+  if (n == 0) return PTG_OK;
+  if (c->require_precomputations) TRY(do_precompute(c));
+  int32_t* d_list;
+  double *d_P, *d_out;
+  TRY(joint_buffers(c, n, &d_list, &d_P, &d_out));
+  CK(c, hipMemcpyAsync(d_list, igrain, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipMemcpyAsync(d_P, P, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  CK(c, launch_joint(*c, d_list, n, d_P, lens, grad || jtj, d_out));
+  const int st = joint_out_stride();
+  std::vector<double> out((size_t)n * st);
+  CK(c, hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < n; k++) {
+    const double* o = &out[(size_t)k * st];
+    f[k] = o[0];
+    if (grad)
+      for (int q = 0; q < 5; q++) grad[5 * k + q] = o[2 + q];
+    if (jtj)
+      for (int q = 0; q < 15; q++) jtj[15 * k + q] = o[7 + q];
+  }
+  return PTG_OK;
+}
+
+namespace {
+// A symmetric positive definite 3 x 3 system A X = B (B with m columns) by Cholesky; false
+// when A is not positive definite.
+bool chol3_solve(const double A[9], double* B, int m) {
+  double L[9] = {0};
+  for (int j = 0; j < 3; j++) {
+    double d = A[3 * j + j];
+    for (int k = 0; k < j; k++) d -= L[3 * j + k] * L[3 * j + k];
+    if (!(d > 0.0)) return false;
+    L[3 * j + j] = sqrt(d);
+    for (int i = j + 1; i < 3; i++) {
+      double v = A[3 * i + j];
+      for (int k = 0; k < j; k++) v -= L[3 * i + k] * L[3 * j + k];
+      L[3 * i + j] = v / L[3 * j + j];
+    }
+  }
+  for (int col = 0; col < m; col++) {
+    double y[3];
+    for (int i = 0; i < 3; i++) {
+      double v = B[3 * col + i];
+      for (int k = 0; k < i; k++) v -= L[3 * i + k] * y[k];
+      y[i] = v / L[3 * i + i];
+    }
+    for (int i = 2; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < 3; k++) v -= L[3 * k + i] * B[3 * col + k];
+      B[3 * col + i] = v / L[3 * i + i];
+    }
+  }
+  return true;
+}
+}  // namespace
+
+int ptg_refine_joint_distortion(void* p, double* lens, double ncc_min, int32_t max_iter, double* f_ini,
+                                double* f_end, int32_t* iters, int32_t* ngrains) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!lens || max_iter < 0) return c->fail(PTG_ERR_ARG, "bad joint refinement arguments");
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (c->require_precomputations) TRY(do_precompute(c));
+  if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
+  // the grains that take part: unmasked, with a subpixel NCC of at least ncc_min
+  std::vector<int32_t> list;
+  for (int g = 0; g < c->G; g++)
+    if (!c->h_grains[g].masked && c->h_grains[g].NCC_subpix >= ncc_min) list.push_back(g);
+  const int n = (int)list.size();
+  if (ngrains) *ngrains = n;
+  if (n == 0) {
+    if (f_ini) *f_ini = 0.0;
+    if (f_end) *f_end = 0.0;
+    if (iters) *iters = 0;
+    return PTG_OK;
+  }
+  int32_t* d_list;
+  double *d_P, *d_out;
+  TRY(joint_buffers(c, n, &d_list, &d_P, &d_out));
+  CK(c, hipMemcpyAsync(d_list, list.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  std::vector<double> P(3 * (size_t)n), Pt(3 * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    const ptg_grain& r = c->h_grains[list[k]];
+    P[3 * k] = r.dx;
+    P[3 * k + 1] = r.dy;
+    P[3 * k + 2] = r.drot;
+  }
+  // the lens in relative units: kappa1 = k1 R^2, kappa2 = k2 R^4, R the distance from the
+  // centre to the farthest frame corner (|kappa| is then the lens's relative displacement there)
+  const Image& im = c->img[c->icur];
+  const double xc = lens[2], yc = lens[3];
+  const double R2 = std::max(std::max(xc * xc, (im.dimx - 1 - xc) * (im.dimx - 1 - xc)), 0.0) +
+                    std::max(yc * yc, (im.dimy - 1 - yc) * (im.dimy - 1 - yc));
+  const double s1 = R2, s2 = R2 * R2;
+  double kap[2] = {lens[0] * s1, lens[1] * s2};
+  const int st = joint_out_stride();
+  std::vector<double> out((size_t)n * st);
+  auto eval = [&](const std::vector<double>& Pv, const double* kp, bool derivs, double* F) -> int {
+    const double L[4] = {kp[0] / s1, kp[1] / s2, xc, yc};
+    CK(c, hipMemcpyAsync(d_P, Pv.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    CK(c, launch_joint(*c, d_list, n, d_P, L, derivs, d_out));
+    CK(c, hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+    double t = 0.0;
+    for (int k = 0; k < n; k++) t += out[(size_t)k * st];
+    *F = t;
+    return PTG_OK;
+  };
+  double F = 0.0;
+  TRY(eval(P, kap, true, &F));
+  if (f_ini) *f_ini = F;
+  if (!std::isfinite(F)) return c->fail(PTG_ERR_ARG, "the joint objective is not finite at the start");
+  std::vector<double> H = out;  // derivatives at the current point
+  double lambda = 1e-3;
+  int it = 0;
+  std::vector<double> Y(9 * (size_t)n);  // per grain: A^-1 [B | b] (3 x 3 column-major)
+  for (; it < max_iter; it++) {
+    bool accepted = false, done = false;
+    while (!accepted) {
+      double S[4] = {0, 0, 0, 0}, sv[2] = {0, 0}, C[4] = {0, 0, 0, 0}, rk[2] = {0, 0};
+      bool ok = true;
+      for (int k = 0; k < n && ok; k++) {
+        const double* o = &H[(size_t)k * st];
+        const double* g = o + 2;
+        const double* h = o + 7;  // upper triangle of (Tx, Ty, th, k1, k2)
+        // h index of (p, q), p <= q: rows 0..4 hold 5, 4, 3, 2, 1 entries
+        auto hp = [&](int a, int b2) {
+          if (a > b2) std::swap(a, b2);
+          static const int row0[5] = {0, 5, 9, 12, 14};
+          return h[row0[a] + (b2 - a)];
+        };
+        const double sc[5] = {1.0, 1.0, 1.0, 1.0 / s1, 1.0 / s2};
+        double A[9];
+        for (int a = 0; a < 3; a++)
+          for (int b2 = 0; b2 < 3; b2++) A[3 * a + b2] = hp(a, b2);
+        for (int a = 0; a < 3; a++) A[4 * a] *= 1.0 + lambda;
+        double* y = &Y[9 * (size_t)k];
+        for (int a = 0; a < 3; a++) {
+          y[a] = hp(a, 3) * sc[3];
+          y[3 + a] = hp(a, 4) * sc[4];
+          y[6 + a] = g[a];
+        }
+        double Bk[6] = {y[0], y[1], y[2], y[3], y[4], y[5]}, bk[3] = {g[0], g[1], g[2]};
+        if (!chol3_solve(A, y, 3)) {
+          ok = false;
+          break;
+        }
+        for (int a = 0; a < 2; a++) {
+          for (int b2 = 0; b2 < 2; b2++) {
+            double t = 0.0;
+            for (int q = 0; q < 3; q++) t += Bk[3 * a + q] * y[3 * b2 + q];
+            S[2 * a + b2] += t;
+          }
+          double t = 0.0;
+          for (int q = 0; q < 3; q++) t += Bk[3 * a + q] * y[6 + q];
+          sv[a] += t;
+          rk[a] += g[3 + a] * sc[3 + a];
+          for (int b2 = 0; b2 < 2; b2++) C[2 * a + b2] += hp(3 + a, 3 + b2) * sc[3 + a] * sc[3 + b2];
+        }
+        (void)bk;
+      }
+      double dk[2] = {0.0, 0.0};
+      if (ok) {
+        double M[4] = {C[0] * (1.0 + lambda) - S[0], C[1] - S[1], C[2] - S[2], C[3] * (1.0 + lambda) - S[3]};
+        const double r0 = -(rk[0] - sv[0]), r1 = -(rk[1] - sv[1]);
+        const double det = M[0] * M[3] - M[1] * M[2];
+        if (!(det > 0.0) || !(M[0] > 0.0)) ok = false;
+        else {
+          dk[0] = (M[3] * r0 - M[1] * r1) / det;
+          dk[1] = (M[0] * r1 - M[2] * r0) / det;
+        }
+      }
+      if (!ok) {
+        lambda *= 4.0;
+        if (lambda > 1e16) {
+          done = true;
+          break;
+        }
+        continue;
+      }
+      double step = std::fabs(dk[0]) + std::fabs(dk[1]);
+      for (int k = 0; k < n; k++) {
+        const double* y = &Y[9 * (size_t)k];
+        for (int a = 0; a < 3; a++) {
+          const double d = -(y[6 + a] + y[a] * dk[0] + y[3 + a] * dk[1]);
+          Pt[3 * k + a] = P[3 * k + a] + d;
+          step = std::max(step, std::fabs(d));
+        }
+      }
+      const double kt[2] = {kap[0] + dk[0], kap[1] + dk[1]};
+      double Ft = 0.0;
+      TRY(eval(Pt, kt, true, &Ft));
+      if (std::isfinite(Ft) && Ft < F) {
+        const bool small = (F - Ft) <= 1e-13 * F || step < 1e-10;
+        P.swap(Pt);
+        kap[0] = kt[0];
+        kap[1] = kt[1];
+        F = Ft;
+        H = out;
+        lambda = std::max(lambda / 3.0, 1e-12);
+        accepted = true;
+        done = small;
+      } else {
+        lambda *= 4.0;
+        if (lambda > 1e16 || step < 1e-12) {
+          done = true;
+          break;
+        }
+      }
+    }
+    if (done) {
+      it++;
+      break;
+    }
+  }
+  // the refined motion into the grain records (the last accepted point's outputs are in H)
+  for (int k = 0; k < n; k++) {
+    ptg_grain& r = c->h_grains[list[k]];
+    r.dx = P[3 * k];
+    r.dy = P[3 * k + 1];
+    r.drot = P[3 * k + 2];
+    r.NCC_subpix = H[(size_t)k * st + 1];
+  }
+  CK(c, hipMemcpyAsync(c->d_grains, c->h_grains.data(), sizeof(ptg_grain) * c->G, hipMemcpyHostToDevice,
+                       c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  lens[0] = kap[0] / s1;
+  lens[1] = kap[1] / s2;
+  if (f_end) *f_end = F;
   if (iters) *iters = it;
   return PTG_OK;
 }

@@ -162,6 +162,10 @@ struct Context {
   // device memory, not owned by the context)
   void* d_noalloc = nullptr;
 
+  // joint refinement: grain list, parameters, per-grain outputs
+  void* d_joint = nullptr;
+  size_t joint_cap = 0;
+
   // timing
   bool timing = false;
   struct Pending { int stage; hipEvent_t a, b; };
@@ -240,5 +244,12 @@ hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, co
 hipError_t launch_interp_batch(Context& c, int index, int32_t n, const double* d_xy, double* d_out);
 hipError_t launch_undistort_points(Context& c, const double* params, int32_t n, const double* d_in, double* d_out);
 hipError_t launch_remap(Context& c, int src, int dst, const double* params);
+
+// joint refinement (ptg_joint.hip): per entry of d_list, from its (Tx, Ty, theta) in d_P and
+// the radial lens (k1, k2, xc, yc), joint_out_stride() doubles: f, NCC, df/dp (5), the
+// Gauss-Newton matrix's upper triangle (15), C1C1, a clamped flag (derivatives only when asked)
+hipError_t launch_joint(Context& c, const int32_t* d_list, int32_t nlist, const double* d_P, const double* lens,
+                        bool derivs, double* d_out);
+int joint_out_stride();
 
 }  // namespace ptg

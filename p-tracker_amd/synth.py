@@ -16,7 +16,8 @@ Workloads (seed = 1000 + number):
   3  the frames of 2; search +-20 px with 20 steps each side of 0.005 rad (the
      reference loop yields 40 steps); also an 8-bit variant (values >> 4).
   4  the frames of 2 warped by radial distortion k1 = -2e-9, k2 = 1e-16 about the frame
-     centre.
+     centre (the scene, grain motion included, lives in undistorted coordinates; the grain
+     records sit at the distorted images of the grain centres).
 The grain radius gives the pattern as `pattern grain_diameter 1.0 1`
 (src/p-tracker.cpp:1662-1667): a disk of radius int(radius).
 """
@@ -203,12 +204,13 @@ class SynthWorkload:
 
 
 def workload(cfg: int, seed: int | None = None, n_grains: int | None = None, size=None,
-             search_method=SEARCH_PLAIN, subpix_method=SUBPIX_POWELL, max_disp=None, far=None) -> SynthWorkload:
+             search_method=SEARCH_PLAIN, subpix_method=SUBPIX_POWELL, max_disp=None, far=None,
+             lens=None) -> SynthWorkload:
     """Builds workload `cfg` (1-4). n_grains / size shrink it for tests; max_disp overrides
     the displacement range (to force rescues, for example). far = (n, lo, hi), configs 2-4:
     n grains drawn at random move by U(lo, hi) px, either sign, along one axis drawn at
     random (the other keeps its U(-15, 15)), e.g. beyond the main search zone but inside
-    the rescue zone; truth["far"] lists them."""
+    the rescue zone; truth["far"] lists them. lens = (k1, k2) replaces config 4's radial lens."""
     seed = 1000 + cfg if seed is None else seed
     rng = np.random.default_rng(seed)
     if cfg == 1:
@@ -256,19 +258,24 @@ def workload(cfg: int, seed: int | None = None, n_grains: int | None = None, siz
         zr = zone(40, num_rot=z.num_rot, inc_rot=z.inc_rot)
         prm = track_params(z, zr, zr, NCC_min=0.7, rescue_level=1, subpixel=1, search_method=search_method,
                            subpix=default_subpix(subpix_method))
+        lens14 = None
         if cfg == 4:
-            from types import SimpleNamespace  # noqa: F401
-            k1, k2 = -2e-9, 1e-16
-            X = np.array([0.5 * W, 0.5 * H, k1, k2, 0, 0, 0, 0, 0.5 * W, 0.5 * H, 0, 0, 0, 0], dtype=np.float64)
-            I0 = distort_frame(I0, X, bits)
-            I1 = distort_frame(I1, X, bits)
+            k1, k2 = (-2e-9, 1e-16) if lens is None else lens
+            lens14 = np.array([0.5 * W, 0.5 * H, k1, k2, 0, 0, 0, 0, 0.5 * W, 0.5 * H, 0, 0, 0, 0], dtype=np.float64)
+            I0 = distort_frame(I0, lens14, bits)
+            I1 = distort_frame(I1, lens14, bits)
     else:
         raise ValueError(f"unknown workload {cfg}")
-    grains = make_grains(centres, radii)
+    # the grains sit where the frames show them: through the lens for config 4 (the scene's
+    # grain centres are undistorted coordinates)
+    seen = centres if cfg != 4 else np.array([_distort_point(lens14, c[0], c[1]) for c in centres])
+    grains = make_grains(seen, radii)
     pxy, pb = make_patterns(radii)
     truth = {"centres": centres, "disp": disp, "rot": rot, "radii": radii}
     if cfg in (2, 3, 4):
         truth["far"] = far_idx
+    if cfg == 4:
+        truth["lens"] = lens14  # undistor's 14 parameters; centres, disp, rot are undistorted coordinates
     return SynthWorkload(I0, I1, grains, pxy, pb, bits, truth, prm, f"cfg{cfg}")
 
 

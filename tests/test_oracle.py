@@ -444,6 +444,57 @@ def test_distort_point_inverts_undistor():
         assert abs(x2 - xu) < 1e-9 and abs(y2 - yu) < 1e-9
 
 
+def test_joint_objective_without_lens_is_the_reference_objective():
+    # the differentiation form's objective at K1 = K2 = 0 is NCC_to_minimize_xyR at
+    # X = P - (dx, dy, drot), bit for bit (the lens maps are then the identity; undistor and
+    # its inverse are pinned separately), and its batch is the single evaluations
+    from ptracker_amd import synth
+
+    w = synth.workload(2, n_grains=12, size=(700, 600), seed=41)
+    W = ptref.Workload(w.I0, w.I1, w.grains, w.pattern_xy, w.pattern_begin, w.bits)
+    gp, p0 = ptref.precompute(W)
+    rng = np.random.default_rng(3)
+    glist = np.arange(12, dtype=np.int32)
+    P = np.stack([gp["dx"] + rng.uniform(-1, 1, 12), gp["dy"] + rng.uniform(-1, 1, 12),
+                  gp["drot"] + rng.uniform(-0.02, 0.02, 12)], 1)
+    lens = [0.0, 0.0, 350.0, 300.0]
+    f = ptref.joint_objective_batch(W, gp, p0, glist, P, lens, 2)
+    for g in range(12):
+        X = P[g] - np.array([gp[g]["dx"], gp[g]["dy"], gp[g]["drot"]])
+        fr = ptref.ncc_objective(W, gp, p0, g, X)
+        assert f[g] == ptref.joint_objective(W, gp, p0, g, P[g], lens)
+        assert abs(f[g] - fr) < 1e-12
+
+
+def test_joint_objective_lens_moves_samples_as_undistor():
+    # with a lens, moving the scene point of each pattern pixel by T = U(q + t) - U(q) for a
+    # whole-pixel t (the motion that maps each pixel onto pixel q + t) is, for a pure
+    # translation of the frame content by t, a perfect match: 1 - NCC vanishes up to the
+    # rounding of the lens round trip
+    rng = np.random.default_rng(8)
+    I0 = rng.integers(100, 4000, size=(200, 220)).astype(np.uint16)
+    t = (3, -2)
+    I1 = np.roll(I0, (t[1], t[0]), (0, 1))
+    pat = ptref.circ_pattern(6).astype(np.int32)
+    gr = np.zeros(1, dtype=pk.GRAIN_DTYPE)
+    gr["refcoord_xpix"], gr["refcoord_ypix"] = 110, 100
+    W = ptref.Workload(I0, I1, gr, pat, np.array([0, len(pat)], dtype=np.int64), 12)
+    gp, p0 = ptref.precompute(W)
+    X = np.zeros(14)
+    X[0], X[1], X[2], X[3], X[8], X[9] = 90.0, 80.0, 0.0, 0.0, 90.0, 80.0
+    assert ptref.joint_objective(W, gp, p0, 0, [t[0], t[1], 0.0], [0.0, 0.0, 90.0, 80.0]) < 1e-12
+    # a lens with the scene motion of the reference point's image: every pixel moves by t to
+    # within the lens's local deformation, far below a pixel over the pattern
+    k1 = 1e-8
+    lens = [k1, 0.0, 90.0, 80.0]
+    X[2] = k1
+    ux, uy = ptref.undistor(X, 110.0, 100.0)
+    vx, vy = ptref.undistor(X, 110.0 + t[0], 100.0 + t[1])
+    f = ptref.joint_objective(W, gp, p0, 0, [vx - ux, vy - uy, 0.0], lens)
+    fwrong = ptref.joint_objective(W, gp, p0, 0, [t[0], t[1], 0.0], [1e-5, 0.0, 90.0, 80.0])
+    assert f < 1e-4 < fwrong
+
+
 def test_remap_identity():
     rng = np.random.default_rng(30)
     f = rng.integers(0, 4096, size=(40, 50)).astype(np.uint16)
