@@ -11,8 +11,14 @@
 // A is a correlation of the rotated template T_k (pixel values scattered onto their
 // rounded offsets, duplicates summed) with the window, computed as a GEMM on
 // v_mfma_i32_16x16x64_i8: rows = (step, template digit), columns = vertical shifts of
-// one shift column, K = template cells. Values are split into 7-bit digits (T < 2^14,
-// I1 < 2^12), so every product is exact in the int32 accumulators.
+// one shift column, K = template cells. The operands are int8 digits chosen by the frames'
+// declared depth, so every product is exact in the int32 accumulators:
+//  - 8-bit frames: I1 as one offset digit b = I1 - 128 (the byte with its top bit flipped),
+//    T as two 7-bit digits: A = 128 P_1 + P_0 + 128 Sa (Sa = sum of T): 2 products;
+//  - 12-bit frames: I1 and T as two 7-bit digits each: 4 products;
+//  - 16-bit frames: I1 as two offset bytes, I1 = 256 (h - 128) + (l - 128) + 32896 (each byte
+//    with its top bit flipped), T as three 7-bit digits (T < 2^21): 6 products, and num,
+//    d1 formed in int64 (N A reaches 2^61).
 //  - Template fragments are built once per search in global memory in fragment order
 //    (one 1 KiB block per M-tile and K-chunk) and read coalesced; all shift columns of
 //    a grain run on one XCD so its L2 keeps them.
@@ -76,7 +82,7 @@ struct ExactArgs {
   double thr;                  // the stage's queueing threshold (NCC_min, or NCC_min_super)
   int32_t part_cap;
   int32_t dcap;                // largest slot dcap of the search
-  int32_t nmt;                 // M-tiles (8 steps each)
+  int32_t nmt;                 // M-tiles (vm_spt steps each)
   int32_t nsub;                // shift blocks per grain
   int32_t nsu;                 // shift columns per block (= waves)
   int32_t vp;                  // vertical shifts padded to 16
@@ -96,6 +102,8 @@ struct ExactArgs {
                                // an overflow then raises flags[0] bit 0 and voids the stage)
   int32_t* ovf_list;           // overflowed slots, flags[1] of them, searched by the plain kernel
   int32_t base_vb;             // k_exact_base_disk: vertical shifts per workgroup
+  int32_t vmode;               // value mode (ExactMode)
+  int32_t vmax;                // its largest window value
   int32_t pad_b;
 };
 
@@ -125,26 +133,38 @@ __host__ __device__ inline double ncc_error_bound(double N, double R0, double R1
 // Grains whose best q is below 2 kQmin go to the plain kernel: the window on s = num^2 / d1
 // (= q^2 d0) below is relative and holds for q* >= kQmin.
 constexpr double kQmin = 1.0 / 64.0;
-constexpr int kExactVmax = 4095;  // window values the exact kernels take (larger: plain kernel)
-// Patterns of at most this many pixels have N A, Sa B, N C and B^2 below 2^53 (N^2 4095^2 < 2^53),
-// so the MFMA kernel forms num and d1 exactly in FP64; larger ones (radius above about 85 px)
-// go to the plain kernel.
+// Patterns of at most this many pixels have N A, Sa B, N C and B^2 below 2^53 for 12-bit
+// values (N^2 4095^2 < 2^53), so the MFMA kernel forms num and d1 exactly in FP64, and below
+// 2^62 for 16-bit values (formed in int64); larger ones (radius above about 85 px) go to the
+// plain kernel.
 constexpr long long kDoubleExactN = 23170;
+
+// Value modes of the exact search, from the frames' declared depth (the larger of the two
+// frames'): window values above the mode's vmax send the grain to the plain kernel.
+enum ExactMode : int { kVm8 = 0, kVm12 = 1, kVm16 = 2 };
+__host__ __device__ constexpr int vm_vmax(int vm) { return vm == kVm8 ? 255 : vm == kVm12 ? 4095 : 65535; }
+// template digit rows per rotation step, steps per 16-row M-tile, window digit planes
+__host__ __device__ constexpr int vm_digits(int vm) { return vm == kVm16 ? 3 : 2; }
+__host__ __device__ constexpr int vm_spt(int vm) { return vm == kVm16 ? 5 : 8; }
+__host__ __device__ constexpr int vm_planes(int vm) { return vm == kVm8 ? 1 : 2; }
+// the offset the window's digits leave out: A = (digit products) + vm_offset Sa
+__host__ __device__ constexpr double vm_offset(int vm) { return vm == kVm8 ? 128.0 : vm == kVm12 ? 0.0 : 32896.0; }
 
 // The relative window on s = num^2 / d1: a candidate with s < s* lo cannot be the reference's
 // pick when q* >= kQmin, since (q* - W)^2 / q*^2 >= (1 - W / kQmin)^2; 16u more cover the
-// roundings of s (5u each) and of the comparisons.
-__host__ __device__ inline double near_window_q(long long N, long long Sa, long long d0) {
+// roundings of s (5u each) and of the comparisons. vmax bounds the window values; q is within
+// 6u of rho when num and d1 are rounded to FP64 from int64 (16-bit frames), 4u otherwise.
+__host__ __device__ inline double near_window_q(long long N, long long Sa, long long d0, int vmax) {
   const double u = 1.1102230246251565e-16;
   const double Nd = (double)N;
   const double R0 = (double)Sa / sqrt((double)d0);
-  const double R1 = N > 1 ? (double)kExactVmax * Nd / sqrt(Nd - 1.0) : 0.0;
-  return 2.0 * (ncc_error_bound(Nd, R0, R1) + 4.0 * u);
+  const double R1 = N > 1 ? (double)vmax * Nd / sqrt(Nd - 1.0) : 0.0;
+  return 2.0 * (ncc_error_bound(Nd, R0, R1) + 6.0 * u);
 }
 
-__host__ __device__ inline double near_window_lo(long long N, long long Sa, long long d0) {
+__host__ __device__ inline double near_window_lo(long long N, long long Sa, long long d0, int vmax) {
   const double u = 1.1102230246251565e-16;
-  const double r = 1.0 - near_window_q(N, Sa, d0) / kQmin;
+  const double r = 1.0 - near_window_q(N, Sa, d0, vmax) / kQmin;
   return r * r * (1.0 - 16.0 * u);
 }
 
@@ -251,14 +271,14 @@ __device__ __forceinline__ void overflow_slot(const ExactArgs& e, int li) {
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
 // LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
 // bytes = count of the base mask (from k_exact_mid or k_exact_base_disk, which run first). Rows beyond the
-// last step are written as zeros.
+// last step, and an M-tile's rows beyond its vm_spt steps, are written as zeros.
 __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int bi = blockIdx.x;
   if (bi >= e.nbucket) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
-  const int k = blockIdx.y;  // 0 .. 8 * nmt - 1
+  const int k = blockIdx.y;  // 0 .. vm_spt * nmt - 1
   const int g = e.s.list[li];
   const ExactSlot sl = e.slots[li];
   const int rho = sl.rho, side = 2 * rho + 1, ncell = side * side;
@@ -306,23 +326,29 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   if (differs) *dflag = 1;
   __syncthreads();
   differs = *dflag;
-  // fragments: rows 2k (digit 1) and 2k + 1 (digit 0) of M-tile k / 8
-  const int mt = k >> 3, m = (2 * k) & 15;
+  // fragments: rows nd j + d of M-tile k / spt (j = k mod spt) hold digit nd - 1 - d of the
+  // template (7-bit digits, most significant first); the M-tile's last step also writes its
+  // padding rows (16 - nd spt of them) as zeros
+  const int nd = vm_digits(e.vmode), spt = vm_spt(e.vmode);
+  const int mt = k / spt, kj = k - mt * spt, m = nd * kj;
+  const int nr = nd + (kj == spt - 1 ? 16 - nd * spt : 0);
+  const int vlim = 1 << (7 * nd);
   const int kc = sl.kc;
   bool overflow = false;
-  for (int t = threadIdx.x; t < kc * 4 * 2; t += blockDim.x) {
-    const int d = t & 1, gg = (t >> 1) & 3, ch = t >> 3;
+  for (int t = threadIdx.x; t < kc * 4 * nr; t += blockDim.x) {
+    const int d = t % nr, r4 = t / nr, gg = r4 & 3, ch = r4 >> 2;
     const int segv = e.seg[sl.seg_off + 4 * ch + gg];
     const int qyr = segv >> 16, c0 = segv & 0x7fff;  // qy + rho, first cell column
     const bool empty = (segv & 0x8000) != 0;          // padding segment: zero template bytes
     unsigned int w[4] = {0u, 0u, 0u, 0u};
-    if (real && !empty) {
+    if (real && !empty && d < nd) {
+      const int sh = 7 * (nd - 1 - d);
       for (int j = 0; j < 16; j++) {
         const int col = c0 + j;  // qx + rho
         if (col < side) {
           const int v = packed[qyr * side + col] & 0xffffff;
-          if (v >= (1 << 14)) overflow = true;
-          const unsigned int dig = d == 0 ? (unsigned)(v >> 7) & 127u : (unsigned)v & 127u;
+          if (v >= vlim) overflow = true;
+          const unsigned int dig = (unsigned)(v >> sh) & 127u;
           w[j >> 2] |= dig << (8 * (j & 3));
         }
       }
@@ -371,7 +397,10 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
 // K runs over (qy, x) in 16-cell segments, and the B operands are the window's digit planes
 // starting at column su0: I1 = 128 h + l and I1^2 = sum_d 128^d q_d (7-bit digits), so
 //   B_mid = 128 acc_h + acc_l,  C_mid = sum_d 128^d acc_qd,
-// every product at most 4 * 127 and every sum exact in int32.
+// every product at most 4 * 127 and every sum exact in int32. For 16-bit frames the planes
+// are the bytes of I1 and I1^2 with their top bits flipped (byte - 128, an int8), so
+//   B_mid = 256 (acc_1 + 128 n) + (acc_0 + 128 n),  C_mid = sum_d 256^d (acc_qd + 128 n),
+// n = the sum of the mask's multiplicities (the pattern's pixel count).
 // One workgroup per (slot, column block); wave w takes N-tiles w, w + 4, ...
 // LDS: NPL digit planes [WHm rows x stride] | mask rows [side x MLR] bytes (16 zero bytes on
 // the left) | segment table. Large patterns stage the six planes in two passes; each
@@ -458,6 +487,8 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   const int ntile = vp >> 4;
   __shared__ int big;
   if (threadIdx.x == 0) big = 0;
+  const bool bytes = e.vmode == kVm16;
+  const uint32_t vmax = (uint32_t)e.vmax;
   for (int p0 = 0; p0 < kMidPlanes; p0 += NPL) {
     // the window's digit planes p0 .. p0 + NPL - 1 for columns su0 .. su0 + stride - 1
     if (p0 > 0) __syncthreads();  // every wave is done with the previous pass's planes
@@ -469,10 +500,20 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
-        over |= v > 4095u;
+        over |= v > vmax;
         const uint32_t q = v * v;
-        const uint32_t dig[kMidPlanes] = {(v >> 7) & 127u, v & 127u, q & 127u, (q >> 7) & 127u, (q >> 14) & 127u,
-                                          (q >> 21) & 127u};
+        uint32_t dig[kMidPlanes];
+        if (bytes) {
+          dig[0] = ((v >> 8) & 255u) ^ 128u;
+          dig[1] = (v & 255u) ^ 128u;
+#pragma unroll
+          for (int d = 0; d < 4; d++) dig[2 + d] = ((q >> (8 * d)) & 255u) ^ 128u;
+        } else {
+          dig[0] = (v >> 7) & 127u;
+          dig[1] = v & 127u;
+#pragma unroll
+          for (int d = 0; d < 4; d++) dig[2 + d] = (q >> (7 * d)) & 127u;
+        }
 #pragma unroll
         for (int p = 0; p < kMidPlanes; p++) w[p] |= dig[p] << (8 * j);
       }
@@ -483,8 +524,8 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
     if (over) big = 1;
     __syncthreads();
     if (big) {
-      // a window value beyond 12 bits (the frame's declared depth is not checked against
-      // its data): the 7-bit digits would drop bits, so the plain kernel takes this grain
+      // a window value beyond the declared depth (the frame's data is not checked against
+      // it): the digits would drop bits, so the plain kernel takes this grain
       if (threadIdx.x == 0) overflow_slot(e, li);
       return;
     }
@@ -547,10 +588,17 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 #pragma unroll
           for (int p = 0; p < NPL; p++) {
             const int pl = p0 + p;
-            const long long a = acc[p][i];
-            if (pl == 0) Bv += a * 128;
-            else if (pl == 1) Bv += a;
-            else Cv += a << (7 * (pl - 2));
+            if (bytes) {
+              const long long a = (long long)acc[p][i] + 128ll * n;
+              if (pl == 0) Bv += a * 256;
+              else if (pl == 1) Bv += a;
+              else Cv += a << (8 * (pl - 2));
+            } else {
+              const long long a = acc[p][i];
+              if (pl == 0) Bv += a * 128;
+              else if (pl == 1) Bv += a;
+              else Cv += a << (7 * (pl - 2));
+            }
           }
           const int64_t t = (int64_t)su * V + svv;
           if (p0 > 0) {
@@ -571,8 +619,9 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 //   B_mid(s) = sum_qy P(row, hi) - P(row, lo),  C_mid likewise on I1^2,
 // 2 (2 r + 1) LDS reads per shift instead of a GEMM over the disk's cells. The prefix of I1^2
 // is kept modulo 2^32: a run has at most 2 rho - 1 <= 255 cells of at most 4095^2, so its
-// true sum is below 2^32 and the modular difference is exact (a window value above 4095,
-// beyond the frames' declared 12 bits, sends the grain to the plain kernel). Also writes the
+// true sum is below 2^32 and the modular difference is exact (a window value above 4095, or
+// above the frames' declared depth, sends the grain to the plain kernel; 16-bit frames take
+// k_exact_mid instead). Also writes the
 // disk's multiplicity grid and, in the first block, the reference moments. One workgroup per
 // (slot, block of base_vb vertical shifts); LDS: the window rows of the block (base_vb +
 // 2 rho) x (U + 2 rho + 1) prefix pairs, then the run half-widths.
@@ -624,7 +673,7 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
       const int x = x0 + lane;
       unsigned int v = 0u;
       if (x < ncol) v = src[clampi(X0 + x, 0, e.s.dimx - 1)];
-      over |= v > 4095u;
+      over |= v > (unsigned int)min(e.vmax, 4095);
       unsigned int a = v, b = v * v;
 #pragma unroll
       for (int m = 1; m < 64; m <<= 1) {
@@ -642,7 +691,7 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
   if (over) big = 1;
   __syncthreads();
   if (big) {
-    // beyond 12 bits: this grain goes to the plain kernel (its MFMA search returns at once)
+    // beyond the depth: this grain goes to the plain kernel (its MFMA search returns at once)
     if (threadIdx.x == 0) overflow_slot(e, li);
     return;
   }
@@ -723,11 +772,26 @@ __host__ __forceinline__ size_t base_disk_lds_bytes(int rho, int U, int vb) {
 }
 
 // ---- the MFMA search ---------------------------------------------------------------
+// Sum of the squares of eight window values: in 32 bits for values of at most 12 bits
+// (8 x 4095^2 < 2^32), term by term in 64 bits for 16-bit values.
+template <bool W64>
+__device__ __forceinline__ unsigned long long sq8(unsigned int w0, unsigned int w1, unsigned int w2, unsigned int w3,
+                                                  unsigned int w4, unsigned int w5, unsigned int w6, unsigned int w7) {
+  if constexpr (W64) {
+    typedef unsigned long long u64;
+    return (((u64)(w0 * w0) + (u64)(w1 * w1)) + ((u64)(w2 * w2) + (u64)(w3 * w3))) +
+           (((u64)(w4 * w4) + (u64)(w5 * w5)) + ((u64)(w6 * w6) + (u64)(w7 * w7)));
+  } else {
+    return (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
+  }
+}
+
 // Sums of I1 and I1^2 at this lane's shift over the entries [lo, lo + len) of one Delta list
 // held in LDS from dent[o] (lo a multiple of 4; 4 entries per 16-byte read, 16 window reads
 // in flight) and, with tail set, over the list's entries [nl, n) beyond the LDS capacity from
-// global memory (dg). A list has at most 1023 cells, so the sum of I1 is exact in 32 bits;
-// I1^2 is summed in 32 bits per group of at most 8 cells, then in 64.
+// global memory (dg). A list has at most 1023 cells, so the sum of I1 is exact in 32 bits
+// (1023 x 65535 < 2^26); I1^2 is summed as sq8 sums it, then in 64.
+template <bool W64>
 __device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* dent, const int32_t* dg, int o,
                                           int lo, int len, bool tail, int nl, int n, uint32_t wbase, int base,
                                           unsigned int& bsum, unsigned long long& csum) {
@@ -745,8 +809,8 @@ __device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* d
     const unsigned int w8 = px(e2.x), w9 = px(e2.y), w10 = px(e2.z), w11 = px(e2.w);
     const unsigned int w12 = px(e3.x), w13 = px(e3.y), w14 = px(e3.z), w15 = px(e3.w);
     bsum += (((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))) + (((w8 + w9) + (w10 + w11)) + ((w12 + w13) + (w14 + w15)));
-    csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
-    csum += (unsigned int)(((w8 * w8 + w9 * w9) + (w10 * w10 + w11 * w11)) + ((w12 * w12 + w13 * w13) + (w14 * w14 + w15 * w15)));
+    csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
+    csum += sq8<W64>(w8, w9, w10, w11, w12, w13, w14, w15);
   }
   for (; q + 8 <= len; q += 8) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
@@ -754,7 +818,7 @@ __device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* d
     const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
     const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
     bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
-    csum += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
+    csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
   }
   for (; q < len; q += 4) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
@@ -763,30 +827,32 @@ __device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* d
     const unsigned int w2 = q + 2 < len ? px(e0.z) : 0u;
     const unsigned int w3 = q + 3 < len ? px(e0.w) : 0u;
     bsum += (w0 + w1) + (w2 + w3);
-    csum += (unsigned int)((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3));
+    csum += sq8<W64>(w0, w1, w2, w3, 0u, 0u, 0u, 0u);
   }
   if (tail)
     for (int r = nl; r < n; r++) {
       const unsigned int w = win[base + dg[r]];
       bsum += w;
-      csum += w * w;
+      csum += (unsigned long long)(w * w);
     }
 }
 
 // The whole list ks = 2 step + sign at this lane's shift; an inactive lane (act false) sums
 // nothing.
+template <bool W64>
 __device__ __forceinline__ void delta_sums(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
                                            const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
                                            bool act, int ks, uint32_t wbase, int base, unsigned int& bsum,
                                            unsigned long long& csum) {
   const int n = act ? dcnt[ks] : 0, o = act ? dofs[ks] : 0;
   const int nl = max(0, min(n, e.dl_lds_cap - o));
-  delta_seg(win, dent, e.dlist + dl_off + (int64_t)ks * dcap, o, 0, nl, true, nl, n, wbase, base, bsum, csum);
+  delta_seg<W64>(win, dent, e.dlist + dl_off + (int64_t)ks * dcap, o, 0, nl, true, nl, n, wbase, base, bsum, csum);
 }
 
 // Both lists of step k (cells gained, cells lost) at this lane's shift, for a wave whose lanes
 // share the step: their common length is walked side by side, 8 entries of each list per
 // iteration, so that the two lists' LDS latencies overlap; then the rest of each.
+template <bool W64>
 __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* win, const int32_t* dcnt,
                                            const int32_t* dofs, const uint32_t* dent, int64_t dl_off, int dcap,
                                            int k, uint32_t wbase, int base, unsigned int& bp,
@@ -806,18 +872,19 @@ __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* w
     const unsigned int v0 = px(b0.x), v1 = px(b0.y), v2 = px(b0.z), v3 = px(b0.w);
     const unsigned int v4 = px(b1.x), v5 = px(b1.y), v6 = px(b1.z), v7 = px(b1.w);
     bp += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
-    cp += (unsigned int)(((w0 * w0 + w1 * w1) + (w2 * w2 + w3 * w3)) + ((w4 * w4 + w5 * w5) + (w6 * w6 + w7 * w7)));
+    cp += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
     bn += ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7));
-    cn += (unsigned int)(((v0 * v0 + v1 * v1) + (v2 * v2 + v3 * v3)) + ((v4 * v4 + v5 * v5) + (v6 * v6 + v7 * v7)));
+    cn += sq8<W64>(v0, v1, v2, v3, v4, v5, v6, v7);
   }
-  delta_seg(win, dent, e.dlist + dl_off + (int64_t)(2 * k) * dcap, o0, nc, nl0 - nc, true, nl0, n0, wbase, base,
-            bp, cp);
-  delta_seg(win, dent, e.dlist + dl_off + (int64_t)(2 * k + 1) * dcap, o1, nc, nl1 - nc, true, nl1, n1, wbase,
-            base, bn, cn);
+  delta_seg<W64>(win, dent, e.dlist + dl_off + (int64_t)(2 * k) * dcap, o0, nc, nl0 - nc, true, nl0, n0, wbase,
+                 base, bp, cp);
+  delta_seg<W64>(win, dent, e.dlist + dl_off + (int64_t)(2 * k + 1) * dcap, o1, nc, nl1 - nc, true, nl1, n1,
+                 wbase, base, bn, cn);
 }
 
-template <int NMT, int NTP>
+template <int NMT, int NTP, int VM>
 __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
+  constexpr int SPT = vm_spt(VM), NP = vm_planes(VM);
   extern __shared__ __align__(16) unsigned char smem[];
   // XCD-aware job mapping: all shift blocks of a grain share blockIdx % 8
   const int bx = blockIdx.x;
@@ -842,14 +909,16 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
 
   uint16_t* win = reinterpret_cast<uint16_t*>(smem);
   const size_t win_bytes = ((size_t)e.ww * e.wh_pad * 2 + 15) / 16 * 16;
-  const size_t stage_bytes = ((size_t)2 * e.wh_pad * e.cs_max + 15) / 16 * 16;
+  // (the same layout as the host's LDS size for the launch: NP digit planes per wave)
+  const size_t stage_bytes = ((size_t)NP * e.wh_pad * e.cs_max + 15) / 16 * 16;
   uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * stage_bytes;
   unsigned char* tail = smem + win_bytes + (size_t)e.nsu * stage_bytes;
   // the epilogue's exchange buffer: the wave's own staging area when the column takes one
   // vertical pass (the staged planes are dead once its MFMAs are issued), else its own area
+  constexpr int ABR = VM == kVm16 ? 16 : 8;  // its rows: one per M-tile row (16-bit) or per step
   double* abuf = e.abuf_alias ? reinterpret_cast<double*>(stage)
-                              : reinterpret_cast<double*>(tail) + (size_t)wave * 8 * (NTP * 16);
-  if (!e.abuf_alias) tail += (size_t)e.nsu * 8 * (NTP * 16) * 8;
+                              : reinterpret_cast<double*>(tail) + (size_t)wave * ABR * (NTP * 16);
+  if (!e.abuf_alias) tail += (size_t)e.nsu * ABR * (NTP * 16) * 8;
   int32_t* bflag = reinterpret_cast<int32_t*>(tail);  // the block's hand-over flag
   tail += 16;
   int32_t* segl = reinterpret_cast<int32_t*>(tail);
@@ -873,11 +942,11 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     X = clampi(X, 0, e.s.dimx - 1);
     Y = clampi(Y, 0, e.s.dimy - 1);
     const uint16_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + X];
-    over |= v > 4095;
+    over |= v > e.vmax;
     win[q] = v;
   }
   if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
-  // a window value beyond 12 bits, or a pattern beyond FP64's exact integers: the plain
+  // a window value beyond the declared depth, or a pattern beyond the exact integers: the plain
   // kernel takes this grain (the flag word was cleared before the staging's barrier)
   if (over || (threadIdx.x == 0 && e.mom[3 * li] > kDoubleExactN)) *bflag = 1;
   __syncthreads();
@@ -920,7 +989,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const long long d0i = N * Saa - Sa * Sa;
   // the relative near-tie window (none for a flat pattern, whose reference NCC is NaN at
   // every candidate: the list stays empty)
-  const double lo = d0i > 0 ? near_window_lo(N, Sa, d0i) : 2.0;
+  const double lo = d0i > 0 ? near_window_lo(N, Sa, d0i, e.vmax) : 2.0;
   Near2 nl = near2_none();  // this lane's list
   double s1lo = -1.0;       // its best's s times lo: candidates below are rejected at once
   const double Nd = (double)N, Sad = (double)Sa;
@@ -930,7 +999,9 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const int cpb = (U + e.nsub - 1) / e.nsub;
   const int c_lo = blk * cpb, c_hi = min(U, c_lo + cpb);
   for (int su = c_lo + wave; su < c_hi; su += e.nsu) {
-    // stage this shift column: cell c of staged row y is window column su + c
+    // stage this shift column: cell c of staged row y is window column su + c; plane 0 holds
+    // the high digit (12-bit: v >> 7; 16-bit: the high byte, top bit flipped) or, for 8-bit
+    // frames, the only one (v - 128), plane 1 the low digit
     const int nq = WH * (cw >> 2);
     for (int q = lane; q < nq; q += 64) {
       const int y = q / (cw >> 2), c4 = (q - y * (cw >> 2)) * 4;
@@ -938,11 +1009,18 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const unsigned int v = win[y * WW + su + c4 + j];
-        hi |= ((v >> 7) & 127u) << (8 * j);
-        lo |= (v & 127u) << (8 * j);
+        if constexpr (VM == kVm8) {
+          hi |= ((v & 255u) ^ 128u) << (8 * j);
+        } else if constexpr (VM == kVm12) {
+          hi |= ((v >> 7) & 127u) << (8 * j);
+          lo |= (v & 127u) << (8 * j);
+        } else {
+          hi |= (((v >> 8) & 255u) ^ 128u) << (8 * j);
+          lo |= ((v & 255u) ^ 128u) << (8 * j);
+        }
       }
       *reinterpret_cast<unsigned int*>(stage + (size_t)y * cs + c4) = hi;
-      *reinterpret_cast<unsigned int*>(stage + (size_t)e.wh_pad * cs + (size_t)y * cs + c4) = lo;
+      if constexpr (NP == 2) *reinterpret_cast<unsigned int*>(stage + (size_t)e.wh_pad * cs + (size_t)y * cs + c4) = lo;
     }
     // the stage is this wave's alone: a wave-level barrier orders its writes and reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -951,17 +1029,17 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   {
     const int ntile = vp >> 4;
     for (int p0 = 0; p0 < ntile; p0 += NTP) {
-      v4i acc[NMT][NTP][2];
+      v4i acc[NMT][NTP][NP];
 #pragma unroll
       for (int a = 0; a < NMT; a++)
 #pragma unroll
         for (int j = 0; j < NTP; j++)
 #pragma unroll
-          for (int p = 0; p < 2; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
+          for (int p = 0; p < NP; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
       // Software pipeline: the template fragments (global memory) of chunk ch + 1 are
       // requested before chunk ch multiplies (two named buffers, scheduling barriers keep
       // the loads ahead of their use); the window fragments (LDS) are read per chunk.
-      v4i fa0[NMT], fa1[NMT], fb[NTP][2];
+      v4i fa0[NMT], fa1[NMT], fb[NTP][NP];
       auto fetch_a = [&](int chq, v4i* fa) {
         const int chc = chq < kc ? chq : kc - 1;
 #pragma unroll
@@ -975,17 +1053,17 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         for (int j = 0; j < NTP; j++) {
           const uint8_t* bp = stage + (size_t)((p0 + j) * 16 + col) * cs + rowoff;
           fb[j][0] = *reinterpret_cast<const v4i*>(bp);
-          fb[j][1] = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
+          if constexpr (NP == 2) fb[j][NP - 1] = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
         }
       };
       auto mult = [&](const v4i* fa) {
 #pragma unroll
         for (int j = 0; j < NTP; j++)
 #pragma unroll
-          for (int a = 0; a < NMT; a++) {
-            acc[a][j][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][0], acc[a][j][0], 0, 0, 0);
-            acc[a][j][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][1], acc[a][j][1], 0, 0, 0);
-          }
+          for (int a = 0; a < NMT; a++)
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+              acc[a][j][p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][p], acc[a][j][p], 0, 0, 0);
       };
       fetch_a(0, fa0);
       for (int ch = 0; ch < kc; ch += 2) {  // kc is even (host pads with empty chunks)
@@ -1022,32 +1100,69 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
       }
 #pragma unroll
       for (int a = 0; a < NMT; a++) {
-        // lane holds rows 4 gg .. 4 gg + 3 = steps 8 a + 2 gg + h, digits (1, 0) per step
+        // lane holds rows 4 gg + i (i < 4) of the M-tile. 8- and 12-bit frames: rows 2 h, 2 h + 1
+        // of the lane are step 8 a + 2 gg + h, template digits 1 and 0, and A = T I1 goes to
+        // abuf row 2 gg + h. 16-bit frames: row r = 3 j + d is step 5 a + j, template digit
+        // 2 - d; its two planes' products, weighted, go to abuf row r, and a step's A is the sum
+        // of its three rows. Every value is an integer below 2^53, exact in double.
 #pragma unroll
-        for (int j = 0; j < NTP; j++)
+        for (int j = 0; j < NTP; j++) {
+          if constexpr (VM == kVm16) {
 #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            // T I1 from its four digit products, exact in double (< 2^53)
-            const double A = (double)acc[a][j][0][2 * h] * 16384.0 +
-                             ((double)acc[a][j][1][2 * h] + (double)acc[a][j][0][2 * h + 1]) * 128.0 +
-                             (double)acc[a][j][1][2 * h + 1];
-            abuf[(2 * gg + h) * (NTP * 16) + j * 16 + col] = A;
+            for (int i = 0; i < 4; i++) {
+              const int r = 4 * gg + i, d = r % 3;
+              const double wgt = d == 0 ? 16384.0 : d == 1 ? 128.0 : 1.0;
+              abuf[r * (NTP * 16) + j * 16 + col] = ((double)acc[a][j][0][i] * 256.0 + (double)acc[a][j][1][i]) * wgt;
+            }
+          } else {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              double A;
+              if constexpr (VM == kVm12)
+                A = (double)acc[a][j][0][2 * h] * 16384.0 +
+                    ((double)acc[a][j][1][2 * h] + (double)acc[a][j][0][2 * h + 1]) * 128.0 +
+                    (double)acc[a][j][1][2 * h + 1];
+              else
+                A = (double)acc[a][j][0][2 * h] * 128.0 + (double)acc[a][j][0][2 * h + 1];
+              abuf[(2 * gg + h) * (NTP * 16) + j * 16 + col] = A;
+            }
           }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // T I1 of step 8 a + kl (5 a + kl) at position pos of the pass, the offset put back
+        auto a_of = [&](int kl, int pos) -> double {
+          if constexpr (VM == kVm16)
+            return ((abuf[(3 * kl) * TW + pos] + abuf[(3 * kl + 1) * TW + pos]) + abuf[(3 * kl + 2) * TW + pos]) +
+                   vm_offset(VM) * Sad;
+          else if constexpr (VM == kVm8)
+            return abuf[kl * TW + pos] + vm_offset(VM) * Sad;
+          else
+            return abuf[kl * TW + pos];
+        };
         // one candidate (step k, shift sidx) with its Delta sums: num = N A - Sa B and
         // d1 = N C - B^2 exactly (every term an integer below 2^53, patterns of at most
-        // kDoubleExactN pixels), then s = num^2 / d1 = q^2 d0 in FP64 (5 roundings). Only
+        // kDoubleExactN pixels; for 16-bit frames in int64, then rounded to double), then
+        // s = num^2 / d1 = q^2 d0 in FP64 (5 roundings). Only
         // num > 0 can win, as best_NCC starts at 0 (src/p-tracker.cpp:680); d1 = 0 is a flat
         // window, whose reference NCC is NaN. A candidate below the window of the lane's best
         // is dropped here; the rest (a few per lane) go through the out-of-line insertion
         auto consider = [&](int k, int sidx, double A, long long Bm, long long Cm, unsigned int bp, unsigned int bn,
                             unsigned long long cp, unsigned long long cn) {
-          const double B = (double)Bm + ((double)bp - (double)bn);
-          const double C = (double)Cm + ((double)cp - (double)cn);
-          const double nd = Nd * A - Sad * B;
-          const double dd = Nd * C - B * B;
+          double nd, dd;
+          if constexpr (VM == kVm16) {
+            const long long Ai = (long long)A;
+            const long long Bi = Bm + (long long)bp - (long long)bn;
+            const long long Ci = Cm + (long long)cp - (long long)cn;
+            nd = (double)(N * Ai - Sa * Bi);
+            dd = (double)(N * Ci - Bi * Bi);
+          } else {
+            const double B = (double)Bm + ((double)bp - (double)bn);
+            const double C = (double)Cm + ((double)cp - (double)cn);
+            nd = Nd * A - Sad * B;
+            dd = Nd * C - B * B;
+          }
           if (nd > 0.0 && dd > 0.0 && nd * nd >= s1lo * dd) {
             nl = near2_insert(nl, nd, dd, k * S + sidx, lo);
             s1lo = nl.s1 * lo;
@@ -1055,22 +1170,22 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         };
         if constexpr (kUniform) {
           const uint32_t wbase = 2u * (uint32_t)base0;  // byte offset of this lane's window origin
-          for (int kl = 0; kl < 8; kl++) {
-            const int k = 8 * a + kl;
+          for (int kl = 0; kl < SPT; kl++) {
+            const int k = SPT * a + kl;
             if (k >= nsteps) break;
             if ((rep >> k) & 1) continue;  // a repeated step
             unsigned int bp = 0u, bn = 0u;
             unsigned long long cp = 0ull, cn = 0ull;
-            delta_pair(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, k, wbase, base0, bp, cp, bn, cn);
-            if (lane_ok) consider(k, sidx0, abuf[kl * TW + lane], Bm0, Cm0, bp, bn, cp, cn);
+            delta_pair<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, k, wbase, base0, bp, cp, bn, cn);
+            if (lane_ok) consider(k, sidx0, a_of(kl, lane), Bm0, Cm0, bp, bn, cp, cn);
           }
         } else {
-          const int nk = min(8, nsteps - 8 * a);
+          const int nk = min(SPT, nsteps - SPT * a);
           const int nitem = nk > 0 ? nk * vpass : 0;
           for (int it0 = 0; it0 < nitem; it0 += 64) {
             const int it = it0 + lane;
             const int kl = it < nitem ? it / vpass : 0, vl = it < nitem ? it - kl * vpass : 0;
-            const int k = 8 * a + kl, svl = p0 * 16 + vl;
+            const int k = SPT * a + kl, svl = p0 * 16 + vl;
             const bool act = it < nitem && !((rep >> k) & 1);  // a repeated step is skipped
             const int sidx = su * V + svl, base = svl * WW + su;
             long long Bm = 0, Cm = 0;
@@ -1081,9 +1196,10 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
             const uint32_t wbase = 2u * (uint32_t)base;
             unsigned int bp = 0u, bn = 0u;
             unsigned long long cp = 0ull, cn = 0ull;
-            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k, wbase, base, bp, cp);
-            delta_sums(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k + 1, wbase, base, bn, cn);
-            if (act) consider(k, sidx, abuf[kl * TW + vl], Bm, Cm, bp, bn, cp, cn);
+            delta_sums<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k, wbase, base, bp, cp);
+            delta_sums<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k + 1, wbase, base, bn,
+                                    cn);
+            if (act) consider(k, sidx, a_of(kl, vl), Bm, Cm, bp, bn, cp, cn);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1194,7 +1310,7 @@ __global__ __launch_bounds__(64 * kMergeWaves) void k_exact_merge(ExactArgs e) {
   const long long N = mom[0], Sa = mom[1], Saa = mom[2];
   const long long d0 = N * Saa - Sa * Sa;
   if (d0 <= 0) return;
-  const double lo = near_window_lo(N, Sa, d0);
+  const double lo = near_window_lo(N, Sa, d0, e.vmax);
   Near2 m = near2_none();
   const ExactBest* xb = e.xout + (int64_t)li * e.part_cap;
   for (int b = lane; b < e.part_cap; b += 64) {
@@ -1210,7 +1326,7 @@ __global__ __launch_bounds__(64 * kMergeWaves) void k_exact_merge(ExactArgs e) {
     return;
   }
   const bool two = m.i2 != INT_MAX && m.s2 >= m.s1 * lo;
-  if (!two && fabs(q - e.thr) > near_window_q(N, Sa, d0)) {
+  if (!two && fabs(q - e.thr) > near_window_q(N, Sa, d0, e.vmax)) {
     StepBest sb;
     sb.ncc = q;
     sb.idx = m.i1;
@@ -1357,16 +1473,26 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 }
 
 // ---- host side -----------------------------------------------------------------------
-template <int NMT, int NTP>
+template <int NMT, int NTP, int VM>
 static hipError_t launch_main(const ExactArgs& e, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   if (lds > 65536) {
-    hipError_t r = hipFuncSetAttribute((const void*)k_search_exact<NMT, NTP>,
+    hipError_t r = hipFuncSetAttribute((const void*)k_search_exact<NMT, NTP, VM>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (r != hipSuccess) return r;
   }
-  k_search_exact<NMT, NTP><<<grid, block, lds, st>>>(e);
+  k_search_exact<NMT, NTP, VM><<<grid, block, lds, st>>>(e);
   return hipGetLastError();
 }
+
+// The instances: M-tiles x N-tiles per pass for each value mode. 8- and 12-bit frames take
+// up to 6 M-tiles (48 steps), 16-bit frames up to 8 (40 steps, 5 per M-tile).
+#define PTG_EXACT_SHAPES(X, VM)                                                                                \
+  X(1, 3, VM) X(2, 3, VM) X(3, 3, VM) X(4, 2, VM) X(5, 2, VM) X(6, 2, VM) X(1, 2, VM) X(2, 2, VM) X(3, 2, VM) \
+  X(1, 1, VM) X(2, 1, VM) X(3, 1, VM) X(4, 1, VM) X(5, 1, VM) X(6, 1, VM)
+#define PTG_EXACT_INSTANCES(X)                                                                          \
+  PTG_EXACT_SHAPES(X, kVm8) PTG_EXACT_SHAPES(X, kVm12) PTG_EXACT_SHAPES(X, kVm16) X(7, 1, kVm16) \
+  X(8, 1, kVm16)
+constexpr int kExactMaxMt = 8;
 
 // Vector registers and static LDS bytes of a k_search_exact instance (from its code object),
 // for occupancy.
@@ -1374,34 +1500,31 @@ struct MainAttr {
   int vgprs = 256;
   size_t static_lds = 1024;
 };
-static MainAttr main_attr(int nmt, int ntp) {
+static MainAttr main_attr(int vm, int nmt, int ntp) {
   const void* fn = nullptr;
-#define PTG_FN(M, T) \
-  if (nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T>;
-  PTG_FN(1, 3) PTG_FN(2, 3) PTG_FN(3, 3) PTG_FN(4, 2) PTG_FN(5, 2) PTG_FN(6, 2)
-  PTG_FN(1, 2) PTG_FN(2, 2) PTG_FN(3, 2) PTG_FN(1, 1) PTG_FN(2, 1) PTG_FN(3, 1) PTG_FN(4, 1) PTG_FN(5, 1) PTG_FN(6, 1)
+#define PTG_FN(M, T, VM) \
+  if (vm == VM && nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T, VM>;
+  PTG_EXACT_INSTANCES(PTG_FN)
 #undef PTG_FN
-  static MainAttr cache[8][4] = {};
-  static bool known[8][4] = {};
+  static MainAttr cache[3][kExactMaxMt + 1][4] = {};
+  static bool known[3][kExactMaxMt + 1][4] = {};
   if (!fn) return MainAttr{};
-  if (!known[nmt][ntp]) {
+  if (!known[vm][nmt][ntp]) {
     hipFuncAttributes at;
     if (hipFuncGetAttributes(&at, fn) == hipSuccess && at.numRegs > 0) {
-      cache[nmt][ntp].vgprs = at.numRegs;
-      cache[nmt][ntp].static_lds = at.sharedSizeBytes;
+      cache[vm][nmt][ntp].vgprs = at.numRegs;
+      cache[vm][nmt][ntp].static_lds = at.sharedSizeBytes;
     }
-    known[nmt][ntp] = true;
+    known[vm][nmt][ntp] = true;
   }
-  return cache[nmt][ntp];
+  return cache[vm][nmt][ntp];
 }
 
-static hipError_t dispatch_main(int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
+static hipError_t dispatch_main(int vm, int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
                                 hipStream_t st) {
-#define PTG_CASE(M, T) \
-  if (nmt == M && ntp == T) return launch_main<M, T>(e, grid, block, lds, st);
-  PTG_CASE(1, 3) PTG_CASE(2, 3) PTG_CASE(3, 3) PTG_CASE(4, 2) PTG_CASE(5, 2) PTG_CASE(6, 2)
-  PTG_CASE(1, 2) PTG_CASE(2, 2) PTG_CASE(3, 2) PTG_CASE(1, 1) PTG_CASE(2, 1) PTG_CASE(3, 1) PTG_CASE(4, 1)
-  PTG_CASE(5, 1) PTG_CASE(6, 1)
+#define PTG_CASE(M, T, VM) \
+  if (vm == VM && nmt == M && ntp == T) return launch_main<M, T, VM>(e, grid, block, lds, st);
+  PTG_EXACT_INSTANCES(PTG_CASE)
 #undef PTG_CASE
   return hipErrorInvalidValue;
 }
@@ -1432,7 +1555,7 @@ struct Bucket {
 // the search zone's shape only, so a keyed list reuses them across calls.
 struct ExactPlan {
   uint64_t key = 0;
-  int nlist = 0, U = 0, V = 0, nsteps = 0;
+  int nlist = 0, U = 0, V = 0, nsteps = 0, vm = 0;
   int nmt = 0, vp = 0, dcap = 0;
   int64_t dl_total = 0;
   std::vector<ExactSlot> slots;
@@ -1482,12 +1605,13 @@ void free_exact_plans(Context& c) {
 }
 
 static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t>& list, int nlist, int U, int V,
-                       int nsteps) {
+                       int nsteps, int vm) {
   P.nlist = nlist;
   P.U = U;
   P.V = V;
   P.nsteps = nsteps;
-  P.nmt = (2 * nsteps + 15) / 16;
+  P.vm = vm;
+  P.nmt = (nsteps + vm_spt(vm) - 1) / vm_spt(vm);
   P.vp = (V + 15) / 16 * 16;
   const int S = U * V;
   P.slots.assign(nlist, ExactSlot{});
@@ -1585,35 +1709,38 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
                      uint64_t plan_key, double stage_thr, StepBest* out_best, int* nparts_out,
                      const int32_t** flag_dev, bool time_kernel) {
-  // frames of at most 12 bits (7-bit digits of I1 and I1^2), at most 48 rotation steps
+  // the value mode from the frames' declared depth; at most 48 rotation steps (40 for 16-bit
+  // frames)
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
-  if (i0.bits > 12 || i1.bits > 12) return kExactUnfit;
+  const int bits = std::max(i0.bits, i1.bits);
+  const int vm = bits <= 8 ? kVm8 : bits <= 12 ? kVm12 : kVm16;
   const int nsteps = sa.nsteps;
-  const int nmt = (2 * nsteps + 15) / 16;
-  if (nmt > 6) return kExactUnfit;
+  const int nmt = (nsteps + vm_spt(vm) - 1) / vm_spt(vm);
+  if (nmt > (vm == kVm16 ? kExactMaxMt : 6)) return kExactUnfit;
   const int U = sa.left + sa.right + 1, V = sa.up + sa.down + 1;
   if ((int64_t)nsteps * U * V >= INT_MAX / 2) return kExactUnfit;
-  // N-tiles per pass: 3 (2 for more than 3 M-tiles, the accumulators' register budget), and
-  // no more than the vertical shifts need (a zone of +-8 has 17: 2 tiles, not 3)
-  const int ntp = std::min(nmt <= 3 ? 3 : 2, (V + 15) / 16);
+  // N-tiles per pass: 3 (2 for more than 3 M-tiles, 1 for more than 6: the accumulators'
+  // register budget), and no more than the vertical shifts need (a zone of +-8 has 17: 2
+  // tiles, not 3)
+  const int ntp = std::min(nmt <= 3 ? 3 : nmt <= 6 ? 2 : 1, (V + 15) / 16);
   // the host plan: cached for a keyed list, else built for this call
   ExactPlan local;
   ExactPlan* P = &local;
   if (plan_key != 0) {
     ExactPlan* cached = static_cast<ExactPlan*>(c.exact_plan);
     if (!cached || cached->key != plan_key || cached->nlist != nlist || cached->U != U || cached->V != V ||
-        cached->nsteps != nsteps) {
+        cached->nsteps != nsteps || cached->vm != vm) {
       if (!cached) {
         cached = new ExactPlan();
         c.exact_plan = cached;
       }
-      build_plan(c, *cached, list, nlist, U, V, nsteps);
+      build_plan(c, *cached, list, nlist, U, V, nsteps, vm);
       cached->key = plan_key;
       c.exact_dev_key = 0;
     }
     P = cached;
   } else {
-    build_plan(c, local, list, nlist, U, V, nsteps);
+    build_plan(c, local, list, nlist, U, V, nsteps, vm);
   }
   for (const Bucket& bk : P->buckets)
     if (!bk.slots.empty() && bk.rho_max > 120)  // pattern words hold offsets as int8
@@ -1642,7 +1769,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     // disk base masks from row prefix sums: all vertical shifts per workgroup when that
     // keeps two workgroups per CU, else as many as the LDS holds (none: k_exact_mid's)
     l.vb = 0;
-    if (bk.ndisk > 0) {
+    if (bk.ndisk > 0 && vm != kVm16) {  // (its prefix sums of I1^2 hold 12-bit values)
       int vb = V;
       while (vb > 1 && base_disk_lds_bytes(rho, U, vb) > 80 * 1024) vb--;
       if (vb < 8 && vb < V) {
@@ -1660,8 +1787,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     l.ww = win_width(U, cw);
     l.wh = P->vp + 2 * rho;
     const size_t win_bytes = ((size_t)l.ww * l.wh * 2 + 15) / 16 * 16;
-    const size_t stage_bytes = ((size_t)2 * l.wh * l.cs + 15) / 16 * 16;
-    const size_t abuf_bytes = (size_t)8 * (ntp * 16) * 8;
+    const size_t stage_bytes = ((size_t)vm_planes(vm) * l.wh * l.cs + 15) / 16 * 16;
+    const size_t abuf_bytes = (size_t)(vm == kVm16 ? 16 : 8) * (ntp * 16) * 8;
     l.alias = P->vp / 16 <= ntp && abuf_bytes <= stage_bytes;
     const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
     // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
@@ -1677,7 +1804,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int cpb = (U + l.nsub - 1) / l.nsub;
     // waves per workgroup: the most waves resident per CU under the register limit (a
     // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
-    const MainAttr ma = main_attr(nmt, ntp);
+    const MainAttr ma = main_attr(vm, nmt, ntp);
     const int wave_cap = 4 * std::max(1, 512 / std::max(ma.vgprs, 1));
     int best_res = 0;
     l.nsu = 1;
@@ -1734,6 +1861,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   if (!plain_ok) e.ovf_slot = e.ovf_list = nullptr;
   e.dcap = P->dcap;
   e.nmt = nmt;
+  e.vmode = vm;
+  e.vmax = vm_vmax(vm);
   e.vp = P->vp;
   hipError_t he;
   // the tables: uploaded unless the scratch already holds this keyed plan's
@@ -1831,7 +1960,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
         (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)l.lds_t)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
-    k_exact_template<<<dim3(nb, 8 * nmt), 256, l.lds_t, prep>>>(e);
+    k_exact_template<<<dim3(nb, vm_spt(vm) * nmt), 256, l.lds_t, prep>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
     if ((he = hipEventRecord(c.aux_events[bi], prep)) != hipSuccess) return c.check(he, "hipEventRecord");
   }
@@ -1869,7 +1998,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
     hipEvent_t ev = nullptr;
     if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev, sx);
-    if ((he = dispatch_main(nmt, ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
+    if ((he = dispatch_main(vm, nmt, ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev, sx);
   }

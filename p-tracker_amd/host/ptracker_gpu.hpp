@@ -14,6 +14,7 @@
 #include <iomanip>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ptgpu.h"
@@ -244,6 +245,14 @@ class ptracker_gpu {
     log << std::endl;
     for (int q = 0; q < 14; q++) s.disto_parameters[q] = X[q];
     return X;
+  }
+
+  // Searches (whole stages) and grains the exact form has handed to the plain FP64 form
+  // since the tracker was created (the frames' depth, zone or pattern beyond its limits).
+  std::pair<int64_t, int64_t> exact_fallbacks() {
+    int64_t searches = 0, grains = 0;
+    check(ptg_get_exact_fallbacks(ctx_, &searches, &grains), "ptg_get_exact_fallbacks");
+    return {searches, grains};
   }
 
  private:

@@ -44,6 +44,9 @@ int main(int argc, char** argv) {
     } else if (s.procedure == "particle_tracking") {
       const int n = pt.particle_tracking(s);
       std::cout << "tracked " << pt.grain.size() << " grains over " << n << " images" << std::endl;
+      const auto fb = pt.exact_fallbacks();
+      std::cout << "plain-form fallbacks of the exact search: " << fb.first << " searches, " << fb.second
+                << " grains" << std::endl;
     } else {
       std::cerr << "procedure " << s.procedure << " is not part of this program" << std::endl;
       return 1;

@@ -74,9 +74,16 @@ def test_example_program_matches_oracle(tmp_path, subpix):
     sp = pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13) if subpix == "lm" else pk.default_subpix(pk.SUBPIX_POWELL)
     prm = pk.track_params(z, zr, zs, NCC_min=0.8, NCC_min_super=0.7, rescue_level=2, subpixel=1, subpix=sp)
     # images 1 -> 1, then 1 -> 2 from the first pair's grains (iref 1, ibeg 1, iend 2)
-    o1, _, _ = ptref.track_pair(ptref.Workload(I1, I1, g, pxy, pb, 16), prm, nthreads=8)
+    o1, counts1, _ = ptref.track_pair(ptref.Workload(I1, I1, g, pxy, pb, 16), prm, nthreads=8)
     o2, counts, _ = ptref.track_pair(ptref.Workload(I1, I2, o1, pxy, pb, 16), prm, nthreads=8)
     assert counts[0] == 1  # one grain goes to the rescue search
+    # the 16-bit frames' main searches (+-6 px x 13 steps) run on the exact MFMA kernels with
+    # no grain handed over; the rescue stages (139 steps, beyond the exact form's 40 for 16-bit
+    # frames) run wholly on the plain form, one search per stage with a non-empty queue
+    fb = [l for l in r.stdout.splitlines() if l.startswith("plain-form fallbacks of the exact search:")]
+    assert len(fb) == 1, r.stdout
+    nsearch, ngrain = (int(t) for t in fb[0].split(":")[1].replace("searches", "").replace("grains", "").split(","))
+    assert ngrain == 0 and nsearch == sum(int(c[0] > 0) + int(c[1] > 0) for c in (counts1, counts)), fb[0]
     tol = {"dx": 1e-3, "dy": 1e-3, "drot": 1e-5, "upix": 1e-3, "vpix": 1e-3, "rot_inc": 1e-5, "NCC": 1e-12,
            "NCC_rescue": 1e-12, "NCC_subpix": 1e-6}
     for num, ref in ((1, o1), (2, o2)):

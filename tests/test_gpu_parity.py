@@ -324,15 +324,18 @@ def test_rescue_exact_picks_equal_oracle(tracker):
     assert np.abs(got["NCC_rescue"] - ref["NCC_rescue"]).max() < 1e-12
 
 
-@pytest.mark.parametrize("bits", [12, 8])
+@pytest.mark.parametrize("bits", [12, 8, 16])
 def test_cfg3_exact_40_steps_picks_equal_oracle(tracker, bits):
     # config 3: +-20 px x 20 num_rot at inc 0.005, which is 40 steps in the reference's
-    # loop; 12-bit frames and their 8-bit requantisation (v >> 4). Zero argmax mismatches.
+    # loop; 12-bit frames, their 8-bit requantisation (v >> 4: one offset-INT8 plane) and
+    # their spread over 16 bits (8 M-tiles of 5 steps). Zero argmax mismatches.
     w = synth.workload(3, n_grains=20, size=(1000, 800), seed=3033)
     if bits == 8:
         w.I0 = (w.I0 >> 4).astype(np.uint16)
         w.I1 = (w.I1 >> 4).astype(np.uint16)
         w.bits = 8
+    elif bits == 16:
+        w.I0, w.I1, w.bits = to16(w.I0, 330), to16(w.I1, 331), 16
     z = w.params.search_zone
     assert len(ptref.rotation_steps(z.inc_rot, z.num_rot)) == 40
     prm = pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT)
@@ -348,22 +351,70 @@ def test_cfg3_exact_40_steps_picks_equal_oracle(tracker, bits):
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
 
 
-def test_exact_falls_back_to_plain_on_16bit_frames(tracker, cfg1):
-    # 16-bit frames are beyond the exact form's 7-bit digits: the search runs the plain
-    # form, whose picks and NCC are the reference's
-    say("exact search on 16-bit frames runs the plain form")
-    tracker.set_image(0, cfg1.I0, 16)
-    tracker.set_image(1, cfg1.I1, 16)
+def to16(img, seed):
+    """12-bit frame data spread over the whole 16-bit range: v * 16 + U{0, ..., 15}."""
+    rng = np.random.default_rng(seed)
+    return (img.astype(np.uint32) * 16 + rng.integers(0, 16, img.shape)).astype(np.uint16)
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2s"])
+def test_exact_16bit_frames_picks_equal_oracle(tracker, which, request):
+    # 16-bit frames on the matrix cores (the window as two offset bytes, the template as three
+    # 7-bit digits, num and d1 in int64): config 1's 8-bit data declared 16-bit, and the
+    # config-2 texture spread over 0 .. 65535; no grain leaves the exact kernels
+    w = request.getfixturevalue(which)
+    I0, I1 = (w.I0, w.I1) if which == "cfg1" else (to16(w.I0, 160), to16(w.I1, 161))
+    say(f"exact search on 16-bit frames, {which}: G={w.G}, values up to {int(max(I0.max(), I1.max()))}")
+    tracker.set_image(0, I0, 16)
+    tracker.set_image(1, I1, 16)
     tracker.set_image_indices(0, 1)
-    tracker.set_grains(cfg1.grains, cfg1.pattern_xy, cfg1.pattern_begin)
+    tracker.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
     n0 = tracker.exact_fallbacks()
-    z = cfg1.params.search_zone
+    z = w.params.search_zone
     tracker.track_pair(pk.track_params(z, rescue_level=0, subpixel=0, search_method=pk.SEARCH_EXACT))
     got = tracker.get_grains()
-    assert tracker.exact_fallbacks()[0] == n0[0] + 1
-    ref, _, _ = ptref.track_pair(oracle_workload(cfg1), pk.track_params(z, rescue_level=0, subpixel=0), nthreads=4)
-    for col in INT_COLS:
+    assert tracker.exact_fallbacks() == n0
+    W = ptref.Workload(I0, I1, w.grains, w.pattern_xy, w.pattern_begin, 16)
+    ref, _, _ = ptref.track_pair(W, pk.track_params(z, rescue_level=0, subpixel=0), nthreads=8)
+    for col in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
         assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+
+
+def test_exact_16bit_rescue_and_super_rescue_equal_oracle(tracker):
+    # the rescue stages on 16-bit frames, with disks and squares (the squares' base mask is
+    # their middle rotation step, from the base-mask GEMM on the offset byte planes)
+    w = synth.workload(2, n_grains=24, size=(800, 700), max_disp=6.0, seed=77)
+    I0, I1 = to16(w.I0, 770), to16(w.I1, 771)
+    pats = []
+    for g in range(w.G):
+        P = w.pattern_xy[w.pattern_begin[g]:w.pattern_begin[g + 1]]
+        if g % 3 == 1:
+            r = math.isqrt(int((P.astype(np.int64) ** 2).sum(1).max())) // 2
+            ys, xs = np.mgrid[-r:r + 1, -r:r + 1]
+            P = np.stack([xs.ravel(), ys.ravel()], 1)
+        pats.append(P.astype(np.int32))
+    pxy = np.concatenate(pats).astype(np.int32)
+    pb = np.concatenate([[0], np.cumsum([len(p) for p in pats])]).astype(np.int64)
+    z, zr, zs = pk.zone(2, num_rot=2, inc_rot=0.02), pk.zone(4, num_rot=2, inc_rot=0.02), pk.zone(8, num_rot=1, inc_rot=0.05)
+    prm = pk.track_params(z, zr, zs, NCC_min=0.7, NCC_min_super=0.6, rescue_level=2, subpixel=0,
+                          search_method=pk.SEARCH_EXACT)
+    say("rescue + super-rescue exact, 16-bit frames, disks and squares")
+    tracker.set_image(0, I0, 16)
+    tracker.set_image(1, I1, 16)
+    tracker.set_image_indices(0, 1)
+    tracker.set_grains(w.grains, pxy, pb)
+    n0 = tracker.exact_fallbacks()
+    tracker.track_pair(prm)
+    got = tracker.get_grains()
+    assert tracker.exact_fallbacks() == n0
+    ref, counts, _ = ptref.track_pair(ptref.Workload(I0, I1, w.grains, pxy, pb, 16), prm, nthreads=4)
+    assert counts[0] > 0 and counts[1] > 0
+    assert tracker.queue_counts() == counts
+    for col in ("dx", "dy", "drot"):
+        assert bits_equal(got[col], ref[col]), col
+    assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+    assert np.abs(got["NCC_rescue"] - ref["NCC_rescue"]).max() < 1e-12
 
 
 # ---- frame sequences (batching form) -------------------------------------------------------
@@ -471,13 +522,17 @@ def test_exact_overflowing_grains_go_to_plain_kernel(tracker):
     assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
 
 
-def test_exact_hands_values_beyond_12_bits_to_plain_kernel(tracker):
-    # frames declared 12-bit whose right half holds 16x larger values (12-bit data stored
-    # shifted up in 16-bit words): the exact kernels find window values above 4095 while
-    # staging and send those grains to the plain kernel (disks through the prefix-sum
+@pytest.mark.parametrize("bits", [12, 8])
+def test_exact_hands_values_beyond_the_declared_depth_to_plain_kernel(tracker, bits):
+    # frames declared 12-bit (8-bit) whose right half holds 16x larger values (data stored
+    # shifted up in wider words): the exact kernels find window values above 4095 (255)
+    # while staging and send those grains to the plain kernel (disks through the prefix-sum
     # base mask, squares through the base-mask GEMM); every pick is the reference's
     w = synth.workload(2, n_grains=24, size=(900, 700), seed=313)
     I0, I1 = w.I0.copy(), w.I1.copy()
+    if bits == 8:
+        I0 >>= 4
+        I1 >>= 4
     half = I0.shape[1] // 2
     I0[:, half:] *= 16
     I1[:, half:] *= 16
@@ -495,9 +550,9 @@ def test_exact_hands_values_beyond_12_bits_to_plain_kernel(tracker):
     left = w.grains["refcoord_xpix"] + 80 < half
     assert right.any() and left.any()
     z = w.params.search_zone
-    say(f"exact search, 12-bit frames with 16-bit values in the right half: G={w.G}")
-    tracker.set_image(0, I0, 12)
-    tracker.set_image(1, I1, 12)
+    say(f"exact search, {bits}-bit frames with {bits + 4}-bit values in the right half: G={w.G}")
+    tracker.set_image(0, I0, bits)
+    tracker.set_image(1, I1, bits)
     tracker.set_image_indices(0, 1)
     tracker.set_grains(w.grains, pxy, pb)
     n0 = tracker.exact_fallbacks()

@@ -69,28 +69,34 @@ def ncc_error_bound(N, R0, R1):
     return 2.0 * (2.0 * U * (R0 + R1) + 7.0 * U + 2.0 * gN) + 8.0 * U * U * (R0 + sn) * (R1 + sn)
 
 
-@pytest.mark.parametrize("case", ["textured", "high_mean_low_variance", "large_pattern"])
+@pytest.mark.parametrize("case", ["textured", "high_mean_low_variance", "large_pattern", "sixteen_bit"])
 def test_reference_ncc_error_within_the_search_window_bound(case):
     # every candidate's reference NCC (the oracle's two-pass FP64 sums) against its exact value
     # num / sqrt(d0 d1): within half the bound (the bound is doubled for margin), and the FP64
-    # value the exact search forms from the exact integers within 4u
+    # value the exact search forms from the exact integers within 4u (6u for 16-bit frames,
+    # whose num and d1 reach 2^61 and are rounded to FP64 first)
     getcontext().prec = 60
-    rng = np.random.default_rng({"textured": 1, "high_mean_low_variance": 2, "large_pattern": 3}[case])
+    rng = np.random.default_rng({"textured": 1, "high_mean_low_variance": 2, "large_pattern": 3,
+                                 "sixteen_bit": 4}[case])
     H, W = 90, 90
+    bits = 12
     if case == "textured":
         r, I0 = 6, rng.integers(0, 4096, size=(H, W))
         I1 = np.clip(np.roll(I0, (1, 2), (0, 1)) + rng.integers(-40, 41, size=(H, W)), 0, 4095)
     elif case == "high_mean_low_variance":
         r, I0 = 6, 4000 + rng.integers(0, 3, size=(H, W))
         I1 = 4000 + rng.integers(0, 3, size=(H, W))
-    else:
+    elif case == "large_pattern":
         r, I0 = 25, 3000 + rng.integers(0, 50, size=(H, W))
         I1 = np.clip(np.roll(I0, (2, -1), (0, 1)) + rng.integers(-20, 21, size=(H, W)), 0, 4095)
+    else:
+        r, bits, I0 = 12, 16, rng.integers(0, 65536, size=(H, W))
+        I1 = np.clip(np.roll(I0, (1, -2), (0, 1)) + rng.integers(-600, 601, size=(H, W)), 0, 65535)
     I0, I1 = I0.astype(np.uint16), I1.astype(np.uint16)
     pat = ptref.circ_pattern(r).astype(np.int32)
     gr = np.zeros(1, dtype=pk.GRAIN_DTYPE)
     gr["refcoord_xpix"], gr["refcoord_ypix"] = W // 2, H // 2
-    w = ptref.Workload(I0, I1, gr, pat, np.array([0, len(pat)], dtype=np.int64), 12)
+    w = ptref.Workload(I0, I1, gr, pat, np.array([0, len(pat)], dtype=np.int64), bits)
     gp, p0 = ptref.precompute(w)
     z = pk.zone(8, num_rot=0, inc_rot=1.0)
     fp = ptref.candidate_ncc(w, gp, p0, 0, z)
@@ -109,7 +115,7 @@ def test_reference_ncc_error_within_the_search_window_bound(case):
             assert err <= Decimal(tau / 2), (u, v, float(err), tau)
             worst = max(worst, float(err) / tau)
             q = float(num) / math.sqrt(float(d0) * float(d1))
-            assert abs(Decimal(q) - rho) <= Decimal(4 * U) * abs(rho) + Decimal(1e-300)
+            assert abs(Decimal(q) - rho) <= Decimal((6 if bits == 16 else 4) * U) * abs(rho) + Decimal(1e-300)
             k += 1
     print(f"{case}: largest error / bound {worst:.3g}")
     assert worst > 0.0  # the reference's rounding is visible, and the bound covers it
