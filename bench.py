@@ -27,6 +27,7 @@ import json
 import os
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -312,6 +313,67 @@ def rescue_variant(pk, synth, args):
     return out
 
 
+def depth_variants(pk, w, args, flops):
+    """The same pair requantised to 8 bits (v >> 4: one offset-INT8 digit plane) and spread
+    over 16 bits (v * 16 + U{0..15}: two offset-byte planes, three template digits): the fast
+    form's pair time, its main-stage search kernel on one stream (HIP events) against the
+    int8 peak, and its integer picks against the plain form's."""
+    out = {}
+    rng = np.random.default_rng(16)
+    for bits in (8, 16):
+        if bits == 8:
+            I0, I1 = (w.I0 >> 4).astype(np.uint16), (w.I1 >> 4).astype(np.uint16)
+        else:
+            I0 = (w.I0.astype(np.uint32) * 16 + rng.integers(0, 16, w.I0.shape)).astype(np.uint16)
+            I1 = (w.I1.astype(np.uint32) * 16 + rng.integers(0, 16, w.I1.shape)).astype(np.uint16)
+        wd = types.SimpleNamespace(I0=I0, I1=I1, bits=bits, grains=w.grains, pattern_xy=w.pattern_xy,
+                                   pattern_begin=w.pattern_begin, params=w.params)
+        tr = pk.Tracker(0)
+        tr.set_image(0, I0, bits)
+        tr.set_image(1, I1, bits)
+        tr.set_image_indices(0, 1)
+        tr.set_grains(w.grains, w.pattern_xy, w.pattern_begin)
+        z = w.params
+        prm = pk.track_params(z.search_zone, z.search_zone_rescue, z.search_zone_super_rescue, NCC_min=0.7,
+                              rescue_level=1, subpixel=1, search_method=pk.SEARCH_EXACT,
+                              subpix=pk.default_subpix(pk.SUBPIX_LM))
+        steps = max(1, args.steps)
+        f0 = tr.exact_fallbacks()
+        for serial in (False, True):
+            tr.set_option(pk.OPT_SEARCH_STREAMS, 1 if serial else 2)
+            tr.put_grains(w.grains)
+            tr.track_pair(prm)
+            tr.sync()
+            tr.set_timing(True)
+            tr.reset_stage_times()
+            t = time.perf_counter()
+            for _ in range(steps):
+                tr.put_grains(w.grains)
+                tr.track_pair(prm)
+                tr.get_grains()
+            tr.sync()
+            el = time.perf_counter() - t
+            ms, _ = tr.stage_times()
+            tr.set_timing(False)
+            if not serial:
+                rec = {"ms_per_pair": 1e3 * el / steps, "grains_per_s": w.G * steps / el,
+                       "stage_ms": {"search_main": ms[pk.T_SEARCH_MAIN] / steps, "subpixel": ms[pk.T_SUBPIXEL] / steps}}
+            else:
+                k_ms = ms[pk.T_SEARCH_KERNEL] / steps
+                ach = flops / (k_ms * 1e-3) / 1e12
+                rec["search_kernel"] = {"ms_per_pair_one_stream": k_ms, "achieved_tops": ach,
+                                        "frac_of_int8_peak": ach / INT8_PEAK_TOPS}
+        f1 = tr.exact_fallbacks()
+        rec["exact_fallbacks"] = {"searches": int(f1[0] - f0[0]), "grains": int(f1[1] - f0[1])}
+        rec["picks_vs_plain"] = pick_mismatches(pk, tr, wd)
+        tr.close()
+        out[f"{bits}bit"] = rec
+        log(f"{bits}-bit frames: exact+lm {rec['ms_per_pair']:.2f} ms/pair, search kernel "
+            f"{rec['search_kernel']['ms_per_pair_one_stream']:.2f} ms ({rec['search_kernel']['achieved_tops']:.0f} TOPS), "
+            f"pick mismatches vs plain {rec['picks_vs_plain']['mismatches']}, fallbacks {rec['exact_fallbacks']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,6 +388,8 @@ def main():
                     help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU the process may use)")
     ap.add_argument("--no-rescue-variant", action="store_true",
                     help="skip the rescue-zone variant of the pair (300 grains moved beyond the main zone)")
+    ap.add_argument("--no-depths", action="store_true",
+                    help="skip the pair requantised to 8 bits and spread over 16 bits")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the exact-vs-plain pick comparison (configs 2 and 3) after the timed run")
     ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
@@ -526,6 +590,9 @@ def main():
         log("exact vs plain integer picks: " + ", ".join(f"{k}: {v['mismatches']} of {v['grains']}" for k, v in prec.items()))
     else:
         tr.close()
+
+    if method == "exact" and not args.no_depths:
+        result["bit_depths"] = depth_variants(pk, w, args, flops)
 
     if not args.no_rescue_variant:
         result["rescue_variant"] = rescue_variant(pk, synth, args)
