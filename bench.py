@@ -313,6 +313,37 @@ def rescue_variant(pk, synth, args):
     return out
 
 
+def autotune_section(pk, tr, w, prm, step, args):
+    """The exact search's autotuner on this pair: per-pair wall times (median, mean, largest)
+    with the built-in launch shapes and with the tuned ones, in the same process, and the
+    main-stage search span the tuner measured for both."""
+    def times(n):
+        for _ in range(2):
+            step(prm)
+        tr.sync()
+        ts = []
+        for _ in range(n):
+            t = time.perf_counter()
+            step(prm)
+            ts.append(1e3 * (time.perf_counter() - t))
+        tr.sync()
+        ts = np.array(ts)
+        return {"median": float(np.median(ts)), "mean": float(ts.mean()), "max": float(ts.max())}
+    n = max(10, args.steps)
+    tr.clear_tuning()
+    heur = times(n)
+    t5 = time.perf_counter()
+    n_t, ms_h, ms_t = tr.autotune_search(w.params.search_zone)
+    sec = time.perf_counter() - t5
+    tuned = times(n)
+    tr.clear_tuning()
+    out = {"searches_run": n_t, "seconds": sec, "search_span_ms_heuristic_shapes": ms_h,
+           "search_span_ms_tuned_shapes": ms_t, "pair_ms_heuristic_shapes": heur, "pair_ms_tuned_shapes": tuned}
+    log(f"autotune: {n_t} searches in {sec:.1f} s; search span {ms_h:.3f} -> {ms_t:.3f} ms; pair median "
+        f"{heur['median']:.3f} -> {tuned['median']:.3f} ms, mean {heur['mean']:.3f} -> {tuned['mean']:.3f} ms")
+    return out
+
+
 def depth_variants(pk, w, args, flops):
     """The same pair requantised to 8 bits (v >> 4: one offset-INT8 digit plane) and spread
     over 16 bits (v * 16 + U{0..15}: two offset-byte planes, three template digits): the fast
@@ -388,6 +419,8 @@ def main():
                     help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU the process may use)")
     ap.add_argument("--no-rescue-variant", action="store_true",
                     help="skip the rescue-zone variant of the pair (300 grains moved beyond the main zone)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="also autotune the exact search's launch shapes and time the tuned pairs")
     ap.add_argument("--no-depths", action="store_true",
                     help="skip the pair requantised to 8 bits and spread over 16 bits")
     ap.add_argument("--no-precision", action="store_true",
@@ -459,6 +492,7 @@ def main():
         tr.put_grains(w.grains)
         tr.track_pair(p)
         return tr.get_grains()
+
 
     for _ in range(args.warmup):
         step(prm)
@@ -557,6 +591,8 @@ def main():
         "stage_ms": stage_ms,
         "roofline": roof,
     }
+    if method == "exact" and args.autotune:
+        result["autotune"] = autotune_section(pk, tr, w, prm, step, args)
 
     # the plain form (FP64 two-pass search + Powell) on the same workload
     if args.plain_steps > 0 and not (method == "plain" and subpix == "powell"):

@@ -285,6 +285,21 @@ int ptg_set_timing(void* ctx, int enable);
  *  that per-launch timings (PTG_T_SEARCH_KERNEL, a profiler's trace) do not overlap. */
 enum { PTG_OPT_SEARCH_STREAMS = 0 };
 int ptg_set_option(void* ctx, int option, int64_t value);
+/* The exact search's autotuner; no counterpart in the reference (it replaces the launch
+ * shapes of the built-in heuristic, which picks them from the LDS and register budgets).
+ * For each radius bucket of a main-stage search over `zone` on the current frames and
+ * grains, tries each candidate shape: N-tiles of 16 vertical shifts per pass (1-3), waves per
+ * workgroup (1-8) and workgroups per grain (1, 2, 4), timing the whole search's GPU span
+ * (HIP events on the context's stream, the buckets overlapped as in any search); a shape
+ * replaces the heuristic's when the median of three spans is shorter, and is kept under the
+ * bucket's key (value mode, rotation steps, zone, largest pattern radius, grain count).
+ * Later exact searches with a bucket of the same key use it; picks and NCC do not depend on
+ * the shape. The grains are left as they were. ntried: searches run; ms_before, ms_after:
+ * the search's span with the heuristic's shapes and with the chosen ones (medians of three;
+ * any output may be NULL). */
+int ptg_autotune_search(void* ctx, const ptg_search_zone* zone, int32_t* ntried, double* ms_before, double* ms_after);
+/* Forgets every autotuned shape (the heuristic's shapes apply again). */
+int ptg_clear_tuning(void* ctx);
 /* Accumulated milliseconds and launch counts since the last reset (synchronises). */
 int ptg_get_stage_times(void* ctx, double* ms, int64_t* launches, int n);
 int ptg_reset_stage_times(void* ctx);

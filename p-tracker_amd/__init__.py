@@ -147,6 +147,9 @@ def load_library(path: str = LIB_PATH):
         "ptg_remap": ([vp, c_int, c_int, vp], c_int),
         "ptg_set_timing": ([vp, c_int], c_int),
         "ptg_set_option": ([vp, c_int, c_int64], c_int),
+        "ptg_autotune_search": ([vp, POINTER(SearchZone), POINTER(c_int32), POINTER(c_double), POINTER(c_double)],
+                                c_int),
+        "ptg_clear_tuning": ([vp], c_int),
         "ptg_get_stage_times": ([vp, vp, vp, c_int], c_int),
         "ptg_reset_stage_times": ([vp], c_int),
         "ptg_get_subpix_evals": ([vp, POINTER(c_int64)], c_int),
@@ -174,7 +177,7 @@ def exported_symbols():
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
-            "ptg_set_timing", "ptg_set_option", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
+            "ptg_set_timing", "ptg_set_option", "ptg_autotune_search", "ptg_clear_tuning", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
             "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
             "ptg_equiproj_eval", "ptg_calibrate_distortion", "ptg_joint_eval", "ptg_refine_joint_distortion"]
 
@@ -389,6 +392,18 @@ class Tracker:
 
     def set_option(self, option: int, value: int):
         self._ck(self.lib.ptg_set_option(self.h, option, value), "ptg_set_option")
+
+    def autotune_search(self, z: SearchZone):
+        """Times the exact search's launch shapes per radius bucket of a main-stage search over
+        z and keeps the fastest; returns (searches run, kernel ms with the heuristic's shapes,
+        kernel ms with the chosen ones)."""
+        n, b, a = c_int32(), c_double(), c_double()
+        self._ck(self.lib.ptg_autotune_search(self.h, ctypes.byref(z), ctypes.byref(n), ctypes.byref(b),
+                                              ctypes.byref(a)), "ptg_autotune_search")
+        return n.value, b.value, a.value
+
+    def clear_tuning(self):
+        self._ck(self.lib.ptg_clear_tuning(self.h), "ptg_clear_tuning")
 
     def set_timing(self, on: bool):
         self._ck(self.lib.ptg_set_timing(self.h, 1 if on else 0), "ptg_set_timing")

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -92,6 +93,18 @@ static hipError_t fill_i32(Context& c, int32_t* p, int n, int32_t v) {
 __global__ void k_fill_i64(long long* p, int n, long long v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+hipError_t ensure_kernel_lds(int dev, const void* fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> set;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = set[{dev, fn}];
+  if (bytes <= have) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) have = bytes;
+  return e;
 }
 
 }  // namespace ptg
@@ -195,6 +208,7 @@ struct Ctx : Context {
 };
 
 Ctx* as_ctx(void* p) { return reinterpret_cast<Ctx*>(p); }
+
 
 #define CK(c, expr)                                                  \
   do {                                                               \
@@ -524,7 +538,7 @@ int ptg_destroy(void* p) {
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
-                  c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
+                  c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
                   c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac};
   for (void* b : more)
     if (b) (void)hipFree(b);
@@ -537,6 +551,8 @@ int ptg_destroy(void* p) {
     (void)hipEventDestroy(pe.b);
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->tune_ev)
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->aux_events) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->search_joins) (void)hipEventDestroy(e);
   for (hipStream_t st : c->search_streams) (void)hipStreamDestroy(st);
@@ -923,6 +939,112 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
   TRY(ptg_end_pair(p));
   // a stage that could not be served exactly is reported here, after the pair's work
   if (c->any_flags_pending()) TRY(c->sync_and_check());
+  return PTG_OK;
+}
+
+// The exact search's autotuner: for each radius bucket of the main stage, every launch shape
+// of a candidate set (N-tiles per pass 1-3, waves per workgroup 1-8, workgroups per grain 1, 2
+// or 4) is tried on the current frames and grains, and the whole search's GPU span (events on
+// the library stream, the radius buckets overlapped over their streams as in any search) is
+// timed; a shape replaces the heuristic's only when the median of three spans is shorter. The
+// grains and the fallback counters are left as they were.
+int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, double* ms_before, double* ms_after) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!z) return c->fail(PTG_ERR_ARG, "null search zone");
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  int32_t tried = 0;
+  double before = 0.0, after = 0.0;
+  if (c->G > 0) {
+    if (c->require_precomputations) TRY(do_precompute(c));
+    const size_t gb = sizeof(ptg_grain) * (size_t)c->G;
+    TRY(c->ensure(&c->d_tune_save, &c->tune_save_cap, gb));
+    CK(c, hipMemcpyAsync(c->d_tune_save, c->d_grains, gb, hipMemcpyDeviceToDevice, c->stream));
+    const int64_t fb0 = c->exact_fallbacks, fg0 = c->exact_grain_fallbacks;
+    c->tune_run = true;
+    // one main-stage search from the saved grains: its GPU span in ms
+    auto run = [&](float* ms) -> int {
+      CK(c, hipMemcpyAsync(c->d_grains, c->d_tune_save, gb, hipMemcpyDeviceToDevice, c->stream));
+      c->snap = Ctx::SNAP_INVALID;
+      TRY(ptg_begin_pair(p));
+      c->tune_keys.clear();
+      TRY(do_search(c, PTG_STAGE_MAIN, z, 0.7, 0.5, PTG_SEARCH_EXACT));
+      TRY(c->sync_and_check());
+      tried++;
+      *ms = 0.0f;
+      if (!c->tune_keys.empty()) CK(c, hipEventElapsedTime(ms, c->tune_ev[0], c->tune_ev[1]));
+      return PTG_OK;
+    };
+    auto median3 = [&](float* ms) -> int {
+      float t[3];
+      for (int r = 0; r < 3; r++) TRY(run(&t[r]));
+      std::sort(t, t + 3);
+      *ms = t[1];
+      return PTG_OK;
+    };
+    auto body = [&]() -> int {
+      c->tune_bucket = -1;
+      float ms = 0.0f;
+      TRY(run(&ms));  // the buckets and their keys
+      if (c->tune_keys.empty()) return PTG_OK;  // the search ran the plain form: nothing to tune
+      for (uint64_t k : c->tune_keys) c->exact_tuned.erase(k);
+      const std::vector<uint64_t> keys = c->tune_keys;
+      float best_ms = 0.0f;
+      TRY(median3(&best_ms));  // the heuristic's shapes
+      before = best_ms;
+      for (size_t bi = 0; bi < keys.size(); bi++) {
+        if (keys[bi] == 0) continue;  // an empty bucket
+        c->tune_bucket = -1;
+        TRY(run(&ms));
+        const ExactShape base_shape = c->tune_used[bi];
+        ExactShape best = base_shape;
+        for (int ntp = 1; ntp <= 3; ntp++)
+          for (int nsub : {1, 2, 4})
+            for (int nsu = 1; nsu <= 8; nsu++) {
+              const ExactShape want{ntp, nsu, nsub};
+              if (want.ntp == base_shape.ntp && want.nsu == base_shape.nsu && want.nsub == base_shape.nsub) continue;
+              c->tune_bucket = (int)bi;
+              c->tune_shape = want;
+              TRY(run(&ms));
+              const ExactShape& got = c->tune_used[bi];
+              if (got.ntp != want.ntp || got.nsu != want.nsu || got.nsub != want.nsub) continue;  // did not fit
+              if (ms < best_ms) {
+                TRY(median3(&ms));
+                if (ms < best_ms) {
+                  best_ms = ms;
+                  best = want;
+                }
+              }
+            }
+        // later buckets are tried with this one's choice in place
+        c->exact_tuned[keys[bi]] = best;
+      }
+      c->tune_bucket = -1;
+      TRY(median3(&best_ms));
+      after = best_ms;
+      return PTG_OK;
+    };
+    const int rc = body();
+    c->tune_run = false;
+    c->tune_bucket = -1;
+    c->exact_fallbacks = fb0;
+    c->exact_grain_fallbacks = fg0;
+    CK(c, hipMemcpyAsync(c->d_grains, c->d_tune_save, gb, hipMemcpyDeviceToDevice, c->stream));
+    c->snap = Ctx::SNAP_INVALID;
+    CK(c, hipStreamSynchronize(c->stream));
+    TRY(rc);
+  }
+  if (ntried) *ntried = tried;
+  if (ms_before) *ms_before = before;
+  if (ms_after) *ms_after = after;
+  return PTG_OK;
+}
+
+int ptg_clear_tuning(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  c->exact_tuned.clear();
   return PTG_OK;
 }
 

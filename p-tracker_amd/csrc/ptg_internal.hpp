@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ptgpu.h"
@@ -61,6 +62,13 @@ struct SearchArgs {
   int32_t off_cap;          // LDS elements reserved for rotated offsets
   int32_t tile_u, tile_v;   // plain kernel: shifts per tile (U, V when the window fits the LDS)
   StepBest* best;           // list slot x nsteps
+};
+
+// A launch shape of the exact search's MFMA kernel for one radius bucket: N-tiles of 16
+// vertical shifts per pass, waves (shift columns in flight) per workgroup, workgroups per
+// grain. Zero fields take the built-in heuristic's choice.
+struct ExactShape {
+  int32_t ntp = 0, nsu = 0, nsub = 0;
 };
 
 struct Context {
@@ -138,6 +146,19 @@ struct Context {
   void* exact_plan = nullptr;
   uint64_t exact_dev_key = 0;       // plan whose tables the scratch holds (0: none)
   uint64_t unmasked_version = 0;    // bumped whenever the unmasked list is rebuilt
+  // launch shapes chosen by ptg_autotune_search, by bucket key (value mode, steps, zone,
+  // largest pattern radius and grain count of the bucket); a tuning run probes tune_shape on
+  // bucket tune_bucket, records per bucket its key and the shape used, and brackets the
+  // whole search with two events on the library stream
+  std::unordered_map<uint64_t, ExactShape> exact_tuned;
+  int tune_bucket = -1;
+  ExactShape tune_shape;
+  bool tune_run = false;
+  std::vector<uint64_t> tune_keys;
+  std::vector<ExactShape> tune_used;
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
+  void* d_tune_save = nullptr;  // the grains, restored before each tuning run
+  size_t tune_save_cap = 0;
   int64_t exact_fallbacks = 0;      // exact-form searches that ran the plain form instead
   int64_t exact_grain_fallbacks = 0;  // grains of exact-form searches served by the plain kernel
 
@@ -181,6 +202,11 @@ struct Context {
   void mark_begin(int stage, hipEvent_t* ev, hipStream_t on = nullptr);
   void mark_end(int stage, hipEvent_t ev, hipStream_t on = nullptr);
 };
+
+// Raises kernel fn's dynamic-LDS limit on device dev to at least `bytes` when that exceeds
+// the 64 KiB default: the runtime is called only when the size grows past the largest set
+// before for (dev, fn), since a call per launch costs enough host time to show in a pair's.
+hipError_t ensure_kernel_lds(int dev, const void* fn, size_t bytes);
 
 // launchers (each returns hipError_t of the launch)
 hipError_t launch_begin_pair(Context& c);

@@ -376,11 +376,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_search_plain_sub(SearchArgs a
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes) {
   if (list_cap == 0) return hipSuccess;
   const int64_t blocks = (int64_t)list_cap * a.nsteps;
-  if (lds_bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_search_plain, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes);
-    if (e != hipSuccess) return e;
-  }
+  if (hipError_t e = ensure_kernel_lds(c.device, (const void*)k_search_plain, lds_bytes); e != hipSuccess) return e;
   k_search_plain<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a);
   return hipGetLastError();
 }
@@ -390,11 +386,8 @@ hipError_t launch_search_plain_sub(Context& c, const SearchArgs& a, int list_cap
   if (list_cap == 0) return hipSuccess;
   // up to 256 entries in flight; a workgroup with no entry returns at once
   const int64_t blocks = (int64_t)std::min(list_cap, 256) * a.nsteps;
-  if (lds_bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_search_plain_sub, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes);
-    if (e != hipSuccess) return e;
-  }
+  if (hipError_t e = ensure_kernel_lds(c.device, (const void*)k_search_plain_sub, lds_bytes); e != hipSuccess)
+    return e;
   k_search_plain_sub<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a, sub, sub_count, best_stride);
   return hipGetLastError();
 }

@@ -1474,12 +1474,9 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 
 // ---- host side -----------------------------------------------------------------------
 template <int NMT, int NTP, int VM>
-static hipError_t launch_main(const ExactArgs& e, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
-  if (lds > 65536) {
-    hipError_t r = hipFuncSetAttribute((const void*)k_search_exact<NMT, NTP, VM>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (r != hipSuccess) return r;
-  }
+static hipError_t launch_main(int dev, const ExactArgs& e, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  if (hipError_t r = ensure_kernel_lds(dev, (const void*)k_search_exact<NMT, NTP, VM>, lds); r != hipSuccess)
+    return r;
   k_search_exact<NMT, NTP, VM><<<grid, block, lds, st>>>(e);
   return hipGetLastError();
 }
@@ -1520,10 +1517,10 @@ static MainAttr main_attr(int vm, int nmt, int ntp) {
   return cache[vm][nmt][ntp];
 }
 
-static hipError_t dispatch_main(int vm, int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block, size_t lds,
-                                hipStream_t st) {
+static hipError_t dispatch_main(int dev, int vm, int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block,
+                                size_t lds, hipStream_t st) {
 #define PTG_CASE(M, T, VM) \
-  if (vm == VM && nmt == M && ntp == T) return launch_main<M, T, VM>(e, grid, block, lds, st);
+  if (vm == VM && nmt == M && ntp == T) return launch_main<M, T, VM>(dev, e, grid, block, lds, st);
   PTG_EXACT_INSTANCES(PTG_CASE)
 #undef PTG_CASE
   return hipErrorInvalidValue;
@@ -1537,10 +1534,30 @@ struct BucketLaunch {
   size_t lds_d = 0;              // its LDS
   size_t lds_m = 0, lds_t = 0;   // LDS of k_exact_mid, k_exact_template
   int cs = 0, ww = 0, wh = 0;    // k_search_exact: staging stride, window width and height
-  int dl_cap = 0, nsub = 0, nsu = 0;
+  int dl_cap = 0, nsub = 0, nsu = 0, ntp = 0;
   bool alias = false;
   size_t lds = 0;
+  uint64_t key = 0;              // the bucket's key in the autotuned shapes
 };
+
+// Whether k_search_exact has an instance for nmt M-tiles x ntp N-tiles in value mode vm.
+static bool exact_instance(int vm, int nmt, int ntp) {
+#define PTG_HAS(M, T, VM) \
+  if (vm == VM && nmt == M && ntp == T) return true;
+  PTG_EXACT_INSTANCES(PTG_HAS)
+#undef PTG_HAS
+  return false;
+}
+
+// The key of a radius bucket's launch shape: everything the shape's cost depends on.
+static uint64_t bucket_key(int vm, int nsteps, int U, int V, int rho, int nb) {
+  uint64_t h = 1469598103934665603ull;
+  for (const int64_t v : {(int64_t)vm, (int64_t)nsteps, (int64_t)U, (int64_t)V, (int64_t)rho, (int64_t)nb}) {
+    h ^= (uint64_t)v;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
 
 struct Bucket {
   int rho_max = 0;
@@ -1787,9 +1804,16 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     l.ww = win_width(U, cw);
     l.wh = P->vp + 2 * rho;
     const size_t win_bytes = ((size_t)l.ww * l.wh * 2 + 15) / 16 * 16;
+    // the launch shape: the autotuned one for this bucket's key (or the shape a tuning run
+    // probes), else the heuristic's below
+    l.key = bucket_key(vm, nsteps, U, V, rho, nb);
+    ExactShape want;
+    if (auto it = c.exact_tuned.find(l.key); it != c.exact_tuned.end()) want = it->second;
+    if (c.tune_run && c.tune_bucket == (int)bi) want = c.tune_shape;
+    l.ntp = want.ntp > 0 && want.ntp <= 3 && exact_instance(vm, nmt, want.ntp) ? want.ntp : ntp;
     const size_t stage_bytes = ((size_t)vm_planes(vm) * l.wh * l.cs + 15) / 16 * 16;
-    const size_t abuf_bytes = (size_t)(vm == kVm16 ? 16 : 8) * (ntp * 16) * 8;
-    l.alias = P->vp / 16 <= ntp && abuf_bytes <= stage_bytes;
+    const size_t abuf_bytes = (size_t)(vm == kVm16 ? 16 : 8) * (l.ntp * 16) * 8;
+    l.alias = P->vp / 16 <= l.ntp && abuf_bytes <= stage_bytes;
     const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
     // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
     // 16-byte read); typical lists total about 1.5 rho entries per rotation step
@@ -1800,11 +1824,11 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return win_bytes + w * (stage_bytes + (l.alias ? 0 : abuf_bytes)) + 16 + seg_bytes + dl_bytes;
     };
     // shift columns split over nsub workgroups when the bucket has few grains
-    l.nsub = std::max(1, std::min(U, (512 + nb - 1) / nb));
+    l.nsub = want.nsub > 0 ? std::min(U, want.nsub) : std::max(1, std::min(U, (512 + nb - 1) / nb));
     const int cpb = (U + l.nsub - 1) / l.nsub;
     // waves per workgroup: the most waves resident per CU under the register limit (a
     // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
-    const MainAttr ma = main_attr(vm, nmt, ntp);
+    const MainAttr ma = main_attr(vm, nmt, l.ntp);
     const int wave_cap = 4 * std::max(1, 512 / std::max(ma.vgprs, 1));
     int best_res = 0;
     l.nsu = 1;
@@ -1816,8 +1840,17 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
         l.nsu = w;
       }
     }
+    if (want.nsu > 0 && want.nsu <= std::min(8, cpb) && need(want.nsu) <= kLdsDynMax) l.nsu = want.nsu;
     l.lds = need(l.nsu);
     if (l.lds_m > kLdsDynMax || l.lds_t > kLdsDynMax || l.lds > kLdsDynMax) return kExactUnfit;
+  }
+  if (c.tune_run) {
+    c.tune_keys.assign(P->buckets.size(), 0);
+    c.tune_used.assign(P->buckets.size(), ExactShape{});
+    for (size_t bi = 0; bi < P->buckets.size(); bi++) {
+      c.tune_keys[bi] = L[bi].key;
+      c.tune_used[bi] = ExactShape{L[bi].ntp, L[bi].nsu, L[bi].nsub};
+    }
   }
   int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
   if (rc != PTG_OK) return rc;
@@ -1913,6 +1946,12 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // that per-launch timings do not overlap)
   const size_t nb_all = P->buckets.size();
   const bool serial = c.serial_search;
+  if (c.tune_run) {
+    // a tuning run: GPU events around the whole search on the library stream
+    if (!c.tune_ev[0] && (!(c.tune_ev[0] = c.get_event()) || !(c.tune_ev[1] = c.get_event())))
+      return c.fail(PTG_ERR_HIP, "hipEventCreate");
+    if ((he = hipEventRecord(c.tune_ev[0], c.stream)) != hipSuccess) return c.check(he, "hipEventRecord");
+  }
   if (int rca = c.ensure_aux(nb_all + 1); rca != PTG_OK) return rca;
   hipStream_t prep = serial ? c.stream : c.aux_stream;
   hipEvent_t fork = c.aux_events[nb_all];
@@ -1930,9 +1969,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     off += nb;
     const int rho = bk.rho_max;
     if (l.vb > 0) {
-      if (l.lds_d > 65536 && (he = hipFuncSetAttribute((const void*)k_exact_base_disk,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)l.lds_d)) != hipSuccess)
+      if ((he = ensure_kernel_lds(c.device, (const void*)k_exact_base_disk, l.lds_d)) != hipSuccess)
         return c.check(he, "hipFuncSetAttribute");
       ExactArgs ed = e;
       ed.base_vb = l.vb;
@@ -1941,9 +1978,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     }
     if (l.vb == 0 || bk.ndisk < nb) {
       const void* fm = l.npl == 6 ? (const void*)k_exact_mid<6> : (const void*)k_exact_mid<3>;
-      if (l.lds_m > 65536 &&
-          (he = hipFuncSetAttribute(fm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_m)) != hipSuccess)
-        return c.check(he, "hipFuncSetAttribute");
+      if ((he = ensure_kernel_lds(c.device, fm, l.lds_m)) != hipSuccess) return c.check(he, "hipFuncSetAttribute");
       ExactArgs em = e;
       em.mid_skip_disk = l.vb > 0 ? 1 : 0;
       em.mid_cbw = l.cbw;
@@ -1956,9 +1991,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       else k_exact_mid<3><<<grid_m, block_m, l.lds_m, prep>>>(em);
       if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_mid");
     }
-    if (l.lds_t > 65536 &&
-        (he = hipFuncSetAttribute((const void*)k_exact_template, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)l.lds_t)) != hipSuccess)
+    if ((he = ensure_kernel_lds(c.device, (const void*)k_exact_template, l.lds_t)) != hipSuccess)
       return c.check(he, "hipFuncSetAttribute");
     k_exact_template<<<dim3(nb, vm_spt(vm) * nmt), 256, l.lds_t, prep>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
@@ -1998,7 +2031,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
     hipEvent_t ev = nullptr;
     if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev, sx);
-    if ((he = dispatch_main(vm, nmt, ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
+    if ((he = dispatch_main(c.device, vm, nmt, l.ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev, sx);
   }
@@ -2018,6 +2051,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     k_exact_rescore<<<nlist, kRescoreThreads, 0, c.stream>>>(ea, out_best);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_rescore");
   }
+  if (c.tune_run && (he = hipEventRecord(c.tune_ev[1], c.stream)) != hipSuccess) return c.check(he, "hipEventRecord");
   if (e.ovf_slot) {
     if ((he = launch_search_plain_sub(c, sa, nlist, e.ovf_list, e.flags + 1, part_cap, lds_plain)) != hipSuccess)
       return c.check(he, "k_search_plain_sub");

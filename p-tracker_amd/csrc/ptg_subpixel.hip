@@ -1028,17 +1028,13 @@ static size_t sp_lds_bytes(const SpLayout& lay) {
   return kSpOffTile + sp_align(sizeof(uint16_t) * lay.tile_elems);
 }
 
-static hipError_t sp_set_lds(const void* fn, size_t lds) {
-  if (lds > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  return hipSuccess;
-}
 
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
   if (c.n_unmasked == 0) return hipSuccess;
   const Image& im = c.img[c.icur];
   const SpLayout lay = sp_layout(c.max_prad_unmasked, p.subpix_displacement_max);
   const size_t lds = sp_lds_bytes(lay);
-  hipError_t e = sp_set_lds((const void*)k_subpixel, lds);
+  hipError_t e = ensure_kernel_lds(c.device, (const void*)k_subpixel, lds);
   if (e != hipSuccess) return e;
   k_subpixel<<<c.n_unmasked, kSpThreads, lds, c.stream>>>(
       im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0,
@@ -1111,7 +1107,7 @@ hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, co
   for (int32_t r : c.h_prad) max_prad = std::max(max_prad, r);
   const SpLayout lay = sp_layout(max_prad, dmax);
   const size_t lds = sp_lds_bytes(lay);
-  hipError_t e = sp_set_lds((const void*)k_objective, lds);
+  hipError_t e = ensure_kernel_lds(c.device, (const void*)k_objective, lds);
   if (e != hipSuccess) return e;
   k_objective<<<n, kSpThreads, lds, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin,
                                                 c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0, n, d_ig, d_X, dmax,
