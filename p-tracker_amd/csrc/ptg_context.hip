@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -337,7 +339,11 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   double* cs = (double*)c->pin_cs.p;
   memcpy(c->pin_steps.p, c->h_steps.data(), sizeof(double) * nsteps);
   const double* steps = c->h_steps.data();
-#pragma omp parallel for schedule(static)
+  // the CPUs this process may use less four, left to the GPU runtime's threads and the
+  // caller's (with every CPU taken, one preempted thread holds the whole loop for a
+  // scheduler time slice); chunks of 8 grains taken in turn
+  const int nthr = std::max(1, std::min(omp_get_max_threads(), omp_get_num_procs() - 4));
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
   for (int li = 0; li < nlist; li++) {
     const int g = list[li];
     const ptg_grain& gr = c->h_grains[g];
