@@ -50,16 +50,20 @@ enum { PTG_STAGE_MAIN = 0, PTG_STAGE_RESCUE = 1, PTG_STAGE_SUPER = 2 };
 
 /* Integer-search arithmetic.
  *  PLAIN: per candidate, the reference's two-pass FP64 sums in the reference's pixel
- *         order, unfused; NCC is bit-identical to follow_pattern_pixel().
+ *         order, unfused; NCC is bit-identical to follow_pattern_pixel(). Patterns of any
+ *         size: offsets and window staged in LDS, read from the frame through the caches
+ *         when they exceed it (radius above about 85 px).
  *  EXACT: per candidate, exact integer moments (sum I1, sum I1^2, sum I0*I1) from an
  *         INT8 MFMA correlation, then the ratio in FP64. Every candidate whose exact NCC
  *         lies within a proven bound of the reference's FP64 rounding error of the best is
  *         re-scored in the reference's two-pass arithmetic, so the picks are the
  *         reference's (ties included); NCC is then the reference's value bit for bit, and
- *         elsewhere within a few units in the last place of it. Grains the exact kernels
- *         cannot settle (more than two such candidates, a best NCC below 1/32, window values
- *         above 12 bits, patterns above 23,170 pixels) are searched by the plain kernel in
- *         the same stage. */
+ *         elsewhere within a few units in the last place of it. The digits follow the frames'
+ *         declared depth (the larger of the two): 8-bit windows as one offset-INT8 digit,
+ *         12-bit as two 7-bit digits, 16-bit as two offset bytes (num and d1 in int64).
+ *         Grains the exact kernels cannot settle (more than two such candidates, a best NCC
+ *         below 1/32, window values above the declared depth, patterns above 23,170 pixels)
+ *         are searched by the plain kernel in the same stage. */
 enum { PTG_SEARCH_PLAIN = 0, PTG_SEARCH_EXACT = 1 };
 
 /* Subpixel optimizer: POWELL is derivative-free, as the reference
@@ -309,11 +313,12 @@ int ptg_get_subpix_evals(void* ctx, int64_t* evals);
 int ptg_get_subpix_evals_per_grain(void* ctx, int32_t* per_grain);
 /* Work the exact form handed to the plain form since the context was created (the picks
  * are the same; NCC comes from the two-pass formula). searches: PTG_SEARCH_EXACT searches
- * run wholly by the plain kernel (frames above 12 bits, more than 48 rotation steps, a
+ * run wholly by the plain kernel (more than 48 rotation steps, 40 for 16-bit frames, a
  * pattern radius above 120 px, or a pattern and window beyond the exact kernels' LDS).
  * grains: grains of exact searches searched by the plain kernel within the same stage:
  * rotated templates that differ from the base mask in more cells than the exact kernel
- * holds (large patterns at large reference rotations), window values above 12 bits,
+ * holds (large patterns at large reference rotations), window values above the declared
+ * depth,
  * patterns above 23,170 pixels, near-tie sets of more than two candidates, or a best NCC
  * below 1/32. Synchronises. */
 int ptg_get_exact_fallbacks(void* ctx, int64_t* searches, int64_t* grains);

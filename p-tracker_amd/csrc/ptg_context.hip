@@ -446,12 +446,13 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     }
   }
   if (method == PTG_SEARCH_PLAIN) {
+    // a pattern whose offsets and smallest window tile exceed the LDS: the kernel that reads
+    // the frame through the caches
     const size_t lds = (size_t)(win_cap * 2 + 15) / 16 * 16 + (size_t)off_cap * 4;
-    if (lds > kMaxLds)
-      return c->fail(PTG_ERR_LIMIT, "search window + pattern need %zu B of LDS (max %zu)", lds, kMaxLds);
     hipEvent_t ek;
     if (stage == PTG_STAGE_MAIN) c->mark_begin(PTG_T_SEARCH_KERNEL, &ek);
-    CK(c, launch_search_plain(*c, a, nlist, lds));
+    if (lds > kMaxLds) CK(c, launch_search_plain_global(*c, a, nlist));
+    else CK(c, launch_search_plain(*c, a, nlist, lds));
     if (stage == PTG_STAGE_MAIN) c->mark_end(PTG_T_SEARCH_KERNEL, ek);
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;

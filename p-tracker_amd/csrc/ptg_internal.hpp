@@ -212,6 +212,9 @@ hipError_t ensure_kernel_lds(int dev, const void* fn, size_t bytes);
 hipError_t launch_begin_pair(Context& c);
 hipError_t launch_precompute(Context& c);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
+// The plain kernel for patterns whose offsets and window exceed the LDS: the same candidates
+// and arithmetic, the frame read through the caches.
+hipError_t launch_search_plain_global(Context& c, const SearchArgs& a, int list_cap);
 // The plain kernel over a device list of *sub_count slots of a's list (sub[j] < list_cap),
 // step k of slot li written to a.best[li * best_stride + k].
 hipError_t launch_search_plain_sub(Context& c, const SearchArgs& a, int list_cap, const int32_t* sub,

@@ -214,6 +214,8 @@ __device__ __forceinline__ bool better(double ncc, int idx, double best, int bid
 constexpr int kPlainThreads = 256;
 constexpr int kPlainCPT = 4;  // candidates per lane per round
 
+__device__ __forceinline__ void plain_argmax_write(double best, int bidx, int k, int S, StepBest* out);
+
 // One (list slot li, step k): the workgroup's best candidate to out.
 __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int k, StepBest* out) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -320,7 +322,12 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
     }
   }
   if (clamped) atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
-  // workgroup argmax with the same ordering
+  plain_argmax_write(best, bidx, k, U * V, out);
+}
+
+// The workgroup's argmax of its lanes' (best, bidx), in the reference's order, written to
+// out with the global candidate index k S + bidx.
+__device__ __forceinline__ void plain_argmax_write(double best, int bidx, int k, int S, StepBest* out) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     const double ob = __shfl_xor(best, m, 64);
@@ -346,10 +353,99 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
       }
     StepBest sb;
     sb.ncc = best;
-    sb.idx = bidx == INT_MAX ? INT_MAX : k * (U * V) + bidx;  // global candidate index
+    sb.idx = bidx == INT_MAX ? INT_MAX : k * S + bidx;  // global candidate index
     sb.pad = 0;
     *out = sb;
   }
+}
+
+// The same candidates, in the same order and arithmetic, for a pattern whose rotated offsets
+// and window exceed the LDS (radius above about 85 px): each lane rotates the pattern pixels
+// itself and reads the current frame through the caches, clamped at its edges as the staged
+// window is; the grain is marked clamped when the window's box leaves the frame, as there.
+__device__ __forceinline__ void plain_slot_step_global(const SearchArgs& a, int li, int k, StepBest* out) {
+  const int nsteps = a.nsteps;
+  const int g = a.list[li];
+  const ptg_grain& gr = a.grains[g];
+  const int64_t b = a.pbegin[g];
+  const int n = (int)(a.pbegin[g + 1] - b);
+  const int rho = a.prad[g];
+  const int gdx = (int)round(gr.dx), gdy = (int)round(gr.dy);
+  const int U = a.left + a.right + 1, V = a.up + a.down + 1;
+  const int X0 = gr.refcoord_xpix + gdx - a.left, Y0 = gr.refcoord_ypix + gdy - a.up;  // shift (0, 0)
+  const double C0C0 = gr.C0C0;
+  const double c = a.cs[2 * ((int64_t)g * nsteps + k)];
+  const double s = a.cs[2 * ((int64_t)g * nsteps + k) + 1];
+  const int32_t* __restrict__ pat = a.pattern + 2 * b;
+  const double* __restrict__ d0 = a.d0 + b;
+  const double dn = (double)n;
+  auto px = [&](int X, int Y) -> unsigned int {
+    return a.I1[(int64_t)clampi(Y, 0, a.dimy - 1) * a.pitch1 + clampi(X, 0, a.dimx - 1)];
+  };
+  double best = 0.0;
+  int bidx = INT_MAX;
+  const int S = U * V;
+  for (int r0 = 0; r0 < S; r0 += kPlainThreads * kPlainCPT) {
+    int cx[kPlainCPT], cy[kPlainCPT], cidx[kPlainCPT];
+    bool valid[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      const int w = r0 + q * kPlainThreads + threadIdx.x;
+      valid[q] = w < S;
+      const int ww_ = valid[q] ? w : 0;
+      const int v = ww_ / U, u = ww_ - v * U;
+      cx[q] = X0 + u;
+      cy[q] = Y0 + v;
+      cidx[q] = u * V + v;
+    }
+    unsigned int isum[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) isum[q] = 0u;
+    for (int i = 0; i < n; i++) {
+      int x, y;
+      rotate_px(pat[2 * i], pat[2 * i + 1], c, s, x, y);
+#pragma unroll
+      for (int q = 0; q < kPlainCPT; q++) isum[q] += px(cx[q] + x, cy[q] + y);
+    }
+    double m1[kPlainCPT], c01[kPlainCPT], c11[kPlainCPT];
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      m1[q] = (double)isum[q] / dn;
+      c01[q] = 0.0;
+      c11[q] = 0.0;
+    }
+    for (int i = 0; i < n; i++) {
+      int x, y;
+      rotate_px(pat[2 * i], pat[2 * i + 1], c, s, x, y);
+      const double dd = d0[i];
+#pragma unroll
+      for (int q = 0; q < kPlainCPT; q++) {
+        const double diffC1 = (double)px(cx[q] + x, cy[q] + y) - m1[q];
+        c01[q] = c01[q] + dd * diffC1;
+        c11[q] = c11[q] + diffC1 * diffC1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPlainCPT; q++) {
+      if (!valid[q]) continue;
+      const double ncc = c01[q] / sqrt(C0C0 * c11[q]);
+      if (better(ncc, cidx[q], best, bidx)) {
+        best = ncc;
+        bidx = cidx[q];
+      }
+    }
+  }
+  const int bx = X0 - rho, by = Y0 - rho;
+  if (threadIdx.x == 0 && (bx < 0 || by < 0 || bx + U + 2 * rho > a.dimx || by + V + 2 * rho > a.dimy))
+    atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
+  plain_argmax_write(best, bidx, k, S, out);
+}
+
+__global__ __launch_bounds__(kPlainThreads) void k_search_plain_global(SearchArgs a) {
+  const int li = blockIdx.x / a.nsteps;
+  const int k = blockIdx.x - li * a.nsteps;
+  if (li >= *a.count) return;
+  plain_slot_step_global(a, li, k, &a.best[(int64_t)li * a.nsteps + k]);
 }
 
 __global__ __launch_bounds__(kPlainThreads) void k_search_plain(SearchArgs a) {
@@ -378,6 +474,13 @@ hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, si
   const int64_t blocks = (int64_t)list_cap * a.nsteps;
   if (hipError_t e = ensure_kernel_lds(c.device, (const void*)k_search_plain, lds_bytes); e != hipSuccess) return e;
   k_search_plain<<<dim3((unsigned)blocks), kPlainThreads, lds_bytes, c.stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_plain_global(Context& c, const SearchArgs& a, int list_cap) {
+  if (list_cap == 0) return hipSuccess;
+  const int64_t blocks = (int64_t)list_cap * a.nsteps;
+  k_search_plain_global<<<dim3((unsigned)blocks), kPlainThreads, 0, c.stream>>>(a);
   return hipGetLastError();
 }
 

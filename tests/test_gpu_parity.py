@@ -479,6 +479,31 @@ def test_sequence_beyond_exact_limits_falls_back_to_plain(tracker):
     assert tracker.exact_fallbacks()[0] == n0[0] + 1
 
 
+@pytest.mark.parametrize("method", [pk.SEARCH_PLAIN, pk.SEARCH_EXACT])
+def test_patterns_beyond_the_lds_equal_oracle(tracker, method):
+    # radius 140-160 (about 70,000 pattern pixels): the rotated offsets alone exceed the LDS,
+    # so the search runs the plain kernel that reads the frame through the caches (the
+    # exact form hands the whole stage to it); picks bit-identical, and the LM subpixel
+    # within tolerance of the FP64 oracle
+    S = synth.sequence(n_frames=2, n_grains=3, size=(1000, 900), seed=511, radius=(140.0, 160.0), margin=200,
+                       max_step=2.0, max_rot=0.01)
+    z = pk.zone(2, num_rot=1, inc_rot=0.01)
+    say(f"patterns beyond the LDS: G={S.G}, P max {np.diff(S.pattern_begin).max()}, method {method}")
+    prm = pk.track_params(z, rescue_level=0, subpixel=0, search_method=method)
+    oprm = pk.track_params(z, rescue_level=0, subpixel=0)
+    for k, got, ref in _sequence_vs_oracle(tracker, S, prm, oprm, 1):
+        for col in INT_COLS:
+            assert bits_equal(got[col], ref[col]), col
+        assert not (got["status"] & 1).any()  # every window inside the frame
+    prm = pk.track_params(z, rescue_level=0, subpixel=1, search_method=method,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    oprm = pk.track_params(z, rescue_level=0, subpixel=1, subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+    for k, got, ref in _sequence_vs_oracle(tracker, S, prm, oprm, 1):
+        assert np.abs(got["dx"] - ref["dx"]).max() < TOL_PX and np.abs(got["dy"] - ref["dy"]).max() < TOL_PX
+        assert np.abs(got["drot"] - ref["drot"]).max() < TOL_RAD
+        assert np.abs(got["NCC_subpix"] - ref["NCC_subpix"]).max() < TOL_NCC
+
+
 def test_sequence_exact_lm_within_tolerance_of_fp64_oracle(tracker):
     # the batching workload's shape at a small size: radii 6-60, +-8 px x 9 steps, LM
     # subpixel, three pairs; every grain within 1e-3 px / 1e-5 rad of the FP64 minimizer
