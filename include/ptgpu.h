@@ -253,6 +253,12 @@ int ptg_refine_joint_distortion(void* ctx, double* lens, double ncc_min, int32_t
  * (the reference skips the pair). Copies the data to the device. */
 int ptg_equiproj_set_data(void* ctx, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
                           const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min);
+/* The POWELL_ITER form of the criterion, src/distortion_correction.cpp:316-354: A1, B1 are
+ * the given positions orig_xy (G x 2; the reference's orig_x / orig_y: the reference pixels
+ * at its first iteration, their undistor() images at the previous parameters afterwards)
+ * instead of undistor() of the reference pixels, and carry no derivative. NULL returns to the
+ * standard form, as does the next ptg_equiproj_set_data. */
+int ptg_equiproj_set_origins(void* ctx, const double* orig_xy);
 /* At params14 (undistor's 14 parameters, taken as given): f = the sum of squared residuals
  * (the reference's Powell functor, :277-303); jtr = J^T r (14) and jtj = J^T J (14 x 14,
  * row major) of the residuals' analytic Jacobian J; res = the residuals (nimg * npairs,

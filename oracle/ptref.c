@@ -692,6 +692,33 @@ void ptref_equiproj_residuals(const double* X, int32_t G, const double* ref_xy, 
   }
 }
 
+/* src/distortion_correction.cpp:316-354 (POWELL_ITER): A1 = orig_i, B1 = orig_j. */
+void ptref_equiproj_residuals_orig(const double* X, int32_t G, const double* orig_xy, const double* ref_xy,
+                                   int32_t nimg, const double* disp, const double* ncc, int32_t npairs,
+                                   const int32_t* pairs, double NCC_min, double* res) {
+  for (int32_t k = 0; k < nimg; k++) {
+    const double* d = disp + 2 * (int64_t)k * G;
+    const double* nc = ncc + (int64_t)k * G;
+    for (int32_t l = 0; l < npairs; l++) {
+      const int32_t i = pairs[2 * l], j = pairs[2 * l + 1];
+      double* r = &res[(int64_t)k * npairs + l];
+      *r = 0.0;
+      if (nc[i] < NCC_min || nc[j] < NCC_min) continue;
+      double A2x, A2y, B2x, B2y;
+      const double A1x = orig_xy[2 * i], A1y = orig_xy[2 * i + 1];
+      ptref_undistor(X, ref_xy[2 * i] + d[2 * i], ref_xy[2 * i + 1] + d[2 * i + 1], &A2x, &A2y);
+      const double B1x = orig_xy[2 * j], B1y = orig_xy[2 * j + 1];
+      ptref_undistor(X, ref_xy[2 * j] + d[2 * j], ref_xy[2 * j + 1] + d[2 * j + 1], &B2x, &B2y);
+      double ABx = B1x - A1x, ABy = B1y - A1y;
+      const double AB = sqrt(ABx * ABx + ABy * ABy);
+      ABx /= AB;
+      ABy /= AB;
+      const double BBx = B2x - B1x, BBy = B2y - B1y, AAx = A2x - A1x, AAy = A2y - A1y;
+      *r = (ABx * (BBx - AAx)) + (ABy * (BBy - AAy));
+    }
+  }
+}
+
 /* Jacobian of undistor with respect to (xd, yd), by differentiating the expressions
  * above: J = Jparallax * Jbrown. */
 static void undistor_jac(const double* X, double xd, double yd, double J[4]) {

@@ -114,6 +114,13 @@ int64_t ptref_precompute_pairs(int32_t G, const double* ref_xy, double dmin, dou
  * grain of pair l has NCC_subpix < NCC_min in image k. */
 void ptref_equiproj_residuals(const double* X, int32_t G, const double* ref_xy, int32_t nimg, const double* disp,
                               const double* ncc, int32_t npairs, const int32_t* pairs, double NCC_min, double* res);
+/* The residuals of the POWELL_ITER functor, src/distortion_correction.cpp:316-354: the first
+ * positions A1, B1 are given (orig_xy, G x 2: the reference pixels at the first iteration,
+ * their undistorted positions at the previous parameters afterwards) and only the tracked
+ * positions ref + disp are undistorted. */
+void ptref_equiproj_residuals_orig(const double* X, int32_t G, const double* orig_xy, const double* ref_xy,
+                                   int32_t nimg, const double* disp, const double* ncc, int32_t npairs,
+                                   const int32_t* pairs, double NCC_min, double* res);
 
 /* Remap of a whole frame (the library's ptg_remap): for every output pixel (x, y),
  * Newton iteration on undistor() for the distorted point, then a bicubic sample of src

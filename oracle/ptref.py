@@ -67,6 +67,8 @@ def lib():
             "ptref_distort_point": ([vp, c_double, c_double, vp, vp], None),
             "ptref_precompute_pairs": ([c_int32, vp, c_double, c_double, vp, c_int64], c_int64),
             "ptref_equiproj_residuals": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double, vp], None),
+            "ptref_equiproj_residuals_orig": ([vp, c_int32, vp, vp, c_int32, vp, vp, c_int32, vp, c_double, vp],
+                                              None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -298,6 +300,21 @@ def equiproj_residuals(X14, ref_xy, disp, ncc, pairs, NCC_min=0.7):
     pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
     res = np.zeros(len(d) * len(pr))
     lib().ptref_equiproj_residuals(_p(X), G, _p(ref), len(d), _p(d), _p(n), len(pr), _p(pr), NCC_min, _p(res))
+    return res
+
+
+def equiproj_residuals_orig(X14, orig_xy, ref_xy, disp, ncc, pairs, NCC_min=0.7):
+    """The POWELL_ITER residuals: the first positions given (orig_xy), the tracked ones undistorted."""
+    X = np.ascontiguousarray(X14, dtype=np.float64)
+    ref = np.ascontiguousarray(np.asarray(ref_xy, dtype=np.float64).reshape(-1, 2))
+    org = np.ascontiguousarray(np.asarray(orig_xy, dtype=np.float64).reshape(-1, 2))
+    G = len(ref)
+    d = np.ascontiguousarray(np.asarray(disp, dtype=np.float64).reshape(-1, G, 2))
+    n = np.ascontiguousarray(np.asarray(ncc, dtype=np.float64).reshape(-1, G))
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    res = np.zeros(len(d) * len(pr))
+    lib().ptref_equiproj_residuals_orig(_p(X), G, _p(org), _p(ref), len(d), _p(d), _p(n), len(pr), _p(pr), NCC_min,
+                                        _p(res))
     return res
 
 

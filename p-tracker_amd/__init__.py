@@ -156,6 +156,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_get_subpix_evals_per_grain": ([vp, vp], c_int),
         "ptg_get_exact_fallbacks": ([vp, POINTER(c_int64), POINTER(c_int64)], c_int),
         "ptg_equiproj_set_data": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double], c_int),
+        "ptg_equiproj_set_origins": ([vp, vp], c_int),
         "ptg_equiproj_eval": ([vp, vp, vp, vp, vp, vp, vp], c_int),
         "ptg_calibrate_distortion": ([vp, vp, vp, c_int32, vp, vp, vp], c_int),
         "ptg_joint_eval": ([vp, c_int32, vp, vp, vp, vp, vp, vp], c_int),
@@ -178,7 +179,7 @@ def exported_symbols():
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_set_option", "ptg_autotune_search", "ptg_clear_tuning", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
-            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data",
+            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_equiproj_set_data", "ptg_equiproj_set_origins",
             "ptg_equiproj_eval", "ptg_calibrate_distortion", "ptg_joint_eval", "ptg_refine_joint_distortion"]
 
 
@@ -334,6 +335,14 @@ class Tracker:
         self._eq_m = len(d) * len(pr)
         self._ck(self.lib.ptg_equiproj_set_data(self.h, G, _ptr(ref), len(d), _ptr(d), _ptr(n), len(pr), _ptr(pr),
                                                 NCC_min), "ptg_equiproj_set_data")
+
+    def equiproj_set_origins(self, orig_xy=None):
+        """The POWELL_ITER form: given first positions (G x 2), or None for the standard form."""
+        if orig_xy is None:
+            self._ck(self.lib.ptg_equiproj_set_origins(self.h, None), "ptg_equiproj_set_origins")
+            return
+        o = np.ascontiguousarray(np.asarray(orig_xy, dtype=np.float64).reshape(-1, 2))
+        self._ck(self.lib.ptg_equiproj_set_origins(self.h, _ptr(o)), "ptg_equiproj_set_origins")
 
     def equiproj_eval(self, params14, full=False):
         """(f, J^T r, J^T J) and, with full, (residuals, Jacobian)."""

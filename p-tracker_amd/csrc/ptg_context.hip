@@ -343,6 +343,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   // caller's (with every CPU taken, one preempted thread holds the whole loop for a
   // scheduler time slice); chunks of 8 grains taken in turn
   const int nthr = std::max(1, std::min(omp_get_max_threads(), omp_get_num_procs() - 4));
+  (void)nthr;  // (the device compilation pass ignores the pragma)
 #pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
   for (int li = 0; li < nlist; li++) {
     const int g = list[li];
@@ -546,7 +547,7 @@ int ptg_destroy(void* p) {
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
                   c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
-                  c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac};
+                  c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac, c->d_eq_orig};
   for (void* b : more)
     if (b) (void)hipFree(b);
   free_exact_plans(*c);
@@ -1212,6 +1213,25 @@ int ptg_equiproj_set_data(void* p, int32_t G, const double* ref_xy, int32_t nimg
   c->eq_nimg = nimg;
   c->eq_npairs = npairs;
   c->eq_NCC_min = NCC_min;
+  c->eq_orig_set = false;  // new data: the first positions are undistorted again
+  return PTG_OK;
+}
+
+int ptg_equiproj_set_origins(void* p, const double* orig_xy) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!c->d_eq_out) return c->fail(PTG_ERR_STATE, "ptg_equiproj_set_data not called");
+  if (!orig_xy) {
+    c->eq_orig_set = false;
+    return PTG_OK;
+  }
+  if (c->eq_G > 0) {
+    TRY(c->ensure((void**)&c->d_eq_orig, &c->eq_orig_cap, sizeof(double) * 2 * (size_t)c->eq_G));
+    CK(c, hipMemcpyAsync(c->d_eq_orig, orig_xy, sizeof(double) * 2 * (size_t)c->eq_G, hipMemcpyHostToDevice,
+                         c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+  }
+  c->eq_orig_set = true;
   return PTG_OK;
 }
 
@@ -1225,6 +1245,7 @@ static int equiproj_eval(Ctx* c, const double* X, double* f, double* jtr, double
   a.pad = 0;
   a.pairs = c->d_eq_pairs;
   a.ref = c->d_eq_ref;
+  a.orig = c->eq_orig_set && c->eq_G > 0 ? c->d_eq_orig : nullptr;
   a.disp = c->d_eq_disp;
   a.ncc = c->d_eq_ncc;
   a.NCC_min = c->eq_NCC_min;

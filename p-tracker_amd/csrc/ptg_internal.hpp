@@ -173,6 +173,9 @@ struct Context {
   double* d_eq_out = nullptr;
   double* d_eq_res = nullptr;
   double* d_eq_jac = nullptr;
+  double* d_eq_orig = nullptr;  // given first positions (ptg_equiproj_set_origins)
+  bool eq_orig_set = false;
+  size_t eq_orig_cap = 0;
   size_t eq_pairs_cap = 0, eq_ref_cap = 0, eq_disp_cap = 0, eq_ncc_cap = 0, eq_partial_cap = 0, eq_out_cap = 0,
          eq_res_cap = 0, eq_jac_cap = 0;
 
@@ -256,6 +259,7 @@ struct EquiprojArgs {
   int32_t G, nimg, npairs, pad;
   const int32_t* pairs;  // npairs x (i, j)
   const double* ref;     // G x 2: reference pixels
+  const double* orig;    // G x 2: given first positions (the POWELL_ITER form), or null: undistor(ref)
   const double* disp;    // nimg x G x 2: tracked displacements
   const double* ncc;     // nimg x G: NCC_subpix
   double NCC_min;

@@ -248,7 +248,9 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 }
 
 // One residual per thread (index (k, il), image-major as the reference's fvec); residual,
-// Jacobian row and workgroup partial sums.
+// Jacobian row and workgroup partial sums. With a.orig set (the POWELL_ITER form,
+// src/distortion_correction.cpp:316-354) the first positions A1, B1 are those given, which do
+// not depend on the parameters.
 __global__ __launch_bounds__(kEqThreads) void k_equiproj(EquiprojArgs a, DistoParams P) {
   __shared__ double wsum[kEqThreads / 64][kEqSums];
   const int64_t t = (int64_t)blockIdx.x * kEqThreads + threadIdx.x;
@@ -266,9 +268,16 @@ __global__ __launch_bounds__(kEqThreads) void k_equiproj(EquiprojArgs a, DistoPa
       const double xi = a.ref[2 * i], yi = a.ref[2 * i + 1], xj = a.ref[2 * j], yj = a.ref[2 * j + 1];
       const double xi2 = xi + d[2 * i], yi2 = yi + d[2 * i + 1], xj2 = xj + d[2 * j], yj2 = yj + d[2 * j + 1];
       double A1x, A1y, A2x, A2y, B1x, B1y, B2x, B2y;
-      undistor_d(P, xi, yi, A1x, A1y);
+      if (a.orig) {
+        A1x = a.orig[2 * i];
+        A1y = a.orig[2 * i + 1];
+        B1x = a.orig[2 * j];
+        B1y = a.orig[2 * j + 1];
+      } else {
+        undistor_d(P, xi, yi, A1x, A1y);
+        undistor_d(P, xj, yj, B1x, B1y);
+      }
       undistor_d(P, xi2, yi2, A2x, A2y);
-      undistor_d(P, xj, yj, B1x, B1y);
       undistor_d(P, xj2, yj2, B2x, B2y);
       double ABx = B1x - A1x, ABy = B1y - A1y;
       const double AB = sqrt(ABx * ABx + ABy * ABy);
@@ -279,9 +288,11 @@ __global__ __launch_bounds__(kEqThreads) void k_equiproj(EquiprojArgs a, DistoPa
       r = (ABx * Dx) + (ABy * Dy);
       // dr/d(B1 - A1) = (D - e r) / |AB|, dr/dD = e
       const double wx = (Dx - ABx * r) / AB, wy = (Dy - ABy * r) / AB;
-      undistor_vjp(P, xi, yi, -wx + ABx, -wy + ABy, g);
+      if (!a.orig) {
+        undistor_vjp(P, xi, yi, -wx + ABx, -wy + ABy, g);
+        undistor_vjp(P, xj, yj, wx - ABx, wy - ABy, g);
+      }
       undistor_vjp(P, xi2, yi2, -ABx, -ABy, g);
-      undistor_vjp(P, xj, yj, wx - ABx, wy - ABy, g);
       undistor_vjp(P, xj2, yj2, ABx, ABy, g);
     }
     if (a.res) a.res[t] = r;

@@ -8,16 +8,20 @@
 // message.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "../../include/ptgpu.h"
+#include "powell_minimize.hpp"
 #include "ptracker_io.hpp"
 #include "tiff_reader.hpp"
 
@@ -171,14 +175,17 @@ class ptracker_gpu {
     return n;
   }
 
-  // correction_distortion, src/distortion_correction.cpp:135-479: the images
+  // correction_distortion, src/distortion_correction.cpp:135-608: the images
   // image_numbers_corrDisto[1..] tracked from image_numbers_corrDisto[0] (every grain
   // starting at the imposed displacement of the image; search and subpixel, no rescue),
   // the grain pairs between equiProj_dist_min and equiProj_dist_max apart, then the
-  // equiprojectivity criterion minimised on the GPU over the parameters whose
-  // perturbation (d*_corrDistor) is non-zero, as those the reference's Powell moves.
-  // Writes dic_out_<num>.txt per image and corrDisto_log.txt (the reference's log lines)
-  // into out_dir; returns the 14 parameters.
+  // equiprojectivity criterion (evaluated on the GPU) minimised over the parameters whose
+  // perturbation (d*_corrDistor) is non-zero, by optim_algo: POWELL (Powell's method from
+  // those perturbations, ftol 1e-12, restarted until a run no longer improves),
+  // POWELL_ITER (powell_num_iter such minimisations with the first positions re-centred
+  // between them) or LEVMAR (Levenberg-Marquardt on the analytic Jacobian). Writes dic_out_<num>.txt per image, corrDisto_log.txt (the reference's log
+  // lines), errors.txt, distobox.txt and, for POWELL_ITER, convergence.txt into out_dir;
+  // returns the 14 parameters.
   std::vector<double> correction_distortion(tracker_settings& s, const std::string& out_dir = "") {
     const std::vector<int>& im = s.image_numbers_corrDisto;
     if (im.size() < 2) throw std::runtime_error("image_numbers_corrDisto needs at least two images");
@@ -228,8 +235,63 @@ class ptracker_gpu {
     int32_t free14[14];
     for (int q = 0; q < 14; q++) free14[q] = s.disto_parameters_perturb[q] != 0.0 ? 1 : 0;
     double f_ini = 0.0, f_end = 0.0;
-    int32_t iters = 0;
-    check(ptg_calibrate_distortion(ctx_, X.data(), free14, 200, &f_ini, &f_end, &iters), "ptg_calibrate_distortion");
+    // the criterion (sum of squared residuals) on the GPU, and Powell's initial directions:
+    // the perturbations of the parameters it moves (the reference's minimize(X, perturb))
+    auto crit = [&](const std::vector<double>& P) {
+      double f = 0.0;
+      check(ptg_equiproj_eval(ctx_, P.data(), &f, nullptr, nullptr, nullptr, nullptr), "ptg_equiproj_eval");
+      return f;
+    };
+    std::vector<std::vector<double>> dirs;
+    for (int q = 0; q < 14; q++)
+      if (s.disto_parameters_perturb[q] != 0.0) {
+        dirs.emplace_back(14, 0.0);
+        dirs.back()[q] = s.disto_parameters_perturb[q];
+      }
+    if (s.optim_algo == OPTIM_LEVMAR) {
+      // Levenberg-Marquardt on the analytic Jacobian in place of lmdif (:401-475)
+      int32_t iters = 0;
+      check(ptg_calibrate_distortion(ctx_, X.data(), free14, 200, &f_ini, &f_end, &iters),
+            "ptg_calibrate_distortion");
+    } else if (s.optim_algo == OPTIM_POWELL) {
+      // :241-300, Powell's method restarted until a run no longer improves (one run can stop
+      // short of the minimum in the K1-K2 valley)
+      f_ini = crit(X);
+      powell_minimize_restarted(crit, X, dirs, 1e-12);
+      f_end = crit(X);
+    } else {
+      // POWELL_ITER, :302-400: the first positions start at the reference pixels and, after
+      // each Powell run, become their undistor() images at its result; the tracked
+      // displacements move by the same correction; one line of convergence.txt per run
+      std::vector<double> orig = ref, und(ref.size());
+      const std::vector<double> disp0 = disp;
+      check(ptg_equiproj_set_origins(ctx_, orig.data()), "ptg_equiproj_set_origins");
+      f_ini = crit(X);
+      std::ofstream conv(out_dir + "convergence.txt");
+      conv << std::scientific << std::setprecision(15);
+      for (int iter = 1; iter <= s.powell_num_iter; iter++) {
+        powell_minimize_restarted(crit, X, dirs, 1e-12);
+        const double idx = crit(X) / nimg;
+        std::cout << std::scientific << std::setprecision(15) << "iteration " << iter << ", index = " << idx
+                  << std::endl;
+        conv << iter << ' ' << idx;
+        for (int q = 0; q < 14; q++) conv << ' ' << X[q];
+        conv << ' ' << std::endl;
+        check(ptg_undistort_points(ctx_, X.data(), G, ref.data(), und.data()), "ptg_undistort_points");
+        for (int g = 0; g < G; g++)
+          for (int k = 0; k < nimg; k++) {
+            const size_t o = 2 * ((size_t)k * G + g);
+            disp[o] = disp0[o] + (und[2 * g] - ref[2 * g]);
+            disp[o + 1] = disp0[o + 1] + (und[2 * g + 1] - ref[2 * g + 1]);
+          }
+        orig = und;
+        check(ptg_equiproj_set_data(ctx_, G, ref.data(), nimg, disp.data(), ncc.data(), (int32_t)(pairs.size() / 2),
+                                    pairs.data(), s.NCC_min),
+              "ptg_equiproj_set_data");
+        check(ptg_equiproj_set_origins(ctx_, orig.data()), "ptg_equiproj_set_origins");
+      }
+      f_end = crit(X);
+    }
     // the reference reports the criterion divided by the number of image pairs (:234-236)
     const double index_ini = f_ini / nimg, index_end = f_end / nimg;
     std::ofstream log(out_dir + "corrDisto_log.txt");
@@ -243,8 +305,72 @@ class ptracker_gpu {
                                           "A1_corrParallax     ", "angle2_corrParallax ", "A2_corrParallax     "};
     for (int q = 0; q < 14; q++) log << names[q] << X[q] << (q == 13 ? "" : "\n");
     log << std::endl;
+    write_correction_field(X, f0.width, f0.height, out_dir, log);
     for (int q = 0; q < 14; q++) s.disto_parameters[q] = X[q];
     return X;
+  }
+
+  // The correction field of the fitted parameters, :515-608: errors.txt, undistor(p) - p on
+  // a grid of steps dimx / 50, dimy / 50 (at least 1) with a blank line after each column,
+  // the largest components and length in the log, and distobox.txt along the frame's edge.
+  void write_correction_field(const std::vector<double>& X, int dimx, int dimy, const std::string& out_dir,
+                              std::ostream& log) {
+    const double dxd = std::max(1.0, (double)dimx / 50.0), dyd = std::max(1.0, (double)dimy / 50.0);
+    auto undistort = [&](const std::vector<double>& pts) {
+      std::vector<double> u(pts.size());
+      if (!pts.empty())
+        check(ptg_undistort_points(ctx_, X.data(), (int32_t)(pts.size() / 2), pts.data(), u.data()),
+              "ptg_undistort_points");
+      return u;
+    };
+    std::vector<double> grid;
+    std::vector<size_t> col_end;
+    for (double xd = 0.0; xd < dimx; xd += dxd) {
+      for (double yd = 0.0; yd < dimy; yd += dyd) {
+        grid.push_back(xd);
+        grid.push_back(yd);
+      }
+      col_end.push_back(grid.size() / 2);
+    }
+    const std::vector<double> gu = undistort(grid);
+    std::ofstream ef(out_dir + "errors.txt");
+    double xmax = -1e20, ymax = -1e20, emax = -1e20;
+    size_t c = 0;
+    for (size_t i = 0; i < grid.size() / 2; i++) {
+      const double xe = gu[2 * i] - grid[2 * i], ye = gu[2 * i + 1] - grid[2 * i + 1];
+      const double e = sqrt(xe * xe + ye * ye);
+      xmax = std::max(xmax, std::fabs(xe));
+      ymax = std::max(ymax, std::fabs(ye));
+      emax = std::max(emax, e);
+      ef << grid[2 * i] << ' ' << grid[2 * i + 1] << ' ' << xe << ' ' << ye << std::endl;
+      if (i + 1 == col_end[c]) {
+        ef << std::endl;
+        c++;
+      }
+    }
+    log << "max |xu - xd| " << xmax << '\n';
+    log << "max |yu - yd| " << ymax << '\n';
+    log << "errormax " << emax << std::endl;
+    std::cout << std::scientific << std::setprecision(15) << "max |xu - xd| " << xmax << '\n'
+              << "max |yu - yd| " << ymax << '\n'
+              << "errormax      " << emax << std::endl;
+    // the edge: top left to right, right side down, bottom right to left, left side up, and
+    // the first corner again
+    std::vector<double> box;
+    auto add = [&](double x, double y) {
+      box.push_back(x);
+      box.push_back(y);
+    };
+    for (double xd = 0; xd < dimx; xd += dxd) add(xd, 0.0);
+    for (double yd = 0; yd < dimy; yd += dyd) add(dimx - 1, yd);
+    for (double xd = dimx - 1; xd >= 0; xd -= dxd) add(xd, dimy - 1);
+    for (double yd = dimy - 1; yd >= 0; yd -= dyd) add(0.0, yd);
+    add(0.0, 0.0);
+    const std::vector<double> bu = undistort(box);
+    std::ofstream bf(out_dir + "distobox.txt");
+    for (size_t i = 0; i < box.size() / 2; i++)
+      bf << box[2 * i] << " " << box[2 * i + 1] << " " << bu[2 * i] - box[2 * i] << " " << bu[2 * i + 1] - box[2 * i + 1]
+         << std::endl;
   }
 
   // Searches (whole stages) and grains the exact form has handed to the plain FP64 form

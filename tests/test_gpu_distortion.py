@@ -85,6 +85,39 @@ def test_jacobian_against_central_differences(tracker, views):
         assert np.abs(jac[:, j] - cd).max() < 1e-6 * scale, (j, np.abs(jac[:, j] - cd).max() / scale)
 
 
+def test_residuals_with_given_first_positions_equal_oracle(tracker, views):
+    # the POWELL_ITER form (src/distortion_correction.cpp:316-354): A1, B1 given, only the
+    # tracked positions undistorted; residuals bit-identical to the oracle's, the Jacobian
+    # (the tracked points' terms only) against central differences, and back to the
+    # standard form when the origins are cleared
+    X, ref, disp, ncc, pairs = views
+    P = every_term()
+    rng = np.random.default_rng(22)
+    orig = ref + rng.normal(0, 2.0, ref.shape)
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, NCC_min=0.7)
+    tracker.equiproj_set_origins(orig)
+    f, jtr, jtj, res, jac = tracker.equiproj_eval(P, full=True)
+    exp = ptref.equiproj_residuals_orig(P, orig, ref, disp, ncc, pairs, NCC_min=0.7)
+    assert bits_equal(res, exp)
+    assert np.allclose(jtr, jac.T @ res, rtol=1e-11, atol=0.0)
+    h = np.array([0.1, 0.1, 2e-12, 1e-19, 3e-27, 2e-10, 1e-10, 3e-10, 1e-3, 1e-3, 1e-4, 1e-5, 1e-4, 2e-5])
+    for j in range(14):
+        Pp, Pm = P.copy(), P.copy()
+        Pp[j] += h[j]
+        Pm[j] -= h[j]
+        cd = (ptref.equiproj_residuals_orig(Pp, orig, ref, disp, ncc, pairs, 0.7) -
+              ptref.equiproj_residuals_orig(Pm, orig, ref, disp, ncc, pairs, 0.7)) / (2 * h[j])
+        if j in (8, 9):
+            # a shift of the parallax centre moves both tracked points alike: no dependence
+            assert np.abs(jac[:, j]).max() <= 1e-9 * np.abs(jac).max() and np.abs(cd).max() < 1e-6
+            continue
+        scale = np.abs(cd).max()
+        assert np.abs(jac[:, j] - cd).max() < 1e-6 * scale, (j, np.abs(jac[:, j] - cd).max() / scale)
+    tracker.equiproj_set_origins(None)
+    _, _, _, res2, _ = tracker.equiproj_eval(P, full=True)
+    assert bits_equal(res2, ptref.equiproj_residuals(P, ref, disp, ncc, pairs, NCC_min=0.7))
+
+
 def test_jacobian_against_minpack_forward_differences(tracker, views):
     _, ref, disp, ncc, pairs = views
     P = every_term()
@@ -196,9 +229,10 @@ def test_differentiation_form_on_distorted_frames(tracker):
     assert abs(got[2] - C.params14[2]) < 0.1 * abs(C.params14[2])
 
 
-def test_program_correction_distortion(tracker, tmp_path):
-    # procedure correction_distortion of the tracking program on TIFF frames: the same fit
-    # as through the C ABI on the same decoded frames
+def _calibration_program(tmp_path, algo, extra=""):
+    """Writes the calibration set as TIFF frames with a command file (procedure
+    correction_distortion, optim_algo algo) and runs the tracking program on them; returns the
+    set, the frames as the program decodes them and the log's entries."""
     import os
     import subprocess
 
@@ -230,20 +264,18 @@ pattern grain_diameter 1.0 4
 image_numbers_corrDisto {len(C.frames)} {' '.join(str(k) for k in range(len(C.frames)))}
 imposedDisplApprox_corrDisto {shifts}
 equiProj_dist_range 0 1000
+optim_algo {algo}
 K1_corrDistor 0
 K2_corrDistor 0
-""" + "".join(f"d{n} 0\n" for n in ("xc_corrDistor", "yc_corrDistor", "K3_corrDistor", "P1_corrDistor",
-                                     "P2_corrDistor", "P3_corrDistor", "xc_corrParallax", "yc_corrParallax",
-                                     "angle1_corrParallax", "A1_corrParallax", "angle2_corrParallax",
-                                     "A2_corrParallax")))
-    say("program: correction_distortion")
-    r = subprocess.run([exe, "--lm", "commands.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+""" + extra + "".join(f"d{n} 0\n" for n in ("xc_corrDistor", "yc_corrDistor", "K3_corrDistor", "P1_corrDistor",
+                                             "P2_corrDistor", "P3_corrDistor", "xc_corrParallax", "yc_corrParallax",
+                                             "angle1_corrParallax", "A1_corrParallax", "angle2_corrParallax",
+                                             "A2_corrParallax")))
+    say(f"program: correction_distortion, {algo}")
+    r = subprocess.run([exe, "--lm", "commands.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    log = dict(l.rsplit(" ", 1) for l in (tmp_path / "corrDisto_log.txt").read_text().splitlines() if l.strip())
-    K1, K2 = float(log["K1_corrDistor"]), float(log["K2_corrDistor"])
-    assert float(log["Final distorsion index"]) < float(log["Initial distorsion index"])
-    assert os.path.exists(tmp_path / "dic_out_3.txt")
-    # the same steps through the C ABI on the frames as the program decodes them
+    log = {k.strip(): v for k, v in (l.rsplit(" ", 1) for l in (tmp_path / "corrDisto_log.txt").read_text()
+                                      .splitlines() if l.strip())}
     frames = []
     for k in range(len(C.frames)):
         subprocess.run([io, "tiff", str(tmp_path / f"cal{k}.tif"), str(tmp_path / "f.raw")], check=True)
@@ -252,6 +284,12 @@ K2_corrDistor 0
         l2 = b.index(b"\n", l1 + 1)
         w, h = map(int, b[:l1].split())
         frames.append(np.frombuffer(b[l2 + 1:], dtype="<u2").reshape(h, w).copy())
+    return C, frames, log
+
+
+def _calibration_data(tracker, C, frames):
+    """The program's tracking steps through the C ABI: reference pixels, displacements,
+    NCC_subpix and pairs."""
     pats = [synth.circ_pattern(max(int(r), 4)) for r in C.grains["radius_pix"]]
     pxy = np.ascontiguousarray(np.concatenate(pats), dtype=np.int32)
     pb = np.zeros(C.G + 1, dtype=np.int64)
@@ -272,12 +310,135 @@ K2_corrDistor 0
         disp.append(np.stack([out["dx"], out["dy"]], axis=1))
         ncc.append(out["NCC_subpix"])
     ref = np.stack([C.grains["refcoord_xpix"], C.grains["refcoord_ypix"]], axis=1).astype(np.float64)
-    tracker.equiproj_set_data(ref, np.array(disp), np.array(ncc), ptref.precompute_pairs(ref, 0.0, 1000.0), 0.7)
+    return ref, np.array(disp), np.array(ncc), ptref.precompute_pairs(ref, 0.0, 1000.0)
+
+
+def _lm_fit(tracker, C, ref, disp, ncc, pairs):
+    tracker.equiproj_set_data(ref, disp, ncc, pairs, 0.7)
     free = np.zeros(14, dtype=np.int32)
     free[[2, 3]] = 1
     start = C.params14.copy()
     start[2] = start[3] = 0.0
-    got, _, _, _ = tracker.calibrate_distortion(start, free, max_iter=200)
+    got, _, f_end, _ = tracker.calibrate_distortion(start, free, max_iter=200)
+    return got, f_end
+
+
+def _expected_field(X, dimx, dimy):
+    """errors.txt and distobox.txt as the reference writes them (src/distortion_correction.cpp:
+    515-608: default stream formatting, that of printf's %g), from the oracle's undistor."""
+    g = lambda v: "%g" % v  # noqa: E731
+    dxd, dyd = max(1.0, dimx / 50.0), max(1.0, dimy / 50.0)
+    lines, box = [], []
+    errs = []
+    xd = 0.0
+    while xd < dimx:
+        yd = 0.0
+        while yd < dimy:
+            xu, yu = ptref.undistor(X, xd, yd)
+            errs.append((xu - xd, yu - yd))
+            lines.append(f"{g(xd)} {g(yd)} {g(xu - xd)} {g(yu - yd)}")
+            yd += dyd
+        lines.append("")
+        xd += dxd
+    pts = []
+    xd = 0.0
+    while xd < dimx:
+        pts.append((xd, 0.0))
+        xd += dxd
+    yd = 0.0
+    while yd < dimy:
+        pts.append((dimx - 1.0, yd))
+        yd += dyd
+    xd = dimx - 1.0
+    while xd >= 0:
+        pts.append((xd, dimy - 1.0))
+        xd -= dxd
+    yd = dimy - 1.0
+    while yd >= 0:
+        pts.append((0.0, yd))
+        yd -= dyd
+    pts.append((0.0, 0.0))
+    for x, y in pts:
+        xu, yu = ptref.undistor(X, x, y)
+        box.append(f"{g(x)} {g(y)} {g(xu - x)} {g(yu - y)}")
+    e = np.array(errs)
+    return "\n".join(lines) + "\n", "\n".join(box) + "\n", np.abs(e[:, 0]).max(), np.abs(e[:, 1]).max(), \
+        np.hypot(e[:, 0], e[:, 1]).max()
+
+
+def test_program_correction_distortion(tracker, tmp_path):
+    # procedure correction_distortion of the tracking program on TIFF frames, LEVMAR: the
+    # same fit as through the C ABI on the same decoded frames; the correction field files
+    # byte for byte as the reference writes them from the oracle's undistor
+    C, frames, log = _calibration_program(tmp_path, "LEVMAR")
+    K1, K2 = float(log["K1_corrDistor"]), float(log["K2_corrDistor"])
+    assert float(log["Final distorsion index"]) < float(log["Initial distorsion index"])
+    assert (tmp_path / "dic_out_3.txt").exists()
+    ref, disp, ncc, pairs = _calibration_data(tracker, C, frames)
+    got, _ = _lm_fit(tracker, C, ref, disp, ncc, pairs)
     say(f"program K1 {K1:.6e} K2 {K2:.6e}; C ABI K1 {got[2]:.6e} K2 {got[3]:.6e}")
     assert abs(K1 - got[2]) <= 1e-9 * abs(got[2]) and abs(K2 - got[3]) <= 1e-9 * abs(got[3])
     assert abs(K1 - C.params14[2]) < 0.2 * abs(C.params14[2])
+    # the held parameters are the set's own (centres at the frame centre, the rest 0); K1, K2
+    # the fitted doubles (the log prints 16 digits)
+    X = C.params14.copy()
+    X[2], X[3] = got[2], got[3]
+    for q, name in ((0, "xc_corrDistor"), (1, "yc_corrDistor"), (4, "K3_corrDistor"), (8, "xc_corrParallax"),
+                    (11, "A1_corrParallax")):
+        assert float(log[name]) == X[q], name
+    errors, box, xm, ym, em = _expected_field(X, frames[0].shape[1], frames[0].shape[0])
+    assert (tmp_path / "errors.txt").read_text() == errors
+    assert (tmp_path / "distobox.txt").read_text() == box
+    assert abs(float(log["max |xu - xd|"]) - xm) <= 1e-12 * xm and abs(float(log["max |yu - yd|"]) - ym) <= 1e-12 * ym
+    assert abs(float(log["errormax"]) - em) <= 1e-12 * em
+
+
+def test_program_correction_distortion_powell(tracker, tmp_path):
+    # optim_algo POWELL: Powell's method from the perturbations reaches the minimum the
+    # Levenberg-Marquardt fit finds on the same data
+    C, frames, log = _calibration_program(tmp_path, "POWELL")
+    K1, K2 = float(log["K1_corrDistor"]), float(log["K2_corrDistor"])
+    ref, disp, ncc, pairs = _calibration_data(tracker, C, frames)
+    got, f_lm = _lm_fit(tracker, C, ref, disp, ncc, pairs)
+    f_pw = float(log["Final distorsion index"]) * (len(frames) - 1)
+    say(f"Powell K1 {K1:.6e} K2 {K2:.6e}, f {f_pw:.6e}; LM K1 {got[2]:.6e} K2 {got[3]:.6e}, f {f_lm:.6e}")
+    assert abs(K1 - got[2]) <= 1e-4 * abs(got[2]) and abs(K2 - got[3]) <= 1e-4 * abs(got[3])
+    assert f_pw <= f_lm * (1 + 1e-8)
+    assert (tmp_path / "errors.txt").exists() and (tmp_path / "distobox.txt").exists()
+
+
+def test_program_correction_distortion_powell_iter(tracker, tmp_path):
+    # optim_algo POWELL_ITER with 3 runs: each run's index and parameters (convergence.txt)
+    # against the same loop in Python, each run's minimum found by MINPACK (scipy leastsq) on
+    # the oracle's residuals of that form, the first positions re-centred between runs
+    from scipy.optimize import leastsq
+
+    C, frames, log = _calibration_program(tmp_path, "POWELL_ITER", extra="powell_num_iter 3\n")
+    rows = [l.split() for l in (tmp_path / "convergence.txt").read_text().splitlines()]
+    assert len(rows) == 3 and all(len(r) == 16 for r in rows)
+    ref, disp0, ncc, pairs = _calibration_data(tracker, C, frames)
+    nimg = len(frames) - 1
+    X = C.params14.copy()
+    X[2] = X[3] = 0.0
+    orig = ref.copy()
+    disp = disp0.copy()
+    for it, row in enumerate(rows, start=1):
+        def fun(k, orig=orig, disp=disp):
+            P = X.copy()
+            P[2], P[3] = k[0] * 1e-8, k[1] * 1e-14
+            return ptref.equiproj_residuals_orig(P, orig, ref, disp, ncc, pairs, 0.7)
+        k, _ = leastsq(fun, [X[2] / 1e-8, X[3] / 1e-14], xtol=1e-14, ftol=1e-15)
+        X[2], X[3] = k[0] * 1e-8, k[1] * 1e-14
+        idx = (fun(k) ** 2).sum() / nimg
+        got = np.array([float(v) for v in row])
+        say(f"run {it}: index {got[1]:.9e} (MINPACK {idx:.9e}), K1 {got[4]:.6e} ({X[2]:.6e}), K2 {got[5]:.6e} ({X[3]:.6e})")
+        assert int(got[0]) == it
+        assert abs(got[1] - idx) <= 1e-8 * idx
+        assert abs(got[4] - X[2]) <= 1e-4 * abs(X[2]) and abs(got[5] - X[3]) <= 1e-4 * abs(X[3])
+        # the program continues from its own parameters
+        X[2], X[3] = got[4], got[5]
+        und = np.array([ptref.undistor(X, x, y) for x, y in ref])
+        disp = disp0 + (und - ref)[None, :, :]
+        orig = und
+    f_end = (ptref.equiproj_residuals_orig(X, orig, ref, disp, ncc, pairs, 0.7) ** 2).sum() / nimg
+    assert abs(float(log["Final distorsion index"]) - f_end) <= 1e-8 * f_end

@@ -672,6 +672,31 @@ def test_equiproj_vanishes_for_rigid_motion_at_the_true_lens_only():
     assert (r[1, drop] == 0.0).all() and (r[0, drop] != 0.0).all()
 
 
+def test_equiproj_residuals_with_given_first_positions():
+    # the POWELL_ITER form: with the first positions set to the undistorted reference pixels it
+    # is the standard form bit for bit; with them set to the reference pixels it differs by
+    # moving only the tracked points, which a hand evaluation of one pair confirms
+    X = _lens()
+    rng = np.random.default_rng(6)
+    ref = np.round(rng.uniform(300, 5700, (20, 2)))
+    disp = _rigid_views(X, ref, [(36.0, 11.0), (-29.0, 33.0)])
+    pairs = ptref.precompute_pairs(ref, 0.0, 2000.0)
+    ncc = np.ones((2, 20))
+    ncc[0, 4] = 0.2
+    und = np.array([ptref.undistor(X, x, y) for x, y in ref])
+    std = ptref.equiproj_residuals(X, ref, disp, ncc, pairs)
+    assert np.array_equal(ptref.equiproj_residuals_orig(X, und, ref, disp, ncc, pairs), std)
+    r = ptref.equiproj_residuals_orig(X, ref, ref, disp, ncc, pairs).reshape(2, -1)
+    k, l = 1, 7
+    i, j = pairs[l]
+    a2 = np.array(ptref.undistor(X, *(ref[i] + disp[k][i])))
+    b2 = np.array(ptref.undistor(X, *(ref[j] + disp[k][j])))
+    e = (ref[j] - ref[i]) / np.hypot(*(ref[j] - ref[i]))
+    assert abs(r[k, l] - e @ ((b2 - ref[j]) - (a2 - ref[i]))) < 1e-9
+    drop = (pairs[:, 0] == 4) | (pairs[:, 1] == 4)
+    assert (r[0, drop] == 0.0).all()
+
+
 def test_minpack_least_squares_recovers_the_lens():
     # the reference's LEVMAR path is MINPACK lmdif (forward-difference Jacobian); scipy's
     # leastsq is that routine. On scaled K1, K2 it recovers the lens from rigid views.
