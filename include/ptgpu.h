@@ -292,9 +292,25 @@ int ptg_set_timing(void* ctx, int enable);
  *  PTG_OPT_SEARCH_STREAMS [2]: 2 overlaps an exact search's radius buckets over three
  *  streams (each bucket's preparation kernels ahead of the searches, consecutive searches
  *  on two streams); 1 runs every kernel of the search in order on the context's stream, so
- *  that per-launch timings (PTG_T_SEARCH_KERNEL, a profiler's trace) do not overlap. */
-enum { PTG_OPT_SEARCH_STREAMS = 0 };
+ *  that per-launch timings (PTG_T_SEARCH_KERNEL, a profiler's trace) do not overlap.
+ *  PTG_OPT_ROT_TABLES [0]: where the rotation tables cos/sin(refrot + drot + step)
+ *  (src/p-tracker.cpp:636-637, 687-689) are formed. 0: on the device, from the device
+ *  library's cosine and sine; every (grain, step) with a rotated pattern coordinate within
+ *  (|dx| + |dy| + 2) 2^-44 of a half-integer (where a cosine or sine differing from the host C
+ *  library's by up to 2^-50 could change a rounded offset) is listed, and after the call's
+ *  stages the host compares those with its own C library's values and the offsets they give;
+ *  if any offset differs, the call's records are put back and its work is redone on the host's
+ *  tables (the rotated offsets, hence the picks, are the reference's either way). 1: on the
+ *  host, as the reference (a host round trip per stage).
+ *  PTG_OPT_ROT_GUARD_TEST [0]: a test of that check: 1 lists every (grain, step), 2 also
+ *  redoes every call whose list is not empty. */
+enum { PTG_OPT_SEARCH_STREAMS = 0, PTG_OPT_ROT_TABLES = 1, PTG_OPT_ROT_GUARD_TEST = 2 };
 int ptg_set_option(void* ctx, int option, int64_t value);
+/* The rotation guard's counters since the context was created: guard entries the host
+ * compared, calls redone on the host's tables, and the largest difference between a device
+ * cosine or sine and the host C library's among the compared entries (the guard assumes at
+ * most 2^-50). Synchronises. */
+int ptg_get_rot_guard_stats(void* ctx, int64_t* checked, int64_t* redone, double* max_diff);
 /* The exact search's autotuner; no counterpart in the reference (it replaces the launch
  * shapes of the built-in heuristic, which picks them from the LDS and register budgets).
  * For each radius bucket of a main-stage search over `zone` on the current frames and

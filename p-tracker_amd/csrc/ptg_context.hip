@@ -274,7 +274,17 @@ int do_precompute(Ctx* c) {
   TRY(need_grains(c));
   if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, false));
+  if (!c->rot_host) {
+    // cos/sin(refrot) on the device, guarded (checked by the caller after its stages)
+    hipEvent_t ev;
+    c->mark_begin(PTG_T_PRECOMPUTE, &ev);
+    CK(c, launch_precompute(*c, true));
+    c->mark_end(PTG_T_PRECOMPUTE, ev);
+    c->require_precomputations = false;
+    return PTG_OK;
+  }
   // cos/sin(refrot) per grain from the host's libm, as do_precomputations (:636-637)
+  if (c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
   TRY(c->wait_staging());
   TRY(c->pin_cs0.ensure(*c, sizeof(double) * 2 * c->G));
   double* cs0 = (double*)c->pin_cs0.p;
@@ -286,13 +296,49 @@ int do_precompute(Ctx* c) {
   c->mark_begin(PTG_T_PRECOMPUTE, &ev);
   CK(c, hipMemcpyAsync(c->d_cs0, cs0, sizeof(double) * 2 * c->G, hipMemcpyHostToDevice, c->stream));
   TRY(c->mark_staging());
-  CK(c, launch_precompute(*c));
+  CK(c, launch_precompute(*c, false));
   c->mark_end(PTG_T_PRECOMPUTE, ev);
   c->require_precomputations = false;
   return PTG_OK;
 }
 
 constexpr size_t kMaxLds = 160 * 1024 - 1024;  // dynamic LDS; the rest for the kernels' static arrays
+
+// Rotation tables of a stage from the host's libm: cos/sin(refrot + drot + step) per (grain,
+// step) of the list, evaluated as the reference (:687-689), uploaded with the step values.
+static int host_rot_tables(Ctx* c, const std::vector<int32_t>& list, int stage) {
+  const int nlist = (int)list.size(), nsteps = (int)c->h_steps.size();
+  TRY(c->pin_cs.ensure(*c, sizeof(double) * 2 * (size_t)c->G * nsteps));
+  double* cs = (double*)c->pin_cs.p;
+  const double* steps = c->h_steps.data();
+  // the CPUs this process may use less four, left to the GPU runtime's threads and the
+  // caller's (with every CPU taken, one preempted thread holds the whole loop for a
+  // scheduler time slice); chunks of 8 grains taken in turn
+  const int nthr = std::max(1, std::min(omp_get_max_threads(), omp_get_num_procs() - 4));
+  (void)nthr;  // (the device compilation pass ignores the pragma)
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
+  for (int li = 0; li < nlist; li++) {
+    const int g = list[li];
+    const ptg_grain& gr = c->h_grains[g];
+    double* row = cs + 2 * (size_t)g * nsteps;
+    for (int k = 0; k < nsteps; k++) {
+      double total_rot = gr.refrot + gr.drot + steps[k];
+      row[2 * k] = cos(total_rot);
+      row[2 * k + 1] = sin(total_rot);
+    }
+  }
+  // upload only the rows in use: for the main stage that is nearly all of them
+  if (stage == PTG_STAGE_MAIN) {
+    CK(c, hipMemcpyAsync(c->d_cs, cs, sizeof(double) * 2 * (size_t)c->G * nsteps, hipMemcpyHostToDevice, c->stream));
+  } else {
+    for (int li = 0; li < nlist; li++) {
+      const size_t off = 2 * (size_t)list[li] * nsteps;
+      CK(c, hipMemcpyAsync(c->d_cs + off, cs + off, sizeof(double) * 2 * nsteps, hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
+  return c->mark_staging();
+}
 
 int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, double NCC_min_super, int method) {
   TRY(need_grains(c));
@@ -316,11 +362,11 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   // next queue counter starts at zero (enqueued ahead of the wait for this stage's list)
   if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
   if (stage == PTG_STAGE_MAIN) {
-    if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
+    if (c->rot_host && c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
     list = c->h_unmasked;  // built at set/put time
   } else {
     // the queue was filled on the device by the previous stage
-    if (c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
+    if (c->rot_host && c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
     int32_t n = 0;
     CK(c, hipMemcpyAsync(&n, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     TRY(c->sync_and_check());
@@ -331,45 +377,34 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   const int nlist = (int)list.size();
   if (nlist == 0) return PTG_OK;
 
-  // rotation tables: cos/sin(refrot + drot + step) per (grain, step) from the host's
-  // libm, evaluated as the reference (:687-689)
+  // rotation tables: cos/sin(refrot + drot + step) per (grain, step), evaluated as the
+  // reference (:687-689), on the device (guarded) or from the host's libm
   TRY(c->wait_staging());
-  TRY(c->pin_cs.ensure(*c, sizeof(double) * 2 * (size_t)c->G * nsteps));
   TRY(c->pin_steps.ensure(*c, sizeof(double) * nsteps));
-  double* cs = (double*)c->pin_cs.p;
   memcpy(c->pin_steps.p, c->h_steps.data(), sizeof(double) * nsteps);
-  const double* steps = c->h_steps.data();
-  // the CPUs this process may use less four, left to the GPU runtime's threads and the
-  // caller's (with every CPU taken, one preempted thread holds the whole loop for a
-  // scheduler time slice); chunks of 8 grains taken in turn
-  const int nthr = std::max(1, std::min(omp_get_max_threads(), omp_get_num_procs() - 4));
-  (void)nthr;  // (the device compilation pass ignores the pragma)
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
-  for (int li = 0; li < nlist; li++) {
-    const int g = list[li];
-    const ptg_grain& gr = c->h_grains[g];
-    double* row = cs + 2 * (size_t)g * nsteps;
-    for (int k = 0; k < nsteps; k++) {
-      double total_rot = gr.refrot + gr.drot + steps[k];
-      row[2 * k] = cos(total_rot);
-      row[2 * k + 1] = sin(total_rot);
-    }
-  }
   TRY(c->ensure((void**)&c->d_cs, &c->cs_cap, sizeof(double) * 2 * (size_t)c->G * nsteps));
   TRY(c->ensure((void**)&c->d_steps, &c->steps_cap, sizeof(double) * nsteps));
   const int nparts_cap = std::max(nsteps, z->left + z->right + 1);
   TRY(c->ensure((void**)&c->d_best, &c->best_cap, sizeof(StepBest) * (size_t)nlist * nparts_cap));
-  // upload only the rows in use: for the main stage that is nearly all of them
-  if (stage == PTG_STAGE_MAIN) {
-    CK(c, hipMemcpyAsync(c->d_cs, cs, sizeof(double) * 2 * (size_t)c->G * nsteps, hipMemcpyHostToDevice, c->stream));
+  if (!c->rot_host) {
+    CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
+    TRY(c->mark_staging());
+    RotArgs r;
+    r.grains = c->d_grains;
+    r.pattern = c->d_pattern;
+    r.pbegin = c->d_pbegin;
+    r.list = d_list;
+    r.count = d_count;
+    r.steps = c->d_steps;
+    r.nsteps = nsteps;
+    r.all = c->rot_guard_test ? 1 : 0;
+    r.cs = c->d_cs;
+    r.guard = c->d_guard;
+    r.guard_count = c->d_guard_count;
+    CK(c, launch_rot_tables(*c, r, nlist));
   } else {
-    for (int li = 0; li < nlist; li++) {
-      const size_t off = 2 * (size_t)list[li] * nsteps;
-      CK(c, hipMemcpyAsync(c->d_cs + off, cs + off, sizeof(double) * 2 * nsteps, hipMemcpyHostToDevice, c->stream));
-    }
+    TRY(host_rot_tables(c, list, stage));
   }
-  CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
-  TRY(c->mark_staging());
 
   // LDS of the plain kernel: the largest window and pattern of the list; a window beyond
   // the LDS is searched in square tiles of shifts
@@ -467,6 +502,80 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   return PTG_OK;
 }
 
+// The records at the start of work done on device rotation tables, for a redo.
+int save_grains(Ctx* c) {
+  if (c->rot_host || c->G == 0) return PTG_OK;
+  TRY(c->ensure((void**)&c->d_grains_save, &c->grains_save_cap, sizeof(ptg_grain) * c->G));
+  CK(c, hipMemcpyAsync(c->d_grains_save, c->d_grains, sizeof(ptg_grain) * c->G, hipMemcpyDeviceToDevice, c->stream));
+  return PTG_OK;
+}
+
+// After work on device rotation tables: each guard entry's device cosine and sine against the
+// host C library's for the same angle and, where they differ, the rotated offsets of the grain's
+// pattern with both (src/p-tracker.cpp:615-621). *redo when an offset differs, when the list
+// overflowed, or when the test option asks for it; the list is emptied.
+int check_rot_guard(Ctx* c, bool* redo) {
+  *redo = false;
+  if (c->rot_host) return PTG_OK;
+  int32_t n = 0;
+  CK(c, hipMemcpyAsync(&n, c->d_guard_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  if (n == 0) return PTG_OK;
+  std::vector<RotGuardEntry> ent(std::min(n, kRotGuardCap));
+  CK(c, hipMemcpyAsync(ent.data(), c->d_guard, sizeof(RotGuardEntry) * ent.size(), hipMemcpyDeviceToHost,
+                       c->stream));
+  CK(c, fill_i32(*c, c->d_guard_count, 1, 0));
+  CK(c, hipStreamSynchronize(c->stream));
+  c->rot_checked += (int64_t)ent.size();
+  if (n > kRotGuardCap) {
+    *redo = true;
+    return PTG_OK;
+  }
+  if (c->rot_guard_test == 2) *redo = true;
+  for (const RotGuardEntry& e : ent) {
+    const double ch = cos(e.theta), sh = sin(e.theta);
+    c->rot_max_diff = std::max(c->rot_max_diff, std::max(fabs(ch - e.c), fabs(sh - e.s)));
+    if (ch == e.c && sh == e.s) continue;
+    for (int64_t i = c->h_pbegin[e.g]; i < c->h_pbegin[e.g + 1]; i++) {
+      const double dx = (double)c->h_pattern[2 * i], dy = (double)c->h_pattern[2 * i + 1];
+      const double xd = e.c * dx + e.s * dy, yd = -e.s * dx + e.c * dy;
+      const double xh = ch * dx + sh * dy, yh = -sh * dx + ch * dy;
+      if (round(xd) != round(xh) || round(yd) != round(yh)) {
+        *redo = true;
+        break;
+      }
+    }
+  }
+  return PTG_OK;
+}
+
+// Puts back the records saved by save_grains and marks the precomputations stale, before a
+// redo on the host's tables.
+int restore_grains(Ctx* c) {
+  CK(c, hipMemcpyAsync(c->d_grains, c->d_grains_save, sizeof(ptg_grain) * c->G, hipMemcpyDeviceToDevice, c->stream));
+  c->snap = Ctx::SNAP_INVALID;
+  c->require_precomputations = true;
+  return PTG_OK;
+}
+
+// Runs fn() on device rotation tables and checks the guard; when an entry's offsets differ,
+// puts the records back and runs fn() again on the host's tables.
+template <class F>
+int with_rot_guard(Ctx* c, F fn) {
+  if (c->rot_host || c->G == 0) return fn();
+  TRY(save_grains(c));
+  TRY(fn());
+  bool redo = false;
+  TRY(check_rot_guard(c, &redo));
+  if (!redo) return PTG_OK;
+  TRY(restore_grains(c));
+  c->rot_host = true;
+  const int rc = fn();
+  c->rot_host = false;
+  c->rot_redos++;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -503,15 +612,19 @@ int ptg_create(void** out, int device) {
   auto undo = [c]() {
     if (c->d_counts) (void)hipFree(c->d_counts);
     if (c->d_evals) (void)hipFree(c->d_evals);
+    if (c->d_guard_count) (void)hipFree(c->d_guard_count);
     (void)hipStreamDestroy(c->stream);
     delete c;
   };
   if ((e = hipMalloc((void**)&c->d_counts, sizeof(int32_t) * 4)) != hipSuccess ||
-      (e = hipMalloc((void**)&c->d_evals, sizeof(int64_t))) != hipSuccess) {
+      (e = hipMalloc((void**)&c->d_evals, sizeof(int64_t))) != hipSuccess ||
+      (e = hipMalloc((void**)&c->d_guard_count, 64 + sizeof(RotGuardEntry) * kRotGuardCap)) != hipSuccess) {
     undo();
     return create_fail(PTG_ERR_NOMEM, "hipMalloc", e);
   }
+  c->d_guard = reinterpret_cast<RotGuardEntry*>(reinterpret_cast<char*>(c->d_guard_count) + 64);
   k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_counts, 4, 0);
+  k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_guard_count, 1, 0);
   k_fill_i64<<<1, 64, 0, c->stream>>>((long long*)c->d_evals, 1, 0);
   if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     undo();
@@ -547,7 +660,8 @@ int ptg_destroy(void* p) {
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
                   c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
-                  c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac, c->d_eq_orig};
+                  c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac, c->d_eq_orig, c->d_guard_count,
+                  c->d_grains_save};
   for (void* b : more)
     if (b) (void)hipFree(b);
   free_exact_plans(*c);
@@ -828,16 +942,18 @@ int ptg_begin_pair(void* p) {
 int ptg_precompute(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
-  return do_precompute(c);
+  return with_rot_guard(c, [c]() -> int { return do_precompute(c); });
 }
 
 int ptg_search(void* p, int stage, const ptg_search_zone* zone, double NCC_min, double NCC_min_super, int method) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
-  TRY(do_search(c, stage, zone, NCC_min, NCC_min_super, method));
-  // an exact search reports its capacity flag before returning
-  if (c->any_flags_pending()) TRY(c->sync_and_check());
-  return PTG_OK;
+  return with_rot_guard(c, [&]() -> int {
+    TRY(do_search(c, stage, zone, NCC_min, NCC_min_super, method));
+    // an exact search reports its capacity flag before returning
+    if (c->any_flags_pending()) TRY(c->sync_and_check());
+    return PTG_OK;
+  });
 }
 
 // Zeroes the subpixel stage's evaluation counters (the total and the per-grain counts).
@@ -882,7 +998,7 @@ int ptg_subpixel(void* p, const ptg_subpix_params* prm) {
   TRY(need_grains(c));
   if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, true));
-  return subpixel_run(c, prm, true);
+  return with_rot_guard(c, [&]() -> int { return subpixel_run(c, prm, true); });
 }
 
 int ptg_end_pair(void* p) {
@@ -905,6 +1021,14 @@ int ptg_change_reference(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   TRY(need_grains(c));
+  if (!c->rot_host) {
+    // nothing on the host needs the records: the device tables read them where they are
+    CK(c, launch_change_reference(*c));
+    c->snap = Ctx::SNAP_INVALID;
+    std::swap(c->iref, c->icur);
+    c->require_precomputations = true;
+    return PTG_OK;
+  }
   if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
   CK(c, launch_change_reference(*c));
   // the same update on the host snapshot keeps it valid (src/p-tracker.cpp:598-609)
@@ -923,14 +1047,14 @@ int ptg_change_reference(void* p) {
   return PTG_OK;
 }
 
-int ptg_track_pair(void* p, const ptg_track_params* prm) {
-  Ctx* c = as_ctx(p);
-  if (!c) return PTG_ERR_ARG;
-  if (!prm) return c->fail(PTG_ERR_ARG, "null track params");
-  TRY(need_grains(c));
-  TRY(need_images(c, true, true));
-  if (prm->subpixel) TRY(subpixel_check(c, &prm->subpix));
-  if (c->G == 0) return PTG_OK;
+// The precomputations, when stale, for work outside a guarded stage: checked at once.
+static int ensure_precomputed(Ctx* c) {
+  if (!c->require_precomputations) return PTG_OK;
+  return with_rot_guard(c, [c]() -> int { return do_precompute(c); });
+}
+
+static int track_pair_run(Ctx* c, const ptg_track_params* prm) {
+  void* p = c;
   if (c->require_precomputations) TRY(do_precompute(c));
   if (prm->rescue_level >= 0) {
     TRY(ptg_begin_pair(p));
@@ -950,6 +1074,17 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
   return PTG_OK;
 }
 
+int ptg_track_pair(void* p, const ptg_track_params* prm) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!prm) return c->fail(PTG_ERR_ARG, "null track params");
+  TRY(need_grains(c));
+  TRY(need_images(c, true, true));
+  if (prm->subpixel) TRY(subpixel_check(c, &prm->subpix));
+  if (c->G == 0) return PTG_OK;
+  return with_rot_guard(c, [&]() -> int { return track_pair_run(c, prm); });
+}
+
 // The exact search's autotuner: for each radius bucket of the main stage, every launch shape
 // of a candidate set (N-tiles per pass 1-3, waves per workgroup 1-8, workgroups per grain 1, 2
 // or 4) is tried on the current frames and grains, and the whole search's GPU span (events on
@@ -965,7 +1100,7 @@ int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, doub
   int32_t tried = 0;
   double before = 0.0, after = 0.0;
   if (c->G > 0) {
-    if (c->require_precomputations) TRY(do_precompute(c));
+    TRY(ensure_precomputed(c));
     const size_t gb = sizeof(ptg_grain) * (size_t)c->G;
     TRY(c->ensure(&c->d_tune_save, &c->tune_save_cap, gb));
     CK(c, hipMemcpyAsync(c->d_tune_save, c->d_grains, gb, hipMemcpyDeviceToDevice, c->stream));
@@ -1040,6 +1175,8 @@ int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, doub
     c->exact_grain_fallbacks = fg0;
     CK(c, hipMemcpyAsync(c->d_grains, c->d_tune_save, gb, hipMemcpyDeviceToDevice, c->stream));
     c->snap = Ctx::SNAP_INVALID;
+    // the tuning searches' guard entries concern records that were put back
+    CK(c, fill_i32(*c, c->d_guard_count, 1, 0));
     CK(c, hipStreamSynchronize(c->stream));
     TRY(rc);
   }
@@ -1083,7 +1220,7 @@ static int objective_common(Ctx* c, int32_t n, const int32_t* igrain, const doub
   for (int k = 0; k < n; k++)
     if (igrain[k] < 0 || igrain[k] >= c->G) return c->fail(PTG_ERR_ARG, "grain index %d out of range", igrain[k]);
   if (n == 0) return PTG_OK;
-  if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(ensure_precomputed(c));
   const size_t bi = ((size_t)n * 4 + 255) / 256 * 256, bx = (size_t)n * 24, bf = (size_t)n * 8, bg = (size_t)n * 24;
   TRY(tmp_upload(c, bi + bx + bf + bg));
   char* base = (char*)c->d_tmp;
@@ -1407,7 +1544,7 @@ int ptg_joint_eval(void* p, int32_t n, const int32_t* igrain, const double* P, c
   for (int k = 0; k < n; k++)
     if (igrain[k] < 0 || igrain[k] >= c->G) return c->fail(PTG_ERR_ARG, "grain index %d out of range", igrain[k]);
   if (n == 0) return PTG_OK;
-  if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(ensure_precomputed(c));
   int32_t* d_list;
   double *d_P, *d_out;
   TRY(joint_buffers(c, n, &d_list, &d_P, &d_out));
@@ -1469,7 +1606,7 @@ int ptg_refine_joint_distortion(void* p, double* lens, double ncc_min, int32_t m
   if (!lens || max_iter < 0) return c->fail(PTG_ERR_ARG, "bad joint refinement arguments");
   TRY(need_grains(c));
   TRY(need_images(c, true, true));
-  if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(ensure_precomputed(c));
   if (c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
   // the grains that take part: unmasked, with a subpixel NCC of at least ncc_min
   std::vector<int32_t> list;
@@ -1683,7 +1820,32 @@ int ptg_set_option(void* p, int option, int64_t value) {
     c->serial_search = value == 1;
     return PTG_OK;
   }
+  if (option == PTG_OPT_ROT_TABLES) {
+    if (value != 0 && value != 1) return c->fail(PTG_ERR_ARG, "PTG_OPT_ROT_TABLES takes 0 or 1, not %lld",
+                                                  (long long)value);
+    TRY(c->sync_and_check());
+    bool redo = false;
+    TRY(check_rot_guard(c, &redo));  // (entries of unguarded work are dropped)
+    c->rot_host = value == 1;
+    c->snap = Ctx::SNAP_INVALID;
+    return PTG_OK;
+  }
+  if (option == PTG_OPT_ROT_GUARD_TEST) {
+    if (value < 0 || value > 2) return c->fail(PTG_ERR_ARG, "PTG_OPT_ROT_GUARD_TEST takes 0, 1 or 2");
+    c->rot_guard_test = (int)value;
+    return PTG_OK;
+  }
   return c->fail(PTG_ERR_ARG, "unknown option %d", option);
+}
+
+int ptg_get_rot_guard_stats(void* p, int64_t* checked, int64_t* redone, double* max_diff) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(c->sync_and_check());
+  if (checked) *checked = c->rot_checked;
+  if (redone) *redone = c->rot_redos;
+  if (max_diff) *max_diff = c->rot_max_diff;
+  return PTG_OK;
 }
 
 int ptg_set_timing(void* p, int enable) {

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ptg_internal.hpp"
+
 namespace ptg {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -19,6 +21,36 @@ __device__ __forceinline__ void rotate_px(int dx, int dy, double c, double s, in
   double ypix = -s * (double)dx + c * (double)dy;
   xo = (int)round(xpix);
   yo = (int)round(ypix);
+}
+
+// Whether the rotation of (dx, dy) by (c, s) could round differently with a cosine and sine that
+// differ from c and s by up to 2^-50 (8 units in the last place of values in [0.5, 1): the device
+// library and the host C library are each accurate to a few units): x = c dx + s dy then moves by
+// at most (|dx| + |dy|) 2^-50 plus the roundings of the products and the sum (each below
+// (|dx| + |dy|) 2^-52), well inside m = (|dx| + |dy| + 2) 2^-44, and round(x) can change only if x
+// lies within m of a half-integer. Likewise y = -s dx + c dy.
+__device__ __forceinline__ bool rot_near_half(int dx, int dy, double c, double s) {
+  const double x = c * (double)dx + s * (double)dy;
+  const double y = -s * (double)dx + c * (double)dy;
+  const double m = (double)(abs(dx) + abs(dy) + 2) * 0x1p-44;
+  const double fx = x - floor(x), fy = y - floor(y);
+  return fabs(fx - 0.5) <= m || fabs(fy - 0.5) <= m;
+}
+
+// Appends one entry to the rotation guard list (entries beyond its capacity are counted only:
+// the host then redoes the work with its own tables).
+__device__ __forceinline__ void rot_guard_push(RotGuardEntry* guard, int32_t* count, int g, int k, double theta,
+                                               double c, double s) {
+  const int pos = atomicAdd(count, 1);
+  if (pos < kRotGuardCap) {
+    RotGuardEntry e;
+    e.g = g;
+    e.k = k;
+    e.theta = theta;
+    e.c = c;
+    e.s = s;
+    guard[pos] = e;
+  }
 }
 
 // CubicInterpolate, src/interpol.cpp:42-53, in the reference's operation order.

@@ -39,6 +39,31 @@ struct ExactBest {
   int32_t ovf, pad;
 };
 
+// A (grain, rotation step) whose device rotation might round differently from the host C
+// library's: the angle and the device's cosine and sine, for the host to compare with its own
+// (k = -1: the reference rotation refrot of the precomputation).
+struct RotGuardEntry {
+  int32_t g, k;
+  double theta, c, s;
+};
+// Entries the guard list holds; more than that sends the stage back to the host tables.
+constexpr int kRotGuardCap = 65536;
+
+// Arguments of the device rotation tables (k_rot_tables).
+struct RotArgs {
+  const ptg_grain* grains;
+  const int32_t* pattern;   // (dx, dy) pairs
+  const int64_t* pbegin;    // G + 1
+  const int32_t* list;      // grains of the stage
+  const int32_t* count;     // device count of list entries
+  const double* steps;      // rotation-step values
+  int32_t nsteps;
+  int32_t all;              // list every (grain, step) in the guard (a test of the host check)
+  double* cs;               // out: per grain, steps x (cos, sin)
+  RotGuardEntry* guard;
+  int32_t* guard_count;
+};
+
 // Arguments of the integer-search kernels (plain and exact forms).
 struct SearchArgs {
   const uint16_t* I0;
@@ -132,6 +157,17 @@ struct Context {
   StepBest* d_best = nullptr;
   size_t best_cap = 0;
   double* d_cs0 = nullptr;  // cos, sin of refrot per grain (precompute)
+  // rotation tables: computed on the device (default) with a guard list checked by the host
+  // after each stage, or on the host with its C library (rot_host, also the guard's fallback)
+  bool rot_host = false;
+  int rot_guard_test = 0;  // 1: list every (grain, step); 2: also treat every entry as differing
+  int32_t* d_guard_count = nullptr;  // the guard list's count, then its entries
+  RotGuardEntry* d_guard = nullptr;
+  int64_t rot_redos = 0;   // stages (or pairs) redone with the host tables
+  int64_t rot_checked = 0; // guard entries the host compared
+  double rot_max_diff = 0.0;  // largest |device - host| cosine or sine among them
+  ptg_grain* d_grains_save = nullptr;  // the records at the start of the work a redo repeats
+  size_t grains_save_cap = 0;
 
   // subpixel
   int64_t* d_evals = nullptr;
@@ -213,7 +249,11 @@ hipError_t ensure_kernel_lds(int dev, const void* fn, size_t bytes);
 
 // launchers (each returns hipError_t of the launch)
 hipError_t launch_begin_pair(Context& c);
-hipError_t launch_precompute(Context& c);
+// dev_trig: cos/sin(refrot) on the device, with the guard of k_rot_tables (k = -1); else the
+// host's from d_cs0.
+hipError_t launch_precompute(Context& c, bool dev_trig);
+// The rotation tables of a stage's grains on the device (grid: list_cap x step blocks).
+hipError_t launch_rot_tables(Context& c, const RotArgs& r, int list_cap);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);
 // The plain kernel for patterns whose offsets and window exceed the LDS: the same candidates
 // and arithmetic, the frame read through the caches.

@@ -90,16 +90,25 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
                                                             const double* __restrict__ cs0, int32_t* __restrict__ p0,
                                                             double* __restrict__ d0, double* __restrict__ sd0,
                                                             uint32_t* __restrict__ pwords,
-                                                            uint32_t* __restrict__ spwords, double* __restrict__ m0) {
+                                                            uint32_t* __restrict__ spwords, double* __restrict__ m0,
+                                                            int dev_trig, int guard_all,
+                                                            RotGuardEntry* __restrict__ guard,
+                                                            int32_t* __restrict__ guard_count) {
   __shared__ __align__(16) double chunk[2][kPreChunk];
   __shared__ long long part[kPreThreads / 64];
+  __shared__ int near_flag;
   const int g = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   ptg_grain& gr = grains[g];
   const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
-  const double c0 = cs0[2 * g], s0 = cs0[2 * g + 1];
+  // cos/sin(refrot) (:636-637): the device library's, guarded as the rotation tables are,
+  // or the host's
+  const double c0 = dev_trig ? cos(gr.refrot) : cs0[2 * g], s0 = dev_trig ? sin(gr.refrot) : cs0[2 * g + 1];
+  if (tid == 0) near_flag = dev_trig && guard_all;
+  __syncthreads();
+  bool near = false;
   long long s = 0;
   bool clamped = false;
   // four pixels per thread and pass, their loads issued together
@@ -115,6 +124,7 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
     }
 #pragma unroll
     for (int u = 0; u < kU; u++) {
+      if (dev_trig && rot_near_half(px[u], py[u], c0, s0)) near = true;
       rotate_px(px[u], py[u], c0, s0, x[u], y[u]);
       int X = rx + x[u], Y = ry + y[u];
       if (i0 + u * kPreThreads < n && (X < 0 || Y < 0 || X >= dimx || Y >= dimy)) clamped = true;
@@ -140,6 +150,7 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
     }
   }
   if (clamped) atomicOr(&gr.status, (int)PTG_STATUS_CLAMPED);
+  if (near) near_flag = 1;  // (every writer stores the same value)
   s = wave_sum_i64(s);
   if (lane == 0) part[wave] = s;
   __syncthreads();  // also makes every thread's packed words visible to the whole workgroup
@@ -191,15 +202,67 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
     gr.C0C0 = C0C0;
     sd0[g] = sd;
     m0[g] = mean0;
+    if (near_flag) rot_guard_push(guard, guard_count, g, -1, gr.refrot, c0, s0);
   }
 }
 
-hipError_t launch_precompute(Context& c) {
+hipError_t launch_precompute(Context& c, bool dev_trig) {
   if (c.G == 0) return hipSuccess;
   const Image& im = c.img[c.iref];
   k_precompute<<<c.G, kPreThreads, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern,
                                                   c.d_pbegin, c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords,
-                                                  c.d_spwords, c.d_m0);
+                                                  c.d_spwords, c.d_m0, dev_trig ? 1 : 0,
+                                                  c.rot_guard_test ? 1 : 0, c.d_guard, c.d_guard_count);
+  return hipGetLastError();
+}
+
+// Rotation tables of a stage on the device: cos and sin of refrot + drot + step_k, summed left to
+// right as src/p-tracker.cpp:687-689 does, from the device library. One workgroup per (list
+// entry, block of kRotSteps steps): the first lanes form the block's values, then every lane
+// checks its pattern pixels against them (rot_near_half) and a step with a pixel near a rounding
+// boundary goes on the guard list.
+constexpr int kRotThreads = 256;
+constexpr int kRotSteps = 8;
+
+__global__ __launch_bounds__(kRotThreads) void k_rot_tables(RotArgs r) {
+  __shared__ double tc[kRotSteps], ts[kRotSteps], tth[kRotSteps];
+  __shared__ int flag[kRotSteps];
+  const int li = blockIdx.x;
+  if (li >= *r.count) return;
+  const int g = r.list[li];
+  const int k0 = blockIdx.y * kRotSteps;
+  const int nk = min(kRotSteps, r.nsteps - k0);
+  const int t = threadIdx.x;
+  if (t < nk) {
+    const ptg_grain& gr = r.grains[g];
+    const double th = gr.refrot + gr.drot + r.steps[k0 + t];
+    const double c = cos(th), s = sin(th);
+    tc[t] = c;
+    ts[t] = s;
+    tth[t] = th;
+    flag[t] = r.all;
+    r.cs[2 * ((int64_t)g * r.nsteps + k0 + t)] = c;
+    r.cs[2 * ((int64_t)g * r.nsteps + k0 + t) + 1] = s;
+  }
+  __syncthreads();
+  const int64_t b = r.pbegin[g];
+  const int n = (int)(r.pbegin[g + 1] - b);
+  int near = 0;  // bit j: step k0 + j has a pixel near a boundary
+  for (int i = t; i < n; i += kRotThreads) {
+    const int dx = r.pattern[2 * (b + i)], dy = r.pattern[2 * (b + i) + 1];
+    for (int j = 0; j < nk; j++)
+      if (rot_near_half(dx, dy, tc[j], ts[j])) near |= 1 << j;
+  }
+  if (near)
+    for (int j = 0; j < nk; j++)
+      if ((near >> j) & 1) flag[j] = 1;  // (every writer stores the same value)
+  __syncthreads();
+  if (t < nk && flag[t]) rot_guard_push(r.guard, r.guard_count, g, k0 + t, tth[t], tc[t], ts[t]);
+}
+
+hipError_t launch_rot_tables(Context& c, const RotArgs& r, int list_cap) {
+  if (list_cap == 0 || r.nsteps == 0) return hipSuccess;
+  k_rot_tables<<<dim3(list_cap, (r.nsteps + kRotSteps - 1) / kRotSteps), kRotThreads, 0, c.stream>>>(r);
   return hipGetLastError();
 }
 
