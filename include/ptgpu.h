@@ -303,8 +303,15 @@ int ptg_set_timing(void* ctx, int enable);
  *  tables (the rotated offsets, hence the picks, are the reference's either way). 1: on the
  *  host, as the reference (a host round trip per stage).
  *  PTG_OPT_ROT_GUARD_TEST [0]: a test of that check: 1 lists every (grain, step), 2 also
- *  redoes every call whose list is not empty. */
-enum { PTG_OPT_SEARCH_STREAMS = 0, PTG_OPT_ROT_TABLES = 1, PTG_OPT_ROT_GUARD_TEST = 2 };
+ *  redoes every call whose list is not empty.
+ *  PTG_OPT_DEVICE_QUEUES [0]: 1, with device tables: a rescue stage's queue (the grains the
+ *  previous stage left below its threshold, src/p-tracker.cpp:732-735, 865-866) stays on the
+ *  device: the exact search is planned once over all unmasked grains and serves the queued
+ *  ones, its launches sized for all, with no host round trip between the stages (what a captured
+ *  pair needs; an empty queue still costs the stage's launches). 0 reads the queue back and
+ *  plans the stage for it (an empty queue: no launch).
+ */
+enum { PTG_OPT_SEARCH_STREAMS = 0, PTG_OPT_ROT_TABLES = 1, PTG_OPT_ROT_GUARD_TEST = 2, PTG_OPT_DEVICE_QUEUES = 3 };
 int ptg_set_option(void* ctx, int option, int64_t value);
 /* The rotation guard's counters since the context was created: guard entries the host
  * compared, calls redone on the host's tables, and the largest difference between a device

@@ -170,7 +170,15 @@ struct Ctx : Context {
   // kernel, copied to pin_flags[2 stage], pin_flags[2 stage + 1] and read at the next
   // synchronisation of the stream
   bool flags_pending[3] = {false, false, false};
+  // a rescue stage whose queue stayed on the device ran the plain form in place of the exact
+  // one: its queue count, copied to pin_flags[6 + stage], says whether a search took place
+  bool unfit_pending[3] = {false, false, false};
   int check_flags() {
+    for (int st = 0; st < 3; st++) {
+      if (!unfit_pending[st]) continue;
+      unfit_pending[st] = false;
+      if (static_cast<const int32_t*>(pin_flags.p)[6 + st] > 0) exact_fallbacks++;
+    }
     for (int st = 0; st < 3; st++) {
       if (!flags_pending[st]) continue;
       flags_pending[st] = false;
@@ -189,7 +197,10 @@ struct Ctx : Context {
     if (e != hipSuccess) return check(e, "hipStreamSynchronize");
     return check_flags();
   }
-  bool any_flags_pending() const { return flags_pending[0] || flags_pending[1] || flags_pending[2]; }
+  bool any_flags_pending() const {
+    return flags_pending[0] || flags_pending[1] || flags_pending[2] || unfit_pending[0] || unfit_pending[1] ||
+           unfit_pending[2];
+  }
   enum { SNAP_INVALID = 0, SNAP_VALID = 1, SNAP_QUEUES = 2 } snap = SNAP_INVALID;
   std::vector<double> h_steps;
   // recorded after the last upload from the pinned staging buffers, waited on before
@@ -361,9 +372,14 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   int32_t* d_count = c->d_counts + stage;
   // next queue counter starts at zero (enqueued ahead of the wait for this stage's list)
   if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
+  // a rescue stage on device tables keeps its queue on the device: its exact search is planned
+  // over the unmasked list and serves the queued grains among them (no host round trip)
+  const bool devq = stage != PTG_STAGE_MAIN && !c->rot_host && c->device_queues;
   if (stage == PTG_STAGE_MAIN) {
     if (c->rot_host && c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
     list = c->h_unmasked;  // built at set/put time
+  } else if (devq) {
+    list = c->h_unmasked;  // (the launches are sized for it; the queue is read on the device)
   } else {
     // the queue was filled on the device by the previous stage
     if (c->rot_host && c->snap == Ctx::SNAP_INVALID) TRY(sync_snapshot(c));
@@ -445,6 +461,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   a.prad = c->d_prad;
   a.list = d_list;
   a.count = d_count;
+  a.act = nullptr;
   a.cs = c->d_cs;
   a.steps = c->d_steps;
   a.nsteps = nsteps;
@@ -463,19 +480,37 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   c->mark_begin(tstage, &ev);
   int nparts = nsteps;
   const int32_t* flag_dev = nullptr;
+  SearchArgs ca = a;  // the commit's list: the stage's, or the plan's with the queued slots marked
   if (method != PTG_SEARCH_PLAIN) {
-    // the unmasked list is the same on every main-stage call until it is rebuilt
-    const uint64_t key = stage == PTG_STAGE_MAIN ? c->unmasked_version : 0;
+    // the unmasked list is the same on every call until it is rebuilt
+    const uint64_t key = stage == PTG_STAGE_MAIN || devq ? c->unmasked_version : 0;
     const double thr = stage == PTG_STAGE_MAIN ? NCC_min : NCC_min_super;
-    const int rc =
-        run_search_exact(*c, a, nlist, list, key, thr, c->d_best, &nparts, &flag_dev, stage == PTG_STAGE_MAIN);
+    SearchArgs ax = a;
+    if (devq) {
+      ax.list = c->d_list[0];
+      ax.count = c->d_counts;
+    }
+    const int32_t* act = nullptr;
+    const int rc = run_search_exact(*c, ax, nlist, list, key, thr, c->d_best, &nparts, &flag_dev,
+                                    stage == PTG_STAGE_MAIN, stage, devq ? d_list : nullptr, devq ? d_count : nullptr,
+                                    &act);
     if (rc == kExactUnfit) {
       method = PTG_SEARCH_PLAIN;
       nparts = nsteps;
-      c->exact_fallbacks++;
+      if (devq) {
+        // counted at the next synchronisation if the queue held grains
+        TRY(c->pin_flags.ensure(*c, 9 * sizeof(int32_t)));
+        CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + 6 + stage, d_count, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+        c->unfit_pending[stage] = true;
+      } else {
+        c->exact_fallbacks++;
+      }
     } else {
+      ca = ax;
+      ca.act = act;
       TRY(rc);
-      TRY(c->pin_flags.ensure(*c, 6 * sizeof(int32_t)));
+      TRY(c->pin_flags.ensure(*c, 9 * sizeof(int32_t)));
       CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + 2 * stage, flag_dev, 2 * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
       c->flags_pending[stage] = true;
@@ -493,7 +528,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   }
   int32_t* next_list = stage < PTG_STAGE_SUPER ? c->d_list[stage + 1] : nullptr;
   int32_t* next_count = stage < PTG_STAGE_SUPER ? c->d_counts + stage + 1 : nullptr;
-  CK(c, launch_search_commit(*c, a, nlist, stage, NCC_min, NCC_min_super, next_list, next_count, c->d_best,
+  CK(c, launch_search_commit(*c, ca, nlist, stage, NCC_min, NCC_min_super, next_list, next_count, c->d_best,
                              nparts, flag_dev));
   c->mark_end(tstage, ev);
   // successful grains changed their dx/drot; queued ones did not, so the snapshot
@@ -659,7 +694,7 @@ int ptg_destroy(void* p) {
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
-                  c->d_exact_scratch, c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
+                  c->d_exact_scratch[0], c->d_exact_scratch[1], c->d_exact_scratch[2], c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
                   c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac, c->d_eq_orig, c->d_guard_count,
                   c->d_grains_save};
   for (void* b : more)
@@ -1828,6 +1863,11 @@ int ptg_set_option(void* p, int option, int64_t value) {
     TRY(check_rot_guard(c, &redo));  // (entries of unguarded work are dropped)
     c->rot_host = value == 1;
     c->snap = Ctx::SNAP_INVALID;
+    return PTG_OK;
+  }
+  if (option == PTG_OPT_DEVICE_QUEUES) {
+    if (value != 0 && value != 1) return c->fail(PTG_ERR_ARG, "PTG_OPT_DEVICE_QUEUES takes 0 or 1");
+    c->device_queues = value == 1;
     return PTG_OK;
   }
   if (option == PTG_OPT_ROT_GUARD_TEST) {

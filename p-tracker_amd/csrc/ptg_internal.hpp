@@ -79,6 +79,7 @@ struct SearchArgs {
   const int32_t* prad;      // per grain: every rotated offset lies in [-prad, prad]^2
   const int32_t* list;      // grains of this stage
   const int32_t* count;     // device count of list entries
+  const int32_t* act;       // per list slot: searched in this stage (null: every slot)
   const double* cs;         // per list slot g: steps x (cos, sin) of refrot + drot + step
   const double* steps;      // rotation-step values
   int32_t nsteps;
@@ -161,6 +162,8 @@ struct Context {
   // after each stage, or on the host with its C library (rot_host, also the guard's fallback)
   bool rot_host = false;
   int rot_guard_test = 0;  // 1: list every (grain, step); 2: also treat every entry as differing
+  // rescue stages' queues stay on the device (with device tables; else they are read back)
+  bool device_queues = false;
   int32_t* d_guard_count = nullptr;  // the guard list's count, then its entries
   RotGuardEntry* d_guard = nullptr;
   int64_t rot_redos = 0;   // stages (or pairs) redone with the host tables
@@ -177,10 +180,12 @@ struct Context {
   double* d_m0 = nullptr;   // per grain: the mean0 d0 was formed with (d0 = I0(p0) - m0)
 
   // exact-form scratch and the cached host plan of the last keyed list
-  void* d_exact_scratch = nullptr;
-  size_t exact_scratch_cap = 0;
-  void* exact_plan = nullptr;
-  uint64_t exact_dev_key = 0;       // plan whose tables the scratch holds (0: none)
+  // (one of each per search stage: the stages' zones differ, and their device tables stay
+  // where a captured pair's kernels find them)
+  void* d_exact_scratch[3] = {nullptr, nullptr, nullptr};
+  size_t exact_scratch_cap[3] = {0, 0, 0};
+  void* exact_plan[3] = {nullptr, nullptr, nullptr};
+  uint64_t exact_dev_key[3] = {0, 0, 0};  // plan whose tables the scratch holds (0: none)
   uint64_t unmasked_version = 0;    // bumped whenever the unmasked list is rebuilt
   // launch shapes chosen by ptg_autotune_search, by bucket key (value mode, steps, zone,
   // largest pattern radius and grain count of the bucket); a tuning run probes tune_shape on
@@ -289,9 +294,15 @@ constexpr int kExactUnfit = 1;
 // NCC_min_super): a grain whose NCC lies within rounding of it is re-scored in the
 // reference's arithmetic.
 // time_kernel: bracket each MFMA kernel launch with the PTG_T_SEARCH_KERNEL timer.
+// stage: the plan and scratch slot (each stage keeps its own). queue/qcount (device, or null):
+// the stage's grains are the queued ones among the list's (a rescue stage searched without a
+// host read-back of its queue): the slots of the queued grains are marked in a.act (which the
+// call sets) and listed per radius bucket on the device, and every launch is sized for the whole
+// list, its surplus workgroups returning at once.
 int run_search_exact(Context& c, const SearchArgs& a, int nlist, const std::vector<int32_t>& list,
                      uint64_t plan_key, double stage_thr, StepBest* out_best, int* nparts_out,
-                     const int32_t** flag_dev, bool time_kernel);
+                     const int32_t** flag_dev, bool time_kernel, int stage, const int32_t* queue,
+                     const int32_t* qcount, const int32_t** act_out);
 void free_exact_plans(Context& c);
 
 // Equiprojectivity criterion of the distortion calibration (device data, see ptgpu.h).

@@ -567,6 +567,7 @@ __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double 
                                 const int32_t* __restrict__ abort_flag) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= *a.count) return;
+  if (a.act && !a.act[li]) return;  // a slot of the plan that is not in this stage's queue
   // a search that could not be served exactly (capacity flag set) changes nothing
   if (abort_flag && *abort_flag) return;
   const int g = a.list[li];

@@ -67,7 +67,8 @@ struct ExactArgs {
   SearchArgs s;
   const ExactSlot* slots;      // per list slot
   const int32_t* bucket;       // list slots of this launch
-  int32_t nbucket;
+  int32_t nbucket;             // their number (with nbucket_dev: the most there can be)
+  const int32_t* nbucket_dev;  // device count of the launch's slots, or null (nbucket of them)
   const int32_t* seg;          // segment table: (qy + rho) << 16 | c0
   uint8_t* tmpl;               // template fragments
   int32_t* dlist;              // Delta cell offsets: per step dcap cells gained, then dcap cells lost (per slot)
@@ -268,6 +269,15 @@ __device__ __forceinline__ void overflow_slot(const ExactArgs& e, int li) {
   else if (atomicExch(&e.ovf_slot[li], 1) == 0) e.ovf_list[atomicAdd(&e.flags[1], 1)] = li;
 }
 
+// Slots of this launch: the device count when the launch serves a rescue stage's queue.
+__device__ __forceinline__ int bucket_n(const ExactArgs& e) { return e.nbucket_dev ? *e.nbucket_dev : e.nbucket; }
+
+// Workgroups per grain of the MFMA search when a radius bucket has n grains: enough for about
+// 512 workgroups in all, at most one per shift column.
+__host__ __device__ __forceinline__ int exact_nsub_of(int n, int U) {
+  return n > 0 ? max(1, min(U, (512 + n - 1) / n)) : 1;
+}
+
 // ---- template fragments and Delta lists: one workgroup per (slot, step) -------------
 // LDS: packed[(2 rho + 1)^2] = sum of a_i | count << 24 for step k; mid[(2 rho + 1)^2]
 // bytes = count of the base mask (from k_exact_mid or k_exact_base_disk, which run first). Rows beyond the
@@ -275,7 +285,7 @@ __device__ __forceinline__ void overflow_slot(const ExactArgs& e, int li) {
 __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int bi = blockIdx.x;
-  if (bi >= e.nbucket) return;
+  if (bi >= bucket_n(e)) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
   const int k = blockIdx.y;  // 0 .. vm_spt * nmt - 1
@@ -428,7 +438,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ long long red[2][16];
   const int bi = blockIdx.x;
-  if (bi >= e.nbucket) return;
+  if (bi >= bucket_n(e)) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
   const int g = e.s.list[li];
@@ -630,7 +640,7 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
   __shared__ long long red[2][8];
   __shared__ int big;
   const int bi = blockIdx.x;
-  if (bi >= e.nbucket) return;
+  if (bi >= bucket_n(e)) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
   const ExactSlot sl = e.slots[li];
@@ -889,9 +899,13 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   // XCD-aware job mapping: all shift blocks of a grain share blockIdx % 8
   const int bx = blockIdx.x;
   const int xcd = bx & 7, t = bx >> 3;
-  const int grp = t / e.nsub, blk = t - grp * e.nsub;
+  const int nbk = bucket_n(e);
+  // workgroups per grain: the launch's, or for a rescue stage's queue the heuristic's for the
+  // queued count (the grid covers the largest count)
+  const int nsub = e.nbucket_dev ? exact_nsub_of(nbk, e.s.left + e.s.right + 1) : e.nsub;
+  const int grp = t / nsub, blk = t - grp * nsub;
   const int bi = grp * 8 + xcd;
-  if (bi >= e.nbucket) return;
+  if (bi >= nbk) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
   if (e.ovf_slot && e.ovf_slot[li]) return;  // served by the plain kernel
@@ -996,7 +1010,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const uint8_t* tmpl = e.tmpl + sl.tmpl_off;
   const int32_t* segs = segl;
   // this workgroup's shift columns [c_lo, c_hi); wave w takes c_lo + w, c_lo + w + nsu, ...
-  const int cpb = (U + e.nsub - 1) / e.nsub;
+  const int cpb = (U + nsub - 1) / nsub;
   const int c_lo = blk * cpb, c_hi = min(U, c_lo + cpb);
   for (int su = c_lo + wave; su < c_hi; su += e.nsu) {
     // stage this shift column: cell c of staged row y is window column su + c; plane 0 holds
@@ -1264,26 +1278,31 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   }
 }
 
-__global__ void k_fill_best(StepBest* p, int n) {
+// Clears a search's outputs and counters in one launch: the partial picks and the blocks'
+// near-tie lists (n of each), the overflow flags of nlist slots (when kept), the capacity flags
+// and, for a rescue stage's queue, the slots' marks and the buckets' counts.
+__global__ void k_exact_reset(StepBest* __restrict__ best, ExactBest* __restrict__ xout, int n,
+                              int32_t* __restrict__ ovf_slot, int32_t* __restrict__ act, int nlist,
+                              int32_t* __restrict__ flags, int32_t* __restrict__ bcount, int nbuckets) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     StepBest sb;
     sb.ncc = 0.0;
     sb.idx = INT_MAX;
     sb.pad = 0;
-    p[i] = sb;
-  }
-}
-
-__global__ void k_fill_exact_best(ExactBest* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+    best[i] = sb;
     ExactBest xb;
     xb.s1 = xb.s2 = -1.0;
     xb.i1 = xb.i2 = INT_MAX;
     xb.ovf = xb.pad = 0;
-    p[i] = xb;
+    xout[i] = xb;
   }
+  if (i < nlist) {
+    if (ovf_slot) ovf_slot[i] = 0;
+    if (act) act[i] = 0;
+  }
+  if (i < 4) flags[i] = 0;
+  if (bcount && i < nbuckets) bcount[i] = 0;
 }
 
 // ---- the grain's near-tie set and its re-scoring --------------------------------------
@@ -1304,6 +1323,7 @@ __global__ __launch_bounds__(64 * kMergeWaves) void k_exact_merge(ExactArgs e) {
   if (bi >= e.nbucket) return;
   const int li = e.bucket[bi];
   if (li >= *e.s.count) return;
+  if (e.s.act && !e.s.act[li]) return;  // not in this rescue stage's queue
   if (lane == 0) e.cand[2 * li] = e.cand[2 * li + 1] = INT_MAX;
   if (e.ovf_slot && e.ovf_slot[li]) return;  // already the plain kernel's
   const int64_t* mom = e.mom + 3 * li;
@@ -1355,6 +1375,7 @@ __global__ __launch_bounds__(kRescoreThreads) void k_exact_rescore(ExactArgs e, 
   if ((int)blockIdx.x >= e.nbucket) return;
   const int li = e.bucket[blockIdx.x];
   if (li >= *e.s.count) return;
+  if (e.s.act && !e.s.act[li]) return;
   if (e.ovf_slot && e.ovf_slot[li]) return;
   const int ca[2] = {e.cand[2 * li], e.cand[2 * li + 1]};
   if (ca[0] == INT_MAX) return;
@@ -1467,9 +1488,20 @@ __global__ __launch_bounds__(kRescoreThreads) void k_exact_rescore(ExactArgs e, 
   }
 }
 
-__global__ void k_zero_i32(int32_t* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// A rescue stage's queued grains as the plan's slots: act[slot] = 1 and the slot appended to its
+// radius bucket's list (bucket_act[bucket_off[b] ...], bcount[b] of them). act and bcount start
+// at zero.
+__global__ void k_exact_activate(const int32_t* __restrict__ queue, const int32_t* __restrict__ qcount,
+                                 const int32_t* __restrict__ slot_of, const int32_t* __restrict__ slot_bucket,
+                                 const int32_t* __restrict__ bucket_off, int32_t* __restrict__ act,
+                                 int32_t* __restrict__ bcount, int32_t* __restrict__ bucket_act) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *qcount) return;
+  const int li = slot_of[queue[j]];
+  if (li < 0) return;  // (a queue holds only the list's grains)
+  act[li] = 1;
+  const int b = slot_bucket[li];
+  bucket_act[bucket_off[b] + atomicAdd(&bcount[b], 1)] = li;
 }
 
 // ---- host side -----------------------------------------------------------------------
@@ -1580,6 +1612,10 @@ struct ExactPlan {
   std::vector<int32_t> msegs;
   std::vector<Bucket> buckets;
   std::vector<int32_t> bucket_cat;
+  // a rescue stage's queue to slots: grain -> slot (-1: not in the list), slot -> bucket,
+  // bucket -> first position of its slots in bucket_cat
+  std::vector<int32_t> slot_of, slot_bucket, bucket_off;
+  size_t b_slot_of = 0, b_slot_bucket = 0, b_bucket_off = 0, b_act = 0, b_bcount = 0, b_bact = 0;
   size_t b_mseg = 0, b_mmask = 0;
   size_t b_slots = 0, b_seg = 0, b_bucket = 0, b_tmpl = 0, b_dl = 0, b_dlen = 0, b_mid = 0, b_mom = 0, b_flags = 0;
   size_t b_ovf = 0, b_xout = 0, b_cand = 0;
@@ -1616,9 +1652,11 @@ int Context::ensure_aux(size_t nevents) {
 }
 
 void free_exact_plans(Context& c) {
-  delete static_cast<ExactPlan*>(c.exact_plan);
-  c.exact_plan = nullptr;
-  c.exact_dev_key = 0;
+  for (int st = 0; st < 3; st++) {
+    delete static_cast<ExactPlan*>(c.exact_plan[st]);
+    c.exact_plan[st] = nullptr;
+    c.exact_dev_key[st] = 0;
+  }
 }
 
 static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t>& list, int nlist, int U, int V,
@@ -1702,8 +1740,22 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     if (P.slots[li].base_pat) b.ndisk++;
   }
   P.bucket_cat.clear();
-  for (auto& bk : P.buckets) P.bucket_cat.insert(P.bucket_cat.end(), bk.slots.begin(), bk.slots.end());
+  P.slot_bucket.assign(nlist, 0);
+  P.bucket_off.assign(P.buckets.size(), 0);
+  for (size_t b = 0; b < P.buckets.size(); b++) {
+    P.bucket_off[b] = (int32_t)P.bucket_cat.size();
+    for (int32_t li : P.buckets[b].slots) P.slot_bucket[li] = (int32_t)b;
+    P.bucket_cat.insert(P.bucket_cat.end(), P.buckets[b].slots.begin(), P.buckets[b].slots.end());
+  }
+  P.slot_of.assign(c.G, -1);
+  for (int li = 0; li < nlist; li++) P.slot_of[list[li]] = li;
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  P.b_slot_of = al(sizeof(int32_t) * (size_t)c.G);
+  P.b_slot_bucket = al(sizeof(int32_t) * (size_t)nlist);
+  P.b_bucket_off = al(sizeof(int32_t) * P.buckets.size());
+  P.b_act = al(sizeof(int32_t) * (size_t)nlist);
+  P.b_bcount = al(sizeof(int32_t) * P.buckets.size());
+  P.b_bact = al(sizeof(int32_t) * (size_t)nlist);
   P.b_slots = al(sizeof(ExactSlot) * nlist);
   P.b_seg = al(sizeof(int32_t) * P.segs.size());
   P.b_mseg = al(sizeof(int32_t) * P.msegs.size());
@@ -1719,13 +1771,15 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
   P.b_xout = al(sizeof(ExactBest) * (size_t)nlist * std::max(U, nsteps));  // the blocks' near-tie lists
   P.b_cand = al(sizeof(int32_t) * 2 * (size_t)nlist);                   // the grains' sets to re-score
   P.total = P.b_mmask + P.b_slots + P.b_seg + P.b_mseg + P.b_bucket + P.b_tmpl + P.b_dl + P.b_dlen + P.b_mid + P.b_mom +
-            P.b_flags + P.b_ovf + P.b_xout + P.b_cand;
+            P.b_flags + P.b_ovf + P.b_xout + P.b_cand + P.b_slot_of + P.b_slot_bucket + P.b_bucket_off + P.b_act +
+            P.b_bcount + P.b_bact;
   P.dev_base = nullptr;
 }
 
 int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vector<int32_t>& list,
                      uint64_t plan_key, double stage_thr, StepBest* out_best, int* nparts_out,
-                     const int32_t** flag_dev, bool time_kernel) {
+                     const int32_t** flag_dev, bool time_kernel, int stage, const int32_t* queue,
+                     const int32_t* qcount, const int32_t** act_out) {
   // the value mode from the frames' declared depth; at most 48 rotation steps (40 for 16-bit
   // frames)
   const Image &i0 = c.img[c.iref], &i1 = c.img[c.icur];
@@ -1744,16 +1798,16 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   ExactPlan local;
   ExactPlan* P = &local;
   if (plan_key != 0) {
-    ExactPlan* cached = static_cast<ExactPlan*>(c.exact_plan);
+    ExactPlan* cached = static_cast<ExactPlan*>(c.exact_plan[stage]);
     if (!cached || cached->key != plan_key || cached->nlist != nlist || cached->U != U || cached->V != V ||
         cached->nsteps != nsteps || cached->vm != vm) {
       if (!cached) {
         cached = new ExactPlan();
-        c.exact_plan = cached;
+        c.exact_plan[stage] = cached;
       }
       build_plan(c, *cached, list, nlist, U, V, nsteps, vm);
       cached->key = plan_key;
-      c.exact_dev_key = 0;
+      c.exact_dev_key[stage] = 0;
     }
     P = cached;
   } else {
@@ -1852,9 +1906,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       c.tune_used[bi] = ExactShape{L[bi].ntp, L[bi].nsu, L[bi].nsub};
     }
   }
-  int rc = c.ensure(&c.d_exact_scratch, &c.exact_scratch_cap, P->total);
+  int rc = c.ensure(&c.d_exact_scratch[stage], &c.exact_scratch_cap[stage], P->total);
   if (rc != PTG_OK) return rc;
-  char* base = (char*)c.d_exact_scratch;
+  char* base = (char*)c.d_exact_scratch[stage];
   ExactArgs e;
   e.s = sa;
   e.slots = (const ExactSlot*)base;
@@ -1887,6 +1941,21 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   e.xout = (ExactBest*)base;
   base += P->b_xout;
   e.cand = (int32_t*)base;
+  base += P->b_cand;
+  char* slot_of_dev = base;
+  base += P->b_slot_of;
+  char* slot_bucket_dev = base;
+  base += P->b_slot_bucket;
+  char* bucket_off_dev = base;
+  base += P->b_bucket_off;
+  int32_t* act_dev = (int32_t*)base;
+  base += P->b_act;
+  int32_t* bcount_dev = (int32_t*)base;
+  base += P->b_bcount;
+  int32_t* bact_dev = (int32_t*)base;
+  e.nbucket_dev = nullptr;
+  e.s.act = queue ? act_dev : nullptr;
+  if (act_out) *act_out = e.s.act;
   // grains whose lists overflow are searched by the plain kernel when its LDS takes the
   // list's largest window and pattern (else an overflow voids the stage)
   const size_t lds_plain = ((size_t)sa.win_cap * 2 + 15) / 16 * 16 + (size_t)sa.off_cap * 4;
@@ -1899,10 +1968,16 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   e.vp = P->vp;
   hipError_t he;
   // the tables: uploaded unless the scratch already holds this keyed plan's
-  const bool fresh = plan_key != 0 && c.exact_dev_key == plan_key && P->dev_base == c.d_exact_scratch;
+  const bool fresh = plan_key != 0 && c.exact_dev_key[stage] == plan_key && P->dev_base == c.d_exact_scratch[stage];
   if (!fresh) {
     if ((he = hipMemcpyAsync((void*)e.slots, P->slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
                              c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(slot_of_dev, P->slot_of.data(), sizeof(int32_t) * P->slot_of.size(),
+                             hipMemcpyHostToDevice, c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(slot_bucket_dev, P->slot_bucket.data(), sizeof(int32_t) * P->slot_bucket.size(),
+                             hipMemcpyHostToDevice, c.stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(bucket_off_dev, P->bucket_off.data(), sizeof(int32_t) * P->bucket_off.size(),
+                             hipMemcpyHostToDevice, c.stream)) != hipSuccess ||
         (he = hipMemcpyAsync(seg_dev, P->segs.data(), sizeof(int32_t) * P->segs.size(), hipMemcpyHostToDevice,
                              c.stream)) != hipSuccess ||
         (he = hipMemcpyAsync(mseg_dev, P->msegs.data(), sizeof(int32_t) * P->msegs.size(), hipMemcpyHostToDevice,
@@ -1912,19 +1987,13 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return c.check(he, "exact search tables");
     if (plan_key != 0) {
       // the cached host tables outlive the copies
-      P->dev_base = c.d_exact_scratch;
-      c.exact_dev_key = plan_key;
+      P->dev_base = c.d_exact_scratch[stage];
+      c.exact_dev_key[stage] = plan_key;
     } else {
       // this call's tables are freed on return: let the copies finish first
       if ((he = hipStreamSynchronize(c.stream)) != hipSuccess) return c.check(he, "hipStreamSynchronize");
-      c.exact_dev_key = 0;
+      c.exact_dev_key[stage] = 0;
     }
-  }
-  k_zero_i32<<<1, 64, 0, c.stream>>>(e.flags, 4);
-  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_zero_i32");
-  if (e.ovf_slot) {
-    k_zero_i32<<<(nlist + 255) / 256, 256, 0, c.stream>>>(e.ovf_slot, nlist);
-    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_zero_i32");
   }
   // shift blocks: results of block b of slot li go to out_best[li * part_cap + b]; the
   // plain kernel writes step k of an overflowed slot to out_best[li * part_cap + k]; the
@@ -1934,10 +2003,21 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   e.out = out_best;
   e.thr = stage_thr;
   *nparts_out = part_cap;
-  k_fill_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(out_best, nlist * part_cap);
-  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_best");
-  k_fill_exact_best<<<(nlist * part_cap + 255) / 256, 256, 0, c.stream>>>(e.xout, nlist * part_cap);
-  if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_fill_exact_best");
+  {
+    const int nb_tab = (int)P->buckets.size();
+    const int n_all = std::max(nlist * part_cap, std::max(nlist, nb_tab));
+    k_exact_reset<<<(n_all + 255) / 256, 256, 0, c.stream>>>(out_best, e.xout, nlist * part_cap, e.ovf_slot,
+                                                             queue ? act_dev : nullptr, nlist, e.flags,
+                                                             queue ? bcount_dev : nullptr, nb_tab);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_reset");
+  }
+  if (queue) {
+    // the queued grains' slots, per radius bucket
+    k_exact_activate<<<(nlist + 255) / 256, 256, 0, c.stream>>>(
+        queue, qcount, (const int32_t*)slot_of_dev, (const int32_t*)slot_bucket_dev, (const int32_t*)bucket_off_dev,
+        act_dev, bcount_dev, bact_dev);
+    if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_activate");
+  }
   // the template and base-mask kernels of every bucket run in order on the second
   // stream; each bucket's MFMA kernel runs after its bucket's event, on the library stream
   // and the search streams in turn, so the next bucket's preparation overlaps the current
@@ -1964,8 +2044,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
     const BucketLaunch& l = L[bi];
-    e.bucket = bucket_dev + off;
+    e.bucket = (queue ? bact_dev : bucket_dev) + off;
     e.nbucket = nb;
+    e.nbucket_dev = queue ? bcount_dev + bi : nullptr;
     off += nb;
     const int rho = bk.rho_max;
     if (l.vb > 0) {
@@ -2012,8 +2093,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
     const BucketLaunch& l = L[bi];
-    e.bucket = bucket_dev + off;
+    e.bucket = (queue ? bact_dev : bucket_dev) + off;
     e.nbucket = nb;
+    e.nbucket_dev = queue ? bcount_dev + bi : nullptr;
     off += nb;
     hipStream_t sx = lanes[nlaunch++ % nlanes];
     if (!serial && (he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess)
@@ -2027,8 +2109,13 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     eb.dl_lds_cap = l.dl_cap;
     eb.seg_cap = bk.max_kc;
     eb.abuf_alias = l.alias ? 1 : 0;
-    const int groups = (nb + 7) / 8;
-    dim3 grid(8 * groups * eb.nsub), block(64 * eb.nsu);
+    // the grid: nsub workgroups per grain, or for a queue the most any queued count needs
+    int nwg = 8 * ((nb + 7) / 8) * eb.nsub;
+    if (queue) {
+      nwg = 0;
+      for (int n = 1; n <= nb; n++) nwg = std::max(nwg, 8 * ((n + 7) / 8) * exact_nsub_of(n, U));
+    }
+    dim3 grid(nwg), block(64 * eb.nsu);
     hipEvent_t ev = nullptr;
     if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev, sx);
     if ((he = dispatch_main(c.device, vm, nmt, l.ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
@@ -2044,8 +2131,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // grains the merge hands over join the plain kernel's list)
   {
     ExactArgs ea = e;
-    ea.bucket = bucket_dev;  // the buckets' slot lists, concatenated
+    ea.bucket = bucket_dev;  // the buckets' slot lists, concatenated (a queue's marked in act)
     ea.nbucket = nlist;
+    ea.nbucket_dev = nullptr;
     k_exact_merge<<<(nlist + kMergeWaves - 1) / kMergeWaves, 64 * kMergeWaves, 0, c.stream>>>(ea);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_merge");
     k_exact_rescore<<<nlist, kRescoreThreads, 0, c.stream>>>(ea, out_best);

@@ -130,17 +130,22 @@ def rescue_variant():
     return w, fx
 
 
-@pytest.mark.parametrize("method", ["plain", "exact"])
-def test_full_size_rescue_zone_equals_oracle(tracker, rescue_variant, method):
+@pytest.mark.parametrize("method,devq", [("plain", 0), ("exact", 0), ("exact", 1)])
+def test_full_size_rescue_zone_equals_oracle(tracker, rescue_variant, method, devq):
     # the rescue stage at its own geometry: +-40 px x 21 steps (137,781 candidates per grain)
     # for the grains the +-20 px main search misses; picks bit-identical to the oracle's, NCC
-    # and NCC_rescue bit-identical for the plain form and within 1e-12 for the exact one
+    # and NCC_rescue bit-identical for the plain form and within 1e-12 for the exact one; the
+    # rescue queue read back by the host, or kept on the device (devq)
     w, fx = rescue_variant
     n0 = tracker.exact_fallbacks()
-    got = run(tracker, w, pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN, 0)
+    tracker.set_option(pk.OPT_DEVICE_QUEUES, devq)
+    try:
+        got = run(tracker, w, pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN, 0)
+    finally:
+        tracker.set_option(pk.OPT_DEVICE_QUEUES, 0)
     n1 = tracker.exact_fallbacks()
     counts = list(tracker.queue_counts())
-    say(f"full size rescue variant, {method}: queues {counts}, exact-form fallbacks (searches, grains) "
+    say(f"full size rescue variant, {method}, device queue {devq}: queues {counts}, exact-form fallbacks (searches, grains) "
         f"{n1[0] - n0[0]}, {n1[1] - n0[1]}")
     assert counts == list(fx["counts"]) and counts[0] >= 250
     for c in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):

@@ -6,7 +6,8 @@ tables use the device library's cosine and sine. Each (grain, step) with a rotat
 near a rounding boundary is put on a guard list, which the host checks against its own values
 after the call. The tests here:
   - whole pairs (rescue and super-rescue, both search forms, a sequence with reference swaps,
-    so refrot is not zero) give the same records bit for bit on both tables;
+    so refrot is not zero) give the same records bit for bit on both tables, and with the
+    rescue queues kept on the device or read back by the host;
   - with every (grain, step) on the guard list, the host finds no offset that differs, and no
     device cosine or sine more than 2^-50 from its own (the bound the guard's margin assumes);
   - a forced redo (the host's tables after the device's) gives the same records again.
@@ -70,6 +71,9 @@ def test_device_tables_equal_host_tables(rescue_case, method):
     dev, (checked, redone, _) = _pair(w, prm, {})
     assert same_records(dev, host)
     assert redone == 0
+    say("device tables, rescue queues kept on the device")
+    dq, _ = _pair(w, prm, {pk.OPT_DEVICE_QUEUES: 1})
+    assert same_records(dq, host)
     say("device tables, every (grain, step) checked by the host")
     allg, (checked, redone, max_diff) = _pair(w, prm, {pk.OPT_ROT_GUARD_TEST: 1})
     say(f"  checked {checked}, redone {redone}, max |device - host| {max_diff:.3g}")
