@@ -7,9 +7,11 @@ refinement, and the 9 result columns copied back to the host. Frames are residen
 HBM before timing; each step re-uploads the grains' input records (144 B each) as the
 pair's input. Prints one JSON line.
 
-After the timed steps the same pairs run again with every kernel of the exact search on one
-stream, so that the search kernel's launch durations (HIP events) do not overlap: the roofline
-is computed from those. Then the plain form, the exact-vs-plain pick comparison on configs 2 and
+The timed steps run without stage timers; the stage split comes from as many steps again with
+HIP events around each stage. Then the same pairs run with every kernel of the exact search on
+one stream, so that the search kernel's launch durations (HIP events) do not overlap: the
+roofline is computed from those. Then the pair captured as one HIP graph and replayed, the plain
+form, the exact-vs-plain pick comparison on configs 2 and
 3, the rescue-zone variant of the pair (300 grains moved 21-35 px, beyond the main zone: the
 rescue stage at its own +-40 px geometry) and the oracle's CPU baseline on a sample of grains
 (one warm-up, median of three).
@@ -19,8 +21,8 @@ rescue stage at its own +-40 px geometry) and the oracle's CPU baseline on a sam
 rad (accumulated), tracked pair after pair with the reference frame swapped in between
 (search +-8 px x 9 steps, subpixel). All frames are resident in HBM; a pair's latency runs
 from the reference swap to the 9 result columns on the host. It prints one JSON line with
-grains/s and the p50/p99 pair latency, and checks a sample of grains of every pair against
-the oracle.
+grains/s and the p50/p99 pair latency (pairs of launches, and each pair replaying one HIP
+graph), and checks a sample of grains of every pair against the oracle.
 """
 import argparse
 import json
@@ -142,24 +144,46 @@ def run_sequence(args, pk, synth, ptref):
     lat = []
     outs = None
     f0 = tr.exact_fallbacks()
-    tr.set_timing(True)
-    tr.reset_stage_times()
     t1 = time.perf_counter()
     for r in range(args.seq_repeats):
         o = one_pass(lat, r == 0)
         outs = outs or o
     el = time.perf_counter() - t1
+    f1 = tr.exact_fallbacks()
+    # the stage split from one more pass, with HIP events around each stage
+    tr.set_timing(True)
+    tr.reset_stage_times()
+    one_pass([], False)
     ms, launches = tr.stage_times()
     tr.set_timing(False)
-    f1 = tr.exact_fallbacks()
     stage_names = ["precompute", "search_main", "search_rescue", "search_super", "subpixel", "remap", "search_kernel"]
-    stage_ms = {n: ms[i] / len(lat) for i, n in enumerate(stage_names) if launches[i]}
+    stage_ms = {n: ms[i] / npairs for i, n in enumerate(stage_names) if launches[i]}
     log("stage ms/pair: " + ", ".join(f"{k} {v:.2f}" for k, v in stage_ms.items()) +
         f"; grains handed to the plain kernel: {(f1[1] - f0[1]) / len(lat):.0f} per pair")
     lat_ms = 1e3 * np.array(lat)
     value = S.G * len(lat) / lat_ms.sum() * 1e3
     log(f"sequence: {len(lat)} pairs, {value:.0f} grains/s, pair latency p50 {np.percentile(lat_ms, 50):.2f} ms "
         f"p99 {np.percentile(lat_ms, 99):.2f} ms ({el:.1f}s with the set-up of each pass)")
+    # the same passes with each pair replaying one HIP graph (captured after two passes' pairs
+    # of launches settle the buffers), the records checked against the pairs of launches
+    graph = None
+    if args.graph_steps > 0:
+        tr.set_option(pk.OPT_GRAPH, 1)
+        one_pass([], False)
+        glat = []
+        gouts = one_pass(glat, True)
+        for r in range(1, args.seq_repeats):
+            one_pass(glat, False)
+        captures, glaunches, nodes = tr.graph_stats()
+        tr.set_option(pk.OPT_GRAPH, 0)
+        glat_ms = 1e3 * np.array(glat)
+        same = all(a.tobytes() == b.tobytes() for a, b in zip(gouts, outs))
+        graph = {"grains_per_s": S.G * len(glat) / glat_ms.sum() * 1e3,
+                 "latency_ms": {"p50": float(np.percentile(glat_ms, 50)), "p99": float(np.percentile(glat_ms, 99))},
+                 "nodes_per_pair": nodes, "captures": captures, "graph_launches": glaunches,
+                 "records_equal_launches": same}
+        log(f"sequence, graph replay: p50 {graph['latency_ms']['p50']:.2f} ms p99 {graph['latency_ms']['p99']:.2f} ms, "
+            f"{nodes} nodes per pair, {captures} captures, records equal to launches: {same}")
     tr.close()
 
     # oracle check on a sample of every pair, from the GPU's own input records of that pair
@@ -205,6 +229,7 @@ def run_sequence(args, pk, synth, ptref):
                        "max": float(lat_ms.max()), "min": float(lat_ms.min()),
                        "per_pair_first_pass": [round(float(x), 2) for x in lat_ms[:npairs]]},
         "stage_ms": stage_ms,
+        "graph": graph,
         "plain_kernel_grains_per_pair": (f1[1] - f0[1]) / len(lat),
         "algorithmic_search_flops_per_pair": 2.0 * R * Sh * P.sum(),
         "oracle_check": check,
@@ -311,6 +336,29 @@ def rescue_variant(pk, synth, args):
         f"{out['plain_powell']['ms_per_pair']:.2f} ms/pair (rescue {out['plain_powell']['stage_ms']['search_rescue']:.2f}"
         f" ms); pick mismatches {out['pick_mismatches_exact_vs_plain']}")
     return out
+
+
+def graph_section(pk, tr, w, prm, step, ref, args):
+    """The same pairs with the pair captured once as a HIP graph and replayed (PTG_OPT_GRAPH):
+    ms/pair, the graph's nodes (the pair's launches and copies) and whether the records equal
+    those of the pairs run as launches."""
+    tr.set_option(pk.OPT_GRAPH, 1)
+    for _ in range(3):  # launches twice (new buffers change the key once), then the capture
+        step(prm)
+    tr.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.graph_steps):
+        got = step(prm)
+    tr.sync()
+    el = time.perf_counter() - t0
+    captures, launches, nodes = tr.graph_stats()
+    tr.set_option(pk.OPT_GRAPH, 0)
+    same = got.tobytes() == ref.tobytes()
+    ms = 1e3 * el / args.graph_steps
+    log(f"graph replay: {ms:.2f} ms/pair, {nodes} nodes per pair, records equal to launches: {same}")
+    return {"ms_per_pair": ms, "grains_per_s": w.G * args.graph_steps / el, "steps": args.graph_steps,
+            "nodes_per_pair": nodes, "captures": captures, "graph_launches": launches,
+            "records_equal_launches": same}
 
 
 def autotune_section(pk, tr, w, prm, step, args):
@@ -425,6 +473,8 @@ def main():
                     help="skip the pair requantised to 8 bits and spread over 16 bits")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the exact-vs-plain pick comparison (configs 2 and 3) after the timed run")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="pairs timed with the pair captured as a HIP graph and replayed (0: skip)")
     ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
     ap.add_argument("--seq-frames", type=int, default=16)
     ap.add_argument("--seq-grains", type=int, default=10000)
@@ -497,13 +547,18 @@ def main():
     for _ in range(args.warmup):
         step(prm)
     tr.sync()
-    tr.set_timing(True)
-    tr.reset_stage_times()
+    # the timed pairs, with no stage timers (their events are not part of the work)
     t1 = time.perf_counter()
     for _ in range(args.steps):
         out = step(prm)
     tr.sync()
     el = time.perf_counter() - t1
+    # the stage split from as many pairs again, with HIP events around each stage
+    tr.set_timing(True)
+    tr.reset_stage_times()
+    for _ in range(args.steps):
+        step(prm)
+    tr.sync()
     ms, launches = tr.stage_times()
     tr.set_timing(False)
     nres, _ = tr.queue_counts()
@@ -593,6 +648,8 @@ def main():
     }
     if method == "exact" and args.autotune:
         result["autotune"] = autotune_section(pk, tr, w, prm, step, args)
+    if args.graph_steps > 0:
+        result["graph"] = graph_section(pk, tr, w, prm, step, out, args)
 
     # the plain form (FP64 two-pass search + Powell) on the same workload
     if args.plain_steps > 0 and not (method == "plain" and subpix == "powell"):

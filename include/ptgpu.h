@@ -310,14 +310,29 @@ int ptg_set_timing(void* ctx, int enable);
  *  ones, its launches sized for all, with no host round trip between the stages (what a captured
  *  pair needs; an empty queue still costs the stage's launches). 0 reads the queue back and
  *  plans the stage for it (an empty queue: no launch).
+ *  PTG_OPT_GRAPH [0]: 1 runs ptg_track_pair as one HIP graph: the first pair with new
+ *  parameters, grains or buffers runs as launches, the next is captured from the same launch
+ *  sequence (rescue queues on the device) and it and every later pair with that key replay the
+ *  graph, on the frames the context holds at the time (read through a device frame record), as
+ *  a frame sequence does between reference swaps; stage timers are not kept for replayed pairs
+ *  (with timing on, or host tables, pairs run as launches).
  */
-enum { PTG_OPT_SEARCH_STREAMS = 0, PTG_OPT_ROT_TABLES = 1, PTG_OPT_ROT_GUARD_TEST = 2, PTG_OPT_DEVICE_QUEUES = 3 };
+enum {
+  PTG_OPT_SEARCH_STREAMS = 0,
+  PTG_OPT_ROT_TABLES = 1,
+  PTG_OPT_ROT_GUARD_TEST = 2,
+  PTG_OPT_DEVICE_QUEUES = 3,
+  PTG_OPT_GRAPH = 4
+};
 int ptg_set_option(void* ctx, int option, int64_t value);
 /* The rotation guard's counters since the context was created: guard entries the host
  * compared, calls redone on the host's tables, and the largest difference between a device
  * cosine or sine and the host C library's among the compared entries (the guard assumes at
  * most 2^-50). Synchronises. */
 int ptg_get_rot_guard_stats(void* ctx, int64_t* checked, int64_t* redone, double* max_diff);
+/* Pairs captured as HIP graphs and graph launches (PTG_OPT_GRAPH) since the context was
+ * created, and the nodes of the current graph (the pair's kernel launches and copies). */
+int ptg_get_graph_stats(void* ctx, int64_t* captures, int64_t* launches, int64_t* nodes);
 /* The exact search's autotuner; no counterpart in the reference (it replaces the launch
  * shapes of the built-in heuristic, which picks them from the LDS and register budgets).
  * For each radius bucket of a main-stage search over `zone` on the current frames and

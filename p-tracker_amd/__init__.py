@@ -24,6 +24,7 @@ OPT_SEARCH_STREAMS = 0
 OPT_ROT_TABLES = 1  # 0: device tables with the host-checked guard (default), 1: host tables
 OPT_ROT_GUARD_TEST = 2  # 1: guard every (grain, step); 2: also redo every guarded call
 OPT_DEVICE_QUEUES = 3  # 1: rescue queues stay on the device (with device tables), 0: read back (default)
+OPT_GRAPH = 4  # 1: track_pair captured once as a HIP graph and replayed
 PTG_ERR_ARG, PTG_ERR_HIP, PTG_ERR_STATE, PTG_ERR_NOMEM, PTG_ERR_LIMIT = -1, -2, -3, -4, -5
 PTG_STATUS_CLAMPED = 1
 STAGE_MAIN, STAGE_RESCUE, STAGE_SUPER = 0, 1, 2
@@ -159,6 +160,7 @@ def load_library(path: str = LIB_PATH):
         "ptg_get_subpix_evals_per_grain": ([vp, vp], c_int),
         "ptg_get_exact_fallbacks": ([vp, POINTER(c_int64), POINTER(c_int64)], c_int),
         "ptg_get_rot_guard_stats": ([vp, POINTER(c_int64), POINTER(c_int64), POINTER(c_double)], c_int),
+        "ptg_get_graph_stats": ([vp, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)], c_int),
         "ptg_equiproj_set_data": ([vp, c_int32, vp, c_int32, vp, vp, c_int32, vp, c_double], c_int),
         "ptg_equiproj_set_origins": ([vp, vp], c_int),
         "ptg_equiproj_eval": ([vp, vp, vp, vp, vp, vp, vp], c_int),
@@ -183,7 +185,7 @@ def exported_symbols():
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
             "ptg_ncc_objective_grad_batch", "ptg_interp_cubic", "ptg_undistort_points", "ptg_remap",
             "ptg_set_timing", "ptg_set_option", "ptg_autotune_search", "ptg_clear_tuning", "ptg_get_stage_times", "ptg_reset_stage_times", "ptg_get_subpix_evals",
-            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_get_rot_guard_stats", "ptg_equiproj_set_data", "ptg_equiproj_set_origins",
+            "ptg_get_subpix_evals_per_grain", "ptg_get_exact_fallbacks", "ptg_get_rot_guard_stats", "ptg_get_graph_stats", "ptg_equiproj_set_data", "ptg_equiproj_set_origins",
             "ptg_equiproj_eval", "ptg_calibrate_distortion", "ptg_joint_eval", "ptg_refine_joint_distortion"]
 
 
@@ -434,6 +436,13 @@ class Tracker:
         out = np.zeros(self.G, dtype=np.int32)
         self._ck(self.lib.ptg_get_subpix_evals_per_grain(self.h, _ptr(out)), "ptg_get_subpix_evals_per_grain")
         return out
+
+    def graph_stats(self):
+        """(pairs captured as HIP graphs, graph launches, nodes of the current graph)."""
+        n, m, k = c_int64(), c_int64(), c_int64()
+        self._ck(self.lib.ptg_get_graph_stats(self.h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(k)),
+                 "ptg_get_graph_stats")
+        return n.value, m.value, k.value
 
     def rot_guard_stats(self):
         """(entries the host compared, calls redone on the host's rotation tables, largest

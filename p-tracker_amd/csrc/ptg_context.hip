@@ -33,6 +33,28 @@ int Context::check(hipError_t e, const char* what) {
   return fail(PTG_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+int Context::set_frames() {
+  const Image &i0 = img[iref], &i1 = img[icur];
+  FrameDesc f{};
+  f.I0 = i0.d;
+  f.I1 = i1.d;
+  f.pitch0 = i0.pitch;
+  f.pitch1 = i1.pitch;
+  f.dimx = i1.dimx;
+  f.dimy = i1.dimy;
+  f.dimx0 = i0.dimx;
+  f.dimy0 = i0.dimy;
+  if (frames_written && f.I0 == h_frames.I0 && f.I1 == h_frames.I1 && f.pitch0 == h_frames.pitch0 &&
+      f.pitch1 == h_frames.pitch1 && f.dimx == h_frames.dimx && f.dimy == h_frames.dimy &&
+      f.dimx0 == h_frames.dimx0 && f.dimy0 == h_frames.dimy0)
+    return PTG_OK;
+  hipError_t e = launch_set_frames(*this, f);
+  if (e != hipSuccess) return check(e, "k_set_frames");
+  h_frames = f;
+  frames_written = true;
+  return PTG_OK;
+}
+
 int Context::ensure(void** p, size_t* cap, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (*p && *cap >= bytes) return PTG_OK;
@@ -285,6 +307,7 @@ int do_precompute(Ctx* c) {
   TRY(need_grains(c));
   if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, false));
+  TRY(c->set_frames());
   if (!c->rot_host) {
     // cos/sin(refrot) on the device, guarded (checked by the caller after its stages)
     hipEvent_t ev;
@@ -347,7 +370,22 @@ static int host_rot_tables(Ctx* c, const std::vector<int32_t>& list, int stage) 
       CK(c, hipMemcpyAsync(c->d_cs + off, cs + off, sizeof(double) * 2 * nsteps, hipMemcpyHostToDevice, c->stream));
     }
   }
-  CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
+  return c->mark_staging();
+}
+
+// The stage's rotation-step values on the device, uploaded when they change (each stage keeps
+// its own, where a captured pair's kernels read them).
+static int upload_steps(Ctx* c, int stage) {
+  if (c->d_steps_st[stage] && c->h_steps_st[stage] == c->h_steps) return PTG_OK;
+  if (c->capturing) return c->fail(PTG_ERR_STATE, "rotation steps changed during a pair's capture");
+  const size_t bytes = sizeof(double) * c->h_steps.size();
+  TRY(c->wait_staging());
+  TRY(c->pin_steps.ensure(*c, bytes));
+  memcpy(c->pin_steps.p, c->h_steps.data(), bytes);
+  TRY(c->ensure((void**)&c->d_steps_st[stage], &c->steps_cap_st[stage], bytes));
+  CK(c, hipMemcpyAsync(c->d_steps_st[stage], c->pin_steps.p, bytes, hipMemcpyHostToDevice, c->stream));
+  c->h_steps_st[stage] = c->h_steps;
+  c->table_uploads++;
   return c->mark_staging();
 }
 
@@ -363,6 +401,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   if ((int64_t)(z->left + z->right + 1) * (z->up + z->down + 1) > (1 << 24))
     return c->fail(PTG_ERR_LIMIT, "search zone too large");
   if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(c->set_frames());
   TRY(make_steps(c, z, c->h_steps));
   const int nsteps = (int)c->h_steps.size();
 
@@ -374,7 +413,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   if (stage < PTG_STAGE_SUPER) CK(c, fill_i32(*c, c->d_counts + stage + 1, 1, 0));
   // a rescue stage on device tables keeps its queue on the device: its exact search is planned
   // over the unmasked list and serves the queued grains among them (no host round trip)
-  const bool devq = stage != PTG_STAGE_MAIN && !c->rot_host && c->device_queues;
+  const bool devq = stage != PTG_STAGE_MAIN && !c->rot_host && (c->device_queues || c->graph_pair);
   if (stage == PTG_STAGE_MAIN) {
     if (c->rot_host && c->snap != Ctx::SNAP_VALID) TRY(sync_snapshot(c));
     list = c->h_unmasked;  // built at set/put time
@@ -395,23 +434,18 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
 
   // rotation tables: cos/sin(refrot + drot + step) per (grain, step), evaluated as the
   // reference (:687-689), on the device (guarded) or from the host's libm
-  TRY(c->wait_staging());
-  TRY(c->pin_steps.ensure(*c, sizeof(double) * nsteps));
-  memcpy(c->pin_steps.p, c->h_steps.data(), sizeof(double) * nsteps);
   TRY(c->ensure((void**)&c->d_cs, &c->cs_cap, sizeof(double) * 2 * (size_t)c->G * nsteps));
-  TRY(c->ensure((void**)&c->d_steps, &c->steps_cap, sizeof(double) * nsteps));
   const int nparts_cap = std::max(nsteps, z->left + z->right + 1);
   TRY(c->ensure((void**)&c->d_best, &c->best_cap, sizeof(StepBest) * (size_t)nlist * nparts_cap));
+  TRY(upload_steps(c, stage));
   if (!c->rot_host) {
-    CK(c, hipMemcpyAsync(c->d_steps, c->pin_steps.p, sizeof(double) * nsteps, hipMemcpyHostToDevice, c->stream));
-    TRY(c->mark_staging());
     RotArgs r;
     r.grains = c->d_grains;
     r.pattern = c->d_pattern;
     r.pbegin = c->d_pbegin;
     r.list = d_list;
     r.count = d_count;
-    r.steps = c->d_steps;
+    r.steps = c->d_steps_st[stage];
     r.nsteps = nsteps;
     r.all = c->rot_guard_test ? 1 : 0;
     r.cs = c->d_cs;
@@ -419,6 +453,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     r.guard_count = c->d_guard_count;
     CK(c, launch_rot_tables(*c, r, nlist));
   } else {
+    TRY(c->wait_staging());
     TRY(host_rot_tables(c, list, stage));
   }
 
@@ -445,13 +480,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   win_cap = (int64_t)(tile_u + 2 * rho_max) * (tile_v + 2 * rho_max);
 
   SearchArgs a;
-  const Image &i0 = c->img[c->iref], &i1 = c->img[c->icur];
-  a.I0 = i0.d;
-  a.I1 = i1.d;
-  a.dimx = i1.dimx;
-  a.dimy = i1.dimy;
-  a.pitch0 = i0.pitch;
-  a.pitch1 = i1.pitch;
+  a.fr = c->d_frames;
   a.grains = c->d_grains;
   a.pattern = c->d_pattern;
   a.pbegin = c->d_pbegin;
@@ -463,7 +492,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
   a.count = d_count;
   a.act = nullptr;
   a.cs = c->d_cs;
-  a.steps = c->d_steps;
+  a.steps = c->d_steps_st[stage];
   a.nsteps = nsteps;
   a.left = z->left;
   a.right = z->right;
@@ -648,12 +677,14 @@ int ptg_create(void** out, int device) {
     if (c->d_counts) (void)hipFree(c->d_counts);
     if (c->d_evals) (void)hipFree(c->d_evals);
     if (c->d_guard_count) (void)hipFree(c->d_guard_count);
+    if (c->d_frames) (void)hipFree(c->d_frames);
     (void)hipStreamDestroy(c->stream);
     delete c;
   };
   if ((e = hipMalloc((void**)&c->d_counts, sizeof(int32_t) * 4)) != hipSuccess ||
       (e = hipMalloc((void**)&c->d_evals, sizeof(int64_t))) != hipSuccess ||
-      (e = hipMalloc((void**)&c->d_guard_count, 64 + sizeof(RotGuardEntry) * kRotGuardCap)) != hipSuccess) {
+      (e = hipMalloc((void**)&c->d_guard_count, 64 + sizeof(RotGuardEntry) * kRotGuardCap)) != hipSuccess ||
+      (e = hipMalloc((void**)&c->d_frames, sizeof(FrameDesc))) != hipSuccess) {
     undo();
     return create_fail(PTG_ERR_NOMEM, "hipMalloc", e);
   }
@@ -685,6 +716,8 @@ static void free_grain_buffers(Ctx* c) {
   }
 }
 
+static void drop_graph(Ctx* c);
+
 int ptg_destroy(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_OK;
@@ -693,13 +726,14 @@ int ptg_destroy(void* p) {
   for (int i = 0; i < kMaxImages; i++)
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
   free_grain_buffers(c);
-  void* more[] = {c->d_counts,   c->d_cs,      c->d_steps,   c->d_best,       c->d_evals,  c->d_evals_g,
+  void* more[] = {c->d_counts,   c->d_cs,      c->d_steps_st[0], c->d_steps_st[1], c->d_steps_st[2], c->d_best,       c->d_evals,  c->d_evals_g,
                   c->d_exact_scratch[0], c->d_exact_scratch[1], c->d_exact_scratch[2], c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
                   c->d_eq_partial, c->d_eq_out, c->d_eq_res, c->d_eq_jac, c->d_eq_orig, c->d_guard_count,
-                  c->d_grains_save};
+                  c->d_grains_save, c->d_frames};
   for (void* b : more)
     if (b) (void)hipFree(b);
   free_exact_plans(*c);
+  drop_graph(c);
   PinnedBuf* pins[] = {&c->pin_cs, &c->pin_steps, &c->pin_cs0, &c->pin_list, &c->pin_flags};
   for (PinnedBuf* b : pins)
     if (b->p) (void)hipHostFree(b->p);
@@ -1017,6 +1051,7 @@ static int subpixel_check(Ctx* c, const ptg_subpix_params* prm) {
 // rescue stages' host round trip, so that only the kernel launch follows it)
 static int subpixel_run(Ctx* c, const ptg_subpix_params* prm, bool reset) {
   if (c->require_precomputations) TRY(do_precompute(c));
+  TRY(c->set_frames());
   if (reset) TRY(subpixel_reset(c));
   hipEvent_t ev;
   c->mark_begin(PTG_T_SUBPIXEL, &ev);
@@ -1105,8 +1140,135 @@ static int track_pair_run(Ctx* c, const ptg_track_params* prm) {
   if (prm->subpixel) TRY(subpixel_run(c, &prm->subpix, false));
   TRY(ptg_end_pair(p));
   // a stage that could not be served exactly is reported here, after the pair's work
-  if (c->any_flags_pending()) TRY(c->sync_and_check());
+  if (!c->capturing && c->any_flags_pending()) TRY(c->sync_and_check());
   return PTG_OK;
+}
+
+// The key of a captured pair: everything its launches and their arguments depend on besides the
+// frames (read through the frame record) and the records' values.
+static uint64_t pair_key(const Ctx* c, const ptg_track_params* p) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  auto mixd = [&mix](double d) {
+    uint64_t u;
+    memcpy(&u, &d, sizeof(u));
+    mix(u);
+  };
+  for (const ptg_search_zone* z : {&p->search_zone, &p->search_zone_rescue, &p->search_zone_super_rescue}) {
+    mix((uint64_t)(uint32_t)z->left | (uint64_t)(uint32_t)z->right << 32);
+    mix((uint64_t)(uint32_t)z->up | (uint64_t)(uint32_t)z->down << 32);
+    mix((uint64_t)(uint32_t)z->num_rot);
+    mixd(z->inc_rot);
+  }
+  mixd(p->NCC_min);
+  mixd(p->NCC_min_super);
+  mix((uint64_t)(uint32_t)p->rescue_level | (uint64_t)(uint32_t)p->subpixel << 32);
+  mix((uint64_t)(uint32_t)p->search_method);
+  if (p->subpixel) {
+    const ptg_subpix_params& q = p->subpix;
+    mixd(q.subpix_tol);
+    for (double d : q.initial_direction) mixd(d);
+    mixd(q.subpix_displacement_max);
+    mixd(q.overflow_stiffness);
+    mix((uint64_t)(uint32_t)q.method | (uint64_t)(uint32_t)q.max_iter << 32);
+  }
+  const Image &i0 = c->img[c->iref], &i1 = c->img[c->icur];
+  mix((uint64_t)std::max(i0.bits, i1.bits));
+  mix(c->unmasked_version);
+  mix(c->tune_gen);
+  mix(c->table_uploads);  // (device tables another call replaced)
+  // the buffers the launches address (each grows, and moves, with the work it serves)
+  for (const void* q : {(const void*)c->d_cs, (const void*)c->d_best, (const void*)c->d_steps_st[0],
+                        (const void*)c->d_steps_st[1], (const void*)c->d_steps_st[2],
+                        (const void*)c->d_exact_scratch[0], (const void*)c->d_exact_scratch[1],
+                        (const void*)c->d_exact_scratch[2], (const void*)c->d_evals_g, (const void*)c->pin_flags.p})
+    mix((uint64_t)(uintptr_t)q);
+  mix((uint64_t)c->G);
+  mix((uint64_t)c->require_precomputations | (uint64_t)c->serial_search << 1);
+  return h;
+}
+
+static void drop_graph(Ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key = 0;
+}
+
+// A pair as one HIP graph: the first pair with a new key runs as launches (building every plan,
+// table and buffer the pair needs), the second is captured from the same launch sequence and
+// replayed, and so is every later pair with that key, its frames given by the frame record.
+// Rescue queues stay on the device; the rotation guard and the exact search's flags are checked
+// after each replay as after a pair of launches.
+static int track_pair_graph(Ctx* c, const ptg_track_params* prm) {
+  const uint64_t key = pair_key(c, prm);
+  c->graph_pair = true;
+  struct Reset {
+    Ctx* c;
+    ~Reset() { c->graph_pair = false; }
+  } reset{c};
+  if (key == c->graph_bad_key || (key != c->graph_key && key != c->graph_warm_key)) {
+    // a new key: a pair of launches first
+    if (key != c->graph_bad_key) c->graph_warm_key = key;
+    return with_rot_guard(c, [&]() -> int { return track_pair_run(c, prm); });
+  }
+  TRY(c->set_frames());
+  TRY(save_grains(c));
+  if (key != c->graph_key) {
+    // capture the pair's launches
+    drop_graph(c);
+    const bool pre = c->require_precomputations;
+    for (int st = 0; st < 3; st++) c->flags_pending[st] = c->unfit_pending[st] = false;
+    hipError_t he = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (he != hipSuccess) return c->check(he, "hipStreamBeginCapture");
+    c->capturing = true;
+    const int rc = track_pair_run(c, prm);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    he = hipStreamEndCapture(c->stream, &g);
+    hipGraphExec_t ge = nullptr;
+    if (rc == PTG_OK && he == hipSuccess && g) he = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (rc != PTG_OK || he != hipSuccess || !ge) {
+      // this key stays on launches
+      if (ge) (void)hipGraphExecDestroy(ge);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      c->graph_bad_key = key;
+      c->require_precomputations = pre;
+      for (int st = 0; st < 3; st++) c->flags_pending[st] = c->unfit_pending[st] = false;
+      return with_rot_guard(c, [&]() -> int { return track_pair_run(c, prm); });
+    }
+    c->graph = g;
+    c->graph_exec = ge;
+    c->graph_key = key;
+    for (int st = 0; st < 3; st++) {
+      c->graph_flags[st] = c->flags_pending[st];
+      c->graph_unfit[st] = c->unfit_pending[st];
+    }
+    c->graph_captures++;
+  }
+  CK(c, hipGraphLaunch(c->graph_exec, c->stream));
+  c->graph_launches++;
+  c->require_precomputations = false;
+  c->snap = Ctx::SNAP_INVALID;
+  for (int st = 0; st < 3; st++) {
+    c->flags_pending[st] = c->graph_flags[st];
+    c->unfit_pending[st] = c->graph_unfit[st];
+  }
+  if (c->any_flags_pending()) TRY(c->sync_and_check());
+  bool redo = false;
+  TRY(check_rot_guard(c, &redo));
+  if (!redo) return PTG_OK;
+  TRY(restore_grains(c));
+  c->rot_host = true;
+  const int rc = track_pair_run(c, prm);
+  c->rot_host = false;
+  c->rot_redos++;
+  return rc;
 }
 
 int ptg_track_pair(void* p, const ptg_track_params* prm) {
@@ -1117,6 +1279,7 @@ int ptg_track_pair(void* p, const ptg_track_params* prm) {
   TRY(need_images(c, true, true));
   if (prm->subpixel) TRY(subpixel_check(c, &prm->subpix));
   if (c->G == 0) return PTG_OK;
+  if (c->use_graph && !c->rot_host && !c->timing && c->rot_guard_test == 0) return track_pair_graph(c, prm);
   return with_rot_guard(c, [&]() -> int { return track_pair_run(c, prm); });
 }
 
@@ -1205,6 +1368,7 @@ int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, doub
     };
     const int rc = body();
     c->tune_run = false;
+    c->tune_gen++;
     c->tune_bucket = -1;
     c->exact_fallbacks = fb0;
     c->exact_grain_fallbacks = fg0;
@@ -1225,6 +1389,7 @@ int ptg_clear_tuning(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
   c->exact_tuned.clear();
+  c->tune_gen++;
   return PTG_OK;
 }
 
@@ -1870,12 +2035,35 @@ int ptg_set_option(void* p, int option, int64_t value) {
     c->device_queues = value == 1;
     return PTG_OK;
   }
+  if (option == PTG_OPT_GRAPH) {
+    if (value != 0 && value != 1) return c->fail(PTG_ERR_ARG, "PTG_OPT_GRAPH takes 0 or 1");
+    c->use_graph = value == 1;
+    if (!c->use_graph) {
+      TRY(c->sync_and_check());
+      drop_graph(c);
+      c->graph_warm_key = c->graph_bad_key = 0;
+    }
+    return PTG_OK;
+  }
   if (option == PTG_OPT_ROT_GUARD_TEST) {
     if (value < 0 || value > 2) return c->fail(PTG_ERR_ARG, "PTG_OPT_ROT_GUARD_TEST takes 0, 1 or 2");
     c->rot_guard_test = (int)value;
     return PTG_OK;
   }
   return c->fail(PTG_ERR_ARG, "unknown option %d", option);
+}
+
+int ptg_get_graph_stats(void* p, int64_t* captures, int64_t* launches, int64_t* nodes) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (launches) *launches = c->graph_launches;
+  if (nodes) {
+    size_t n = 0;
+    if (c->graph) CK(c, hipGraphGetNodes(c->graph, nullptr, &n));
+    *nodes = (int64_t)n;
+  }
+  return PTG_OK;
 }
 
 int ptg_get_rot_guard_stats(void* p, int64_t* checked, int64_t* redone, double* max_diff) {

@@ -64,12 +64,20 @@ struct RotArgs {
   int32_t* guard_count;
 };
 
+// The frames of a pair as the kernels of the tracking path read them: a small device record the
+// host rewrites (k_set_frames, in stream order) when the images or their roles change, so that a
+// captured pair replays on whatever frames the context holds.
+struct FrameDesc {
+  const uint16_t* I0;  // reference frame
+  const uint16_t* I1;  // current frame
+  int64_t pitch0, pitch1;
+  int32_t dimx, dimy;    // the current frame's size (the same as the reference's for a search)
+  int32_t dimx0, dimy0;  // the reference frame's size
+};
+
 // Arguments of the integer-search kernels (plain and exact forms).
 struct SearchArgs {
-  const uint16_t* I0;
-  const uint16_t* I1;
-  int32_t dimx, dimy;
-  int64_t pitch0, pitch1;
+  const FrameDesc* fr;      // the pair's frames (device)
   ptg_grain* grains;
   const int32_t* pattern;   // (dx, dy) pairs
   const int64_t* pbegin;    // G + 1
@@ -116,6 +124,10 @@ struct Context {
 
   Image img[kMaxImages];
   int iref = 0, icur = 1;
+  FrameDesc* d_frames = nullptr;  // the frames' device record, and what was last written to it
+  FrameDesc h_frames{};
+  bool frames_written = false;
+  int set_frames();  // writes the record when the images or their roles changed
 
   // grains
   int32_t G = 0;
@@ -153,8 +165,9 @@ struct Context {
   // search scratch
   double* d_cs = nullptr;
   size_t cs_cap = 0;
-  double* d_steps = nullptr;
-  size_t steps_cap = 0;
+  double* d_steps_st[3] = {nullptr, nullptr, nullptr};  // each stage's rotation-step values
+  size_t steps_cap_st[3] = {0, 0, 0};
+  std::vector<double> h_steps_st[3];  // what each holds
   StepBest* d_best = nullptr;
   size_t best_cap = 0;
   double* d_cs0 = nullptr;  // cos, sin of refrot per grain (precompute)
@@ -164,6 +177,18 @@ struct Context {
   int rot_guard_test = 0;  // 1: list every (grain, step); 2: also treat every entry as differing
   // rescue stages' queues stay on the device (with device tables; else they are read back)
   bool device_queues = false;
+  // a pair captured as one HIP graph and replayed (PTG_OPT_GRAPH): the capture's state
+  bool use_graph = false;
+  bool capturing = false;   // inside a capture: no host synchronisation and no upload
+  bool graph_pair = false;  // a pair run for a graph or in one: rescue queues stay on the device
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_key = 0, graph_warm_key = 0, graph_bad_key = 0;
+  bool graph_flags[3] = {false, false, false};  // exact-search flags the captured pair copies out
+  bool graph_unfit[3] = {false, false, false};
+  int64_t graph_captures = 0, graph_launches = 0;
+  uint64_t tune_gen = 0;   // bumped when the tuned launch shapes change
+  uint64_t table_uploads = 0;  // bumped whenever an exact search's tables or a stage's steps are uploaded
   int32_t* d_guard_count = nullptr;  // the guard list's count, then its entries
   RotGuardEntry* d_guard = nullptr;
   int64_t rot_redos = 0;   // stages (or pairs) redone with the host tables
@@ -257,6 +282,7 @@ hipError_t launch_begin_pair(Context& c);
 // dev_trig: cos/sin(refrot) on the device, with the guard of k_rot_tables (k = -1); else the
 // host's from d_cs0.
 hipError_t launch_precompute(Context& c, bool dev_trig);
+hipError_t launch_set_frames(Context& c, const FrameDesc& f);
 // The rotation tables of a stage's grains on the device (grid: list_cap x step blocks).
 hipError_t launch_rot_tables(Context& c, const RotArgs& r, int list_cap);
 hipError_t launch_search_plain(Context& c, const SearchArgs& a, int list_cap, size_t lds_bytes);

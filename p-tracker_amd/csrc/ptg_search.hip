@@ -83,8 +83,8 @@ hipError_t launch_change_reference(Context& c) {
 constexpr int kPreThreads = 256;
 constexpr int kPreChunk = 1024;
 
-__global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __restrict__ I0, int dimx, int dimy,
-                                                            int64_t pitch, ptg_grain* __restrict__ grains,
+__global__ __launch_bounds__(kPreThreads) void k_precompute(const FrameDesc* __restrict__ frames,
+                                                            ptg_grain* __restrict__ grains,
                                                             const int32_t* __restrict__ pattern,
                                                             const int64_t* __restrict__ pbegin,
                                                             const double* __restrict__ cs0, int32_t* __restrict__ p0,
@@ -103,6 +103,10 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
   const int n = (int)(pbegin[g + 1] - b);
   ptg_grain& gr = grains[g];
   const int rx = gr.refcoord_xpix, ry = gr.refcoord_ypix;
+  const FrameDesc fr = *frames;
+  const uint16_t* __restrict__ I0 = fr.I0;
+  const int dimx = fr.dimx0, dimy = fr.dimy0;
+  const int64_t pitch = fr.pitch0;
   // cos/sin(refrot) (:636-637): the device library's, guarded as the rotation tables are,
   // or the host's
   const double c0 = dev_trig ? cos(gr.refrot) : cs0[2 * g], s0 = dev_trig ? sin(gr.refrot) : cs0[2 * g + 1];
@@ -208,11 +212,20 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const uint16_t* __re
 
 hipError_t launch_precompute(Context& c, bool dev_trig) {
   if (c.G == 0) return hipSuccess;
-  const Image& im = c.img[c.iref];
-  k_precompute<<<c.G, kPreThreads, 0, c.stream>>>(im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pattern,
+  k_precompute<<<c.G, kPreThreads, 0, c.stream>>>(c.d_frames, c.d_grains, c.d_pattern,
                                                   c.d_pbegin, c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords,
                                                   c.d_spwords, c.d_m0, dev_trig ? 1 : 0,
                                                   c.rot_guard_test ? 1 : 0, c.d_guard, c.d_guard_count);
+  return hipGetLastError();
+}
+
+// The pair's frame record (FrameDesc), written in stream order.
+__global__ void k_set_frames(FrameDesc* d, FrameDesc f) {
+  if (threadIdx.x == 0) *d = f;
+}
+
+hipError_t launch_set_frames(Context& c, const FrameDesc& f) {
+  k_set_frames<<<1, 64, 0, c.stream>>>(c.d_frames, f);
   return hipGetLastError();
 }
 
@@ -281,6 +294,7 @@ __device__ __forceinline__ void plain_argmax_write(double best, int bidx, int k,
 
 // One (list slot li, step k): the workgroup's best candidate to out.
 __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int k, StepBest* out) {
+  const FrameDesc fr = *a.fr;
   extern __shared__ __align__(16) unsigned char smem[];
   const int nsteps = a.nsteps;
   const int g = a.list[li];
@@ -322,10 +336,10 @@ __device__ __forceinline__ void plain_slot_step(const SearchArgs& a, int li, int
       for (int t = threadIdx.x; t < tw * th; t += kPlainThreads) {
         const int wy = t / tw, wx = t - wy * tw;
         int X = X0 + tu0 + wx, Y = Y0 + tv0 + wy;
-        if (X < 0 || Y < 0 || X >= a.dimx || Y >= a.dimy) clamped = true;
-        X = clampi(X, 0, a.dimx - 1);
-        Y = clampi(Y, 0, a.dimy - 1);
-        win[wy * ww + wx] = a.I1[(int64_t)Y * a.pitch1 + X];
+        if (X < 0 || Y < 0 || X >= fr.dimx || Y >= fr.dimy) clamped = true;
+        X = clampi(X, 0, fr.dimx - 1);
+        Y = clampi(Y, 0, fr.dimy - 1);
+        win[wy * ww + wx] = fr.I1[(int64_t)Y * fr.pitch1 + X];
       }
       __syncthreads();
       const int St = tu * tv;
@@ -427,6 +441,7 @@ __device__ __forceinline__ void plain_argmax_write(double best, int bidx, int k,
 // itself and reads the current frame through the caches, clamped at its edges as the staged
 // window is; the grain is marked clamped when the window's box leaves the frame, as there.
 __device__ __forceinline__ void plain_slot_step_global(const SearchArgs& a, int li, int k, StepBest* out) {
+  const FrameDesc fr = *a.fr;
   const int nsteps = a.nsteps;
   const int g = a.list[li];
   const ptg_grain& gr = a.grains[g];
@@ -443,7 +458,7 @@ __device__ __forceinline__ void plain_slot_step_global(const SearchArgs& a, int 
   const double* __restrict__ d0 = a.d0 + b;
   const double dn = (double)n;
   auto px = [&](int X, int Y) -> unsigned int {
-    return a.I1[(int64_t)clampi(Y, 0, a.dimy - 1) * a.pitch1 + clampi(X, 0, a.dimx - 1)];
+    return fr.I1[(int64_t)clampi(Y, 0, fr.dimy - 1) * fr.pitch1 + clampi(X, 0, fr.dimx - 1)];
   };
   double best = 0.0;
   int bidx = INT_MAX;
@@ -499,7 +514,7 @@ __device__ __forceinline__ void plain_slot_step_global(const SearchArgs& a, int 
     }
   }
   const int bx = X0 - rho, by = Y0 - rho;
-  if (threadIdx.x == 0 && (bx < 0 || by < 0 || bx + U + 2 * rho > a.dimx || by + V + 2 * rho > a.dimy))
+  if (threadIdx.x == 0 && (bx < 0 || by < 0 || bx + U + 2 * rho > fr.dimx || by + V + 2 * rho > fr.dimy))
     atomicOr(&a.grains[g].status, (int)PTG_STATUS_CLAMPED);
   plain_argmax_write(best, bidx, k, S, out);
 }

@@ -435,6 +435,7 @@ __host__ __device__ __forceinline__ size_t mid_lds_bytes(int rho, int vp, int cb
 
 template <int NPL>
 __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
+  const FrameDesc fr = *e.s.fr;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ long long red[2][16];
   const int bi = blockIdx.x;
@@ -505,11 +506,11 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
     bool over = false;
     for (int t = threadIdx.x; t < WHm * (stride / 4); t += blockDim.x) {
       const int y = t / (stride / 4), x4 = (t - y * (stride / 4)) * 4;
-      const int Y = clampi(Y0 + y, 0, e.s.dimy - 1);
+      const int Y = clampi(Y0 + y, 0, fr.dimy - 1);
       uint32_t w[kMidPlanes] = {0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const uint32_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + clampi(X0 + x4 + j, 0, e.s.dimx - 1)];
+        const uint32_t v = fr.I1[(int64_t)Y * fr.pitch1 + clampi(X0 + x4 + j, 0, fr.dimx - 1)];
         over |= v > vmax;
         const uint32_t q = v * v;
         uint32_t dig[kMidPlanes];
@@ -636,6 +637,7 @@ __global__ __launch_bounds__(1024) void k_exact_mid(ExactArgs e) {
 // (slot, block of base_vb vertical shifts); LDS: the window rows of the block (base_vb +
 // 2 rho) x (U + 2 rho + 1) prefix pairs, then the run half-widths.
 __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
+  const FrameDesc fr = *e.s.fr;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ long long red[2][8];
   __shared__ int big;
@@ -674,15 +676,15 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
   // reads clamp to the frame as the MFMA kernels' do
   bool over = false;
   for (int row = wave; row < nrow; row += nwave) {
-    const int Y = clampi(Y0 + row, 0, e.s.dimy - 1);
-    const uint16_t* src = e.s.I1 + (int64_t)Y * e.s.pitch1;
+    const int Y = clampi(Y0 + row, 0, fr.dimy - 1);
+    const uint16_t* src = fr.I1 + (int64_t)Y * fr.pitch1;
     uint2* dst = pre + (size_t)row * pst;
     unsigned int c1 = 0u, c2 = 0u;
     if (lane == 0) dst[0] = make_uint2(0u, 0u);
     for (int x0 = 0; x0 < ncol; x0 += 64) {
       const int x = x0 + lane;
       unsigned int v = 0u;
-      if (x < ncol) v = src[clampi(X0 + x, 0, e.s.dimx - 1)];
+      if (x < ncol) v = src[clampi(X0 + x, 0, fr.dimx - 1)];
       over |= v > (unsigned int)min(e.vmax, 4095);
       unsigned int a = v, b = v * v;
 #pragma unroll
@@ -894,6 +896,7 @@ __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* w
 
 template <int NMT, int NTP, int VM>
 __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
+  const FrameDesc fr = *e.s.fr;
   constexpr int SPT = vm_spt(VM), NP = vm_planes(VM);
   extern __shared__ __align__(16) unsigned char smem[];
   // XCD-aware job mapping: all shift blocks of a grain share blockIdx % 8
@@ -952,10 +955,10 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
     const int y = q / WW, x = q - y * WW;
     int X = X0 + x, Y = Y0 + y;
-    if (X < 0 || Y < 0 || X >= e.s.dimx || Y >= e.s.dimy) clamped = true;
-    X = clampi(X, 0, e.s.dimx - 1);
-    Y = clampi(Y, 0, e.s.dimy - 1);
-    const uint16_t v = e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+    if (X < 0 || Y < 0 || X >= fr.dimx || Y >= fr.dimy) clamped = true;
+    X = clampi(X, 0, fr.dimx - 1);
+    Y = clampi(Y, 0, fr.dimy - 1);
+    const uint16_t v = fr.I1[(int64_t)Y * fr.pitch1 + X];
     over |= v > e.vmax;
     win[q] = v;
   }
@@ -1369,6 +1372,7 @@ constexpr int kRescoreThreads = 256;
 constexpr int kRescoreChunk = 1024;
 
 __global__ __launch_bounds__(kRescoreThreads) void k_exact_rescore(ExactArgs e, StepBest* out) {
+  const FrameDesc fr = *e.s.fr;
   __shared__ __align__(16) double terms[4][kRescoreChunk];  // (candidate, C0C1 | C1C1) x chunk
   __shared__ unsigned long long part[2][kRescoreThreads / 64];
   __shared__ double sums[4];
@@ -1401,8 +1405,8 @@ __global__ __launch_bounds__(kRescoreThreads) void k_exact_rescore(ExactArgs e, 
   auto pixel = [&](int q, int i) -> unsigned int {
     int x, y;
     rotate_px(e.s.pattern[2 * (b + i)], e.s.pattern[2 * (b + i) + 1], c[q], sn[q], x, y);
-    const int X = clampi(bx[q] + x, 0, e.s.dimx - 1), Y = clampi(by[q] + y, 0, e.s.dimy - 1);
-    return e.s.I1[(int64_t)Y * e.s.pitch1 + X];
+    const int X = clampi(bx[q] + x, 0, fr.dimx - 1), Y = clampi(by[q] + y, 0, fr.dimy - 1);
+    return fr.I1[(int64_t)Y * fr.pitch1 + X];
   };
   // mean1 (:698-705): the sum of integers is exact in any order, then one division
   unsigned long long s1[2] = {0ull, 0ull};
@@ -1970,6 +1974,8 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   // the tables: uploaded unless the scratch already holds this keyed plan's
   const bool fresh = plan_key != 0 && c.exact_dev_key[stage] == plan_key && P->dev_base == c.d_exact_scratch[stage];
   if (!fresh) {
+    if (c.capturing) return c.fail(PTG_ERR_STATE, "exact search tables changed during a pair's capture");
+    c.table_uploads++;
     if ((he = hipMemcpyAsync((void*)e.slots, P->slots.data(), sizeof(ExactSlot) * nlist, hipMemcpyHostToDevice,
                              c.stream)) != hipSuccess ||
         (he = hipMemcpyAsync(slot_of_dev, P->slot_of.data(), sizeof(int32_t) * P->slot_of.size(),

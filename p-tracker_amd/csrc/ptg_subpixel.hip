@@ -911,8 +911,8 @@ __device__ __noinline__ void sp_step() {
 
 // follow_pattern_subpixel_xyR, src/p-tracker.cpp:1090-1128: one workgroup per grain of the
 // unmasked list, taken in the order of `list` (largest patterns first).
-__global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_subpixel(const uint16_t* __restrict__ I1, int dimx, int dimy,
-                                                         int64_t pitch, ptg_grain* __restrict__ grains,
+__global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_subpixel(const FrameDesc* __restrict__ frames,
+                                                         ptg_grain* __restrict__ grains,
                                                          const int64_t* __restrict__ pbegin,
                                                          const uint32_t* __restrict__ words,
                                                          const double* __restrict__ m0,
@@ -930,7 +930,8 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   Obj o;
-  load_obj(o, gr, I1, dimx, dimy, pitch, words + b, m0[g], p0 + 2 * b, d0 + b, n, sd0[g],
+  const FrameDesc fr = *frames;
+  load_obj(o, gr, fr.I1, fr.dimx, fr.dimy, fr.pitch1, words + b, m0[g], p0 + 2 * b, d0 + b, n, sd0[g],
            prm.subpix_displacement_max, prm.overflow_stiffness);
   stage_grain(o, gr, prad[g], lay);
   SpState& st = sp_state();
@@ -1031,13 +1032,12 @@ static size_t sp_lds_bytes(const SpLayout& lay) {
 
 hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
   if (c.n_unmasked == 0) return hipSuccess;
-  const Image& im = c.img[c.icur];
   const SpLayout lay = sp_layout(c.max_prad_unmasked, p.subpix_displacement_max);
   const size_t lds = sp_lds_bytes(lay);
   hipError_t e = ensure_kernel_lds(c.device, (const void*)k_subpixel, lds);
   if (e != hipSuccess) return e;
   k_subpixel<<<c.n_unmasked, kSpThreads, lds, c.stream>>>(
-      im.d, im.dimx, im.dimy, im.pitch, c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0,
+      c.d_frames, c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0,
       c.d_list_lpt, c.n_unmasked,
       p, reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, lay, c.d_evals_g);
   return hipGetLastError();
