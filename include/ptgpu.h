@@ -317,12 +317,20 @@ int ptg_set_timing(void* ctx, int enable);
  *  a frame sequence does between reference swaps; stage timers are not kept for replayed pairs
  *  (with timing on, or host tables, pairs run as launches).
  */
+/*  PTG_OPT_SEARCH_ROLES [1]: the wave roles of the exact search's MFMA kernel (where no tuned
+ *  shape says otherwise). 1: every wave multiplies and scores its own shift columns in turn;
+ *  2: producer waves stage shift columns and run the MFMA GEMM, each handing a vertical pass's
+ *  cross terms to a consumer wave through LDS, which adds the window-sum corrections and keeps
+ *  the near-tie candidates while its producer multiplies the next pass (at most 4 + 4 waves).
+ *  The picks are the same; the autotuner tries both.
+ */
 enum {
   PTG_OPT_SEARCH_STREAMS = 0,
   PTG_OPT_ROT_TABLES = 1,
   PTG_OPT_ROT_GUARD_TEST = 2,
   PTG_OPT_DEVICE_QUEUES = 3,
-  PTG_OPT_GRAPH = 4
+  PTG_OPT_GRAPH = 4,
+  PTG_OPT_SEARCH_ROLES = 5
 };
 int ptg_set_option(void* ctx, int option, int64_t value);
 /* The rotation guard's counters since the context was created: guard entries the host

@@ -208,8 +208,8 @@ struct Ctx : Context {
       exact_grain_fallbacks += static_cast<const int32_t*>(pin_flags.p)[2 * st + 1];
       if (f)
         return fail(PTG_ERR_LIMIT,
-                    "exact search capacity exceeded in stage %d (flags %d): the stage changed nothing; use "
-                    "PTG_SEARCH_PLAIN",
+                    "exact search capacity exceeded in stage %d (flags %d; 4: a producer/consumer hand-over "
+                    "wait expired): the stage changed nothing; use PTG_SEARCH_PLAIN",
                     st, f);
     }
     return PTG_OK;
@@ -1340,16 +1340,20 @@ int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, doub
         TRY(run(&ms));
         const ExactShape base_shape = c->tune_used[bi];
         ExactShape best = base_shape;
-        for (int ntp = 1; ntp <= 3; ntp++)
-          for (int nsub : {1, 2, 4})
-            for (int nsu = 1; nsu <= 8; nsu++) {
-              const ExactShape want{ntp, nsu, nsub};
-              if (want.ntp == base_shape.ntp && want.nsu == base_shape.nsu && want.nsub == base_shape.nsub) continue;
+        for (int roles = 1; roles <= 2; roles++)
+          for (int ntp = 1; ntp <= 3; ntp++)
+            for (int nsub : {1, 2, 4})
+              for (int nsu = 1; nsu <= (roles == 2 ? 4 : 8); nsu++) {
+              const ExactShape want{ntp, nsu, nsub, roles};
+              if (want.ntp == base_shape.ntp && want.nsu == base_shape.nsu && want.nsub == base_shape.nsub &&
+                  want.roles == base_shape.roles)
+                continue;
               c->tune_bucket = (int)bi;
               c->tune_shape = want;
               TRY(run(&ms));
               const ExactShape& got = c->tune_used[bi];
-              if (got.ntp != want.ntp || got.nsu != want.nsu || got.nsub != want.nsub) continue;  // did not fit
+              if (got.ntp != want.ntp || got.nsu != want.nsu || got.nsub != want.nsub || got.roles != want.roles)
+                continue;  // did not fit
               if (ms < best_ms) {
                 TRY(median3(&ms));
                 if (ms < best_ms) {
@@ -2028,6 +2032,12 @@ int ptg_set_option(void* p, int option, int64_t value) {
     TRY(check_rot_guard(c, &redo));  // (entries of unguarded work are dropped)
     c->rot_host = value == 1;
     c->snap = Ctx::SNAP_INVALID;
+    return PTG_OK;
+  }
+  if (option == PTG_OPT_SEARCH_ROLES) {
+    if (value != 1 && value != 2) return c->fail(PTG_ERR_ARG, "PTG_OPT_SEARCH_ROLES takes 1 or 2");
+    c->exact_roles = (int)value;
+    c->tune_gen++;  // (a captured pair's launches change)
     return PTG_OK;
   }
   if (option == PTG_OPT_DEVICE_QUEUES) {

@@ -100,9 +100,11 @@ struct SearchArgs {
 
 // A launch shape of the exact search's MFMA kernel for one radius bucket: N-tiles of 16
 // vertical shifts per pass, waves (shift columns in flight) per workgroup, workgroups per
-// grain. Zero fields take the built-in heuristic's choice.
+// grain, and the wave roles (1: every wave multiplies and scores its own columns; 2: producer
+// waves multiply, as many consumer waves score, nsu being the producers). Zero fields take the
+// built-in heuristic's choice.
 struct ExactShape {
-  int32_t ntp = 0, nsu = 0, nsub = 0;
+  int32_t ntp = 0, nsu = 0, nsub = 0, roles = 0;
 };
 
 struct Context {
@@ -120,6 +122,9 @@ struct Context {
   int ensure_aux(size_t nevents);
   // PTG_OPT_SEARCH_STREAMS = 1: every kernel of an exact search in order on `stream`
   bool serial_search = false;
+  // PTG_OPT_SEARCH_ROLES: the MFMA kernel's wave roles where no tuned shape says otherwise
+  // (1: each wave multiplies and scores its own columns; 2: producer and consumer waves)
+  int exact_roles = 1;
   std::string err;
 
   Image img[kMaxImages];

@@ -798,6 +798,18 @@ __device__ __forceinline__ unsigned long long sq8(unsigned int w0, unsigned int 
   }
 }
 
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+// Sums of I1 and I1^2 over pairs of window values of at most 12 bits: v_dot2_u32_u16 adds both
+// values (against (1, 1)) and both squares in one instruction each. A square is below 2^24, so a
+// 32-bit sum of up to 128 squares is exact; callers pass at most 16 values' squares through
+// csq32 before widening it.
+__device__ __forceinline__ void dot2_acc(unsigned int a, unsigned int b, unsigned int& bsum, unsigned int& csq32) {
+  const us2 v = us2{(unsigned short)a, (unsigned short)b};
+  bsum = __builtin_amdgcn_udot2(v, us2{1, 1}, bsum, false);
+  csq32 = __builtin_amdgcn_udot2(v, v, csq32, false);
+}
+
 // Sums of I1 and I1^2 at this lane's shift over the entries [lo, lo + len) of one Delta list
 // held in LDS from dent[o] (lo a multiple of 4; 4 entries per 16-byte read, 16 window reads
 // in flight) and, with tail set, over the list's entries [nl, n) beyond the LDS capacity from
@@ -820,17 +832,39 @@ __device__ __forceinline__ void delta_seg(const uint16_t* win, const uint32_t* d
     const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
     const unsigned int w8 = px(e2.x), w9 = px(e2.y), w10 = px(e2.z), w11 = px(e2.w);
     const unsigned int w12 = px(e3.x), w13 = px(e3.y), w14 = px(e3.z), w15 = px(e3.w);
-    bsum += (((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))) + (((w8 + w9) + (w10 + w11)) + ((w12 + w13) + (w14 + w15)));
-    csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
-    csum += sq8<W64>(w8, w9, w10, w11, w12, w13, w14, w15);
+    if constexpr (W64) {
+      bsum += (((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))) + (((w8 + w9) + (w10 + w11)) + ((w12 + w13) + (w14 + w15)));
+      csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
+      csum += sq8<W64>(w8, w9, w10, w11, w12, w13, w14, w15);
+    } else {
+      unsigned int c32 = 0u;
+      dot2_acc(w0, w1, bsum, c32);
+      dot2_acc(w2, w3, bsum, c32);
+      dot2_acc(w4, w5, bsum, c32);
+      dot2_acc(w6, w7, bsum, c32);
+      dot2_acc(w8, w9, bsum, c32);
+      dot2_acc(w10, w11, bsum, c32);
+      dot2_acc(w12, w13, bsum, c32);
+      dot2_acc(w14, w15, bsum, c32);
+      csum += c32;
+    }
   }
   for (; q + 8 <= len; q += 8) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
     const uint4 e1 = *reinterpret_cast<const uint4*>(dent + o + q + 4);
     const unsigned int w0 = px(e0.x), w1 = px(e0.y), w2 = px(e0.z), w3 = px(e0.w);
     const unsigned int w4 = px(e1.x), w5 = px(e1.y), w6 = px(e1.z), w7 = px(e1.w);
-    bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
-    csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
+    if constexpr (W64) {
+      bsum += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+      csum += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
+    } else {
+      unsigned int c32 = 0u;
+      dot2_acc(w0, w1, bsum, c32);
+      dot2_acc(w2, w3, bsum, c32);
+      dot2_acc(w4, w5, bsum, c32);
+      dot2_acc(w6, w7, bsum, c32);
+      csum += c32;
+    }
   }
   for (; q < len; q += 4) {
     const uint4 e0 = *reinterpret_cast<const uint4*>(dent + o + q);
@@ -883,10 +917,24 @@ __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* w
     const unsigned int w4 = px(a1.x), w5 = px(a1.y), w6 = px(a1.z), w7 = px(a1.w);
     const unsigned int v0 = px(b0.x), v1 = px(b0.y), v2 = px(b0.z), v3 = px(b0.w);
     const unsigned int v4 = px(b1.x), v5 = px(b1.y), v6 = px(b1.z), v7 = px(b1.w);
-    bp += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
-    cp += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
-    bn += ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7));
-    cn += sq8<W64>(v0, v1, v2, v3, v4, v5, v6, v7);
+    if constexpr (W64) {
+      bp += ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+      cp += sq8<W64>(w0, w1, w2, w3, w4, w5, w6, w7);
+      bn += ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7));
+      cn += sq8<W64>(v0, v1, v2, v3, v4, v5, v6, v7);
+    } else {
+      unsigned int cp32 = 0u, cn32 = 0u;
+      dot2_acc(w0, w1, bp, cp32);
+      dot2_acc(v0, v1, bn, cn32);
+      dot2_acc(w2, w3, bp, cp32);
+      dot2_acc(v2, v3, bn, cn32);
+      dot2_acc(w4, w5, bp, cp32);
+      dot2_acc(v4, v5, bn, cn32);
+      dot2_acc(w6, w7, bp, cp32);
+      dot2_acc(v6, v7, bn, cn32);
+      cp += cp32;
+      cn += cn32;
+    }
   }
   delta_seg<W64>(win, dent, e.dlist + dl_off + (int64_t)(2 * k) * dcap, o0, nc, nl0 - nc, true, nl0, n0, wbase,
                  base, bp, cp);
@@ -894,7 +942,13 @@ __device__ __forceinline__ void delta_pair(const ExactArgs& e, const uint16_t* w
                  wbase, base, bn, cn);
 }
 
-template <int NMT, int NTP, int VM>
+// Hand-over waits of the producer/consumer form: a wait that lasts this many polls (each a short
+// sleep, about 0.1 s in all) gives up and voids the search (a hand-over never waits that long).
+constexpr int kWsSpinLimit = 1 << 21;
+
+// WS: producer and consumer waves (e.nsu producers, as many consumers: at most 4 + 4); else every
+// wave takes its own shift columns through the MFMA loop and the epilogue in turn.
+template <int NMT, int NTP, int VM, bool WS>
 __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   const FrameDesc fr = *e.s.fr;
   constexpr int SPT = vm_spt(VM), NP = vm_planes(VM);
@@ -931,13 +985,23 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   uint8_t* stage = reinterpret_cast<uint8_t*>(smem + win_bytes) + (size_t)wave * stage_bytes;
   unsigned char* tail = smem + win_bytes + (size_t)e.nsu * stage_bytes;
   // the epilogue's exchange buffer: the wave's own staging area when the column takes one
-  // vertical pass (the staged planes are dead once its MFMAs are issued), else its own area
+  // vertical pass (the staged planes are dead once its MFMAs are issued), else its own area;
+  // for producer and consumer waves, one slot per pair with every M-tile's rows
   constexpr int ABR = VM == kVm16 ? 16 : 8;  // its rows: one per M-tile row (16-bit) or per step
-  double* abuf = e.abuf_alias ? reinterpret_cast<double*>(stage)
-                              : reinterpret_cast<double*>(tail) + (size_t)wave * ABR * (NTP * 16);
-  if (!e.abuf_alias) tail += (size_t)e.nsu * ABR * (NTP * 16) * 8;
+  double* abuf = nullptr;
+  double* slots = nullptr;
+  if constexpr (WS) {
+    slots = reinterpret_cast<double*>(tail);
+    tail += (size_t)e.nsu * NMT * ABR * (NTP * 16) * 8;
+  } else {
+    abuf = e.abuf_alias ? reinterpret_cast<double*>(stage)
+                        : reinterpret_cast<double*>(tail) + (size_t)wave * ABR * (NTP * 16);
+    if (!e.abuf_alias) tail += (size_t)e.nsu * ABR * (NTP * 16) * 8;
+  }
   int32_t* bflag = reinterpret_cast<int32_t*>(tail);  // the block's hand-over flag
   tail += 16;
+  int32_t* hflag = reinterpret_cast<int32_t*>(tail);  // producer/consumer slot flags, then the abort word
+  if constexpr (WS) tail += 80;
   int32_t* segl = reinterpret_cast<int32_t*>(tail);
   for (int q = threadIdx.x; q < 4 * kc; q += nthreads) segl[q] = e.seg[sl.seg_off + q];
   // Delta lists of this slot in LDS: counts, offsets of the lists in a compact table
@@ -950,6 +1014,8 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   for (int q = threadIdx.x; q < nls; q += nthreads) dcnt[q] = e.dlen[(int64_t)li * nls + q];
 
   if (threadIdx.x == 0) *bflag = 0;
+  if constexpr (WS)
+    if (threadIdx.x < 17) hflag[threadIdx.x] = 0;
   __syncthreads();
   bool clamped = false, over = false;
   for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
@@ -1015,6 +1081,7 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
   // this workgroup's shift columns [c_lo, c_hi); wave w takes c_lo + w, c_lo + w + nsu, ...
   const int cpb = (U + nsub - 1) / nsub;
   const int c_lo = blk * cpb, c_hi = min(U, c_lo + cpb);
+  if constexpr (!WS) {
   for (int su = c_lo + wave; su < c_hi; su += e.nsu) {
     // stage this shift column: cell c of staged row y is window column su + c; plane 0 holds
     // the high digit (12-bit: v >> 7; 16-bit: the high byte, top bit flipped) or, for 8-bit
@@ -1224,6 +1291,234 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
+    }
+  }
+  } else {
+    // Producer and consumer waves: producer w (w < P) stages its shift columns and runs each
+    // vertical pass's MFMA GEMM, then hands the pass's cross terms A (every step, TW shifts)
+    // to consumer P + w through its LDS slot; the consumer does the pass's epilogue (Delta
+    // sums and the near-tie list) while the producer multiplies the next pass. A flag per slot
+    // (0 free, 1 full) orders the hand-over (release store, acquire load, workgroup scope).
+    // Waits are bounded: one that runs out marks the search void (flags[0] bit 2) and tells
+    // every wave to skip the rest, so all reach the end.
+    const int P = e.nsu;
+    const bool producer = wave < P;
+    const int pid = producer ? wave : wave - P;
+    int32_t* hf = hflag + pid;
+    int32_t* habort = hflag + 16;
+    const int ntile = vp >> 4;
+    constexpr int TW = NTP * 16;
+    double* slot = slots + (size_t)pid * (NMT * ABR * TW);
+    auto ws_wait = [&](int want) -> bool {
+      for (int n = 0; n < kWsSpinLimit; n++) {
+        if (__hip_atomic_load(hf, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == want) return true;
+        if (__hip_atomic_load(habort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (lane == 0) atomicOr(&e.flags[0], 4);
+      __hip_atomic_store(habort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      return false;
+    };
+    // the pass's cross term of step k at position pos, the offset put back
+    auto a_at = [&](int k, int pos) -> double {
+      if constexpr (VM == kVm16) {
+        const int a = k / 5, j = k - 5 * a;
+        const double* r = slot + (size_t)(a * 16 + 3 * j) * TW + pos;
+        return ((r[0] + r[TW]) + r[2 * TW]) + vm_offset(VM) * Sad;
+      } else if constexpr (VM == kVm8) {
+        return slot[(size_t)k * TW + pos] + vm_offset(VM) * Sad;
+      } else {
+        return slot[(size_t)k * TW + pos];
+      }
+    };
+    // one candidate with its Delta sums (as the single-role form's)
+    auto consider = [&](int k, int sidx, double A, long long Bm, long long Cm, unsigned int bp, unsigned int bn,
+                        unsigned long long cp, unsigned long long cn) {
+      double nd, dd;
+      if constexpr (VM == kVm16) {
+        const long long Ai = (long long)A;
+        const long long Bi = Bm + (long long)bp - (long long)bn;
+        const long long Ci = Cm + (long long)cp - (long long)cn;
+        nd = (double)(N * Ai - Sa * Bi);
+        dd = (double)(N * Ci - Bi * Bi);
+      } else {
+        const double B = (double)Bm + ((double)bp - (double)bn);
+        const double C = (double)Cm + ((double)cp - (double)cn);
+        nd = Nd * A - Sad * B;
+        dd = Nd * C - B * B;
+      }
+      if (nd > 0.0 && dd > 0.0 && nd * nd >= s1lo * dd) {
+        nl = near2_insert(nl, nd, dd, k * S + sidx, lo);
+        s1lo = nl.s1 * lo;
+      }
+    };
+    bool alive = true;
+    for (int su = c_lo + pid; su < c_hi && alive; su += P) {
+      if (producer) {
+        // stage this shift column: cell c of staged row y is window column su + c; plane 0 holds
+        // the high digit (12-bit: v >> 7; 16-bit: the high byte, top bit flipped) or, for 8-bit
+        // frames, the only one (v - 128), plane 1 the low digit
+        const int nq = WH * (cw >> 2);
+        for (int q = lane; q < nq; q += 64) {
+          const int y = q / (cw >> 2), c4 = (q - y * (cw >> 2)) * 4;
+          unsigned int hi = 0u, lo = 0u;
+    #pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const unsigned int v = win[y * WW + su + c4 + j];
+            if constexpr (VM == kVm8) {
+              hi |= ((v & 255u) ^ 128u) << (8 * j);
+            } else if constexpr (VM == kVm12) {
+              hi |= ((v >> 7) & 127u) << (8 * j);
+              lo |= (v & 127u) << (8 * j);
+            } else {
+              hi |= (((v >> 8) & 255u) ^ 128u) << (8 * j);
+              lo |= ((v & 255u) ^ 128u) << (8 * j);
+            }
+          }
+          *reinterpret_cast<unsigned int*>(stage + (size_t)y * cs + c4) = hi;
+          if constexpr (NP == 2) *reinterpret_cast<unsigned int*>(stage + (size_t)e.wh_pad * cs + (size_t)y * cs + c4) = lo;
+        }
+        // the stage is this wave's alone: a wave-level barrier orders its writes and reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      for (int p0 = 0; p0 < ntile && alive; p0 += NTP) {
+        if (producer) {
+          v4i acc[NMT][NTP][NP];
+    #pragma unroll
+          for (int a = 0; a < NMT; a++)
+    #pragma unroll
+            for (int j = 0; j < NTP; j++)
+    #pragma unroll
+              for (int p = 0; p < NP; p++) acc[a][j][p] = v4i{0, 0, 0, 0};
+          // Software pipeline: the template fragments (global memory) of chunk ch + 1 are
+          // requested before chunk ch multiplies (two named buffers, scheduling barriers keep
+          // the loads ahead of their use); the window fragments (LDS) are read per chunk.
+          v4i fa0[NMT], fa1[NMT], fb[NTP][NP];
+          auto fetch_a = [&](int chq, v4i* fa) {
+            const int chc = chq < kc ? chq : kc - 1;
+    #pragma unroll
+            for (int a = 0; a < NMT; a++)
+              fa[a] = *reinterpret_cast<const v4i*>(tmpl + (((int64_t)a * kc + chc) * 64 + lane) * 16);
+          };
+          auto fetch_b = [&](int chq) {
+            const int segv = segs[4 * chq + gg];
+            const int rowoff = (segv >> 16) * cs + (segv & 0x7fff);  // padding segments read row 0
+    #pragma unroll
+            for (int j = 0; j < NTP; j++) {
+              const uint8_t* bp = stage + (size_t)((p0 + j) * 16 + col) * cs + rowoff;
+              fb[j][0] = *reinterpret_cast<const v4i*>(bp);
+              if constexpr (NP == 2) fb[j][NP - 1] = *reinterpret_cast<const v4i*>(bp + (size_t)e.wh_pad * cs);
+            }
+          };
+          auto mult = [&](const v4i* fa) {
+    #pragma unroll
+            for (int j = 0; j < NTP; j++)
+    #pragma unroll
+              for (int a = 0; a < NMT; a++)
+    #pragma unroll
+                for (int p = 0; p < NP; p++)
+                  acc[a][j][p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[j][p], acc[a][j][p], 0, 0, 0);
+          };
+          fetch_a(0, fa0);
+          for (int ch = 0; ch < kc; ch += 2) {  // kc is even (host pads with empty chunks)
+            fetch_a(ch + 1, fa1);
+            fetch_b(ch);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(fa0);
+            __builtin_amdgcn_sched_barrier(0);
+            fetch_a(ch + 2, fa0);
+            fetch_b(ch + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(fa1);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (!ws_wait(0)) {
+            alive = false;
+            break;
+          }
+          // A of every step of this pass into the slot: 8- and 12-bit frames one row per step,
+          // 16-bit frames the M-tiles' rows (three per step, summed by the consumer)
+#pragma unroll
+          for (int a = 0; a < NMT; a++)
+#pragma unroll
+            for (int j = 0; j < NTP; j++) {
+              if constexpr (VM == kVm16) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                  const int r = 4 * gg + i, d = r % 3;
+                  const double wgt = d == 0 ? 16384.0 : d == 1 ? 128.0 : 1.0;
+                  slot[(size_t)(a * 16 + r) * TW + j * 16 + col] =
+                      ((double)acc[a][j][0][i] * 256.0 + (double)acc[a][j][1][i]) * wgt;
+                }
+              } else {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                  double A;
+                  if constexpr (VM == kVm12)
+                    A = (double)acc[a][j][0][2 * h] * 16384.0 +
+                        ((double)acc[a][j][1][2 * h] + (double)acc[a][j][0][2 * h + 1]) * 128.0 +
+                        (double)acc[a][j][1][2 * h + 1];
+                  else
+                    A = (double)acc[a][j][0][2 * h] * 128.0 + (double)acc[a][j][0][2 * h + 1];
+                  slot[(size_t)(a * 8 + 2 * gg + h) * TW + j * 16 + col] = A;
+                }
+              }
+            }
+          __hip_atomic_store(hf, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          if (!ws_wait(1)) {
+            alive = false;
+            break;
+          }
+          const int vpass = min(TW, V - p0 * 16);  // vertical shifts of this pass
+          if constexpr (NTP == 3) {
+            // lane = vertical shift, one step per round: the step's lists are wave-uniform
+            const int svl0 = p0 * 16 + lane;
+            const bool lane_ok = lane < TW && svl0 < V;
+            long long Bm0 = 0, Cm0 = 0;
+            int base0 = 0, sidx0 = 0;
+            if (lane_ok) {
+              sidx0 = su * V + svl0;
+              Bm0 = e.mid[sl.mid_off + 2 * (int64_t)sidx0];
+              Cm0 = e.mid[sl.mid_off + 2 * (int64_t)sidx0 + 1];
+              base0 = svl0 * WW + su;
+            }
+            const uint32_t wbase = 2u * (uint32_t)base0;
+            for (int k = 0; k < nsteps; k++) {
+              if ((rep >> k) & 1) continue;  // a repeated step
+              unsigned int bp = 0u, bn = 0u;
+              unsigned long long cp = 0ull, cn = 0ull;
+              delta_pair<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, k, wbase, base0, bp, cp, bn, cn);
+              if (lane_ok) consider(k, sidx0, a_at(k, lane), Bm0, Cm0, bp, bn, cp, cn);
+            }
+          } else {
+            // the pass's (step, shift) pairs 64 per round over every step
+            const int nitem = nsteps * vpass;
+            for (int it0 = 0; it0 < nitem; it0 += 64) {
+              const int it = it0 + lane;
+              const int k = it < nitem ? it / vpass : 0, vl = it < nitem ? it - k * vpass : 0;
+              const int svl = p0 * 16 + vl;
+              const bool act = it < nitem && !((rep >> k) & 1);  // a repeated step is skipped
+              const int sidx = su * V + svl, base = svl * WW + su;
+              long long Bm = 0, Cm = 0;
+              if (act) {
+                Bm = e.mid[sl.mid_off + 2 * (int64_t)sidx];
+                Cm = e.mid[sl.mid_off + 2 * (int64_t)sidx + 1];
+              }
+              const uint32_t wbase = 2u * (uint32_t)base;
+              unsigned int bp = 0u, bn = 0u;
+              unsigned long long cp = 0ull, cn = 0ull;
+              delta_sums<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k, wbase, base, bp, cp);
+              delta_sums<VM == kVm16>(e, win, dcnt, dofs, dent, sl.dl_off, sl.dcap, act, 2 * k + 1, wbase, base, bn,
+                                      cn);
+              if (act) consider(k, sidx, a_at(k, vl), Bm, Cm, bp, bn, cp, cn);
+            }
+          }
+          __hip_atomic_store(hf, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
     }
   }
   // the block's list: its best (lanes, then waves, as (s, index) pairs), then the entries
@@ -1509,11 +1804,11 @@ __global__ void k_exact_activate(const int32_t* __restrict__ queue, const int32_
 }
 
 // ---- host side -----------------------------------------------------------------------
-template <int NMT, int NTP, int VM>
+template <int NMT, int NTP, int VM, bool WS>
 static hipError_t launch_main(int dev, const ExactArgs& e, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
-  if (hipError_t r = ensure_kernel_lds(dev, (const void*)k_search_exact<NMT, NTP, VM>, lds); r != hipSuccess)
+  if (hipError_t r = ensure_kernel_lds(dev, (const void*)k_search_exact<NMT, NTP, VM, WS>, lds); r != hipSuccess)
     return r;
-  k_search_exact<NMT, NTP, VM><<<grid, block, lds, st>>>(e);
+  k_search_exact<NMT, NTP, VM, WS><<<grid, block, lds, st>>>(e);
   return hipGetLastError();
 }
 
@@ -1533,30 +1828,33 @@ struct MainAttr {
   int vgprs = 256;
   size_t static_lds = 1024;
 };
-static MainAttr main_attr(int vm, int nmt, int ntp) {
+static MainAttr main_attr(int vm, int nmt, int ntp, bool ws) {
   const void* fn = nullptr;
-#define PTG_FN(M, T, VM) \
-  if (vm == VM && nmt == M && ntp == T) fn = (const void*)k_search_exact<M, T, VM>;
+#define PTG_FN(M, T, VM)                                                                                \
+  if (vm == VM && nmt == M && ntp == T)                                                                 \
+    fn = ws ? (const void*)k_search_exact<M, T, VM, true> : (const void*)k_search_exact<M, T, VM, false>;
   PTG_EXACT_INSTANCES(PTG_FN)
 #undef PTG_FN
-  static MainAttr cache[3][kExactMaxMt + 1][4] = {};
-  static bool known[3][kExactMaxMt + 1][4] = {};
+  static MainAttr cache[2][3][kExactMaxMt + 1][4] = {};
+  static bool known[2][3][kExactMaxMt + 1][4] = {};
   if (!fn) return MainAttr{};
-  if (!known[vm][nmt][ntp]) {
+  if (!known[ws][vm][nmt][ntp]) {
     hipFuncAttributes at;
     if (hipFuncGetAttributes(&at, fn) == hipSuccess && at.numRegs > 0) {
-      cache[vm][nmt][ntp].vgprs = at.numRegs;
-      cache[vm][nmt][ntp].static_lds = at.sharedSizeBytes;
+      cache[ws][vm][nmt][ntp].vgprs = at.numRegs;
+      cache[ws][vm][nmt][ntp].static_lds = at.sharedSizeBytes;
     }
-    known[vm][nmt][ntp] = true;
+    known[ws][vm][nmt][ntp] = true;
   }
-  return cache[vm][nmt][ntp];
+  return cache[ws][vm][nmt][ntp];
 }
 
-static hipError_t dispatch_main(int dev, int vm, int nmt, int ntp, const ExactArgs& e, dim3 grid, dim3 block,
-                                size_t lds, hipStream_t st) {
-#define PTG_CASE(M, T, VM) \
-  if (vm == VM && nmt == M && ntp == T) return launch_main<M, T, VM>(dev, e, grid, block, lds, st);
+static hipError_t dispatch_main(int dev, int vm, int nmt, int ntp, bool ws, const ExactArgs& e, dim3 grid,
+                                dim3 block, size_t lds, hipStream_t st) {
+#define PTG_CASE(M, T, VM)                                                                   \
+  if (vm == VM && nmt == M && ntp == T)                                                      \
+    return ws ? launch_main<M, T, VM, true>(dev, e, grid, block, lds, st)                    \
+              : launch_main<M, T, VM, false>(dev, e, grid, block, lds, st);
   PTG_EXACT_INSTANCES(PTG_CASE)
 #undef PTG_CASE
   return hipErrorInvalidValue;
@@ -1572,6 +1870,7 @@ struct BucketLaunch {
   int cs = 0, ww = 0, wh = 0;    // k_search_exact: staging stride, window width and height
   int dl_cap = 0, nsub = 0, nsu = 0, ntp = 0;
   bool alias = false;
+  bool ws = false;               // producer and consumer waves (nsu producers)
   size_t lds = 0;
   uint64_t key = 0;              // the bucket's key in the autotuned shapes
 };
@@ -1872,6 +2171,9 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const size_t stage_bytes = ((size_t)vm_planes(vm) * l.wh * l.cs + 15) / 16 * 16;
     const size_t abuf_bytes = (size_t)(vm == kVm16 ? 16 : 8) * (l.ntp * 16) * 8;
     l.alias = P->vp / 16 <= l.ntp && abuf_bytes <= stage_bytes;
+    // wave roles: producer/consumer pairs hand each pass's cross terms of every step over in a slot
+    l.ws = (want.roles > 0 ? want.roles : c.exact_roles) == 2;
+    const size_t slot_bytes = (size_t)nmt * abuf_bytes;
     const size_t seg_bytes = (size_t)((4 * bk.max_kc + 3) & ~3) * 4;
     // Delta tables: counts, offsets and up to dl_cap entries (+ 4 of slack for the last
     // 16-byte read); typical lists total about 1.5 rho entries per rotation step
@@ -1879,6 +2181,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     l.dl_cap = std::min(nls * P->dcap, 2 * rho * nsteps + 4 * nls);
     const size_t dl_bytes = (size_t)(((nls + 3) & ~3) + ((nls + 4) & ~3) + l.dl_cap + 4) * 4;
     auto need = [&](int w) {
+      if (l.ws) return win_bytes + w * (stage_bytes + slot_bytes) + 16 + 80 + seg_bytes + dl_bytes;
       return win_bytes + w * (stage_bytes + (l.alias ? 0 : abuf_bytes)) + 16 + seg_bytes + dl_bytes;
     };
     // shift columns split over nsub workgroups when the bucket has few grains
@@ -1886,19 +2189,22 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     const int cpb = (U + l.nsub - 1) / l.nsub;
     // waves per workgroup: the most waves resident per CU under the register limit (a
     // SIMD holds 512 / VGPRs waves) and the LDS limit, fewer and larger workgroups on ties
-    const MainAttr ma = main_attr(vm, nmt, l.ntp);
+    const MainAttr ma = main_attr(vm, nmt, l.ntp, l.ws);
     const int wave_cap = 4 * std::max(1, 512 / std::max(ma.vgprs, 1));
+    // (producer/consumer: w producers and w consumers per workgroup, at most 4 + 4)
+    const int wmax = std::min(l.ws ? 4 : 8, cpb), wpw = l.ws ? 2 : 1;
     int best_res = 0;
     l.nsu = 1;
-    for (int w = 1; w <= std::min(8, cpb); w++) {
+    for (int w = 1; w <= wmax; w++) {
       if (need(w) > kLdsDynMax) break;
-      const int res = std::min(wave_cap, (int)(160 * 1024 / (need(w) + ma.static_lds)) * w);  // workgroups per CU by LDS
+      // waves resident per CU: workgroups by LDS times their waves, under the register limit
+      const int res = std::min(wave_cap, (int)(160 * 1024 / (need(w) + ma.static_lds)) * w * wpw);
       if (res >= best_res) {
         best_res = res;
         l.nsu = w;
       }
     }
-    if (want.nsu > 0 && want.nsu <= std::min(8, cpb) && need(want.nsu) <= kLdsDynMax) l.nsu = want.nsu;
+    if (want.nsu > 0 && want.nsu <= wmax && need(want.nsu) <= kLdsDynMax) l.nsu = want.nsu;
     l.lds = need(l.nsu);
     if (l.lds_m > kLdsDynMax || l.lds_t > kLdsDynMax || l.lds > kLdsDynMax) return kExactUnfit;
   }
@@ -1907,7 +2213,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     c.tune_used.assign(P->buckets.size(), ExactShape{});
     for (size_t bi = 0; bi < P->buckets.size(); bi++) {
       c.tune_keys[bi] = L[bi].key;
-      c.tune_used[bi] = ExactShape{L[bi].ntp, L[bi].nsu, L[bi].nsub};
+      c.tune_used[bi] = ExactShape{L[bi].ntp, L[bi].nsu, L[bi].nsub, L[bi].ws ? 2 : 1};
     }
   }
   int rc = c.ensure(&c.d_exact_scratch[stage], &c.exact_scratch_cap[stage], P->total);
@@ -2121,10 +2427,10 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       nwg = 0;
       for (int n = 1; n <= nb; n++) nwg = std::max(nwg, 8 * ((n + 7) / 8) * exact_nsub_of(n, U));
     }
-    dim3 grid(nwg), block(64 * eb.nsu);
+    dim3 grid(nwg), block(64 * eb.nsu * (l.ws ? 2 : 1));
     hipEvent_t ev = nullptr;
     if (time_kernel) c.mark_begin(PTG_T_SEARCH_KERNEL, &ev, sx);
-    if ((he = dispatch_main(c.device, vm, nmt, l.ntp, eb, grid, block, l.lds, sx)) != hipSuccess)
+    if ((he = dispatch_main(c.device, vm, nmt, l.ntp, l.ws, eb, grid, block, l.lds, sx)) != hipSuccess)
       return c.check(he, "k_search_exact");
     if (time_kernel) c.mark_end(PTG_T_SEARCH_KERNEL, ev, sx);
   }

@@ -56,11 +56,17 @@ def run(tracker, w, method, subpixel, subpix_method=pk.SUBPIX_LM):
     return tracker.get_grains()
 
 
-@pytest.mark.parametrize("method", ["plain", "exact"])
-def test_full_size_search_and_rescue_equal_oracle(tracker, full, method):
+@pytest.mark.parametrize("method,roles", [("plain", 1), ("exact", 1), ("exact", 2)])
+def test_full_size_search_and_rescue_equal_oracle(tracker, full, method, roles):
+    # roles: the exact MFMA kernel's waves multiply and score their own columns (1, the default)
+    # or producer waves hand each pass to consumer waves (2)
     w, fx = full
-    say(f"full size, {method} search + rescue")
-    got = run(tracker, w, pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN, 0)
+    say(f"full size, {method} search + rescue, wave roles {roles}")
+    tracker.set_option(pk.OPT_SEARCH_ROLES, roles)
+    try:
+        got = run(tracker, w, pk.SEARCH_EXACT if method == "exact" else pk.SEARCH_PLAIN, 0)
+    finally:
+        tracker.set_option(pk.OPT_SEARCH_ROLES, 1)
     assert list(tracker.queue_counts()) == list(fx["counts"])
     for c in ("dx", "dy", "drot", "upix", "vpix", "rot_inc"):
         bad = np.flatnonzero(np.ascontiguousarray(got[c]).view(np.uint64) != fx["search_" + c].view(np.uint64))
