@@ -95,3 +95,57 @@ def test_graph_rescue_pair_replays_equal_launches(method):
         captures, launches, nodes = t.graph_stats()
         say(f"  captures {captures}, graph launches {launches}, nodes {nodes}")
         assert captures == 1 and launches >= 3
+
+
+def test_graph_sequence_10000_grains_equals_oracle():
+    # the batching workload at its full size (6000x4000, 10,000 disk grains of radius 6-60 px,
+    # +-8 px x 9 steps, LM subpixel) over 4 pairs with reference swaps: the first two run as
+    # launches (the key settles), the third is captured, the fourth replays it; a sample of every
+    # pair's grains against the oracle run from the GPU's own input records of that pair
+    S = synth.sequence(n_frames=5, n_grains=10000, seed=1005)
+    z = S.params.search_zone
+    prm = pk.track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
+                          subpix=pk.default_subpix(pk.SUBPIX_LM))
+    oprm = pk.track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=pk.SEARCH_PLAIN,
+                           subpix=pk.default_subpix(pk.SUBPIX_LM, subpix_tol=1e-13))
+    sample = np.sort(np.random.default_rng(0).choice(S.G, size=48, replace=False))
+    outs = []
+    with pk.Tracker(0) as t:
+        t.set_option(pk.OPT_GRAPH, 1)
+        t.set_image(0, S.frames[0], S.bits)
+        t.set_image(1, S.frames[1], S.bits)
+        t.set_image_indices(0, 1)
+        t.set_grains(S.grains, S.pattern_xy, S.pattern_begin)
+        for k in range(4):
+            if k > 0:
+                t.change_reference()
+                t.set_image((k + 1) % 2, S.frames[k + 1], S.bits)
+            t.track_pair(prm)
+            outs.append(t.get_grains())
+            say(f"pair {k}: graph stats {t.graph_stats()}")
+        captures, launches, nodes = t.graph_stats()
+    assert captures == 1 and launches == 2
+    # Integer picks: the NCC of the search equal to the oracle's (within the exact form's ulps).
+    # Subpixel: within 1e-3 px / 1e-5 rad / NCC_subpix 1e-6 of the oracle's FP64 minimizer for
+    # grains the search matched well (NCC >= 0.7). A grain matched poorly (heavily overlapped by
+    # larger grains drawn over it) can have an objective with several minima, and two minimizers
+    # may end in different ones: there the GPU's minimum must be at least as good as the oracle's.
+    gin = S.grains
+    other_minima = 0
+    for k in range(4):
+        W = ptref.Workload(S.frames[k], S.frames[k + 1], gin, S.pattern_xy, S.pattern_begin, S.bits)
+        ref, _, _ = ptref.track_pair(W, oprm, nthreads=8, subset=sample)
+        got = outs[k][sample]
+        assert np.abs(got["NCC"] - ref["NCC"]).max() < 1e-12
+        epx = np.hypot(got["dx"] - ref["dx"], got["dy"] - ref["dy"])
+        erad = np.abs(got["drot"] - ref["drot"])
+        encc = np.abs(got["NCC_subpix"] - ref["NCC_subpix"])
+        close = (epx < 1e-3) & (erad < 1e-5) & (encc < 1e-6)
+        good = ref["NCC"] >= 0.7
+        say(f"pair {k}: {good.sum()} well-matched grains within {epx[good].max():.2e} px, {erad[good].max():.2e} rad; "
+            f"{int((~close).sum())} grains in another minimum")
+        assert close[good].all()
+        assert (got["NCC_subpix"][~close] >= ref["NCC_subpix"][~close] - 1e-6).all()
+        other_minima += int((~close).sum())
+        gin = ptref.change_reference(outs[k])
+    assert other_minima <= 2
