@@ -235,7 +235,7 @@ hipError_t launch_set_frames(Context& c, const FrameDesc& f) {
 // checks its pattern pixels against them (rot_near_half) and a step with a pixel near a rounding
 // boundary goes on the guard list.
 constexpr int kRotThreads = 256;
-constexpr int kRotSteps = 8;
+constexpr int kRotSteps = 32;  // (one block covers the usual step counts: the pattern is read once)
 
 __global__ __launch_bounds__(kRotThreads) void k_rot_tables(RotArgs r) {
   __shared__ double tc[kRotSteps], ts[kRotSteps], tth[kRotSteps];
@@ -260,11 +260,11 @@ __global__ __launch_bounds__(kRotThreads) void k_rot_tables(RotArgs r) {
   __syncthreads();
   const int64_t b = r.pbegin[g];
   const int n = (int)(r.pbegin[g + 1] - b);
-  int near = 0;  // bit j: step k0 + j has a pixel near a boundary
+  unsigned int near = 0u;  // bit j: step k0 + j has a pixel near a boundary
   for (int i = t; i < n; i += kRotThreads) {
     const int dx = r.pattern[2 * (b + i)], dy = r.pattern[2 * (b + i) + 1];
     for (int j = 0; j < nk; j++)
-      if (rot_near_half(dx, dy, tc[j], ts[j])) near |= 1 << j;
+      if (rot_near_half(dx, dy, tc[j], ts[j])) near |= 1u << j;
   }
   if (near)
     for (int j = 0; j < nk; j++)
