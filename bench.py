@@ -10,7 +10,9 @@ pair's input. Prints one JSON line.
 The timed steps run without stage timers; the stage split comes from as many steps again with
 HIP events around each stage. Then the same pairs run with every kernel of the exact search on
 one stream, so that the search kernel's launch durations (HIP events) do not overlap: the
-roofline is computed from those. Then the pair captured as one HIP graph and replayed, the plain
+roofline is computed from those. Then the pair captured as one HIP graph and replayed; frames
+from host memory (a 6000x4000 frame copied per pair from pageable memory before the pair, or from
+pinned memory while the pair runs: the PCIe-inclusive rate); the plain
 form, the exact-vs-plain pick comparison on configs 2 and
 3, the rescue-zone variant of the pair (300 grains moved 21-35 px, beyond the main zone: the
 rescue stage at its own +-40 px geometry) and the oracle's CPU baseline on a sample of grains
@@ -361,6 +363,63 @@ def graph_section(pk, tr, w, prm, step, ref, args):
             "records_equal_launches": same}
 
 
+def ingest_section(pk, tr, w, prm, step, ref, args):
+    """Frames from host memory: one 6000x4000 uint16 frame copied into a slot the pair does
+    not use, (a) from pageable memory with the synchronous ptg_set_image, (b) from pinned
+    memory with ptg_set_image_async (the copy alone, waited for), then the pair with a new
+    frame per step, (c) copied synchronously before the pair and (d) copied while the pair
+    runs (ptg_set_image_async queued before ptg_track_pair): the PCIe-inclusive rate of a
+    sequence whose frames come from the host."""
+    n = args.ingest_steps
+    H, W = w.I1.shape
+    nbytes = 2.0 * H * W
+    page = np.ascontiguousarray(w.I1)
+    pinned = [tr.host_frame(H, W), tr.host_frame(H, W)]
+    for p in pinned:
+        p[...] = w.I1
+    tr.set_image(2, page, w.bits)
+    tr.set_image(3, page, w.bits)
+    tr.sync()
+
+    def timed(body):
+        t0 = time.perf_counter()
+        for i in range(n):
+            body(i)
+        tr.sync()
+        return (time.perf_counter() - t0) / n
+
+    def sync_copy(i):
+        tr.set_image(2 + (i & 1), page, w.bits)
+
+    def async_copy(i):
+        tr.set_image_async(2 + (i & 1), pinned[i & 1], w.bits)
+        tr.wait_image(2 + (i & 1))
+
+    def pair_sync_copy(i):
+        tr.set_image(2 + (i & 1), page, w.bits)
+        step(prm)
+
+    got = []
+
+    def pair_overlapped_copy(i):
+        tr.set_image_async(2 + (i & 1), pinned[i & 1], w.bits)
+        got.append(step(prm))
+
+    t_page = timed(sync_copy)
+    t_pin = timed(async_copy)
+    t_ps = timed(pair_sync_copy)
+    t_po = timed(pair_overlapped_copy)
+    same = all(g.tobytes() == ref.tobytes() for g in got)
+    log(f"frame copy: pageable {1e3 * t_page:.2f} ms ({nbytes / t_page / 1e9:.1f} GB/s), pinned "
+        f"{1e3 * t_pin:.2f} ms ({nbytes / t_pin / 1e9:.1f} GB/s); pair + copy {1e3 * t_ps:.2f} ms, pair with the "
+        f"copy overlapped {1e3 * t_po:.2f} ms; records equal: {same}")
+    return {"frame_bytes": int(nbytes), "steps": n,
+            "pageable_sync_copy_ms": 1e3 * t_page, "pageable_GBps": nbytes / t_page / 1e9,
+            "pinned_async_copy_ms": 1e3 * t_pin, "pinned_GBps": nbytes / t_pin / 1e9,
+            "pair_after_sync_copy_ms": 1e3 * t_ps, "pair_with_overlapped_copy_ms": 1e3 * t_po,
+            "grains_per_s_frames_from_host": w.G / t_po, "records_equal": same}
+
+
 def autotune_section(pk, tr, w, prm, step, args):
     """The exact search's autotuner on this pair: per-pair wall times (median, mean, largest)
     with the built-in launch shapes and with the tuned ones, in the same process, and the
@@ -475,6 +534,9 @@ def main():
                     help="skip the exact-vs-plain pick comparison (configs 2 and 3) after the timed run")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="pairs timed with the pair captured as a HIP graph and replayed (0: skip)")
+    ap.add_argument("--ingest-steps", type=int, default=10,
+                    help="pairs timed with a frame copied from host memory per pair, synchronously and "
+                         "overlapped with the pair (0: skip)")
     ap.add_argument("--sequence", action="store_true", help="the batching form over a frame sequence")
     ap.add_argument("--seq-frames", type=int, default=16)
     ap.add_argument("--seq-grains", type=int, default=10000)
@@ -650,6 +712,8 @@ def main():
         result["autotune"] = autotune_section(pk, tr, w, prm, step, args)
     if args.graph_steps > 0:
         result["graph"] = graph_section(pk, tr, w, prm, step, out, args)
+    if args.ingest_steps > 0:
+        result["frames_from_host"] = ingest_section(pk, tr, w, prm, step, out, args)
 
     # the plain form (FP64 two-pass search + Powell) on the same workload
     if args.plain_steps > 0 and not (method == "plain" and subpix == "powell"):

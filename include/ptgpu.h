@@ -25,6 +25,7 @@
 #ifndef PTGPU_H
 #define PTGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -138,6 +139,26 @@ int ptg_set_image(void* ctx, int index, const uint16_t* host, int32_t dimx, int3
 /* Attaches a caller-owned device frame (no copy); it must outlive its use. */
 int ptg_set_image_device(void* ctx, int index, const uint16_t* dev, int32_t dimx, int32_t dimy,
                          int64_t pitch, int32_t bits);
+/* read_image()'s store into image[index] (src/p-tracker.cpp:75-201), which the reference
+ * does between pairs (:426, :446), as a copy that overlaps the context's work: the copy runs
+ * on a copy stream of the context, after every call made before this one (which may still
+ * read the slot's old contents), and the first later call that reads or replaces
+ * image[index] waits for it on the GPU. Returns once the copy is queued: from pinned memory
+ * (ptg_host_alloc) the copy reads `host` while later calls run, so `host` stays unchanged
+ * until ptg_wait_image(ctx, index) or ptg_sync returns; pageable memory is read before
+ * the call returns (the runtime stages it). The usual sequence: image k+1 into a slot the
+ * running pair does not use, then ptg_track_pair on images k-1 and k. */
+int ptg_set_image_async(void* ctx, int index, const uint16_t* host, int32_t dimx, int32_t dimy,
+                        int64_t pitch, int32_t bits);
+/* Waits on the host until image[index]'s last ptg_set_image_async copy has read its host
+ * buffer (no-op without one). */
+int ptg_wait_image(void* ctx, int index);
+/* Pinned (page-locked) host memory for frames given to ptg_set_image_async, and its
+ * release; memory from ptg_host_alloc may be freed only after the copies that read it
+ * (ptg_wait_image, ptg_sync). No counterpart in the reference (its frames live in host
+ * vectors). */
+int ptg_host_alloc(void* ctx, size_t bytes, void** out);
+int ptg_host_free(void* ctx, void* p);
 /* Copies image[index] to a host buffer of row pitch `pitch` elements (synchronises): the
  * counterpart of the reference's direct access to image[i][x][y] (src/p-tracker.hpp:80),
  * e.g. for the frame ptg_remap wrote. */
@@ -177,6 +198,7 @@ int ptg_change_reference(void* ctx);
 int ptg_track_pair(void* ctx, const ptg_track_params* params);
 /* num_to_be_rescued / num_to_be_super_rescued of the last pair (synchronises). */
 int ptg_get_queue_counts(void* ctx, int32_t* num_to_be_rescued, int32_t* num_to_be_super_rescued);
+/* Waits until the context's work and frame copies are done. */
 int ptg_sync(void* ctx);
 
 /* ---- the subpixel API points ----------------------------------------------------- */

@@ -130,6 +130,10 @@ def load_library(path: str = LIB_PATH):
         "ptg_set_image": ([vp, c_int, vp, c_int32, c_int32, c_int64, c_int32], c_int),
         "ptg_set_image_device": ([vp, c_int, vp, c_int32, c_int32, c_int64, c_int32], c_int),
         "ptg_set_image_indices": ([vp, c_int, c_int], c_int),
+        "ptg_set_image_async": ([vp, c_int, vp, c_int32, c_int32, c_int64, c_int32], c_int),
+        "ptg_wait_image": ([vp, c_int], c_int),
+        "ptg_host_alloc": ([vp, ctypes.c_size_t, POINTER(c_void_p)], c_int),
+        "ptg_host_free": ([vp, vp], c_int),
         "ptg_get_image": ([vp, c_int, vp, c_int64], c_int),
         "ptg_set_grains": ([vp, c_int32, vp, vp, vp], c_int),
         "ptg_put_grains": ([vp, vp], c_int),
@@ -180,7 +184,8 @@ def load_library(path: str = LIB_PATH):
 def exported_symbols():
     """The ptg_* names the binding expects; tests compare them with include/ptgpu.h."""
     return ["ptg_abi_version", "ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_image",
-            "ptg_set_image_device", "ptg_set_image_indices", "ptg_get_image", "ptg_set_grains", "ptg_put_grains",
+            "ptg_set_image_device", "ptg_set_image_indices", "ptg_set_image_async", "ptg_wait_image",
+            "ptg_host_alloc", "ptg_host_free", "ptg_get_image", "ptg_set_grains", "ptg_put_grains",
             "ptg_get_grains", "ptg_begin_pair", "ptg_precompute", "ptg_search", "ptg_subpixel",
             "ptg_end_pair", "ptg_change_reference", "ptg_track_pair", "ptg_get_queue_counts",
             "ptg_sync", "ptg_ncc_objective", "ptg_ncc_objective_batch", "ptg_ncc_objective_batch_noalloc",
@@ -210,9 +215,16 @@ class Tracker:
             raise PtgError(f"ptg_create(device={device}) failed with {rc}: {msg.decode() if msg else ''}")
         self.h = h
         self.G = 0
+        self._pinned = []       # (address, array) of ptg_host_alloc buffers
+        self._in_copy = {}      # image slot -> the host array its overlapped copy reads
 
     def close(self):
         if self.h:
+            self.lib.ptg_sync(self.h)
+            self._in_copy.clear()
+            for addr, _ in self._pinned:
+                self.lib.ptg_host_free(self.h, c_void_p(addr))
+            self._pinned = []
             self.lib.ptg_destroy(self.h)
             self.h = None
 
@@ -241,6 +253,29 @@ class Tracker:
     def set_image_device(self, index: int, dev_ptr: int, dimx: int, dimy: int, pitch: int, bits: int = 16):
         self._ck(self.lib.ptg_set_image_device(self.h, index, c_void_p(dev_ptr), dimx, dimy, pitch, bits),
                  "ptg_set_image_device")
+
+    def host_frame(self, height: int, width: int) -> np.ndarray:
+        """A (height, width) uint16 array in pinned host memory (ptg_host_alloc), for
+        set_image_async; released by close()."""
+        addr = c_void_p()
+        n = int(height) * int(width)
+        self._ck(self.lib.ptg_host_alloc(self.h, 2 * n, ctypes.byref(addr)), "ptg_host_alloc")
+        arr = np.ctypeslib.as_array((ctypes.c_uint16 * n).from_address(addr.value)).reshape(height, width)
+        self._pinned.append((addr.value, arr))
+        return arr
+
+    def set_image_async(self, index: int, frame: np.ndarray, bits: int = 16):
+        """ptg_set_image_async: `frame` (C-contiguous uint16) must stay unchanged until
+        wait_image(index) or sync(); the array is kept alive until then."""
+        if frame.dtype != np.uint16 or not frame.flags["C_CONTIGUOUS"]:
+            raise PtgError("set_image_async needs a C-contiguous uint16 array")
+        self._ck(self.lib.ptg_set_image_async(self.h, index, _ptr(frame), frame.shape[1], frame.shape[0],
+                                              frame.shape[1], bits), "ptg_set_image_async")
+        self._in_copy[index] = frame
+
+    def wait_image(self, index: int):
+        self._ck(self.lib.ptg_wait_image(self.h, index), "ptg_wait_image")
+        self._in_copy.pop(index, None)
 
     def set_image_indices(self, iref: int, icur: int):
         self._ck(self.lib.ptg_set_image_indices(self.h, iref, icur), "ptg_set_image_indices")
@@ -290,6 +325,7 @@ class Tracker:
 
     def sync(self):
         self._ck(self.lib.ptg_sync(self.h), "ptg_sync")
+        self._in_copy.clear()
 
     def objective(self, igrain, X, dmax=3.0, stiffness=0.0):
         ig = np.ascontiguousarray(np.atleast_1d(igrain), dtype=np.int32)

@@ -240,6 +240,46 @@ struct Ctx : Context {
       return fail(PTG_ERR_HIP, "hipEventCreate failed");
     return check(hipEventRecord(staging_done, stream), "hipEventRecord");
   }
+  // Frame copies that overlap the context's work (ptg_set_image_async): a copy stream (made
+  // at the first such copy), and per image slot the event recorded after its last copy.
+  // up_pending: the library stream has not yet been told to wait for that copy.
+  hipStream_t up_stream = nullptr;
+  hipEvent_t up_after = nullptr;  // the library stream's work so far, waited on by a copy
+  hipEvent_t up_done[kMaxImages] = {};
+  bool up_pending[kMaxImages] = {};
+  int upload_setup() {
+    if (up_stream) return PTG_OK;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return check(e, "hipStreamCreateWithFlags (frame copies)");
+    hipEvent_t ev[kMaxImages + 1] = {};
+    for (int i = 0; i <= kMaxImages && e == hipSuccess; i++) e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+      (void)hipStreamDestroy(s);
+      return check(e, "hipEventCreateWithFlags (frame copies)");
+    }
+    up_after = ev[kMaxImages];
+    for (int i = 0; i < kMaxImages; i++) up_done[i] = ev[i];
+    up_stream = s;
+    return PTG_OK;
+  }
+  // Work queued from now on waits on the GPU for image `index`'s last copy.
+  int order_after_upload(int index) {
+    if (index < 0 || index >= kMaxImages || !up_pending[index]) return PTG_OK;
+    up_pending[index] = false;
+    return check(hipStreamWaitEvent(stream, up_done[index], 0), "hipStreamWaitEvent (frame copy)");
+  }
+  // The host waits until image `index`'s last copy is done.
+  int finish_upload(int index) {
+    if (index < 0 || index >= kMaxImages || !up_done[index]) return PTG_OK;
+    return check(hipEventSynchronize(up_done[index]), "hipEventSynchronize (frame copy)");
+  }
+  int finish_uploads() {
+    if (!up_stream) return PTG_OK;
+    return check(hipStreamSynchronize(up_stream), "hipStreamSynchronize (frame copies)");
+  }
 };
 
 Ctx* as_ctx(void* p) { return reinterpret_cast<Ctx*>(p); }
@@ -300,6 +340,9 @@ int need_images(Ctx* c, bool ref, bool cur) {
     if (a.dimx != b.dimx || a.dimy != b.dimy)
       return c->fail(PTG_ERR_ARG, "image sizes differ: %dx%d vs %dx%d", a.dimx, a.dimy, b.dimx, b.dimy);
   }
+  // the work the caller queues next reads these frames: after their copies
+  if (ref) TRY(c->order_after_upload(c->iref));
+  if (cur) TRY(c->order_after_upload(c->icur));
   return PTG_OK;
 }
 
@@ -722,9 +765,15 @@ int ptg_destroy(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_OK;
   (void)hipSetDevice(c->device);
+  (void)c->finish_uploads();
   (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < kMaxImages; i++)
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
+  if (c->up_stream) {
+    for (hipEvent_t ev : c->up_done) (void)hipEventDestroy(ev);
+    (void)hipEventDestroy(c->up_after);
+    (void)hipStreamDestroy(c->up_stream);
+  }
   free_grain_buffers(c);
   void* more[] = {c->d_counts,   c->d_cs,      c->d_steps_st[0], c->d_steps_st[1], c->d_steps_st[2], c->d_best,       c->d_evals,  c->d_evals_g,
                   c->d_exact_scratch[0], c->d_exact_scratch[1], c->d_exact_scratch[2], c->d_tmp, c->d_joint, c->d_tune_save, c->d_eq_pairs, c->d_eq_ref, c->d_eq_disp, c->d_eq_ncc,
@@ -769,13 +818,19 @@ static int set_image_common(Ctx* c, int index, int32_t dimx, int32_t dimy, int64
   return PTG_OK;
 }
 
-int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_t dimy, int64_t pitch,
-                  int32_t bits) {
-  Ctx* c = as_ctx(p);
-  if (!c) return PTG_ERR_ARG;
-  CK(c, hipSetDevice(c->device));
-  if (!host) return c->fail(PTG_ERR_ARG, "null host frame");
-  TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
+// A frame copy between host and device rows: one linear copy when both sides' rows are
+// contiguous (the runtime's pitched copy of pageable memory moves a row at a time).
+static hipError_t copy_frame(void* dst, int64_t dpitch, const void* src, int64_t spitch, int32_t dimx, int32_t dimy,
+                             hipMemcpyKind kind, hipStream_t st) {
+  if (dpitch == dimx && spitch == dimx)
+    return hipMemcpyAsync(dst, src, sizeof(uint16_t) * (size_t)dimx * dimy, kind, st);
+  return hipMemcpy2DAsync(dst, sizeof(uint16_t) * dpitch, src, sizeof(uint16_t) * spitch, sizeof(uint16_t) * dimx,
+                          dimy, kind, st);
+}
+
+// image[index] as a context-owned buffer of at least dimx x dimy pixels (pitch dimx), the
+// old one released once no copy or queued work uses it.
+static int image_buffer(Ctx* c, int index, int32_t dimx, int32_t dimy, int32_t bits) {
   Image& im = c->img[index];
   if (!im.owned) {
     im.d = nullptr;
@@ -784,6 +839,8 @@ int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_
   const size_t bytes = sizeof(uint16_t) * (size_t)dimx * dimy;
   if (!im.owned || im.cap < bytes) {
     if (im.owned && im.d) {
+      TRY(c->finish_upload(index));
+      c->up_pending[index] = false;
       CK(c, hipStreamSynchronize(c->stream));
       (void)hipFree(im.d);
     }
@@ -799,9 +856,68 @@ int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_
   im.dimy = dimy;
   im.pitch = dimx;
   im.bits = bits;
-  CK(c, hipMemcpy2DAsync(im.d, sizeof(uint16_t) * dimx, host, sizeof(uint16_t) * pitch, sizeof(uint16_t) * dimx,
-                         dimy, hipMemcpyHostToDevice, c->stream));
+  return PTG_OK;
+}
+
+int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_t dimy, int64_t pitch,
+                  int32_t bits) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  CK(c, hipSetDevice(c->device));
+  if (!host) return c->fail(PTG_ERR_ARG, "null host frame");
+  TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
+  TRY(image_buffer(c, index, dimx, dimy, bits));
+  TRY(c->order_after_upload(index));  // (after an earlier overlapped copy into the slot)
+  Image& im = c->img[index];
+  CK(c, copy_frame(im.d, dimx, host, pitch, dimx, dimy, hipMemcpyHostToDevice, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  return PTG_OK;
+}
+
+int ptg_set_image_async(void* p, int index, const uint16_t* host, int32_t dimx, int32_t dimy, int64_t pitch,
+                        int32_t bits) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  CK(c, hipSetDevice(c->device));
+  if (!host) return c->fail(PTG_ERR_ARG, "null host frame");
+  TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
+  TRY(c->upload_setup());
+  TRY(image_buffer(c, index, dimx, dimy, bits));
+  Image& im = c->img[index];
+  // after every call queued so far (they may read the slot's old contents)
+  CK(c, hipEventRecord(c->up_after, c->stream));
+  CK(c, hipStreamWaitEvent(c->up_stream, c->up_after, 0));
+  CK(c, copy_frame(im.d, dimx, host, pitch, dimx, dimy, hipMemcpyHostToDevice, c->up_stream));
+  CK(c, hipEventRecord(c->up_done[index], c->up_stream));
+  c->up_pending[index] = true;
+  return PTG_OK;
+}
+
+int ptg_wait_image(void* p, int index) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  TRY(valid_index(c, index));
+  return c->finish_upload(index);
+}
+
+int ptg_host_alloc(void* p, size_t bytes, void** out) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!out) return c->fail(PTG_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  CK(c, hipSetDevice(c->device));
+  if (hipHostMalloc(out, bytes < 64 ? 64 : bytes, hipHostMallocDefault) != hipSuccess) {
+    *out = nullptr;
+    return c->fail(PTG_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+  }
+  return PTG_OK;
+}
+
+int ptg_host_free(void* p, void* mem) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PTG_ERR_ARG;
+  if (!mem) return PTG_OK;
+  CK(c, hipHostFree(mem));
   return PTG_OK;
 }
 
@@ -813,9 +929,11 @@ int ptg_set_image_device(void* p, int index, const uint16_t* dev, int32_t dimx, 
   TRY(set_image_common(c, index, dimx, dimy, pitch, bits));
   Image& im = c->img[index];
   if (im.owned && im.d) {
+    TRY(c->finish_upload(index));
     CK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(im.d);
   }
+  c->up_pending[index] = false;
   im.d = const_cast<uint16_t*>(dev);
   im.owned = false;
   im.cap = 0;
@@ -834,8 +952,8 @@ int ptg_get_image(void* p, int index, uint16_t* host, int64_t pitch) {
   if (!im.d) return c->fail(PTG_ERR_STATE, "image[%d] not set", index);
   if (!host) return c->fail(PTG_ERR_ARG, "null host buffer");
   if (pitch < im.dimx) return c->fail(PTG_ERR_ARG, "pitch %lld < dimx %d", (long long)pitch, im.dimx);
-  CK(c, hipMemcpy2DAsync(host, sizeof(uint16_t) * pitch, im.d, sizeof(uint16_t) * im.pitch, sizeof(uint16_t) * im.dimx,
-                         im.dimy, hipMemcpyDeviceToHost, c->stream));
+  TRY(c->order_after_upload(index));
+  CK(c, copy_frame(host, pitch, im.d, im.pitch, im.dimx, im.dimy, hipMemcpyDeviceToHost, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   return PTG_OK;
 }
@@ -1411,6 +1529,7 @@ int ptg_get_queue_counts(void* p, int32_t* nres, int32_t* nsup) {
 int ptg_sync(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
+  TRY(c->finish_uploads());
   return c->sync_and_check();
 }
 
@@ -1991,8 +2110,11 @@ int ptg_remap(void* p, int src, int dst, const double* params14) {
   const Image& s = c->img[src];
   if (!s.d) return c->fail(PTG_ERR_STATE, "image[%d] not set", src);
   Image& d = c->img[dst];
+  TRY(c->order_after_upload(src));
+  TRY(c->order_after_upload(dst));
   const size_t bytes = sizeof(uint16_t) * (size_t)s.dimx * s.dimy;
   if (!d.owned || d.cap < bytes) {
+    TRY(c->finish_upload(dst));
     CK(c, hipStreamSynchronize(c->stream));
     if (d.owned && d.d) (void)hipFree(d.d);
     d.d = nullptr;
