@@ -10,13 +10,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <fstream>
+#include <future>
 #include <iomanip>
 #include <iostream>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -28,6 +33,110 @@
 namespace ptgpu {
 
 static_assert(sizeof(search_zone_type) == sizeof(ptg_search_zone), "search zone layouts differ");
+
+// Frames of a sequence decoded ahead of the tracking loop: up to `ahead` TIFF files decode at
+// once on their own threads, each into a pinned buffer of a pool (ptg_host_alloc) from which
+// ptg_set_image_async copies it; the loop hands a buffer back once its copy is done. The
+// reader threads make no library call (the context belongs to the loop's thread).
+class frame_reader {
+ public:
+  struct frame {
+    const uint16_t* px = nullptr;
+    int width = 0, height = 0;
+    std::string dateTime;
+    int buf = -1;
+  };
+
+  frame_reader(void* ctx, std::vector<std::string> names, int width, int height, int ahead)
+      : ctx_(ctx), names_(std::move(names)), width_(width), height_(height) {
+    ahead = std::max(1, ahead);
+    const size_t bytes = sizeof(uint16_t) * (size_t)width * height;
+    // every decode in flight holds a buffer, and so do the frames the loop has taken whose
+    // copies may still run (the next frame's and the current one's)
+    for (int b = 0; b < ahead + 2; b++) {
+      void* p = nullptr;
+      if (ptg_host_alloc(ctx_, bytes, &p) != PTG_OK) {
+        free_buffers();
+        throw std::runtime_error(std::string("ptg_host_alloc: ") + ptg_last_error(ctx_));
+      }
+      bufs_.push_back(static_cast<uint16_t*>(p));
+      free_.push_back(b);
+    }
+    for (int k = 0; k < ahead; k++) launch();
+  }
+  ~frame_reader() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      closing_ = true;
+    }
+    cv_.notify_all();
+    for (auto& f : pending_)
+      if (f.valid()) f.wait();
+    free_buffers();
+  }
+  frame_reader(const frame_reader&) = delete;
+  frame_reader& operator=(const frame_reader&) = delete;
+
+  // The next frame in order (waits for its decode); throws what the decode threw.
+  frame next() {
+    if (pending_.empty()) throw std::runtime_error("no frame left to read");
+    frame f = pending_.front().get();
+    pending_.pop_front();
+    launch();
+    return f;
+  }
+  // Hands buffer b back (its device copy is done).
+  void release(int b) {
+    if (b < 0) return;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      free_.push_back(b);
+    }
+    cv_.notify_one();
+  }
+
+ private:
+  void launch() {
+    if (next_ >= names_.size()) return;
+    const std::string name = names_[next_++];
+    pending_.push_back(std::async(std::launch::async, [this, name]() {
+      tiff_frame t = read_tiff(name);
+      if (t.width != width_ || t.height != height_)
+        throw std::runtime_error("image " + name + " differs in size from the reference");
+      int b = -1;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [this]() { return closing_ || !free_.empty(); });
+        if (closing_) throw std::runtime_error("frame reader closed");
+        b = free_.back();
+        free_.pop_back();
+      }
+      std::memcpy(bufs_[b], t.gray.data(), sizeof(uint16_t) * t.gray.size());
+      frame f;
+      f.px = bufs_[b];
+      f.width = t.width;
+      f.height = t.height;
+      f.dateTime = t.dateTime;
+      f.buf = b;
+      return f;
+    }));
+  }
+  void free_buffers() {
+    for (uint16_t* p : bufs_) (void)ptg_host_free(ctx_, p);
+    bufs_.clear();
+  }
+
+  void* ctx_;
+  std::vector<std::string> names_;
+  int width_, height_;
+  size_t next_ = 0;
+  std::vector<uint16_t*> bufs_;
+  std::deque<std::future<frame>> pending_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<int> free_;
+  bool closing_ = false;
+};
 
 class ptracker_gpu {
  public:
@@ -44,6 +153,9 @@ class ptracker_gpu {
   int im_index_ref{0}, im_index_current{1};
   int search_method{PTG_SEARCH_EXACT};
   int subpix_method{PTG_SUBPIX_POWELL};
+  // particle_tracking: TIFF decodes running ahead of the pairs, or the reference's serial order
+  int frames_ahead{3};
+  bool serial_frames{false};
 
   explicit ptracker_gpu(int device = 0) {
     if (ptg_create(&ctx_, device) != PTG_OK) throw std::runtime_error("ptg_create failed");
@@ -139,17 +251,100 @@ class ptracker_gpu {
     overflow_stiffness = s.overflow_stiffness;
   }
 
+  // grain[] without the pattern lists (what save_grains writes)
+  std::vector<grain_type_2D> scalar_copy() {
+    std::vector<std::vector<relative_coord_type>> pats(grain.size());
+    for (size_t g = 0; g < grain.size(); g++) pats[g].swap(grain[g].pattern);
+    std::vector<grain_type_2D> out = grain;
+    for (size_t g = 0; g < grain.size(); g++) pats[g].swap(grain[g].pattern);
+    return out;
+  }
+
   // particle_tracking, src/p-tracker.cpp:423-611: images ibeg .. iend (step iinc) against
   // the reference image iref, frames read from the TIFF files named by image_name, one
   // dic_out_<num>.txt per image (in out_dir), and the reference swapped once
   // num_image - iref >= iraz. Frames hold the reference's 16-bit gray levels. Returns the
   // number of images tracked.
+  //
+  // The frames are read ahead (frames_ahead TIFF decodes at once, on their own threads), each
+  // next frame is copied into the image slot the running pair does not use while that pair
+  // runs (ptg_set_image_async from pinned memory), and each dic_out file is written while the
+  // next pair runs; serial_frames runs the reference's order instead (read, copy, track, write),
+  // with the same files.
   int particle_tracking(const tracker_settings& s, const std::string& out_dir = "") {
     if (s.RawImages) throw std::runtime_error("RAW images are not read (libraw is not part of this build)");
     if (s.image_interpolator != "cubic")
       throw std::runtime_error("only the cubic image interpolator is implemented (got " + s.image_interpolator + ")");
     if (s.iinc <= 0 && s.iend >= s.ibeg) throw std::runtime_error("iinc must be positive");
     apply_settings(s);
+    if (serial_frames) return particle_tracking_serial(s, out_dir);
+    int iref = s.iref;
+    const tiff_frame f0 = read_tiff(numbered_name(s.image_name, iref));
+    std::vector<int> nums;
+    std::vector<std::string> names;
+    for (int num = s.ibeg; num <= s.iend; num += s.iinc) {
+      nums.push_back(num);
+      names.push_back(numbered_name(s.image_name, num));
+    }
+    // three image slots: the reference frame, the current one and the next one
+    int slot_ref = 0, slot_cur = 1, slot_next = 2;
+    std::string datetime[3];
+    set_image(slot_ref, f0.gray.data(), f0.width, f0.height, 16);
+    datetime[slot_ref] = f0.dateTime;
+    upload_grains();
+    if (nums.empty()) return 0;
+    frame_reader reader(ctx_, names, f0.width, f0.height, frames_ahead);
+    auto copy_in = [&](int slot, const frame_reader::frame& f) {
+      check(ptg_set_image_async(ctx_, slot, f.px, f.width, f.height, f.width, 16), "ptg_set_image_async");
+      datetime[slot] = f.dateTime;
+    };
+    frame_reader::frame cur = reader.next();
+    copy_in(slot_cur, cur);
+    std::future<void> writing;
+    int n = 0;
+    for (size_t i = 0; i < nums.size(); i++) {
+      const int num = nums[i];
+      // the next frame goes in while this pair runs
+      frame_reader::frame next;
+      if (i + 1 < nums.size()) {
+        next = reader.next();
+        copy_in(slot_next, next);
+      }
+      im_index_ref = slot_ref;
+      im_index_current = slot_cur;
+      track_pair();
+      // the current frame's copy is done (the pair waited for it): its buffer goes back
+      check(ptg_wait_image(ctx_, slot_cur), "ptg_wait_image");
+      reader.release(cur.buf);
+      // dic_out of this pair, written while the next pair runs (from the scalar columns)
+      if (writing.valid()) writing.get();
+      writing = std::async(std::launch::async,
+                           [snap = scalar_copy(), path = out_dir + numbered_name("dic_out_%d.txt", num),
+                            from = numbered_name(s.image_name, iref), from_dt = datetime[slot_ref],
+                            to = names[i], to_dt = datetime[slot_cur]]() {
+                             save_grains(path, snap, false, from, from_dt, to, to_dt);
+                           });
+      n++;
+      const int old_ref = slot_ref, old_cur = slot_cur;
+      if (num - iref >= s.iraz) {
+        change_reference();  // the current frame becomes the reference
+        iref = num;
+        slot_ref = old_cur;
+        slot_cur = slot_next;
+        slot_next = old_ref;
+      } else {
+        slot_cur = slot_next;
+        slot_next = old_cur;
+      }
+      cur = next;
+    }
+    if (writing.valid()) writing.get();
+    return n;
+  }
+
+  // The reference's order: each frame read and copied before its pair, each dic_out file
+  // written after it.
+  int particle_tracking_serial(const tracker_settings& s, const std::string& out_dir) {
     int iref = s.iref;
     std::string datetime[kFrames];
     const tiff_frame f0 = read_tiff(numbered_name(s.image_name, iref));

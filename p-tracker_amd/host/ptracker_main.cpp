@@ -5,6 +5,8 @@
 // Options: --lm     subpixel by the LM form instead of Powell
 //          --plain  integer search by the plain FP64 form instead of the exact MFMA form
 //          --device N
+//          --frames-ahead N  TIFF frames decoded at once ahead of the pairs (default 3)
+//          --serial-frames   read, copy, track and write each image in turn, as the reference
 // Procedures: particle_tracking (run once; the reference's run() runs it a second time
 // for that procedure name, src/p-tracker.cpp:1802-1808) and correction_distortion (the
 // equiprojectivity fit on the GPU by Levenberg-Marquardt in place of Powell / lmdif).
@@ -21,8 +23,12 @@ int main(int argc, char** argv) {
   std::string cmd = "commands.txt";
   int device = 0;
   int search = PTG_SEARCH_EXACT, subpix = PTG_SUBPIX_POWELL;
+  int ahead = 3;
+  bool serial = false;
   for (int i = 1; i < argc; i++) {
     if (!std::strcmp(argv[i], "--lm")) subpix = PTG_SUBPIX_LM;
+    else if (!std::strcmp(argv[i], "--serial-frames")) serial = true;
+    else if (!std::strcmp(argv[i], "--frames-ahead") && i + 1 < argc) ahead = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--plain")) search = PTG_SEARCH_PLAIN;
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) device = std::atoi(argv[++i]);
     else cmd = argv[i];
@@ -37,6 +43,8 @@ int main(int argc, char** argv) {
     pt.grain = grains;
     pt.search_method = search;
     pt.subpix_method = subpix;
+    pt.frames_ahead = ahead;
+    pt.serial_frames = serial;
     if (s.procedure == "correction_distortion") {
       const std::vector<double> X = pt.correction_distortion(s);
       std::cout << std::scientific << std::setprecision(15) << "K1_corrDistor " << X[2] << "\nK2_corrDistor " << X[3]

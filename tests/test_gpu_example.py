@@ -97,3 +97,61 @@ def test_example_program_matches_oracle(tmp_path, subpix):
                 assert np.array_equal(got, ref[c].astype(np.float64)), (num, c)
             else:
                 assert np.abs(got - ref[c]).max() < tol[c], (num, c, np.abs(got - ref[c]).max())
+
+
+@pytest.mark.parametrize("iraz", [0, 3])
+def test_program_frames_read_ahead_equal_serial(tmp_path, iraz):
+    # a 7-image sequence of RGB TIFF files: the program with its frames decoded ahead, copied
+    # during the previous pair and its files written during the next, against the reference's
+    # serial order; every dic_out file byte for byte (iraz 0: the reference swapped after each
+    # image; 3: after every third)
+    from test_host_io import write_tiff
+
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (make -C p-tracker_amd)")
+    S = synth.sequence(n_frames=7, n_grains=24, size=(520, 440), seed=909, radius=(10.0, 10.0), max_step=2.0,
+                       max_rot=0.02, margin=70)
+    for k, f in enumerate(S.frames):
+        v = (f >> 4).astype(np.uint8)
+        write_tiff(tmp_path / f"f{k + 1:02d}.tiff", np.stack([v, v, v], axis=2), 2, 8)
+    lines = [str(S.G), ""] + [f"{g['refcoord_xpix'] + g['dx']:.3f} {g['refcoord_ypix'] + g['dy']:.3f} 0 10"
+                              for g in S.grains]
+    (tmp_path / "grains.txt").write_text("\n".join(lines) + "\n")
+    cmd = open(os.path.join(EXAMPLE, "commands.txt")).read()
+    for key, val in (("image_name", "f%02d.tiff"), ("grains_to_follow", "grains.txt"), ("iref", "1"),
+                     ("ibeg", "2"), ("iend", "7"), ("iraz", str(iraz)), ("rescue_level", "1"),
+                     ("search_zone.left", "4"), ("search_zone.right", "4"), ("search_zone.up", "4"),
+                     ("search_zone.down", "4"), ("search_zone.num_rot", "2"), ("search_zone.inc_rot", "0.01"),
+                     ("search_zone_rescue.left", "8"), ("search_zone_rescue.right", "8"),
+                     ("search_zone_rescue.up", "8"), ("search_zone_rescue.down", "8"),
+                     ("search_zone_rescue.num_rot", "2"), ("search_zone_rescue.inc_rot", "0.01")):
+        out = []
+        for line in cmd.split("\n"):
+            if line.split() and line.split()[0] == key:
+                line = f"{key} {val}"
+            out.append(line)
+        cmd = "\n".join(out)
+    (tmp_path / "commands.txt").write_text(cmd)
+    runs = {}
+    for mode in ("serial", "ahead"):
+        d = tmp_path / mode
+        d.mkdir()
+        for f in os.listdir(tmp_path):
+            if f.endswith((".tiff", ".txt")):
+                shutil.copyfile(tmp_path / f, d / f)
+        args = [EXE, "commands.txt", "--lm"] + (["--serial-frames"] if mode == "serial" else ["--frames-ahead", "3"])
+        print(f"program on 7 images, iraz {iraz}: {mode}", flush=True)
+        r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "tracked 24 grains over 6 images" in r.stdout, r.stdout
+        runs[mode] = {num: (d / f"dic_out_{num}.txt").read_bytes() for num in range(2, 8)}
+    for num in range(2, 8):
+        assert runs["ahead"][num] == runs["serial"][num], num
+    rows, trailer = read_dic_bytes(runs["ahead"][7])
+    assert len(rows) == 24 and trailer[2] == "#___TO_image: f07.tiff"
+
+
+def read_dic_bytes(b):
+    lines = b.decode().split("\n")
+    n = int(lines[0])
+    return [l.split() for l in lines[1:1 + n]], [l for l in lines[1 + n:] if l.startswith("#")]
