@@ -14,6 +14,15 @@ namespace ptg {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A frame pixel read as a global-memory load. Frame pointers reach the kernels through a device
+// record, so the compiler sees them as generic addresses: their loads become flat loads, which
+// it keeps in order with LDS accesses (a generic address could be LDS) and which also count
+// against the LDS wait counter, so a staging loop waits for each load before its LDS store.
+typedef __attribute__((address_space(1))) const uint16_t glb_u16;
+__device__ __forceinline__ unsigned int frame_px(const uint16_t* base, int64_t idx) {
+  return ((const glb_u16*)base)[idx];
+}
+
 // rotate_pixel_pattern, src/p-tracker.cpp:615-621: x' = round(c*dx + s*dy),
 // y' = round(-s*dx + c*dy), rounding half away from zero as std::round does.
 __device__ __forceinline__ void rotate_px(int dx, int dy, double c, double s, int& xo, int& yo) {

@@ -675,29 +675,52 @@ __global__ __launch_bounds__(512) void k_exact_base_disk(ExactArgs e) {
   // exclusive row prefix sums of I1 and I1^2 (mod 2^32), one wave per row; the window
   // reads clamp to the frame as the MFMA kernels' do
   bool over = false;
-  for (int row = wave; row < nrow; row += nwave) {
-    const int Y = clampi(Y0 + row, 0, fr.dimy - 1);
-    const uint16_t* src = fr.I1 + (int64_t)Y * fr.pitch1;
-    uint2* dst = pre + (size_t)row * pst;
-    unsigned int c1 = 0u, c2 = 0u;
-    if (lane == 0) dst[0] = make_uint2(0u, 0u);
-    for (int x0 = 0; x0 < ncol; x0 += 64) {
-      const int x = x0 + lane;
-      unsigned int v = 0u;
-      if (x < ncol) v = src[clampi(X0 + x, 0, fr.dimx - 1)];
-      over |= v > (unsigned int)min(e.vmax, 4095);
-      unsigned int a = v, b = v * v;
+  // kRC 64-column chunks of a row loaded together, two rows at a time, then scanned
+  constexpr int kRC = 4;
+  for (int row0 = wave; row0 < nrow; row0 += 2 * nwave) {
+    unsigned int va[2][kRC];
 #pragma unroll
-      for (int m = 1; m < 64; m <<= 1) {
-        const unsigned int oa = __shfl_up(a, m, 64), ob = __shfl_up(b, m, 64);
-        if (lane >= m) {
-          a += oa;
-          b += ob;
-        }
+    for (int h = 0; h < 2; h++) {
+      const int row = row0 + h * nwave;
+      const int64_t rb = (int64_t)clampi(Y0 + row, 0, fr.dimy - 1) * fr.pitch1;
+#pragma unroll
+      for (int j = 0; j < kRC; j++) {
+        const int x = 64 * j + lane;
+        va[h][j] = row < nrow && x < ncol ? frame_px(fr.I1, rb + clampi(X0 + x, 0, fr.dimx - 1)) : 0u;
       }
-      if (x < ncol) dst[x + 1] = make_uint2(c1 + a, c2 + b);
-      c1 += __shfl(a, 63, 64);
-      c2 += __shfl(b, 63, 64);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = row0 + h * nwave;
+      if (row >= nrow) break;
+      const int64_t rb = (int64_t)clampi(Y0 + row, 0, fr.dimy - 1) * fr.pitch1;
+      uint2* dst = pre + (size_t)row * pst;
+      unsigned int c1 = 0u, c2 = 0u;
+      if (lane == 0) dst[0] = make_uint2(0u, 0u);
+      for (int x0 = 0; x0 < ncol; x0 += 64) {
+        const int x = x0 + lane, j = x0 >> 6;
+        unsigned int v = 0u;
+        if (j < kRC) {
+#pragma unroll
+          for (int jj = 0; jj < kRC; jj++)
+            if (jj == j) v = va[h][jj];
+        } else if (x < ncol) {
+          v = frame_px(fr.I1, rb + clampi(X0 + x, 0, fr.dimx - 1));  // (rows wider than kRC chunks)
+        }
+        over |= v > (unsigned int)min(e.vmax, 4095);
+        unsigned int a = v, b = v * v;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+          const unsigned int oa = __shfl_up(a, m, 64), ob = __shfl_up(b, m, 64);
+          if (lane >= m) {
+            a += oa;
+            b += ob;
+          }
+        }
+        if (x < ncol) dst[x + 1] = make_uint2(c1 + a, c2 + b);
+        c1 += __shfl(a, 63, 64);
+        c2 += __shfl(b, 63, 64);
+      }
     }
   }
   if (over) big = 1;
@@ -1018,15 +1041,36 @@ __global__ __launch_bounds__(512) void k_search_exact(ExactArgs e) {
     if (threadIdx.x < 17) hflag[threadIdx.x] = 0;
   __syncthreads();
   bool clamped = false, over = false;
-  for (int q = threadIdx.x; q < WW * WH; q += nthreads) {
-    const int y = q / WW, x = q - y * WW;
-    int X = X0 + x, Y = Y0 + y;
-    if (X < 0 || Y < 0 || X >= fr.dimx || Y >= fr.dimy) clamped = true;
-    X = clampi(X, 0, fr.dimx - 1);
-    Y = clampi(Y, 0, fr.dimy - 1);
-    const uint16_t v = fr.I1[(int64_t)Y * fr.pitch1 + X];
-    over |= v > e.vmax;
-    win[q] = v;
+  {
+    // the window is a rectangle: some pixel of it lies outside the frame iff one corner does
+    clamped = X0 < 0 || Y0 < 0 || X0 + WW > fr.dimx || Y0 + WH > fr.dimy;
+    auto px_of = [&](int q) -> unsigned int {
+      const int y = q / WW, x = q - y * WW;
+      const int X = clampi(X0 + x, 0, fr.dimx - 1), Y = clampi(Y0 + y, 0, fr.dimy - 1);
+      return fr.I1[(int64_t)Y * fr.pitch1 + X];
+    };
+    // kB loads in flight per thread before their LDS stores, then the rest one at a time; the
+    // largest value is kept for the depth check
+    constexpr int kB = 4;
+    const int nwin = WW * WH;
+    unsigned int vmx = 0u;
+    int q0 = threadIdx.x;
+    for (; q0 + (kB - 1) * nthreads < nwin; q0 += kB * nthreads) {
+      unsigned int v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++) v[j] = px_of(q0 + j * nthreads);
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        vmx = max(vmx, v[j]);
+        win[q0 + j * nthreads] = (uint16_t)v[j];
+      }
+    }
+    for (; q0 < nwin; q0 += nthreads) {
+      const unsigned int v = px_of(q0);
+      vmx = max(vmx, v);
+      win[q0] = (uint16_t)v;
+    }
+    over = vmx > (unsigned int)e.vmax;
   }
   if (clamped) atomicOr(&e.s.grains[g].status, (int)PTG_STATUS_CLAMPED);
   // a window value beyond the declared depth, or a pattern beyond the exact integers: the plain

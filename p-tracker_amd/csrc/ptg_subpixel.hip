@@ -854,10 +854,24 @@ __device__ __forceinline__ void stage_grain(Obj& o, const ptg_grain& gr, int pra
   o.ty0 = (int)floor(o.yr) - h;
   o.tw = o.th = tw;
   if (lay.tile_elems > 0 && tw * tw <= lay.tile_elems && o.xr == o.xr && o.yr == o.yr) {
-    for (int t = threadIdx.x; t < tw * tw; t += kSpThreads) {
-      const int yy = t / tw, xx = t - yy * tw;
-      const int X = clampi(o.tx0 + xx, 0, o.dimx - 1), Y = clampi(o.ty0 + yy, 0, o.dimy - 1);
-      tile[t] = o.I1[(int64_t)Y * o.pitch + X];
+    // kB frame loads in flight per thread, then their LDS stores
+    constexpr int kB = 8;
+    const int nt = tw * tw;
+    for (int t0 = threadIdx.x; t0 < nt; t0 += kB * kSpThreads) {
+      unsigned int v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        const int t = t0 + j * kSpThreads;
+        v[j] = 0u;
+        if (t < nt) {
+          const int yy = t / tw, xx = t - yy * tw;
+          const int X = clampi(o.tx0 + xx, 0, o.dimx - 1), Y = clampi(o.ty0 + yy, 0, o.dimy - 1);
+          v[j] = frame_px(o.I1, (int64_t)Y * o.pitch + X);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kB; j++)
+        if (t0 + j * kSpThreads < nt) tile[t0 + j * kSpThreads] = (uint16_t)v[j];
     }
     o.tile = true;
   }
