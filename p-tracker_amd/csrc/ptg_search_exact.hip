@@ -294,14 +294,16 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
   const int rho = sl.rho, side = 2 * rho + 1, ncell = side * side;
   const int nsteps = e.s.nsteps;
   int32_t* packed = reinterpret_cast<int32_t*>(smem);
-  uint8_t* cmid = reinterpret_cast<uint8_t*>(packed + ncell);
+  uint8_t* cmid = smem + ((4 * ncell + 15) & ~15);
   int32_t* dflag = reinterpret_cast<int32_t*>(cmid + ((ncell + 15) & ~15));
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) packed[t] = 0;
   if (threadIdx.x == 0) *dflag = 0;
   const bool real = k < nsteps;
-  // the base mask's multiplicities (k_exact_mid or k_exact_base_disk)
+  // the base mask's multiplicities (k_exact_mid or k_exact_base_disk), 16 cells per load
+  // (the grids start on 16-byte boundaries and are padded to 16 bytes)
   if (real)
-    for (int t = threadIdx.x; t < ncell; t += blockDim.x) cmid[t] = e.mmask[sl.mmask_off + t];
+    for (int t = threadIdx.x; t < (ncell + 15) >> 4; t += blockDim.x)
+      reinterpret_cast<uint4*>(cmid)[t] = reinterpret_cast<const uint4*>(e.mmask + sl.mmask_off)[t];
   __syncthreads();
   const int64_t b = e.s.pbegin[g];
   const int n = (int)(e.s.pbegin[g + 1] - b);
@@ -318,8 +320,7 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
       cp = e.s.cs[2 * ((int64_t)g * nsteps + k - 1)];
       sp = e.s.cs[2 * ((int64_t)g * nsteps + k - 1) + 1];
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const uint32_t w = e.s.pwords[b + i];
+    auto scatter = [&](uint32_t w) {
       const int px = (int)(int8_t)(w & 0xffu), py = (int)(int8_t)((w >> 8) & 0xffu);
       int x, y;
       rotate_px(px, py, c, s, x, y);
@@ -329,7 +330,19 @@ __global__ __launch_bounds__(256) void k_exact_template(ExactArgs e) {
         rotate_px(px, py, cp, sp, xp, yp);
         differs = (xp != x || yp != y) ? 1 : 0;
       }
+    };
+    // four pattern words in flight per thread before their scatters, then the rest
+    const int nthr = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 3 * nthr < n; i += 4 * nthr) {
+      const uint32_t w0 = e.s.pwords[b + i], w1 = e.s.pwords[b + i + nthr];
+      const uint32_t w2 = e.s.pwords[b + i + 2 * nthr], w3 = e.s.pwords[b + i + 3 * nthr];
+      scatter(w0);
+      scatter(w1);
+      scatter(w2);
+      scatter(w3);
     }
+    for (; i < n; i += nthr) scatter(e.s.pwords[b + i]);
   }
   // the block-wide OR through a word after the two grids (__syncthreads_or would reserve
   // static LDS)
@@ -2042,7 +2055,7 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     s.mid_off = mid_total;
     mid_total += 2 * (int64_t)S;
     s.mmask_off = mmask_total;
-    mmask_total += (int64_t)(2 * rho + 1) * (2 * rho + 1);
+    mmask_total += ((int64_t)(2 * rho + 1) * (2 * rho + 1) + 15) & ~(int64_t)15;  // (16-byte aligned grids)
     // base-mask GEMM: per mask row, 16-aligned segments covering x in
     // [rho - xm, rho + xm + 15] (the row's cells moved right by up to 15)
     s.mseg_off = (int64_t)P.msegs.size();
