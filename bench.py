@@ -118,6 +118,10 @@ def run_sequence(args, pk, synth, ptref):
     prm = pk.track_params(z, z, z, NCC_min=0.0, rescue_level=0, subpixel=1, search_method=pk.SEARCH_EXACT,
                           subpix=pk.default_subpix(pk.SUBPIX_LM))
     tr = pk.Tracker(0)
+    if args.serial_search:
+        tr.set_option(pk.OPT_SEARCH_STREAMS, 1)
+    # radius buckets launched by decreasing work: this grain set's work grows with the radius
+    tr.set_option(pk.OPT_BUCKET_ORDER, args.seq_bucket_order)
     npairs = len(S.frames) - 1
 
     def attach(slot, k):
@@ -226,7 +230,7 @@ def run_sequence(args, pk, synth, ptref):
                                "px (overlapping), per-frame motion U(-5, 5) px, U(-0.03, 0.03) rad accumulated, "
                                "consecutive pairs with reference swap, search +-8 px x 9 steps, subpixel LM",
                    "grains": int(S.G), "pattern_px": int(P.sum()), "rotation_steps": R, "shifts": Sh,
-                   "pairs_timed": len(lat)},
+                   "pairs_timed": len(lat), "bucket_order": args.seq_bucket_order},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
                        "max": float(lat_ms.max()), "min": float(lat_ms.min()),
                        "per_pair_first_pass": [round(float(x), 2) for x in lat_ms[:npairs]]},
@@ -542,6 +546,8 @@ def main():
     ap.add_argument("--seq-grains", type=int, default=10000)
     ap.add_argument("--seq-repeats", type=int, default=2, help="timed passes over the sequence")
     ap.add_argument("--seq-check-grains", type=int, default=48, help="grains per pair checked against the oracle")
+    ap.add_argument("--seq-bucket-order", type=int, default=1, choices=[0, 1],
+                    help="PTG_OPT_BUCKET_ORDER of the batching form (1: radius buckets by decreasing work)")
     ap.add_argument("--serial-search", action="store_true",
                     help="run every kernel of the exact search on one stream throughout (for profiler traces "
                          "and counters whose per-kernel figures must not overlap)")

@@ -345,6 +345,10 @@ int ptg_set_timing(void* ctx, int enable);
  *  cross terms to a consumer wave through LDS, which adds the window-sum corrections and keeps
  *  the near-tie candidates while its producer multiplies the next pass (at most 4 + 4 waves).
  *  The picks are the same; the autotuner tries both.
+ *  PTG_OPT_BUCKET_ORDER [0]: the order of the exact search's radius-bucket launches. 0: by
+ *  increasing pattern radius (the first launches' preparation is the cheapest); 1: by
+ *  decreasing work (pattern pixels in all: the longest launch starts first and the last ones
+ *  are short), which suits grain sets whose work grows with the radius. The picks are the same.
  */
 enum {
   PTG_OPT_SEARCH_STREAMS = 0,
@@ -352,7 +356,8 @@ enum {
   PTG_OPT_ROT_GUARD_TEST = 2,
   PTG_OPT_DEVICE_QUEUES = 3,
   PTG_OPT_GRAPH = 4,
-  PTG_OPT_SEARCH_ROLES = 5
+  PTG_OPT_SEARCH_ROLES = 5,
+  PTG_OPT_BUCKET_ORDER = 6
 };
 int ptg_set_option(void* ctx, int option, int64_t value);
 /* The rotation guard's counters since the context was created: guard entries the host
@@ -370,7 +375,8 @@ int ptg_get_graph_stats(void* ctx, int64_t* captures, int64_t* launches, int64_t
  * workgroup (1-8) and workgroups per grain (1, 2, 4), timing the whole search's GPU span
  * (HIP events on the context's stream, the buckets overlapped as in any search); a shape
  * replaces the heuristic's when the median of three spans is shorter, and is kept under the
- * bucket's key (value mode, rotation steps, zone, largest pattern radius, grain count).
+ * bucket's key (value mode, rotation steps, zone, largest pattern radius, grain count); the
+ * buckets' launch order (PTG_OPT_BUCKET_ORDER) is set to the faster of the two first.
  * Later exact searches with a bucket of the same key use it; picks and NCC do not depend on
  * the shape. The grains are left as they were. ntried: searches run; ms_before, ms_after:
  * the search's span with the heuristic's shapes and with the chosen ones (medians of three;

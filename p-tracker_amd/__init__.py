@@ -26,6 +26,7 @@ OPT_ROT_GUARD_TEST = 2  # 1: guard every (grain, step); 2: also redo every guard
 OPT_DEVICE_QUEUES = 3  # 1: rescue queues stay on the device (with device tables), 0: read back (default)
 OPT_GRAPH = 4  # 1: track_pair captured once as a HIP graph and replayed
 OPT_SEARCH_ROLES = 5  # exact search waves: 1 each wave both (default), 2 producer/consumer
+OPT_BUCKET_ORDER = 6  # exact search bucket launches: 0 by increasing radius (default), 1 by decreasing work
 PTG_ERR_ARG, PTG_ERR_HIP, PTG_ERR_STATE, PTG_ERR_NOMEM, PTG_ERR_LIMIT = -1, -2, -3, -4, -5
 PTG_STATUS_CLAMPED = 1
 STAGE_MAIN, STAGE_RESCUE, STAGE_SUPER = 0, 1, 2

@@ -1452,6 +1452,15 @@ int ptg_autotune_search(void* p, const ptg_search_zone* z, int32_t* ntried, doub
       float best_ms = 0.0f;
       TRY(median3(&best_ms));  // the heuristic's shapes
       before = best_ms;
+      // the radius buckets' launch order (PTG_OPT_BUCKET_ORDER): the faster of the two
+      {
+        const int o0 = c->bucket_order;
+        c->bucket_order = 1 - o0;
+        float ms_other = 0.0f;
+        TRY(median3(&ms_other));
+        if (ms_other < best_ms) best_ms = ms_other;
+        else c->bucket_order = o0;
+      }
       for (size_t bi = 0; bi < keys.size(); bi++) {
         if (keys[bi] == 0) continue;  // an empty bucket
         c->tune_bucket = -1;
@@ -2159,6 +2168,12 @@ int ptg_set_option(void* p, int option, int64_t value) {
   if (option == PTG_OPT_SEARCH_ROLES) {
     if (value != 1 && value != 2) return c->fail(PTG_ERR_ARG, "PTG_OPT_SEARCH_ROLES takes 1 or 2");
     c->exact_roles = (int)value;
+    c->tune_gen++;  // (a captured pair's launches change)
+    return PTG_OK;
+  }
+  if (option == PTG_OPT_BUCKET_ORDER) {
+    if (value != 0 && value != 1) return c->fail(PTG_ERR_ARG, "PTG_OPT_BUCKET_ORDER takes 0 or 1");
+    c->bucket_order = (int)value;
     c->tune_gen++;  // (a captured pair's launches change)
     return PTG_OK;
   }

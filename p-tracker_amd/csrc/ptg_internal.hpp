@@ -125,6 +125,9 @@ struct Context {
   // PTG_OPT_SEARCH_ROLES: the MFMA kernel's wave roles where no tuned shape says otherwise
   // (1: each wave multiplies and scores its own columns; 2: producer and consumer waves)
   int exact_roles = 1;
+  // PTG_OPT_BUCKET_ORDER: the radius buckets' launches by increasing radius (0) or by
+  // decreasing work (1)
+  int bucket_order = 0;
   std::string err;
 
   Image img[kMaxImages];

@@ -1971,6 +1971,7 @@ struct ExactPlan {
   std::vector<int32_t> segs;
   std::vector<int32_t> msegs;
   std::vector<Bucket> buckets;
+  std::vector<int> by_radius, by_work;  // launch orders: increasing radius, decreasing work
   std::vector<int32_t> bucket_cat;
   // a rescue stage's queue to slots: grain -> slot (-1: not in the list), slot -> bucket,
   // bucket -> first position of its slots in bucket_cat
@@ -2098,6 +2099,24 @@ static void build_plan(const Context& c, ExactPlan& P, const std::vector<int32_t
     b.max_kc = std::max(b.max_kc, P.slots[li].kc);
     b.max_mkc = std::max(b.max_mkc, P.slots[li].mkc);
     if (P.slots[li].base_pat) b.ndisk++;
+  }
+  // within a bucket the largest patterns first: their workgroups start first and the launch
+  // ends on small ones
+  for (Bucket& b : P.buckets)
+    std::stable_sort(b.slots.begin(), b.slots.end(), [&](int32_t x, int32_t y) {
+      const int32_t gx = list[x], gy = list[y];
+      return c.h_pbegin[gx + 1] - c.h_pbegin[gx] > c.h_pbegin[gy + 1] - c.h_pbegin[gy];
+    });
+  // launch orders: by increasing radius, and by decreasing work (pattern pixels in all: the
+  // longest launch starts first and the shorter ones fill the CUs its tail leaves)
+  {
+    std::vector<int64_t> work(P.buckets.size(), 0);
+    for (size_t b = 0; b < P.buckets.size(); b++)
+      for (int32_t li : P.buckets[b].slots) work[b] += c.h_pbegin[list[li] + 1] - c.h_pbegin[list[li]];
+    P.by_radius.resize(P.buckets.size());
+    for (size_t b = 0; b < P.buckets.size(); b++) P.by_radius[b] = (int)b;
+    P.by_work = P.by_radius;
+    std::stable_sort(P.by_work.begin(), P.by_work.end(), [&](int x, int y) { return work[x] > work[y]; });
   }
   P.bucket_cat.clear();
   P.slot_bucket.assign(nlist, 0);
@@ -2401,22 +2420,21 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
       return c.fail(PTG_ERR_HIP, "hipEventCreate");
     if ((he = hipEventRecord(c.tune_ev[0], c.stream)) != hipSuccess) return c.check(he, "hipEventRecord");
   }
+  const std::vector<int>& order = c.bucket_order == 1 ? P->by_work : P->by_radius;
   if (int rca = c.ensure_aux(nb_all + 1); rca != PTG_OK) return rca;
   hipStream_t prep = serial ? c.stream : c.aux_stream;
   hipEvent_t fork = c.aux_events[nb_all];
   if (!serial && ((he = hipEventRecord(fork, c.stream)) != hipSuccess ||
                   (he = hipStreamWaitEvent(c.aux_stream, fork, 0)) != hipSuccess))
     return c.check(he, "exact search fork");
-  size_t off = 0;
-  for (size_t bi = 0; bi < nb_all; bi++) {
+  for (const int bi : order) {
     const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
     const BucketLaunch& l = L[bi];
-    e.bucket = (queue ? bact_dev : bucket_dev) + off;
+    e.bucket = (queue ? bact_dev : bucket_dev) + P->bucket_off[bi];
     e.nbucket = nb;
     e.nbucket_dev = queue ? bcount_dev + bi : nullptr;
-    off += nb;
     const int rho = bk.rho_max;
     if (l.vb > 0) {
       if ((he = ensure_kernel_lds(c.device, (const void*)k_exact_base_disk, l.lds_d)) != hipSuccess)
@@ -2447,7 +2465,6 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
     if ((he = hipEventRecord(c.aux_events[bi], prep)) != hipSuccess) return c.check(he, "hipEventRecord");
   }
-  off = 0;
   hipStream_t lanes[kSearchLanes];
   const int nlanes = serial ? 1 : kSearchLanes;
   lanes[0] = c.stream;
@@ -2457,15 +2474,14 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
   }
   int nlaunch = 0;
-  for (size_t bi = 0; bi < nb_all; bi++) {
+  for (const int bi : order) {
     const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
     if (nb == 0) continue;
     const BucketLaunch& l = L[bi];
-    e.bucket = (queue ? bact_dev : bucket_dev) + off;
+    e.bucket = (queue ? bact_dev : bucket_dev) + P->bucket_off[bi];
     e.nbucket = nb;
     e.nbucket_dev = queue ? bcount_dev + bi : nullptr;
-    off += nb;
     hipStream_t sx = lanes[nlaunch++ % nlanes];
     if (!serial && (he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess)
       return c.check(he, "hipStreamWaitEvent");
