@@ -39,8 +39,9 @@ __device__ __forceinline__ void rotate_px(int dx, int dy, double c, double s, in
 // (|dx| + |dy|) 2^-52), well inside m = (|dx| + |dy| + 2) 2^-44, and round(x) can change only if x
 // lies within m of a half-integer. Likewise y = -s dx + c dy.
 __device__ __forceinline__ bool rot_near_half(int dx, int dy, double c, double s) {
-  const double x = c * (double)dx + s * (double)dy;
-  const double y = -s * (double)dx + c * (double)dy;
+  // (fused: x moves by less than one more rounding, far inside m)
+  const double x = fma(c, (double)dx, s * (double)dy);
+  const double y = fma(-s, (double)dx, c * (double)dy);
   const double m = (double)(abs(dx) + abs(dy) + 2) * 0x1p-44;
   const double fx = x - floor(x), fy = y - floor(y);
   return fabs(fx - 0.5) <= m || fabs(fy - 0.5) <= m;
