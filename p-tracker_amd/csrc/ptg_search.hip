@@ -580,22 +580,35 @@ __global__ void k_search_commit(SearchArgs a, int stage, double NCC_min, double 
                                 int32_t* __restrict__ next_list, int32_t* __restrict__ next_count,
                                 const StepBest* __restrict__ parts, int nparts,
                                 const int32_t* __restrict__ abort_flag) {
-  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per slot: its lanes read the slot's partial picks side by side and reduce them;
+  // `better` orders candidates by NCC, then by the reference's loop index, and admits no NaN
+  // and nothing at or below zero, so the reduction in any order gives the in-order loop's pick
+  const int li = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (li >= *a.count) return;
   if (a.act && !a.act[li]) return;  // a slot of the plan that is not in this stage's queue
   // a search that could not be served exactly (capacity flag set) changes nothing
   if (abort_flag && *abort_flag) return;
-  const int g = a.list[li];
-  ptg_grain& gr = a.grains[g];
   double best = 0.0;
   int bidx = INT_MAX;
-  for (int p = 0; p < nparts; p++) {
+  for (int p = lane; p < nparts; p += 64) {
     const StepBest sb = parts[(int64_t)li * nparts + p];
     if (better(sb.ncc, sb.idx, best, bidx)) {
       best = sb.ncc;
       bidx = sb.idx;
     }
   }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double ob = __shfl_xor(best, m, 64);
+    const int oi = __shfl_xor(bidx, m, 64);
+    if (better(ob, oi, best, bidx)) {
+      best = ob;
+      bidx = oi;
+    }
+  }
+  if (lane != 0) return;
+  const int g = a.list[li];
+  ptg_grain& gr = a.grains[g];
   const int U = a.left + a.right + 1, V = a.up + a.down + 1, S = U * V;
   int best_upix = 0, best_vpix = 0;
   double best_drot = 0.0;
@@ -638,8 +651,9 @@ hipError_t launch_search_commit(Context& c, const SearchArgs& a, int list_cap, i
                                 double NCC_min_super, int32_t* next_list, int32_t* next_count,
                                 const StepBest* parts, int nparts, const int32_t* abort_flag) {
   if (list_cap == 0) return hipSuccess;
-  k_search_commit<<<(list_cap + 127) / 128, 128, 0, c.stream>>>(a, stage, NCC_min, NCC_min_super, next_list,
-                                                                 next_count, parts, nparts, abort_flag);
+  // four slots (waves) per workgroup
+  k_search_commit<<<(list_cap + 3) / 4, 256, 0, c.stream>>>(a, stage, NCC_min, NCC_min_super, next_list,
+                                                             next_count, parts, nparts, abort_flag);
   return hipGetLastError();
 }
 
