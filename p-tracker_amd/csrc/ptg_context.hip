@@ -346,7 +346,18 @@ int need_images(Ctx* c, bool ref, bool cur) {
   return PTG_OK;
 }
 
-int do_precompute(Ctx* c) {
+// The ordered sums of the precomputation when they are still to be launched: on the context's
+// stream, before any reader.
+int flush_chain(Ctx* c) {
+  if (!c->chain_pending) return PTG_OK;
+  c->chain_pending = false;
+  CK(c, launch_precompute_chain(*c, c->stream));
+  return PTG_OK;
+}
+
+// defer: leave the ordered sums (C0C0, the sum of d0, d0) to the main stage's exact search, which
+// launches them beside its searches (chain_pending until then)
+int do_precompute(Ctx* c, bool defer) {
   TRY(need_grains(c));
   if (c->G == 0) return PTG_OK;
   TRY(need_images(c, true, false));
@@ -355,8 +366,9 @@ int do_precompute(Ctx* c) {
     // cos/sin(refrot) on the device, guarded (checked by the caller after its stages)
     hipEvent_t ev;
     c->mark_begin(PTG_T_PRECOMPUTE, &ev);
-    CK(c, launch_precompute(*c, true));
+    CK(c, launch_precompute(*c, true, !defer));
     c->mark_end(PTG_T_PRECOMPUTE, ev);
+    c->chain_pending = defer;
     c->require_precomputations = false;
     return PTG_OK;
   }
@@ -373,7 +385,7 @@ int do_precompute(Ctx* c) {
   c->mark_begin(PTG_T_PRECOMPUTE, &ev);
   CK(c, hipMemcpyAsync(c->d_cs0, cs0, sizeof(double) * 2 * c->G, hipMemcpyHostToDevice, c->stream));
   TRY(c->mark_staging());
-  CK(c, launch_precompute(*c, false));
+  CK(c, launch_precompute(*c, false, true));
   c->mark_end(PTG_T_PRECOMPUTE, ev);
   c->require_precomputations = false;
   return PTG_OK;
@@ -443,7 +455,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     return c->fail(PTG_ERR_ARG, "search zone extents must be >= 0");
   if ((int64_t)(z->left + z->right + 1) * (z->up + z->down + 1) > (1 << 24))
     return c->fail(PTG_ERR_LIMIT, "search zone too large");
-  if (c->require_precomputations) TRY(do_precompute(c));
+  if (c->require_precomputations) TRY(do_precompute(c, false));
   TRY(c->set_frames());
   TRY(make_steps(c, z, c->h_steps));
   const int nsteps = (int)c->h_steps.size();
@@ -473,7 +485,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
     CK(c, hipStreamSynchronize(c->stream));
   }
   const int nlist = (int)list.size();
-  if (nlist == 0) return PTG_OK;
+  if (nlist == 0) return flush_chain(c);
 
   // rotation tables: cos/sin(refrot + drot + step) per (grain, step), evaluated as the
   // reference (:687-689), on the device (guarded) or from the host's libm
@@ -588,6 +600,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
       c->flags_pending[stage] = true;
     }
   }
+  TRY(flush_chain(c));  // (the exact search did not take them: before any reader)
   if (method == PTG_SEARCH_PLAIN) {
     // a pattern whose offsets and smallest window tile exceed the LDS: the kernel that reads
     // the frame through the caches
@@ -1129,7 +1142,7 @@ int ptg_begin_pair(void* p) {
 int ptg_precompute(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return PTG_ERR_ARG;
-  return with_rot_guard(c, [c]() -> int { return do_precompute(c); });
+  return with_rot_guard(c, [c]() -> int { return do_precompute(c, false); });
 }
 
 int ptg_search(void* p, int stage, const ptg_search_zone* zone, double NCC_min, double NCC_min_super, int method) {
@@ -1168,7 +1181,7 @@ static int subpixel_check(Ctx* c, const ptg_subpix_params* prm) {
 // reset: the counters were already zeroed in this pair (track_pair does it before the
 // rescue stages' host round trip, so that only the kernel launch follows it)
 static int subpixel_run(Ctx* c, const ptg_subpix_params* prm, bool reset) {
-  if (c->require_precomputations) TRY(do_precompute(c));
+  if (c->require_precomputations) TRY(do_precompute(c, false));
   TRY(c->set_frames());
   if (reset) TRY(subpixel_reset(c));
   hipEvent_t ev;
@@ -1238,12 +1251,19 @@ int ptg_change_reference(void* p) {
 // The precomputations, when stale, for work outside a guarded stage: checked at once.
 static int ensure_precomputed(Ctx* c) {
   if (!c->require_precomputations) return PTG_OK;
-  return with_rot_guard(c, [c]() -> int { return do_precompute(c); });
+  return with_rot_guard(c, [c]() -> int { return do_precompute(c, false); });
 }
 
 static int track_pair_run(Ctx* c, const ptg_track_params* prm) {
   void* p = c;
-  if (c->require_precomputations) TRY(do_precompute(c));
+  // the ordered sums of the precomputation overlap the main stage's exact search; they are
+  // launched on the context's stream before any other reader, whichever way the pair ends
+  struct Flush {
+    Ctx* c;
+    ~Flush() { (void)flush_chain(c); }
+  } flush{c};
+  const bool defer = prm->rescue_level >= 0 && prm->search_method == PTG_SEARCH_EXACT && !c->rot_host;
+  if (c->require_precomputations) TRY(do_precompute(c, defer));
   if (prm->rescue_level >= 0) {
     TRY(ptg_begin_pair(p));
     TRY(do_search(c, PTG_STAGE_MAIN, &prm->search_zone, prm->NCC_min, prm->NCC_min_super, prm->search_method));

@@ -128,6 +128,9 @@ struct Context {
   // PTG_OPT_BUCKET_ORDER: the radius buckets' launches by increasing radius (0) or by
   // decreasing work (1)
   int bucket_order = 0;
+  // k_precompute ran without its ordered sums: k_precompute_chain is still to be launched
+  // (by the main stage's exact search beside its searches, else on `stream` before any reader)
+  bool chain_pending = false;
   std::string err;
 
   Image img[kMaxImages];
@@ -289,7 +292,10 @@ hipError_t ensure_kernel_lds(int dev, const void* fn, size_t bytes);
 hipError_t launch_begin_pair(Context& c);
 // dev_trig: cos/sin(refrot) on the device, with the guard of k_rot_tables (k = -1); else the
 // host's from d_cs0.
-hipError_t launch_precompute(Context& c, bool dev_trig);
+// dev_trig: cos/sin(refrot) on the device (guarded); chain false leaves C0C0, the sum of d0 and
+// d0 itself to launch_precompute_chain (on any stream ordered after it)
+hipError_t launch_precompute(Context& c, bool dev_trig, bool chain);
+hipError_t launch_precompute_chain(Context& c, hipStream_t st);
 hipError_t launch_set_frames(Context& c, const FrameDesc& f);
 // The rotation tables of a stage's grains on the device (grid: list_cap x step blocks).
 hipError_t launch_rot_tables(Context& c, const RotArgs& r, int list_cap);

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const FrameDesc* __r
                                                             uint32_t* __restrict__ spwords, double* __restrict__ m0,
                                                             int dev_trig, int guard_all,
                                                             RotGuardEntry* __restrict__ guard,
-                                                            int32_t* __restrict__ guard_count) {
+                                                            int32_t* __restrict__ guard_count, int do_chain) {
   __shared__ __align__(16) double chunk[2][kPreChunk];
   __shared__ long long part[kPreThreads / 64];
   __shared__ int near_flag;
@@ -162,6 +162,15 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const FrameDesc* __r
 #pragma unroll
   for (int w = 0; w < kPreThreads / 64; w++) tot += part[w];
   const double mean0 = (double)tot / (double)n;
+  if (!do_chain) {
+    // the ordered sums come from k_precompute_chain, launched where they can overlap other work
+    if (tid == 0) {
+      gr.mean0 = mean0;
+      m0[g] = mean0;
+      if (near_flag) rot_guard_push(guard, guard_count, g, -1, gr.refrot, c0, s0);
+    }
+    return;
+  }
   double C0C0 = 0.0, sd = 0.0;
   const int nch = (n + kPreChunk - 1) / kPreChunk;
   for (int ch = 0; ch <= nch; ch++) {
@@ -210,12 +219,92 @@ __global__ __launch_bounds__(kPreThreads) void k_precompute(const FrameDesc* __r
   }
 }
 
-hipError_t launch_precompute(Context& c, bool dev_trig) {
+hipError_t launch_precompute(Context& c, bool dev_trig, bool chain) {
   if (c.G == 0) return hipSuccess;
   k_precompute<<<c.G, kPreThreads, 0, c.stream>>>(c.d_frames, c.d_grains, c.d_pattern,
                                                   c.d_pbegin, c.d_cs0, c.d_p0, c.d_d0, c.d_sd0, c.d_pwords,
                                                   c.d_spwords, c.d_m0, dev_trig ? 1 : 0,
-                                                  c.rot_guard_test ? 1 : 0, c.d_guard, c.d_guard_count);
+                                                  c.rot_guard_test ? 1 : 0, c.d_guard, c.d_guard_count,
+                                                  chain ? 1 : 0);
+  return hipGetLastError();
+}
+
+// The ordered sums of do_precomputations (:645-652) after k_precompute without them: one wave
+// per grain (four per workgroup, so that few wave slots are held while other kernels run):
+// its lanes stage a chunk of d0 = I0 - mean0 in LDS (and store d0), then lane 0 adds the
+// chunk's squares and values in the reference's pixel order.
+constexpr int kChainChunk = 1024;
+
+__global__ __launch_bounds__(256) void k_precompute_chain(ptg_grain* __restrict__ grains,
+                                                          const int64_t* __restrict__ pbegin,
+                                                          const uint32_t* __restrict__ spwords,
+                                                          const double* __restrict__ m0, double* __restrict__ d0,
+                                                          double* __restrict__ sd0, int G) {
+  __shared__ __align__(16) double buf[4][kChainChunk];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + wave;
+  if (g >= G) return;  // (no workgroup barrier below)
+  const int64_t b = pbegin[g];
+  const int n = (int)(pbegin[g + 1] - b);
+  const double mean0 = m0[g];
+  double* cb = buf[wave];
+  double C0C0 = 0.0, sd = 0.0;
+  for (int base = 0; base < n; base += kChainChunk) {
+    const int m = min(kChainChunk, n - base);
+    constexpr int kB = 8;  // words in flight per lane before their stores
+    for (int t0 = lane; t0 < m; t0 += 64 * kB) {
+      uint32_t w[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++) w[j] = t0 + 64 * j < m ? spwords[b + base + t0 + 64 * j] : 0u;
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        const int t = t0 + 64 * j;
+        if (t < m) {
+          const double diff = (double)(w[j] >> 16) - mean0;
+          cb[t] = diff;
+          d0[b + base + t] = diff;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) {
+      const double2* src = reinterpret_cast<const double2*>(cb);
+      int t = 0;
+      for (; t + 16 <= m; t += 16) {
+        double2 q[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) q[j] = src[t / 2 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          C0C0 += q[j].x * q[j].x;
+          sd += q[j].x;
+          C0C0 += q[j].y * q[j].y;
+          sd += q[j].y;
+        }
+      }
+      for (; t < m; t++) {
+        const double diff = cb[t];
+        C0C0 += diff * diff;
+        sd += diff;
+      }
+    }
+    // the next chunk overwrites the buffer after lane 0 has read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane == 0) {
+    grains[g].C0C0 = C0C0;
+    sd0[g] = sd;
+  }
+}
+
+hipError_t launch_precompute_chain(Context& c, hipStream_t st) {
+  if (c.G == 0) return hipSuccess;
+  k_precompute_chain<<<(c.G + 3) / 4, 256, 0, st>>>(c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_d0,
+                                                     c.d_sd0, c.G);
   return hipGetLastError();
 }
 

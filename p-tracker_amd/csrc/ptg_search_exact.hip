@@ -2473,6 +2473,12 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     // the tables and cleared outputs above come first
     if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
   }
+  if (c.chain_pending) {
+    // the precomputation's ordered sums beside the searches: on the last search lane ahead of its
+    // searches, joined back to the library stream with them before k_exact_rescore reads C0C0
+    c.chain_pending = false;
+    if ((he = launch_precompute_chain(c, lanes[nlanes - 1])) != hipSuccess) return c.check(he, "k_precompute_chain");
+  }
   int nlaunch = 0;
   for (const int bi : order) {
     const Bucket& bk = P->buckets[bi];
