@@ -185,6 +185,10 @@ struct PinnedBuf {
   }
 };
 
+// Words of pin_flags: [2 stage, 2 stage + 1] a stage's capacity flags and plain-served grains,
+// [6 + stage] a device-queue stage's count, [9] the rescue queue's count read by track_pair.
+constexpr int kPinFlags = 10;
+
 // The context as seen through the C ABI, with the host-side staging it owns.
 struct Ctx : Context {
   PinnedBuf pin_cs, pin_steps, pin_cs0, pin_list, pin_flags;
@@ -228,6 +232,10 @@ struct Ctx : Context {
   // recorded after the last upload from the pinned staging buffers, waited on before
   // the host rewrites them
   hipEvent_t staging_done = nullptr;
+  // the rescue queue's count copied for the host while the subpixel stage runs (track_pair), and
+  // the stream that subpixel stage runs on beside the rescue stages, with its fork and join
+  hipEvent_t count_ev = nullptr, sp_fork = nullptr, sp_join = nullptr;
+  hipStream_t sp_stream = nullptr;
   int wait_staging() {
     if (staging_done) {
       hipError_t e = hipEventSynchronize(staging_done);
@@ -583,7 +591,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
       nparts = nsteps;
       if (devq) {
         // counted at the next synchronisation if the queue held grains
-        TRY(c->pin_flags.ensure(*c, 9 * sizeof(int32_t)));
+        TRY(c->pin_flags.ensure(*c, kPinFlags * sizeof(int32_t)));
         CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + 6 + stage, d_count, sizeof(int32_t),
                              hipMemcpyDeviceToHost, c->stream));
         c->unfit_pending[stage] = true;
@@ -594,7 +602,7 @@ int do_search(Ctx* c, int stage, const ptg_search_zone* z, double NCC_min, doubl
       ca = ax;
       ca.act = act;
       TRY(rc);
-      TRY(c->pin_flags.ensure(*c, 9 * sizeof(int32_t)));
+      TRY(c->pin_flags.ensure(*c, kPinFlags * sizeof(int32_t)));
       CK(c, hipMemcpyAsync(static_cast<int32_t*>(c->pin_flags.p) + 2 * stage, flag_dev, 2 * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
       c->flags_pending[stage] = true;
@@ -779,6 +787,7 @@ int ptg_destroy(void* p) {
   if (!c) return PTG_OK;
   (void)hipSetDevice(c->device);
   (void)c->finish_uploads();
+  if (c->sp_stream) (void)hipStreamSynchronize(c->sp_stream);
   (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < kMaxImages; i++)
     if (c->img[i].owned && c->img[i].d) (void)hipFree(c->img[i].d);
@@ -810,6 +819,9 @@ int ptg_destroy(void* p) {
   for (hipEvent_t e : c->search_joins) (void)hipEventDestroy(e);
   for (hipStream_t st : c->search_streams) (void)hipStreamDestroy(st);
   if (c->staging_done) (void)hipEventDestroy(c->staging_done);
+  for (hipEvent_t e : {c->count_ev, c->sp_fork, c->sp_join})
+    if (e) (void)hipEventDestroy(e);
+  if (c->sp_stream) (void)hipStreamDestroy(c->sp_stream);
   if (c->aux_stream) (void)hipStreamDestroy(c->aux_stream);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1180,14 +1192,17 @@ static int subpixel_check(Ctx* c, const ptg_subpix_params* prm) {
 
 // reset: the counters were already zeroed in this pair (track_pair does it before the
 // rescue stages' host round trip, so that only the kernel launch follows it)
-static int subpixel_run(Ctx* c, const ptg_subpix_params* prm, bool reset) {
+// sel, thr: every unmasked grain (0), or only those whose main-stage NCC reached thr (1) or the
+// rest (2)
+static int subpixel_run(Ctx* c, const ptg_subpix_params* prm, bool reset, int sel = 0, double thr = 0.0,
+                        hipStream_t st = nullptr) {
   if (c->require_precomputations) TRY(do_precompute(c, false));
   TRY(c->set_frames());
   if (reset) TRY(subpixel_reset(c));
   hipEvent_t ev;
-  c->mark_begin(PTG_T_SUBPIXEL, &ev);
-  CK(c, launch_subpixel(*c, *prm));
-  c->mark_end(PTG_T_SUBPIXEL, ev);
+  c->mark_begin(PTG_T_SUBPIXEL, &ev, st ? st : c->stream);
+  CK(c, launch_subpixel(*c, *prm, sel, thr, st));
+  c->mark_end(PTG_T_SUBPIXEL, ev, st ? st : c->stream);
   c->snap = Ctx::SNAP_INVALID;
   return PTG_OK;
 }
@@ -1269,13 +1284,55 @@ static int track_pair_run(Ctx* c, const ptg_track_params* prm) {
     TRY(do_search(c, PTG_STAGE_MAIN, &prm->search_zone, prm->NCC_min, prm->NCC_min_super, prm->search_method));
   }
   if (prm->subpixel) TRY(subpixel_reset(c));
-  if (prm->rescue_level >= 1)
-    TRY(do_search(c, PTG_STAGE_RESCUE, &prm->search_zone_rescue, prm->NCC_min, prm->NCC_min_super,
-                  prm->search_method));
-  if (prm->rescue_level >= 2)
-    TRY(do_search(c, PTG_STAGE_SUPER, &prm->search_zone_super_rescue, prm->NCC_min, prm->NCC_min_super,
-                  prm->search_method));
-  if (prm->subpixel) TRY(subpixel_run(c, &prm->subpix, false));
+  auto rescue_stages = [&]() -> int {
+    if (prm->rescue_level >= 1)
+      TRY(do_search(c, PTG_STAGE_RESCUE, &prm->search_zone_rescue, prm->NCC_min, prm->NCC_min_super,
+                    prm->search_method));
+    if (prm->rescue_level >= 2)
+      TRY(do_search(c, PTG_STAGE_SUPER, &prm->search_zone_super_rescue, prm->NCC_min, prm->NCC_min_super,
+                    prm->search_method));
+    return PTG_OK;
+  };
+  if (prm->subpixel && prm->rescue_level >= 1 && !c->rot_host && !c->device_queues && !c->graph_pair) {
+    // The rescue queue is read back by the host. The subpixel stage of the grains the main stage
+    // settled (NCC at least NCC_min, which no rescue stage moves) runs on a stream of its own,
+    // beside the rescue stages, while the host reads the queue's count: an empty queue then
+    // needs no more host round trips, else the rescue stages run and the subpixel stage of the
+    // other grains follows them. Each grain's work is the same as in one subpixel stage.
+    if (!c->sp_stream) {
+      hipStream_t st = nullptr;
+      CK(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      c->sp_stream = st;
+    }
+    for (hipEvent_t* e : {&c->count_ev, &c->sp_fork, &c->sp_join})
+      if (!*e) CK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    TRY(c->pin_flags.ensure(*c, kPinFlags * sizeof(int32_t)));
+    int32_t* pc = static_cast<int32_t*>(c->pin_flags.p) + 9;
+    CK(c, hipMemcpyAsync(pc, c->d_counts + PTG_STAGE_RESCUE, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    CK(c, hipEventRecord(c->count_ev, c->stream));
+    CK(c, hipEventRecord(c->sp_fork, c->stream));
+    CK(c, hipStreamWaitEvent(c->sp_stream, c->sp_fork, 0));
+    // the library stream waits for that stage at the end of this block, however it ends
+    struct Join {
+      Ctx* c;
+      ~Join() {
+        (void)hipEventRecord(c->sp_join, c->sp_stream);
+        (void)hipStreamWaitEvent(c->stream, c->sp_join, 0);
+      }
+    } join{c};
+    TRY(subpixel_run(c, &prm->subpix, false, 1, prm->NCC_min, c->sp_stream));
+    CK(c, hipEventSynchronize(c->count_ev));
+    if (*pc > 0) {
+      TRY(rescue_stages());
+      TRY(subpixel_run(c, &prm->subpix, false, 2, prm->NCC_min));
+    } else {
+      // nothing queued, so nothing reaches the super-rescue either: its queue count is zero
+      CK(c, fill_i32(*c, c->d_counts + PTG_STAGE_SUPER, 1, 0));
+    }
+  } else {
+    TRY(rescue_stages());
+    if (prm->subpixel) TRY(subpixel_run(c, &prm->subpix, false));
+  }
   TRY(ptg_end_pair(p));
   // a stage that could not be served exactly is reported here, after the pair's work
   if (!c->capturing && c->any_flags_pending()) TRY(c->sync_and_check());

@@ -362,7 +362,9 @@ hipError_t launch_equiproj(Context& c, const EquiprojArgs& a, const double* para
 int equiproj_partial_blocks(int64_t m);
 
 // subpixel
-hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p);
+// sel 0: every unmasked grain; 1: those whose NCC (the main stage's) is at least thr; 2: the rest
+hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p, int sel = 0, double thr = 0.0,
+                           hipStream_t st = nullptr);
 hipError_t launch_objective_batch(Context& c, int32_t n, const int32_t* d_ig, const double* d_X, double dmax,
                                   double stiff, double* d_f, double* d_grad);
 hipError_t launch_interp_batch(Context& c, int index, int32_t n, const double* d_xy, double* d_out);

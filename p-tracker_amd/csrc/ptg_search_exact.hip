@@ -2427,6 +2427,23 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
   if (!serial && ((he = hipEventRecord(fork, c.stream)) != hipSuccess ||
                   (he = hipStreamWaitEvent(c.aux_stream, fork, 0)) != hipSuccess))
     return c.check(he, "exact search fork");
+  hipStream_t lanes[kSearchLanes];
+  const int nlanes = serial ? 1 : kSearchLanes;
+  lanes[0] = c.stream;
+  for (int q = 1; q < nlanes; q++) {
+    lanes[q] = c.search_streams[q - 1];
+    // the tables and cleared outputs above come first
+    if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
+  }
+  if (c.chain_pending) {
+    // the precomputation's ordered sums beside the searches: on the last search lane ahead of its
+    // searches, joined back to the library stream with them before k_exact_rescore reads C0C0
+    c.chain_pending = false;
+    if ((he = launch_precompute_chain(c, lanes[nlanes - 1])) != hipSuccess) return c.check(he, "k_precompute_chain");
+  }
+  // each bucket's preparation, then its search, enqueued in turn (the first search is queued as
+  // soon as its own preparation, not after every bucket's)
+  int nlaunch = 0;
   for (const int bi : order) {
     const Bucket& bk = P->buckets[bi];
     const int nb = (int)bk.slots.size();
@@ -2464,30 +2481,7 @@ int run_search_exact(Context& c, const SearchArgs& sa, int nlist, const std::vec
     k_exact_template<<<dim3(nb, vm_spt(vm) * nmt), 256, l.lds_t, prep>>>(e);
     if ((he = hipGetLastError()) != hipSuccess) return c.check(he, "k_exact_template");
     if ((he = hipEventRecord(c.aux_events[bi], prep)) != hipSuccess) return c.check(he, "hipEventRecord");
-  }
-  hipStream_t lanes[kSearchLanes];
-  const int nlanes = serial ? 1 : kSearchLanes;
-  lanes[0] = c.stream;
-  for (int q = 1; q < nlanes; q++) {
-    lanes[q] = c.search_streams[q - 1];
-    // the tables and cleared outputs above come first
-    if ((he = hipStreamWaitEvent(lanes[q], fork, 0)) != hipSuccess) return c.check(he, "hipStreamWaitEvent");
-  }
-  if (c.chain_pending) {
-    // the precomputation's ordered sums beside the searches: on the last search lane ahead of its
-    // searches, joined back to the library stream with them before k_exact_rescore reads C0C0
-    c.chain_pending = false;
-    if ((he = launch_precompute_chain(c, lanes[nlanes - 1])) != hipSuccess) return c.check(he, "k_precompute_chain");
-  }
-  int nlaunch = 0;
-  for (const int bi : order) {
-    const Bucket& bk = P->buckets[bi];
-    const int nb = (int)bk.slots.size();
-    if (nb == 0) continue;
-    const BucketLaunch& l = L[bi];
-    e.bucket = (queue ? bact_dev : bucket_dev) + P->bucket_off[bi];
-    e.nbucket = nb;
-    e.nbucket_dev = queue ? bcount_dev + bi : nullptr;
+    // the search
     hipStream_t sx = lanes[nlaunch++ % nlanes];
     if (!serial && (he = hipStreamWaitEvent(sx, c.aux_events[bi], 0)) != hipSuccess)
       return c.check(he, "hipStreamWaitEvent");

@@ -935,12 +935,15 @@ __global__ __launch_bounds__(kSpThreads) __attribute__((amdgpu_waves_per_eu(4)))
                                                          const int32_t* __restrict__ list, int nlist,
                                                          ptg_subpix_params prm, unsigned long long* __restrict__ evals,
                                                          const int32_t* __restrict__ prad, SpLayout lay,
-                                                         int32_t* __restrict__ evals_g) {
+                                                         int32_t* __restrict__ evals_g, int sel, double thr) {
   const int li = blockIdx.x;
   if (li >= nlist) return;
   double* red = reinterpret_cast<double*>(sp_lds + kSpOffRed);
   const int g = list[li];
   ptg_grain& gr = grains[g];
+  // sel 1: only the grains whose main-stage NCC reached thr (those no rescue stage will move);
+  // 2: only the others (the same value in every thread of the workgroup)
+  if ((sel == 1 && !(gr.NCC >= thr)) || (sel == 2 && gr.NCC >= thr)) return;
   const int64_t b = pbegin[g];
   const int n = (int)(pbegin[g + 1] - b);
   Obj o;
@@ -1044,16 +1047,16 @@ static size_t sp_lds_bytes(const SpLayout& lay) {
 }
 
 
-hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p) {
+hipError_t launch_subpixel(Context& c, const ptg_subpix_params& p, int sel, double thr, hipStream_t st) {
   if (c.n_unmasked == 0) return hipSuccess;
   const SpLayout lay = sp_layout(c.max_prad_unmasked, p.subpix_displacement_max);
   const size_t lds = sp_lds_bytes(lay);
   hipError_t e = ensure_kernel_lds(c.device, (const void*)k_subpixel, lds);
   if (e != hipSuccess) return e;
-  k_subpixel<<<c.n_unmasked, kSpThreads, lds, c.stream>>>(
+  k_subpixel<<<c.n_unmasked, kSpThreads, lds, st ? st : c.stream>>>(
       c.d_frames, c.d_grains, c.d_pbegin, c.d_spwords, c.d_m0, c.d_p0, c.d_d0, c.d_sd0,
       c.d_list_lpt, c.n_unmasked,
-      p, reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, lay, c.d_evals_g);
+      p, reinterpret_cast<unsigned long long*>(c.d_evals), c.d_prad, lay, c.d_evals_g, sel, thr);
   return hipGetLastError();
 }
 
