@@ -354,8 +354,10 @@ __device__ __forceinline__ void make_request(const ObjConst& o, const double* X,
     rq->c1 = rc[1];
     rq->s1 = rc[2];
   } else {
-    rq->c1 = cos(rot);
-    rq->s1 = sin(rot);
+    double sv, cv;
+    sincos(rot, &sv, &cv);  // (one argument reduction for both)
+    rq->c1 = cv;
+    rq->s1 = sv;
     if (rc) {
       rc[0] = rot;
       rc[1] = rq->c1;
