@@ -145,9 +145,10 @@ int ptg_set_image_device(void* ctx, int index, const uint16_t* dev, int32_t dimx
  * read the slot's old contents), and the first later call that reads or replaces
  * image[index] waits for it on the GPU. Returns once the copy is queued: from pinned memory
  * (ptg_host_alloc) the copy reads `host` while later calls run, so `host` stays unchanged
- * until ptg_wait_image(ctx, index) or ptg_sync returns; pageable memory is read before
- * the call returns (the runtime stages it). The usual sequence: image k+1 into a slot the
- * running pair does not use, then ptg_track_pair on images k-1 and k. */
+ * until ptg_wait_image(ctx, index) or ptg_sync returns; from any other memory the call
+ * returns once the copy has read `host` (it then waits for the work queued before it). The
+ * usual sequence: image k+1 into a slot the running pair does not use, then ptg_track_pair
+ * on images k-1 and k. */
 int ptg_set_image_async(void* ctx, int index, const uint16_t* host, int32_t dimx, int32_t dimy,
                         int64_t pitch, int32_t bits);
 /* Waits on the host until image[index]'s last ptg_set_image_async copy has read its host

@@ -266,8 +266,9 @@ class Tracker:
         return arr
 
     def set_image_async(self, index: int, frame: np.ndarray, bits: int = 16):
-        """ptg_set_image_async: `frame` (C-contiguous uint16) must stay unchanged until
-        wait_image(index) or sync(); the array is kept alive until then."""
+        """ptg_set_image_async: a `frame` (C-contiguous uint16) in pinned memory (host_frame)
+        must stay unchanged until wait_image(index) or sync(), and is kept alive until then;
+        from other memory the call returns once the copy has read it."""
         if frame.dtype != np.uint16 or not frame.flags["C_CONTIGUOUS"]:
             raise PtgError("set_image_async needs a C-contiguous uint16 array")
         self._ck(self.lib.ptg_set_image_async(self.h, index, _ptr(frame), frame.shape[1], frame.shape[0],

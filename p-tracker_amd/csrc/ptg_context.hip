@@ -899,6 +899,17 @@ int ptg_set_image(void* p, int index, const uint16_t* host, int32_t dimx, int32_
   return PTG_OK;
 }
 
+// Whether host memory is page-locked memory of the HIP runtime (hipHostMalloc). For other
+// memory the query may fail; its error is cleared so that no later launch check reports it.
+static bool pinned_host(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
 int ptg_set_image_async(void* p, int index, const uint16_t* host, int32_t dimx, int32_t dimy, int64_t pitch,
                         int32_t bits) {
   Ctx* c = as_ctx(p);
@@ -915,6 +926,9 @@ int ptg_set_image_async(void* p, int index, const uint16_t* host, int32_t dimx, 
   CK(c, copy_frame(im.d, dimx, host, pitch, dimx, dimy, hipMemcpyHostToDevice, c->up_stream));
   CK(c, hipEventRecord(c->up_done[index], c->up_stream));
   c->up_pending[index] = true;
+  // pageable memory: returns once the copy has read it, whatever the runtime does with the
+  // source of an asynchronous copy from such memory
+  if (!pinned_host(host)) TRY(c->finish_upload(index));
   return PTG_OK;
 }
 

@@ -86,6 +86,11 @@ def test_overlapped_copy_orders_later_calls():
         say("copy from pinned memory, then read back")
         t.set_image_async(2, pinned, 12)
         assert np.array_equal(t.get_image(2, W, H), a)
+        say("pageable source changed as soon as the call returns")
+        src = b.copy()
+        t.set_image_async(2, src, 12)
+        src[...] = 0
+        assert np.array_equal(t.get_image(2, W, H), b)
         say("pageable source, then a synchronous copy into the same slot")
         t.set_image_async(2, np.ascontiguousarray(b), 12)
         t.set_image(2, a, 12)
