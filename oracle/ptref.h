@@ -74,7 +74,7 @@ typedef double (*ptref_fn3)(const double* x, void* user);
 double ptref_powell(ptref_fn3 f, void* user, double* x, const double* dx, double ftol,
                     int32_t max_iter, int64_t* nevals);
 
-/* The FP64 minimizer used to judge the LM form: Powell as above, then restarted from its
+/* The FP64 minimizer the LM form is checked against: Powell as above, then restarted from its
  * result with the initial axis directions until a restart no longer lowers the objective
  * (at most max_restarts). A finished Powell run can stall on a degenerate direction set in
  * a flat valley; a restart resets the set and continues downhill within the same basin. */
@@ -84,7 +84,7 @@ double ptref_minimize_fp64(ptref_fn3 f, void* user, double* x, const double* dx,
 /* follow_pattern_subpixel_xyR for one grain, src/p-tracker.cpp:1090-1128. With
  * prm->method == PTG_SUBPIX_LM the oracle runs ptref_minimize_fp64 instead of one Powell
  * run: the converged minimum of the objective in Powell's basin, which the LM form is
- * judged against. */
+ * checked against. */
 void ptref_subpixel(const uint16_t* I0, const uint16_t* I1, int64_t pitch, ptg_grain* g,
                     int32_t n, const int32_t* p0_xy, const ptg_subpix_params* prm,
                     int64_t* nevals);

@@ -79,7 +79,7 @@ __device__ __forceinline__ bool in_tile(const Obj& o, int ix, int iy, int& off) 
 //   w0 = t (-1/2 + t (1 - t/2)),  w1 = 1 + t^2 (-5/2 + 3t/2),
 //   w2 = t (1/2 + t (2 - 3t/2)),  w3 = t^2 (-1/2 + t/2),
 // and their derivatives in t. The same interpolant as the reference's, evaluated in another
-// order (the subpixel stage is judged to a tolerance, not bit for bit).
+// order (the subpixel stage is checked to a tolerance, not bit for bit).
 __device__ __forceinline__ void cr_weights(double t, double* w) {
   const double t2 = t * t;
   w[0] = t * fma(t, fma(-0.5, t, 1.0), -0.5);

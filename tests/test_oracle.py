@@ -1,5 +1,5 @@
 """Pins the oracle (CPU restatement of the reference path) against facts it did not
-produce: counts and values the survey measured from the reference's loops, the Gauss
+produce: counts and values measured from the reference's loops, the Gauss
 circle counts, the reference's own compiled sampler (oracle/_ref), exact rational
 brute force of the ZNCC argmax, closed forms and invariants. A dropped term, a wrong
 sign or index, or a transposed operand in the oracle fails at least one of these."""
