@@ -443,6 +443,7 @@ def autotune_section(pk, tr, w, prm, step, args):
     n = max(10, args.steps)
     tr.clear_tuning()
     heur = times(n)
+    tr.put_grains(w.grains)  # tuned on the state every timed pair starts from
     t5 = time.perf_counter()
     n_t, ms_h, ms_t = tr.autotune_search(w.params.search_zone)
     sec = time.perf_counter() - t5
